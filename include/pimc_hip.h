@@ -1,0 +1,170 @@
+/*
+ * pimc_hip.h -- C ABI of the MI355X (gfx950) sampling path of PIMC.jl.
+ *
+ * The library owns device memory, one HIP stream and the kernels; the caller
+ * owns every host buffer (copies are synchronous).  No exception or abort
+ * crosses this boundary: every entry point returns PIMC_OK (0) or a negative
+ * PIMC_ERR_* code, and pimc_last_error() describes the last failure.  A
+ * context is used by one host thread at a time; separate contexts may run
+ * concurrently on their own streams.  The library never calls back into the
+ * caller.
+ *
+ * Reference interfaces replaced (paths relative to the PIMC.jl tree):
+ *   System(potential; dim, M, N, L, T, lambda, ...)   src/system.jl:129-167   -> pimc_create
+ *   Particle.r (M x dim per particle)                  src/system.jl:1-6       -> pimc_set_paths / pimc_get_paths
+ *   run!(s, n, updates; Zmeasurements)                 src/simulation.jl:29-42 -> pimc_run
+ *   (u::ReshapeLinear)(s) with levy!                   src/updates/reshape.jl:31-91,
+ *                                                      src/updates/helper.jl:118-181 -> pimc_levy_propose / pimc_reshape_linear
+ *   (u::SingleCenterOfMass)(s), move_polymer!          src/updates/com.jl:136-224,
+ *                                                      src/updates/helper.jl:368-395 -> pimc_com_propose / pimc_center_of_mass
+ *   interaction_action!(w, s, fpcycle, j)              src/updates/helper.jl:306-324 -> pimc_interaction
+ *   (u::Energy)(s)                                     src/measurement.jl:92-122 -> pimc_energy
+ *   (d::Density)(s)                                    src/measurement.jl:45-55 -> pimc_density
+ */
+#ifndef PIMC_HIP_H
+#define PIMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIMC_OK 0
+#define PIMC_ERR_ARG (-1)      /* invalid argument or descriptor */
+#define PIMC_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define PIMC_ERR_STATE (-3)    /* call not valid in the current state */
+#define PIMC_ERR_CAPACITY (-4) /* a fixed-size device structure overflowed (e.g. a cell list) */
+#define PIMC_ERR_NOMEM (-5)
+
+/* external potentials: the reference passes a Julia closure `potential` and
+ * `dV`; a device needs an enum plus parameters */
+#define PIMC_POT_NONE 0
+#define PIMC_POT_HARMONIC 1 /* V = 1/2 omega^2 |r|^2, dV = omega^2 r */
+#define PIMC_POT_LATTICE 2  /* V = V0 sum_d cos^2(pi x_d / a)         */
+
+/* pair potentials, entering through the reference's lnU hook
+ * (src/system.jl:26-28) as a primitive link action */
+#define PIMC_PAIR_NONE 0
+#define PIMC_PAIR_GAUSSIAN 1 /* v(r) = g exp(-r^2 / (2 sigma^2)) */
+
+#define PIMC_FP32 0
+#define PIMC_FP64 1
+
+/* One independent chain (the reference's System).  hbar = k_B = 1. */
+typedef struct pimc_chain_desc {
+    int32_t N, M, dim;       /* particles, time slices, dimension (1 or 2) */
+    int32_t potential;       /* PIMC_POT_* */
+    int32_t pair;            /* PIMC_PAIR_* */
+    int32_t pair_in_accept;  /* 1: pair action enters the Metropolis ratio; 0: the
+                                reference's behaviour, where the pair sums of
+                                interaction_action! are computed and dropped */
+    int32_t ell;             /* staging segment length >= 2 (0: library default) */
+    int32_t reserved;
+    double beta, lambda, L;  /* L is the half-width: box [-L, L)^dim (src/system.jl:101) */
+    double omega;            /* harmonic trap */
+    double V0, a;            /* square lattice */
+    double g, sigma;         /* Gaussian pair repulsion */
+    double r_a;              /* cell-list range: nbins = floor(2L / r_a); 0 -> L/4 (src/system.jl:22-24) */
+    double com_step;         /* initial centre-of-mass step (SingleCenterOfMass(s, step)) */
+} pimc_chain_desc;
+
+typedef struct pimc_options {
+    int32_t precision;         /* PIMC_FP32 (positions and Delta S in FP32) or PIMC_FP64 */
+    int32_t sweeps_per_launch; /* sweeps per kernel launch = one estimator block (0: 50) */
+    int32_t radial_bins;       /* radial density histogram per chain (0: off) */
+    int32_t cart_bins;         /* the reference's Cartesian Density per chain (0: off) */
+    double radial_rmax;        /* radial histogram range [0, rmax) */
+    int32_t tile_beads;        /* beads staged per workgroup (0: autotuned) */
+    int32_t block_threads;     /* workgroup size (0: autotuned) */
+    int32_t use_graph;         /* replay launch sequences from a captured hipGraph */
+    int32_t derivatives;       /* run every chain whole per workgroup and record the
+                                  reweighting sums for d<E>/d omega, d<E>/d g */
+    uint64_t seed;             /* Philox key */
+} pimc_options;
+
+typedef struct pimc_counters {
+    int64_t sweeps;            /* sweeps run by this call */
+    int64_t bead_moves;        /* proposed beads: m-1 per staging try, M per COM try */
+    int64_t levy_tries, levy_acc, com_tries, com_acc;
+    double wall_ms;            /* device time of the whole call (events on the library stream) */
+    double kernel_ms;          /* summed device time of the sweep kernel launches */
+    int64_t kernel_launches;
+} pimc_counters;
+
+/* Estimates of one chain from the measured blocks (block means, standard
+ * error from their spread).  E: thermodynamic energy (src/measurement.jl:111)
+ * plus the pair energy; Ev: virial energy (src/measurement.jl:112) plus the
+ * pair virial. */
+typedef struct pimc_estimates {
+    int64_t n_blocks, n_meas;
+    double E, E_err, Ev, Ev_err;
+    double Epair, Epair_err;
+    double dSdomega, dSdg;             /* <dS/d omega>, <dS/d g> */
+    double dE_domega, dE_domega_err;   /* d<E>/d omega by reweighting */
+    double dE_dg, dE_dg_err;           /* d<E>/d g by reweighting */
+} pimc_estimates;
+
+typedef struct pimc_ctx pimc_ctx;
+
+/* System(...) for a batch of chains (src/system.jl:129-167).  Initial paths
+ * are zero; set them with pimc_set_paths. */
+int pimc_create(pimc_ctx **ctx, const pimc_chain_desc *chains, int32_t n_chains, const pimc_options *opt);
+int pimc_destroy(pimc_ctx *ctx);
+const char *pimc_last_error(const pimc_ctx *ctx);
+int pimc_device_info(char *name, int32_t len, int32_t *cu_count);
+
+/* Particle.r of every particle of one chain: N x dim x M doubles (each
+ * particle an M x dim column-major matrix, src/system.jl:1-6). */
+int pimc_set_paths(pimc_ctx *ctx, int32_t chain, const double *r);
+int pimc_get_paths(pimc_ctx *ctx, int32_t chain, double *r);
+int pimc_get_com_step(pimc_ctx *ctx, int32_t chain, double *step);
+int pimc_set_com_step(pimc_ctx *ctx, int32_t chain, double step);
+
+/* run!(s, n, [(1, SingleCenterOfMass), (1, ReshapeLinear)]; Zmeasurements)
+ * as batched sweeps (src/simulation.jl:29-42).  One sweep: per particle,
+ * staging segments tiling the ring at a random offset, then one centre-of-mass
+ * try.  measure = 0 thermalises (COM step adapted between launches, as
+ * adjust!(::Step) does); measure = 1 records one estimator block per launch. */
+int pimc_run(pimc_ctx *ctx, int32_t n_sweeps, int32_t measure, pimc_counters *out);
+int pimc_get_estimates(pimc_ctx *ctx, int32_t chain, pimc_estimates *out);
+/* per-block means of one chain: E, Ev, Epair per block (each n_blocks long) */
+int pimc_get_blocks(pimc_ctx *ctx, int32_t chain, double *E, double *Ev, double *Epair, int64_t cap, int64_t *n_blocks);
+int pimc_get_radial(pimc_ctx *ctx, int32_t chain, double *hist, int64_t *ndata);
+int pimc_get_cartesian(pimc_ctx *ctx, int32_t chain, double *hist, int64_t *ndata);
+int pimc_reset_estimators(pimc_ctx *ctx);
+
+/* ---- one reference move or estimator on the device (parity hooks) ---- */
+
+/* levy! + the ReshapeLinear weights for particle n, start slice j0 (0-based),
+ * m links, with injected normals ((m-1) x dim, row-major per bead).  Writes
+ * the (m+1) x dim bridge (column-major like r') and
+ * w = {wV_initial, wV_updated, wU_initial, wU_updated}. */
+int pimc_levy_propose(pimc_ctx *ctx, int32_t chain, int32_t n, int32_t j0, int32_t m, const double *normals,
+                      double *r_out, double *w_out);
+/* the same move followed by metropolis(exp(dlnW)) with uniform u; commits on accept */
+int pimc_reshape_linear(pimc_ctx *ctx, int32_t chain, int32_t n, int32_t j0, int32_t m, const double *normals,
+                        double u, int32_t *accepted);
+/* SingleCenterOfMass with displacement d[dim]: M x dim moved worldline and weights */
+int pimc_com_propose(pimc_ctx *ctx, int32_t chain, int32_t n, const double *d, double *r_out, double *w_out);
+int pimc_center_of_mass(pimc_ctx *ctx, int32_t chain, int32_t n, const double *d, double u, int32_t *accepted);
+/* interaction_action!(w, s, fpcycle, j): pair link sum of particle n at slice j */
+int pimc_interaction(pimc_ctx *ctx, int32_t chain, int32_t n, int32_t j, double *w);
+/* Energy(s) on the current paths: out = {E, Ev, Epair, Evpair} */
+int pimc_energy(pimc_ctx *ctx, int32_t chain, double *out);
+/* Density(s) on the current paths, accumulated into dens (nbins^dim,
+ * column-major); *ndata += M */
+int pimc_density(pimc_ctx *ctx, int32_t chain, int32_t nbins, double *dens, int64_t *ndata);
+
+/* Adjoint of the potential action S_pot = sum_links tau/2 (V + V') + S_pair:
+ * dS/d(beads) (N x dim x M), dS/domega and dS/dg on the current paths.
+ * The adjoint kernel recomputes bead distances instead of storing them. */
+int pimc_adjoint(pimc_ctx *ctx, int32_t chain, double *dS_dr, double *dS_dparams, double *S_out);
+
+/* potential action S_pot of one chain (for finite-difference checks) */
+int pimc_action(pimc_ctx *ctx, int32_t chain, double *S_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,983 @@
+/*
+ * pimc_oracle.c -- CPU FP64 restatement of PIMC.jl's sampling path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only the
+ * test-suite, the smoke check and the CPU-baseline leg of the benchmark load
+ * this library; the product never does.
+ *
+ * The reference is Julia and cannot run in this environment (no Julia
+ * toolchain), so this file restates its algorithm line by line.  Each
+ * function cites the reference file:line it follows.  Pinning of each part
+ * (closed forms, brute force, known-answer vectors) lives in tests/.
+ */
+#include "pimc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10, Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11.  Multipliers and Weyl constants are the published
+ * ones.                                                                    */
+/* ------------------------------------------------------------------------ */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)PHILOX_M0 * x0;
+        uint64_t p1 = (uint64_t)PHILOX_M1 * x2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        x0 = hi1 ^ x1 ^ k0;
+        x1 = lo1;
+        x2 = hi0 ^ x3 ^ k1;
+        x3 = lo0;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+double o_u01(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+
+void o_box_muller(uint32_t x0, uint32_t x1, double *n0, double *n1)
+{
+    const double two_pi = 6.283185307179586476925286766559;
+    double u0 = o_u01(x0), u1 = o_u01(x1);
+    double rad = sqrt(-2.0 * log(u0));
+    *n0 = rad * cos(two_pi * u1);
+    *n1 = rad * sin(two_pi * u1);
+}
+
+/* draw purposes: the fourth counter word of every keyed draw */
+enum { PUR_LEVY = 1, PUR_LEVY_ACC = 2, PUR_COM = 3, PUR_OFFSET = 4 };
+
+static void keyed(uint64_t seed, uint32_t c0, uint32_t sweep, uint32_t chain, uint32_t purpose,
+                  uint32_t out[4])
+{
+    uint32_t ctr[4] = {c0, sweep, chain, purpose};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    o_philox4x32_10(ctr, key, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* primitives, src/propagator.jl                                            */
+/* ------------------------------------------------------------------------ */
+
+/* src/propagator.jl:6-9: per-component absolute minimum-image offset */
+double o_distance(double x1, double x2, double L)
+{
+    double dx = fabs(x1 - x2);
+    double a = (2.0 * L) - dx;
+    return a < dx ? a : dx;
+}
+
+/* src/propagator.jl:30-32 */
+double o_teleport(double x, double L)
+{
+    return ((x + L) - floor(x / (2.0 * L) + 0.5) * (2.0 * L)) - L;
+}
+
+/* src/propagator.jl:16-19 (dot of the distance vector, then / (4 lambda tau)) */
+double o_lnK(const double *r1, const double *r2, int dim, double tau, double lambda, double L)
+{
+    double dd = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        double dr = o_distance(r1[d], r2[d], L);
+        dd += dr * dr;
+    }
+    return -dd / ((4.0 * lambda) * tau);
+}
+
+double o_V(const o_pot *p, const double *r, int dim)
+{
+    switch (p->kind) {
+    case O_POT_HARMONIC: {
+        double rr = 0.0;
+        for (int d = 0; d < dim; ++d) rr += r[d] * r[d];
+        return 0.5 * (p->omega * p->omega) * rr;
+    }
+    case O_POT_LATTICE: {
+        const double pi = 3.14159265358979323846264338327950288;
+        double v = 0.0;
+        for (int d = 0; d < dim; ++d) {
+            double c = cos(pi * r[d] / p->a);
+            v += c * c;
+        }
+        return p->V0 * v;
+    }
+    default:
+        return 0.0;
+    }
+}
+
+void o_dV(const o_pot *p, const double *r, int dim, double *out)
+{
+    for (int d = 0; d < dim; ++d) out[d] = 0.0;
+    if (p->kind == O_POT_HARMONIC) {
+        for (int d = 0; d < dim; ++d) out[d] = (p->omega * p->omega) * r[d];
+    } else if (p->kind == O_POT_LATTICE) {
+        const double pi = 3.14159265358979323846264338327950288;
+        /* d/dx cos^2(pi x/a) = -(pi/a) sin(2 pi x / a) */
+        for (int d = 0; d < dim; ++d) out[d] = -p->V0 * (pi / p->a) * sin(2.0 * pi * r[d] / p->a);
+    }
+}
+
+/* src/propagator.jl:26-28 */
+double o_lnV(const double *r1, const double *r2, int dim, double tau, const o_pot *p)
+{
+    return (-0.5 * tau) * (o_V(p, r1, dim) + o_V(p, r2, dim));
+}
+
+double o_vpair(const o_pairpot *pp, double r)
+{
+    if (pp->kind != O_PAIR_GAUSSIAN) return 0.0;
+    return pp->g * exp(-(r * r) / (2.0 * pp->sigma * pp->sigma));
+}
+
+/* src/system.jl:26-28, lnU = log(propint(r1_rel, r2_rel, tau)).  The
+ * primitive Gaussian pair action used by the interacting workloads enters as
+ * propint = exp(-tau/2 (v(|r1_rel|) + v(|r2_rel|))), so its log is taken in
+ * closed form.  r1rel, r2rel are the per-component |minimum-image| offsets of
+ * the pair at slices j and j+1. */
+double o_lnU(const double *r1rel, const double *r2rel, int dim, double tau, const o_pairpot *pp)
+{
+    double a = 0.0, b = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        a += r1rel[d] * r1rel[d];
+        b += r2rel[d] * r2rel[d];
+    }
+    return (-0.5 * tau) * (o_vpair(pp, sqrt(a)) + o_vpair(pp, sqrt(b)));
+}
+
+/* src/updates/helper.jl:3-5 */
+int o_metropolis(double delta, double u) { return (delta >= 1.0) || (delta > u); }
+
+/* src/updates/helper.jl:118-139 */
+void o_levy(double *rp, int nrows, int dim, double tau, double L, double lambda, const double *normals)
+{
+    const int last = nrows - 1;
+    for (int d = 0; d < dim; ++d) {
+        if (fabs(rp[d * nrows] - rp[d * nrows + last]) > L) {
+            double sg = rp[d * nrows] > 0.0 ? 1.0 : (rp[d * nrows] < 0.0 ? -1.0 : 0.0);
+            rp[d * nrows + last] += sg * (2.0 * L);
+        }
+    }
+    const int m = nrows - 2;
+    for (int j = 1; j <= m; ++j) {
+        double alpha = (double)(m + 1 - j) / (double)(m + 2 - j);
+        double sd = sqrt(((2.0 * lambda) * alpha) * tau);
+        for (int d = 0; d < dim; ++d) {
+            double a = alpha * rp[d * nrows + (j - 1)];
+            double b = (1.0 - alpha) * rp[d * nrows + last];
+            double c = normals[(j - 1) * dim + d] * sd;
+            rp[d * nrows + j] = (a + b) + c;
+        }
+    }
+    for (int j = 0; j < nrows; ++j)
+        for (int d = 0; d < dim; ++d) rp[d * nrows + j] = o_teleport(rp[d * nrows + j], L);
+}
+
+/* ------------------------------------------------------------------------ */
+/* system                                                                   */
+/* ------------------------------------------------------------------------ */
+static inline double R(const o_sys *s, int n, int d, int j) { return s->r[((size_t)n * s->dim + d) * s->M + j]; }
+static inline void bead(const o_sys *s, int n, int j, double *out)
+{
+    for (int d = 0; d < s->dim; ++d) out[d] = R(s, n, d, j);
+}
+static inline int mod1(int j, int M) { return ((j % M) + M) % M; } /* 0-based wrap */
+
+void o_sys_refresh_V(o_sys *s)
+{
+    double a[3], b[3];
+    for (int n = 0; n < s->N; ++n)
+        for (int j = 0; j < s->M; ++j) {
+            bead(s, n, j, a);
+            bead(s, n, (j + 1) % s->M, b);
+            s->V[(size_t)n * s->M + j] = o_lnV(a, b, s->dim, s->tau, &s->pot);
+        }
+}
+
+int o_sys_init(o_sys *s, int dim, int M, int N, double lambda, double L, double beta,
+               o_pot pot, o_pairpot pair, double r_a, int pair_in_accept, const double *r)
+{
+    memset(s, 0, sizeof(*s));
+    if (dim < 1 || dim > 2 || M < 3 || N < 1 || !(L > 0) || !(beta > 0) || !(lambda > 0)) return -1;
+    s->dim = dim; s->M = M; s->N = N;
+    s->lambda = lambda; s->L = L; s->beta = beta; s->tau = beta / M;
+    s->pot = pot; s->pair = pair; s->pair_in_accept = pair_in_accept;
+    /* src/system.jl:22-24: without interactions r_a defaults to L/4 */
+    if (r_a <= 0.0) r_a = L / 4.0;
+    s->nbins = (int)floor((2.0 * L) / r_a);          /* src/system.jl:81 */
+    if (s->nbins < 1) s->nbins = 1;
+    s->h = (2.0 * L) / s->nbins;                       /* src/nearest_neighbours.jl:127 */
+    s->r = (double *)malloc(sizeof(double) * (size_t)N * dim * M);
+    s->V = (double *)malloc(sizeof(double) * (size_t)N * M);
+    s->next = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+    if (!s->r || !s->V || !s->next) return -2;
+    memcpy(s->r, r, sizeof(double) * (size_t)N * dim * M);
+    for (int n = 0; n < N; ++n) s->next[n] = n;
+    o_sys_refresh_V(s);
+    return 0;
+}
+
+void o_sys_free(o_sys *s)
+{
+    free(s->r); free(s->V); free(s->next);
+    s->r = NULL; s->V = NULL; s->next = NULL;
+}
+
+/* src/nearest_neighbours.jl:8-33 (bin_index, bin), returned 0-based */
+int o_bin(const o_sys *s, const double *r)
+{
+    double w = (2.0 * s->L) / s->nbins;
+    int b = 0, stride = 1;
+    for (int d = 0; d < s->dim; ++d) {
+        long ib = (long)floor((r[d] + s->L) / w);
+        b += (int)ib * stride;
+        stride *= s->nbins;
+    }
+    return b;
+}
+
+/* periodic Euclidean distance of the BallTree metric (src/nearest_neighbours.jl:126-127) */
+static double periodic_norm(const o_sys *s, const double *a, const double *b)
+{
+    double q = 0.0;
+    for (int d = 0; d < s->dim; ++d) {
+        double dd = o_distance(a[d], b[d], s->L);
+        q += dd * dd;
+    }
+    return sqrt(q);
+}
+
+/* src/nearest_neighbours.jl:55-65 (bin_neighbors) + :72-104 (within_neigbourhood)
+ * + :106-154 (find_nns).  Candidate order: stencil order, then particle index
+ * (the reference's per-cell order depends on its update history; only the
+ * summation order of the pair terms depends on it). */
+int o_find_nns(const o_sys *s, const double *target, int j, int exclude, int *out, int cap)
+{
+    const int nb = s->nbins, dim = s->dim;
+    static const int st2[9][2] = {{0, 0}, {-1, 1}, {0, 1}, {1, 1}, {-1, 0}, {1, 0}, {-1, -1}, {0, -1}, {1, -1}};
+    static const int st1[3] = {0, -1, 1};
+    int b = o_bin(s, target);
+    int bx = b % nb, by = (dim >= 2) ? (b / nb) % nb : 0;
+    int ncells = (dim == 1) ? 3 : 9;
+    int cnt = 0;
+    double rk[3];
+    for (int c = 0; c < ncells; ++c) {
+        int cell;
+        if (dim == 1) {
+            cell = mod1(bx + st1[c], nb);
+        } else {
+            int cx = mod1(bx + st2[c][0], nb), cy = mod1(by + st2[c][1], nb);
+            cell = cx + nb * cy;
+            /* 3D is not part of the reference's stencil; dim 3 is rejected at init */
+        }
+        for (int k = 0; k < s->N; ++k) {
+            if (k == exclude) continue;
+            bead(s, k, j, rk);
+            if (o_bin(s, rk) != cell) continue;
+            if (periodic_norm(s, rk, target) <= s->h) {
+                if (cnt < cap) out[cnt] = k;
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
+/* next(): src/updates/helper.jl:287-300, 0-based (slice M-1 -> next particle, slice 0) */
+static void next_bead(const o_sys *s, int n, int j, int *nn, int *jn)
+{
+    *nn = (j == s->M - 1) ? (int)s->next[n] : n;
+    *jn = (j + 1) % s->M;
+}
+
+#define NNCAP 4096
+
+/* src/updates/helper.jl:306-324 for a 1-cycle (fpcycle(j) == n) */
+double o_interaction_old(const o_sys *s, int n, int j)
+{
+    int list[NNCAP];
+    double rn[3], rk[3], rkn[3], rnn[3], r1[3], r2[3];
+    bead(s, n, j, rn);
+    int cnt = o_find_nns(s, rn, j, n, list, NNCAP);
+    if (cnt > NNCAP) cnt = NNCAP;
+    double w = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+        int k = list[q], kn, kj, pn, pj;
+        next_bead(s, k, j, &kn, &kj);
+        next_bead(s, n, j, &pn, &pj);
+        bead(s, k, j, rk);
+        bead(s, kn, kj, rkn);
+        bead(s, pn, pj, rnn);
+        for (int d = 0; d < s->dim; ++d) {
+            r1[d] = o_distance(rk[d], rn[d], s->L);
+            r2[d] = o_distance(rkn[d], rnn[d], s->L);
+        }
+        w += o_lnU(r1, r2, s->dim, s->tau, &s->pair);
+    }
+    return w;
+}
+
+/* src/updates/helper.jl:326-347: rows jp, jp+1 of r' (0-based row jp), slice
+ * j0 + jp, the moving particle n excluded */
+double o_interaction_new(const o_sys *s, const double *rp, int nrows, int jp, int j0, int n)
+{
+    int list[NNCAP];
+    double t[3], t1[3], rk[3], rkn[3], r1[3], r2[3];
+    int j = mod1(j0 + jp, s->M);
+    for (int d = 0; d < s->dim; ++d) {
+        t[d] = rp[d * nrows + jp];
+        t1[d] = rp[d * nrows + jp + 1];
+    }
+    int cnt = o_find_nns(s, t, j, n, list, NNCAP);
+    if (cnt > NNCAP) cnt = NNCAP;
+    double w = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+        int k = list[q], kn, kj;
+        next_bead(s, k, j, &kn, &kj);
+        if (kn == n) continue; /* nn_nextbead[1] in exceptions */
+        bead(s, k, j, rk);
+        bead(s, kn, kj, rkn);
+        for (int d = 0; d < s->dim; ++d) {
+            r1[d] = o_distance(rk[d], t[d], s->L);
+            r2[d] = o_distance(rkn[d], t1[d], s->L);
+        }
+        w += o_lnU(r1, r2, s->dim, s->tau, &s->pair);
+    }
+    return w;
+}
+
+/* src/updates/helper.jl:349-366 for a 1-cycle: rp_all is M x dim (the
+ * displaced worldline), link j -> j+1 (mnext), neighbour set at slice j */
+double o_interaction_com(const o_sys *s, const double *rp_all, int j, int n)
+{
+    int list[NNCAP];
+    double t[3], t1[3], rk[3], rkn[3], r1[3], r2[3];
+    const int M = s->M;
+    int jn = (j + 1) % M;
+    for (int d = 0; d < s->dim; ++d) {
+        t[d] = rp_all[d * M + j];
+        t1[d] = rp_all[d * M + jn];
+    }
+    int cnt = o_find_nns(s, t, j, n, list, NNCAP);
+    if (cnt > NNCAP) cnt = NNCAP;
+    double w = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+        int k = list[q], kn, kj;
+        next_bead(s, k, j, &kn, &kj);
+        bead(s, k, j, rk);
+        bead(s, kn, kj, rkn);
+        for (int d = 0; d < s->dim; ++d) {
+            r1[d] = o_distance(rk[d], t[d], s->L);
+            r2[d] = o_distance(rkn[d], t1[d], s->L);
+        }
+        w += o_lnU(r1, r2, s->dim, s->tau, &s->pair);
+    }
+    return w;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ReshapeLinear, src/updates/reshape.jl:31-91                              */
+/* ------------------------------------------------------------------------ */
+int o_reshape_propose(const o_sys *s, int n, int j0, int m, const double *normals, double *rp, o_weights *w)
+{
+    const int M = s->M, dim = s->dim, nrows = m + 1;
+    if (m < 2 || m > M) return 0;
+    /* :56-58, 1-cycle: fpcycle(j) == n */
+    int jm = j0 + m;
+    for (int d = 0; d < dim; ++d) {
+        rp[d * nrows + 0] = R(s, n, d, j0);
+        rp[d * nrows + m] = R(s, n, d, mod1(jm, M));
+    }
+    /* :60 hardspherelevy! with a = 0 never re-draws: it equals levy! */
+    o_levy(rp, nrows, dim, s->tau, s->L, s->lambda, normals);
+    /* :66-77 */
+    double wVi = 0.0, wUi = 0.0, wUu = 0.0;
+    double Vp[4096 + 1];
+    double a[3], b[3];
+    for (int j = j0; j < jm; ++j) {
+        int jp = j - j0;
+        wVi += s->V[(size_t)n * M + mod1(j, M)];
+        if (s->pair.kind != O_PAIR_NONE) wUi += o_interaction_old(s, n, mod1(j, M));
+        for (int d = 0; d < dim; ++d) {
+            a[d] = rp[d * nrows + jp];
+            b[d] = rp[d * nrows + jp + 1];
+        }
+        Vp[jp] = o_lnV(a, b, dim, s->tau, &s->pot);
+        if (s->pair.kind != O_PAIR_NONE) wUu += o_interaction_new(s, rp, nrows, jp, j0, n);
+    }
+    double wVu = 0.0;
+    for (int jp = 0; jp < m; ++jp) wVu += Vp[jp];
+    w->wV_initial = wVi; w->wV_updated = wVu;
+    w->wU_initial = wUi; w->wU_updated = wUu;
+    return 1;
+}
+
+int o_reshape_apply(o_sys *s, int n, int j0, int m, const double *normals, double u, int *accepted, o_weights *w)
+{
+    const int M = s->M, dim = s->dim, nrows = m + 1;
+    double *rp = (double *)malloc(sizeof(double) * nrows * dim);
+    o_weights ww;
+    if (!o_reshape_propose(s, n, j0, m, normals, rp, &ww)) { free(rp); *accepted = 0; return 0; }
+    if (w) *w = ww;
+    double lw = ww.wV_updated - ww.wV_initial;
+    if (s->pair_in_accept) lw += ww.wU_updated - ww.wU_initial;
+    *accepted = o_metropolis(exp(lw), u);
+    if (*accepted) {
+        /* :82-86 rows 0..m-1 are written back (the start bead as teleported) */
+        double a[3], b[3];
+        for (int jp = 0; jp < m; ++jp) {
+            int j = mod1(j0 + jp, M);
+            for (int d = 0; d < dim; ++d) {
+                s->r[((size_t)n * dim + d) * M + j] = rp[d * nrows + jp];
+                a[d] = rp[d * nrows + jp];
+                b[d] = rp[d * nrows + jp + 1];
+            }
+            s->V[(size_t)n * M + j] = o_lnV(a, b, dim, s->tau, &s->pot);
+        }
+    }
+    free(rp);
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SingleCenterOfMass, src/updates/com.jl:136-224                           */
+/* ------------------------------------------------------------------------ */
+int o_com_propose(const o_sys *s, int n, const double *dsp, double *rp, o_weights *w)
+{
+    const int M = s->M, dim = s->dim;
+    /* :168-175 */
+    double wVi = 0.0, wUi = 0.0;
+    for (int j = 0; j < M; ++j) wVi += s->V[(size_t)n * M + j];
+    if (s->pair.kind != O_PAIR_NONE)
+        for (int j = 0; j < M; ++j) wUi += o_interaction_old(s, n, j);
+    /* move_polymer!, src/updates/helper.jl:368-395 (a = 0: no re-draw) */
+    for (int j = 0; j < M; ++j)
+        for (int d = 0; d < dim; ++d) rp[d * M + j] = o_teleport(R(s, n, d, j) + dsp[d], s->L);
+    /* :189-201 */
+    double wVu = 0.0, wUu = 0.0, a[3], b[3];
+    for (int j = 0; j < M; ++j) {
+        int jn = (j + 1) % M;
+        for (int d = 0; d < dim; ++d) { a[d] = rp[d * M + j]; b[d] = rp[d * M + jn]; }
+        wVu += o_lnV(a, b, dim, s->tau, &s->pot);
+        if (s->pair.kind != O_PAIR_NONE) wUu += o_interaction_com(s, rp, j, n);
+    }
+    w->wV_initial = wVi; w->wV_updated = wVu;
+    w->wU_initial = wUi; w->wU_updated = wUu;
+    return 1;
+}
+
+int o_com_apply(o_sys *s, int n, const double *dsp, double u, int *accepted, o_weights *w)
+{
+    const int M = s->M, dim = s->dim;
+    double *rp = (double *)malloc(sizeof(double) * M * dim);
+    o_weights ww;
+    o_com_propose(s, n, dsp, rp, &ww);
+    if (w) *w = ww;
+    double lw = ww.wV_updated - ww.wV_initial;
+    if (s->pair_in_accept) lw += ww.wU_updated - ww.wU_initial;
+    *accepted = o_metropolis(exp(lw), u);
+    if (*accepted) {
+        for (int j = 0; j < M; ++j)
+            for (int d = 0; d < dim; ++d) s->r[((size_t)n * dim + d) * M + j] = rp[d * M + j];
+        o_sys_refresh_V(s); /* :211-219 store V' per link: same values */
+    }
+    free(rp);
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* estimators, src/measurement.jl                                           */
+/* ------------------------------------------------------------------------ */
+/* :92-122 */
+void o_energy(const o_sys *s, double *E, double *Ev)
+{
+    const int M = s->M, dim = s->dim;
+    double link = 0.0, pot = 0.0, vkin = 0.0, vpot = 0.0;
+    double a[3], b[3], g[3];
+    for (int i = 0; i < s->N; ++i)
+        for (int j = 0; j < M; ++j) {
+            int inext = (j == M - 1) ? (int)s->next[i] : i;
+            int jnext = (j + 1) % M;
+            bead(s, i, j, a);
+            bead(s, inext, jnext, b);
+            double dd = 0.0;
+            for (int d = 0; d < dim; ++d) {
+                double dr = o_distance(a[d], b[d], s->L);
+                dd += dr * dr;
+            }
+            link += dd;
+            pot += o_V(&s->pot, a, dim) + o_V(&s->pot, b, dim);
+            o_dV(&s->pot, a, dim, g);
+            double rg = 0.0;
+            for (int d = 0; d < dim; ++d) rg += a[d] * g[d];
+            vkin += rg;
+            vpot += o_V(&s->pot, a, dim) + o_V(&s->pot, b, dim);
+        }
+    double tau = s->tau;
+    *E = ((double)(dim * s->N) / (2.0 * tau) - (1.0 / (((4.0 * s->lambda) * (tau * tau)) * M)) * link) +
+         (1.0 / (2.0 * M)) * pot;
+    *Ev = (1.0 / (2.0 * M)) * vkin + (1.0 / (2.0 * M)) * vpot;
+}
+
+/* Pair energy of the link action S_U = -sum lnU (build addition; the
+ * reference's Energy has no pair term).  Thermodynamic: <S_U>/beta.  Virial:
+ * <S_U>/beta + (1/(2 beta)) sum r . dS_U/dr, the gradient taken with the
+ * neighbour indicator held fixed. */
+void o_pair_energy(const o_sys *s, double *Epair, double *Evir_pair)
+{
+    *Epair = 0.0; *Evir_pair = 0.0;
+    if (s->pair.kind == O_PAIR_NONE) return;
+    const int M = s->M, dim = s->dim;
+    int list[NNCAP];
+    double ri[3], rk[3], rin[3], rkn[3];
+    double SU = 0.0, rgrad = 0.0;
+    const double s2 = s->pair.sigma * s->pair.sigma;
+    for (int i = 0; i < s->N; ++i)
+        for (int j = 0; j < M; ++j) {
+            bead(s, i, j, ri);
+            int cnt = o_find_nns(s, ri, j, i, list, NNCAP);
+            if (cnt > NNCAP) cnt = NNCAP;
+            int jn = (j + 1) % M;
+            for (int q = 0; q < cnt; ++q) {
+                int k = list[q];
+                bead(s, k, j, rk);
+                bead(s, i, jn, rin);
+                bead(s, k, jn, rkn);
+                double a2 = 0.0, b2 = 0.0;
+                double da[3], db[3];
+                for (int d = 0; d < dim; ++d) {
+                    /* signed minimum-image separations for the gradient */
+                    double x = ri[d] - rk[d];
+                    x -= (2.0 * s->L) * floor(x / (2.0 * s->L) + 0.5);
+                    double y = rin[d] - rkn[d];
+                    y -= (2.0 * s->L) * floor(y / (2.0 * s->L) + 0.5);
+                    da[d] = x; db[d] = y;
+                    a2 += x * x; b2 += y * y;
+                }
+                double va = o_vpair(&s->pair, sqrt(a2)), vb = o_vpair(&s->pair, sqrt(b2));
+                /* each unordered pair is visited from both ends: weight 1/2 */
+                SU += 0.5 * (0.5 * s->tau) * (va + vb);
+                /* grad of v(|x|) wrt x_i is -x/s2 v; ri.grad_i + rk.grad_k = (ri-rk).grad */
+                double ga = 0.0, gb = 0.0;
+                for (int d = 0; d < dim; ++d) { ga += da[d] * da[d]; gb += db[d] * db[d]; }
+                rgrad += 0.5 * (0.5 * s->tau) * (-(ga / s2) * va - (gb / s2) * vb);
+            }
+        }
+    *Epair = SU / s->beta;
+    *Evir_pair = SU / s->beta + rgrad / (2.0 * s->beta);
+}
+
+/* :45-55 */
+void o_density(const o_sys *s, int nbins, double *dens, int64_t *ndata)
+{
+    const double binw = (2.0 * s->L) / nbins;
+    for (int n = 0; n < s->N; ++n)
+        for (int m = 0; m < s->M; ++m) {
+            long ib[3];
+            int ok = 1;
+            for (int d = 0; d < s->dim; ++d) {
+                ib[d] = (long)floor((R(s, n, d, m) + s->L) / binw);
+                if (!(ib[d] > 0 && ib[d] < nbins + 1)) ok = 0;
+            }
+            if (ok) {
+                /* Julia stores at 1-based index ibin, i.e. 0-based ibin-1;
+                 * dens is nbins^dim column-major like the Julia Array */
+                size_t idx = 0, stride = 1;
+                for (int d = 0; d < s->dim; ++d) { idx += (size_t)(ib[d] - 1) * stride; stride *= (size_t)nbins; }
+                dens[idx] += 1.0;
+            }
+        }
+    *ndata += s->M;
+}
+
+void o_radial(const o_sys *s, int nb, double rmax, double *hist, int64_t *ndata)
+{
+    const double w = rmax / nb;
+    for (int n = 0; n < s->N; ++n)
+        for (int m = 0; m < s->M; ++m) {
+            double rr = 0.0;
+            for (int d = 0; d < s->dim; ++d) rr += R(s, n, d, m) * R(s, n, d, m);
+            double q = sqrt(rr) / w;
+            if (q < nb) hist[(int)q] += 1.0;
+        }
+    *ndata += s->M;
+}
+
+/* ------------------------------------------------------------------------ */
+/* brute-force full actions                                                  */
+/* ------------------------------------------------------------------------ */
+double o_total_lnV(const o_sys *s)
+{
+    double a[3], b[3], w = 0.0;
+    for (int n = 0; n < s->N; ++n)
+        for (int j = 0; j < s->M; ++j) {
+            bead(s, n, j, a);
+            bead(s, n, (j + 1) % s->M, b);
+            w += o_lnV(a, b, s->dim, s->tau, &s->pot);
+        }
+    return w;
+}
+
+double o_total_lnK(const o_sys *s)
+{
+    double a[3], b[3], w = 0.0;
+    for (int n = 0; n < s->N; ++n)
+        for (int j = 0; j < s->M; ++j) {
+            bead(s, n, j, a);
+            bead(s, n, (j + 1) % s->M, b);
+            w += o_lnK(a, b, s->dim, s->tau, s->lambda, s->L);
+        }
+    return w;
+}
+
+/* sum over links of sum over unordered pairs, by O(N^2) brute force with the
+ * slice-j radius test (no cell list): pins the cell-list enumeration */
+double o_total_lnU(const o_sys *s)
+{
+    if (s->pair.kind == O_PAIR_NONE) return 0.0;
+    double ri[3], rk[3], rin[3], rkn[3], r1[3], r2[3], w = 0.0;
+    for (int j = 0; j < s->M; ++j) {
+        int jn = (j + 1) % s->M;
+        for (int i = 0; i < s->N; ++i)
+            for (int k = i + 1; k < s->N; ++k) {
+                bead(s, i, j, ri);
+                bead(s, k, j, rk);
+                if (periodic_norm(s, ri, rk) > s->h) continue;
+                bead(s, i, jn, rin);
+                bead(s, k, jn, rkn);
+                for (int d = 0; d < s->dim; ++d) {
+                    r1[d] = o_distance(rk[d], ri[d], s->L);
+                    r2[d] = o_distance(rkn[d], rin[d], s->L);
+                }
+                w += o_lnU(r1, r2, s->dim, s->tau, &s->pair);
+            }
+    }
+    return w;
+}
+
+/* ------------------------------------------------------------------------ */
+/* the reference schedule, src/simulation.jl:29-42                          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint64_t seed, ctr;
+    uint32_t buf[4];
+    int have;
+} stream_t;
+
+static uint32_t stream_next(stream_t *st)
+{
+    if (st->have == 0) {
+        uint32_t c[4] = {(uint32_t)st->ctr, (uint32_t)(st->ctr >> 32), 0xC0FFEEu, 0u};
+        uint32_t k[2] = {(uint32_t)st->seed, (uint32_t)(st->seed >> 32)};
+        o_philox4x32_10(c, k, st->buf);
+        st->ctr++;
+        st->have = 4;
+    }
+    return st->buf[4 - st->have--];
+}
+static double stream_u(stream_t *st) { return o_u01(stream_next(st)); }
+static int stream_int(stream_t *st, int lo, int hi) /* uniform in lo..hi */
+{
+    uint64_t span = (uint64_t)(hi - lo + 1);
+    return lo + (int)(((uint64_t)stream_next(st) * span) >> 32);
+}
+static double stream_normal(stream_t *st)
+{
+    double n0, n1;
+    uint32_t a = stream_next(st), b = stream_next(st);
+    o_box_muller(a, b, &n0, &n1);
+    return n0;
+}
+
+/* Counter: sliding window of accept flags (src/updates/helper.jl:6-11) */
+typedef struct {
+    unsigned char *q;
+    int range, head, len, sum;
+    int64_t tries;
+} window_t;
+static void win_init(window_t *w, int range) { w->q = (unsigned char *)calloc(range, 1); w->range = range; w->head = 0; w->len = 0; w->sum = 0; w->tries = 0; }
+static void win_free(window_t *w) { free(w->q); }
+/* queue! (src/simulation.jl:3-10) */
+static void win_push(window_t *w, int acc)
+{
+    w->tries++;
+    if (w->len == w->range) { w->sum -= w->q[w->head]; w->q[w->head] = (unsigned char)acc; w->head = (w->head + 1) % w->range; }
+    else { w->q[(w->head + w->len) % w->range] = (unsigned char)acc; w->len++; }
+    w->sum += acc;
+}
+static double win_acceptance(const window_t *w) { return w->len ? (double)w->sum / w->len : 0.0; }
+
+void o_refstate_init(o_refstate *st, const o_sys *s, int cap, double step)
+{
+    memset(st, 0, sizeof(*st));
+    int maxslices = s->M - 2;
+    st->cap = cap < maxslices ? cap : maxslices;       /* src/updates/reshape.jl:25 */
+    st->step = step;
+    st->win_range = 10000;                               /* range = 10_000 */
+    st->adj = 10;                                        /* adj = 10 */
+}
+
+int o_run_reference(o_sys *s, o_refstate *st, int64_t n_iter, uint64_t seed, int ncycle,
+                    double *E, double *Ev, int64_t nE)
+{
+    stream_t rs = {seed, st->rng_counter, {0, 0, 0, 0}, 0};
+    window_t wl, wc;
+    win_init(&wl, st->win_range);
+    win_init(&wc, st->win_range);
+    const int M = s->M, dim = s->dim;
+    int maxslices = M - 2, nctr = 0;
+    double *normals = (double *)malloc(sizeof(double) * (size_t)(M + 1) * dim);
+    for (int64_t it = 0; it < n_iter; ++it) {
+        /* sample(fs, Weights(1 ./ every)) with every = (1, 1): COM first */
+        int is_com = stream_u(&rs) < 0.5;
+        if (is_com) {
+            /* SingleCenterOfMass: all particles are 1-cycles */
+            int n = stream_int(&rs, 0, s->N - 1);
+            double dsp[3];
+            for (int d = 0; d < dim; ++d) dsp[d] = (st->step * 2.0) * (stream_u(&rs) - 0.5);
+            double *rp = (double *)malloc(sizeof(double) * M * dim);
+            o_weights w;
+            o_com_propose(s, n, dsp, rp, &w);
+            double lw = w.wV_updated - w.wV_initial;
+            if (s->pair_in_accept) lw += w.wU_updated - w.wU_initial;
+            double delta = exp(lw);
+            int acc = (delta >= 1.0) || (delta > stream_u(&rs));
+            if (acc) {
+                for (int j = 0; j < M; ++j)
+                    for (int d = 0; d < dim; ++d) s->r[((size_t)n * dim + d) * M + j] = rp[d * M + j];
+                double a[3], b[3];
+                for (int j = 0; j < M; ++j) {
+                    for (int d = 0; d < dim; ++d) { a[d] = rp[d * M + j]; b[d] = rp[d * M + (j + 1) % M]; }
+                    s->V[(size_t)n * M + j] = o_lnV(a, b, dim, s->tau, &s->pot);
+                }
+            }
+            free(rp);
+            win_push(&wc, acc);
+            st->com_tries++; st->com_acc += acc; st->bead_moves += M;
+            /* apply!: adjust every adj tries (src/simulation.jl:19-27, helper.jl:22-32) */
+            if (wc.tries % st->adj == 0) {
+                double a = win_acceptance(&wc);
+                if (a < 0.4) st->step *= 0.9;
+                else if (a > 0.6) st->step *= 1.1;
+                if (st->step < 0.1) st->step = 0.1;
+                if (st->step > s->L / 2) st->step = s->L / 2;
+            }
+        } else {
+            int n = stream_int(&rs, 0, s->N - 1);
+            int j0 = stream_int(&rs, 0, M - 1);
+            int m = stream_int(&rs, 2, st->cap);
+            if (m > maxslices) m = maxslices;
+            for (int q = 0; q < (m - 1) * dim; ++q) normals[q] = stream_normal(&rs);
+            double *rp = (double *)malloc(sizeof(double) * (m + 1) * dim);
+            o_weights w;
+            o_reshape_propose(s, n, j0, m, normals, rp, &w);
+            double lw = w.wV_updated - w.wV_initial;
+            if (s->pair_in_accept) lw += w.wU_updated - w.wU_initial;
+            double delta = exp(lw);
+            int acc = (delta >= 1.0) || (delta > stream_u(&rs));
+            if (acc) {
+                double a[3], b[3];
+                for (int jp = 0; jp < m; ++jp) {
+                    int j = mod1(j0 + jp, M);
+                    for (int d = 0; d < dim; ++d) {
+                        s->r[((size_t)n * dim + d) * M + j] = rp[d * (m + 1) + jp];
+                        a[d] = rp[d * (m + 1) + jp];
+                        b[d] = rp[d * (m + 1) + jp + 1];
+                    }
+                    s->V[(size_t)n * M + j] = o_lnV(a, b, dim, s->tau, &s->pot);
+                }
+            }
+            free(rp);
+            win_push(&wl, acc);
+            st->levy_tries++; st->levy_acc += acc; st->bead_moves += m - 1;
+            if (wl.tries % st->adj == 0) {
+                double a = win_acceptance(&wl);
+                if (a < 0.6) st->cap -= 1;
+                else if (a > 0.8) st->cap += 1;
+                if (st->cap < 2) st->cap = 2;
+                if (st->cap > maxslices) st->cap = maxslices;
+            }
+        }
+        /* measurement_Z_sector, src/measurement.jl:1-17 */
+        if (E) {
+            if (++nctr == ncycle) {
+                nctr = 0;
+                if (st->n_meas < nE) o_energy(s, &E[st->n_meas], &Ev[st->n_meas]);
+                st->n_meas++;
+            }
+        }
+    }
+    st->rng_counter = rs.ctr;
+    free(normals);
+    win_free(&wl);
+    win_free(&wc);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* the sweep schedule of the HIP kernels                                     */
+/* ------------------------------------------------------------------------ */
+int64_t o_sweep_bead_moves(int N, int M, int ell)
+{
+    int64_t per = 0;
+    int nseg = (M + ell - 1) / ell;
+    for (int k = 0; k < nseg; ++k) {
+        int m = (M - k * ell) < ell ? (M - k * ell) : ell;
+        if (m >= 2) per += m - 1;
+    }
+    return (int64_t)N * (per + M);
+}
+
+double o_adapt_step(double step, int64_t acc, int64_t tries, double L)
+{
+    if (tries <= 0) return step;
+    double a = (double)acc / (double)tries;
+    if (a < 0.4) step *= 0.9;
+    else if (a > 0.6) step *= 1.1;
+    if (step < 0.1) step = 0.1;
+    if (step > L / 2) step = L / 2;
+    return step;
+}
+
+/* pair link sum for particle n over link j -> j+1 with its beads at slices j
+ * and j+1 replaced by pj, pj1 (neighbour set from slice j) */
+static double link_pair(const o_sys *s, int n, int j, const double *pj, const double *pj1)
+{
+    int list[NNCAP];
+    double rk[3], rkn[3], r1[3], r2[3];
+    int jn = (j + 1) % s->M;
+    int cnt = o_find_nns(s, pj, j, n, list, NNCAP);
+    if (cnt > NNCAP) cnt = NNCAP;
+    double w = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+        int k = list[q];
+        bead(s, k, j, rk);
+        bead(s, k, jn, rkn);
+        for (int d = 0; d < s->dim; ++d) {
+            r1[d] = o_distance(rk[d], pj[d], s->L);
+            r2[d] = o_distance(rkn[d], pj1[d], s->L);
+        }
+        w += o_lnU(r1, r2, s->dim, s->tau, &s->pair);
+    }
+    return w;
+}
+
+void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
+             o_sweepstats *st)
+{
+    const int M = s->M, dim = s->dim, N = s->N;
+    const int nseg = (M + ell - 1) / ell;
+    const int use_pair = (s->pair.kind != O_PAIR_NONE) && s->pair_in_accept;
+    double *rp = (double *)malloc(sizeof(double) * (size_t)(M + 1) * dim);
+    double *nrm = (double *)malloc(sizeof(double) * (size_t)(M + 1) * dim);
+    uint32_t x[4];
+    /* Levy phase: particles in order; the segments of one particle have
+     * disjoint interiors and are independent given their end beads. */
+    for (int p = 0; p < N; ++p) {
+        keyed(seed, (uint32_t)p, sweep, chain, PUR_OFFSET, x);
+        int off = (int)(((uint64_t)x[0] * (uint64_t)ell) >> 32);
+        for (int k = 0; k < nseg; ++k) {
+            int m = (M - k * ell) < ell ? (M - k * ell) : ell;
+            if (m < 2) continue;
+            int start = (off + k * ell) % M;
+            const int nrows = m + 1;
+            for (int d = 0; d < dim; ++d) {
+                rp[d * nrows + 0] = R(s, p, d, start);
+                rp[d * nrows + m] = R(s, p, d, (start + m) % M);
+            }
+            for (int i = 1; i < m; ++i) {
+                int b = (start + i) % M;
+                keyed(seed, (uint32_t)(p * M + b), sweep, chain, PUR_LEVY, x);
+                double n0, n1;
+                o_box_muller(x[0], x[1], &n0, &n1);
+                nrm[(i - 1) * dim + 0] = n0;
+                if (dim > 1) nrm[(i - 1) * dim + 1] = n1;
+            }
+            /* the bridge of levy!, with the fixed ends kept as stored */
+            double e0[3], em[3];
+            for (int d = 0; d < dim; ++d) { e0[d] = rp[d * nrows]; em[d] = rp[d * nrows + m]; }
+            o_levy(rp, nrows, dim, s->tau, s->L, s->lambda, nrm);
+            for (int d = 0; d < dim; ++d) { rp[d * nrows] = e0[d]; rp[d * nrows + m] = em[d]; }
+            /* Delta ln W = -tau sum_interior (V' - V) + Delta lnU */
+            double dv = 0.0, a[3], b[3];
+            for (int i = 1; i < m; ++i) {
+                int j = (start + i) % M;
+                for (int d = 0; d < dim; ++d) { a[d] = rp[d * nrows + i]; b[d] = R(s, p, d, j); }
+                dv += o_V(&s->pot, a, dim) - o_V(&s->pot, b, dim);
+            }
+            double lw = -s->tau * dv;
+            if (use_pair) {
+                double du = 0.0, c0[3], c1[3];
+                for (int t = 0; t < m; ++t) {
+                    int j = (start + t) % M, jn = (j + 1) % M;
+                    for (int d = 0; d < dim; ++d) {
+                        a[d] = rp[d * nrows + t]; b[d] = rp[d * nrows + t + 1];
+                        c0[d] = R(s, p, d, j); c1[d] = R(s, p, d, jn);
+                    }
+                    du += link_pair(s, p, j, a, b) - link_pair(s, p, j, c0, c1);
+                }
+                lw += du;
+            }
+            keyed(seed, (uint32_t)(p * M + start), sweep, chain, PUR_LEVY_ACC, x);
+            int acc = o_metropolis(exp(lw), o_u01(x[0]));
+            st->levy_tries++;
+            st->bead_moves += m - 1;
+            if (acc) {
+                st->levy_acc++;
+                for (int i = 1; i < m; ++i) {
+                    int j = (start + i) % M;
+                    for (int d = 0; d < dim; ++d) s->r[((size_t)p * dim + d) * M + j] = rp[d * nrows + i];
+                }
+            }
+        }
+    }
+    /* COM phase */
+    for (int p = 0; p < N; ++p) {
+        keyed(seed, (uint32_t)p, sweep, chain, PUR_COM, x);
+        double dsp[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) dsp[d] = (com_step * 2.0) * (o_u01(x[d]) - 0.5);
+        double dv = 0.0, a[3], b[3];
+        for (int j = 0; j < M; ++j)
+            for (int d = 0; d < dim; ++d) rp[d * M + j] = o_teleport(R(s, p, d, j) + dsp[d], s->L);
+        for (int j = 0; j < M; ++j) {
+            for (int d = 0; d < dim; ++d) { a[d] = rp[d * M + j]; b[d] = R(s, p, d, j); }
+            dv += o_V(&s->pot, a, dim) - o_V(&s->pot, b, dim);
+        }
+        double lw = -s->tau * dv;
+        if (use_pair) {
+            double du = 0.0, c0[3], c1[3];
+            for (int j = 0; j < M; ++j) {
+                int jn = (j + 1) % M;
+                for (int d = 0; d < dim; ++d) {
+                    a[d] = rp[d * M + j]; b[d] = rp[d * M + jn];
+                    c0[d] = R(s, p, d, j); c1[d] = R(s, p, d, jn);
+                }
+                du += link_pair(s, p, j, a, b) - link_pair(s, p, j, c0, c1);
+            }
+            lw += du;
+        }
+        int acc = o_metropolis(exp(lw), o_u01(x[2]));
+        st->com_tries++;
+        st->bead_moves += M;
+        if (acc) {
+            st->com_acc++;
+            for (int j = 0; j < M; ++j)
+                for (int d = 0; d < dim; ++d) s->r[((size_t)p * dim + d) * M + j] = rp[d * M + j];
+        }
+    }
+    o_sys_refresh_V(s);
+    free(rp);
+    free(nrm);
+}

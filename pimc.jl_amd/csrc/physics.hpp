@@ -1,0 +1,366 @@
+// Device-side physics of the primitive-action PIMC sampling path.
+//
+// Every function here is shared by the batched sweep kernels and the
+// single-move kernels behind the C ABI, so the parity checks of the single
+// moves also cover the arithmetic of the sweeps.  Reference citations are
+// file:line in the PIMC.jl tree.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.hpp"
+
+namespace pimc {
+
+constexpr int kPotNone = 0, kPotHarmonic = 1, kPotLattice = 2;
+constexpr int kPairNone = 0, kPairGaussian = 1;
+
+// One chain as the kernels see it (device-resident table, 8-byte fields
+// first).  Positions of particle n, coordinate d, slice j sit at
+// pos[pos_off + (n * dim + d) * M + j]: each particle an M x dim column-major
+// block, so consecutive slices are consecutive addresses.
+struct ChainDev {
+    double beta, tau, lambda, L;
+    double omega, V0, a, g;
+    double sigma, h;
+    long long pos_off;  // element offset of the chain in the position buffer
+    long long cell_off; // offset of slice 0, cell 0 in the cell-count array (pair chains)
+    long long mem_off;  // offset of slice 0, cell 0, slot 0 in the cell-member array
+    long long bin_off;  // offset of the chain's per-bead cell index array (pair chains)
+    int N, M, dim, pot;
+    int pair, pair_in_accept, ell, nbins;
+    int cap, ncell, part_base, nseg;
+};
+
+// The chain constants converted to the arithmetic type.
+template <typename T> struct Prm {
+    T tau, lambda, L, twoL, inv2L;
+    T halfw2, w2;        // omega^2 / 2, omega^2
+    T V0, inva;          // lattice depth, 1/a
+    T g, inv2s2, h2;     // pair strength, 1/(2 sigma^2), cut-off radius^2
+    int pot, pair;
+
+    __device__ __forceinline__ static Prm make(const ChainDev &c)
+    {
+        Prm p;
+        p.tau = (T)c.tau;
+        p.lambda = (T)c.lambda;
+        p.L = (T)c.L;
+        p.twoL = (T)(2.0 * c.L);
+        p.inv2L = (T)(1.0 / (2.0 * c.L));
+        p.halfw2 = (T)(0.5 * c.omega * c.omega);
+        p.w2 = (T)(c.omega * c.omega);
+        p.V0 = (T)c.V0;
+        p.inva = (T)(1.0 / c.a);
+        p.g = (T)c.g;
+        p.inv2s2 = (T)(1.0 / (2.0 * c.sigma * c.sigma));
+        p.h2 = (T)(c.h * c.h);
+        p.pot = c.pot;
+        p.pair = c.pair;
+        return p;
+    }
+};
+
+__device__ __forceinline__ double fl(double x) { return floor(x); }
+__device__ __forceinline__ float fl(float x) { return floorf(x); }
+__device__ __forceinline__ double cpi(double x) { return cospi(x); }
+__device__ __forceinline__ float cpi(float x) { return cospif(x); }
+__device__ __forceinline__ double spi(double x) { return sinpi(x); }
+__device__ __forceinline__ float spi(float x) { return sinpif(x); }
+__device__ __forceinline__ double ex(double x) { return exp(x); }
+__device__ __forceinline__ float ex(float x) { return __expf(x); }
+__device__ __forceinline__ double sq(double x) { return sqrt(x); }
+__device__ __forceinline__ float sq(float x) { return __builtin_sqrtf(x); }
+
+// src/propagator.jl:30-32, teleport(x) = (x + L) - floor(x / 2L + 1/2) 2L - L
+template <typename T> __device__ __forceinline__ T teleport(T x, const Prm<T> &p)
+{
+    return ((x + p.L) - fl(x * p.inv2L + T(0.5)) * p.twoL) - p.L;
+}
+
+// src/propagator.jl:6-9, per-component |minimum-image| offset
+template <typename T> __device__ __forceinline__ T mindist(T x1, T x2, const Prm<T> &p)
+{
+    const T dx = fabs(x1 - x2);
+    const T a = p.twoL - dx;
+    return a < dx ? a : dx;
+}
+
+// External potential V(r) (the `potential` closure of System, src/system.jl:129-131).
+template <typename T, int DIM> __device__ __forceinline__ T potV(const T *r, const Prm<T> &p)
+{
+    if (p.pot == kPotHarmonic) {
+        T rr = r[0] * r[0];
+        if (DIM > 1) rr += r[1] * r[1];
+        return p.halfw2 * rr;
+    }
+    if (p.pot == kPotLattice) {
+        T c0 = cpi(r[0] * p.inva);
+        T v = c0 * c0;
+        if (DIM > 1) {
+            T c1 = cpi(r[1] * p.inva);
+            v += c1 * c1;
+        }
+        return p.V0 * v;
+    }
+    return T(0);
+}
+
+// r . grad V (the virial term of src/measurement.jl:107)
+template <typename T, int DIM> __device__ __forceinline__ T rgradV(const T *r, const Prm<T> &p)
+{
+    if (p.pot == kPotHarmonic) {
+        T rr = r[0] * r[0];
+        if (DIM > 1) rr += r[1] * r[1];
+        return p.w2 * rr;
+    }
+    if (p.pot == kPotLattice) {
+        // d/dx V0 cos^2(pi x / a) = -V0 (pi/a) sin(2 pi x / a)
+        const T pia = T(3.14159265358979323846) * p.inva;
+        T s = r[0] * spi(T(2) * r[0] * p.inva);
+        if (DIM > 1) s += r[1] * spi(T(2) * r[1] * p.inva);
+        return -p.V0 * pia * s;
+    }
+    return T(0);
+}
+
+template <typename T, int DIM> __device__ __forceinline__ void gradV(const T *r, const Prm<T> &p, T *g)
+{
+    g[0] = T(0);
+    if (DIM > 1) g[1] = T(0);
+    if (p.pot == kPotHarmonic) {
+        g[0] = p.w2 * r[0];
+        if (DIM > 1) g[1] = p.w2 * r[1];
+    } else if (p.pot == kPotLattice) {
+        const T pia = T(3.14159265358979323846) * p.inva;
+        g[0] = -p.V0 * pia * spi(T(2) * r[0] * p.inva);
+        if (DIM > 1) g[1] = -p.V0 * pia * spi(T(2) * r[1] * p.inva);
+    }
+}
+
+// dV/d omega of the harmonic trap: omega |r|^2
+template <typename T, int DIM> __device__ __forceinline__ T dVdomega(const T *r, double omega)
+{
+    T rr = r[0] * r[0];
+    if (DIM > 1) rr += r[1] * r[1];
+    return (T)omega * rr;
+}
+
+// Gaussian pair potential v as a function of r^2.
+template <typename T> __device__ __forceinline__ T vpair_r2(T r2, const Prm<T> &p)
+{
+    return p.g * ex(-r2 * p.inv2s2);
+}
+
+// src/updates/helper.jl:3-5
+template <typename T> __device__ __forceinline__ bool metropolis(T delta, T u)
+{
+    return (delta >= T(1)) || (delta > u);
+}
+
+// End-bead shift of levy! (src/updates/helper.jl:120-125): if the ends are
+// more than L apart in a coordinate, move the far end by sign(start) 2L.
+template <typename T> __device__ __forceinline__ T shift_end(T e0, T em, const Prm<T> &p)
+{
+    if (fabs(e0 - em) > p.L) {
+        const T sg = e0 > T(0) ? T(1) : (e0 < T(0) ? T(-1) : T(0));
+        em += sg * p.twoL;
+    }
+    return em;
+}
+
+// One step of the Levy bridge (src/updates/helper.jl:129-135): interior bead
+// i of a segment of m links, alpha = (m - i) / (m - i + 1), from the
+// (untelported) previous bead `prev` towards the end bead `em`.
+template <typename T> __device__ __forceinline__ T bridge_alpha(int m, int i) { return T(m - i) / T(m - i + 1); }
+
+template <typename T> __device__ __forceinline__ T bridge_sd(T alpha, const Prm<T> &p)
+{
+    return sq(((T(2) * p.lambda) * alpha) * p.tau);
+}
+
+template <typename T> __device__ __forceinline__ T bridge_step(T prev, T em, T alpha, T xi, T sd)
+{
+    return (alpha * prev + (T(1) - alpha) * em) + xi * sd;
+}
+
+// Cell index of a point (src/nearest_neighbours.jl:26-33), 0-based, x fastest.
+template <typename T, int DIM> __device__ __forceinline__ int cell_of(const T *r, const Prm<T> &p, int nbins)
+{
+    const T w = p.twoL / T(nbins);
+    int bx = (int)fl((r[0] + p.L) / w);
+    bx = bx < 0 ? 0 : (bx >= nbins ? nbins - 1 : bx);
+    if (DIM == 1) return bx;
+    int by = (int)fl((r[1] + p.L) / w);
+    by = by < 0 ? 0 : (by >= nbins ? nbins - 1 : by);
+    return bx + nbins * by;
+}
+
+// Stencil of bin_neighbors (src/nearest_neighbours.jl:55-65), in its order.
+__device__ __forceinline__ int stencil_cell(int c, int s, int nbins, int dim)
+{
+    const int bx = c % nbins;
+    if (dim == 1) {
+        const int dx = (s == 0) ? 0 : (s == 1 ? -1 : 1);
+        return (bx + dx + nbins) % nbins;
+    }
+    const int by = c / nbins;
+    constexpr int sx[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1};
+    constexpr int sy[9] = {0, 1, 1, 1, 0, 0, -1, -1, -1};
+    const int cx = (bx + sx[s] + nbins) % nbins;
+    const int cy = (by + sy[s] + nbins) % nbins;
+    return cx + nbins * cy;
+}
+
+// Accessor for one chain's positions (global or LDS) with the reference layout.
+template <typename T, int DIM> struct Path {
+    T *base;
+    int M;
+    __device__ __forceinline__ T &at(int n, int d, int j) const { return base[((long long)n * DIM + d) * M + j]; }
+    __device__ __forceinline__ void get(int n, int j, T *r) const
+    {
+        r[0] = at(n, 0, j);
+        if (DIM > 1) r[1] = at(n, 1, j);
+    }
+    __device__ __forceinline__ void put(int n, int j, const T *r) const
+    {
+        at(n, 0, j) = r[0];
+        if (DIM > 1) at(n, 1, j) = r[1];
+    }
+};
+
+// Cell lists of one chain: per slice j and cell c a count and up to `cap`
+// particle ids, plus the cell of every bead (the reference's nn grid and
+// Particle.bins, src/system.jl:80-91).
+struct Cells {
+    int *cnt;            // [M][ncell]
+    unsigned short *mem; // [M][ncell][cap]
+    int *bin;            // [N][M]
+    int M, ncell, cap, nbins;
+    __device__ __forceinline__ int &count(int j, int c) const { return cnt[(long long)j * ncell + c]; }
+    __device__ __forceinline__ unsigned short *members(int j, int c) const
+    {
+        return mem + ((long long)j * ncell + c) * cap;
+    }
+};
+
+// Pair link sum of particle n over link j -> j+1 with its beads there
+// replaced by pj, pj1: the neighbour set is every particle k != n of the 9
+// stencil cells around pj at slice j whose periodic distance to pj is <= h
+// (find_nns, src/nearest_neighbours.jl:106-154), each contributing
+// lnU = -tau/2 (v(|r_jk|) + v(|r_j+1,k|)) (src/updates/helper.jl:306-366).
+// Returns the sum of lnU (a log weight, <= 0 for a repulsion).
+template <typename T, int DIM>
+__device__ T pair_link(const Path<T, DIM> &P, const Cells &C, const Prm<T> &p, int n, int j, const T *pj,
+                       const T *pj1)
+{
+    const int M = P.M;
+    const int jn = (j + 1 == M) ? 0 : j + 1;
+    const int c0 = cell_of<T, DIM>(pj, p, C.nbins);
+    const int ns = (DIM == 1) ? 3 : 9;
+    T acc = T(0);
+    for (int s = 0; s < ns; ++s) {
+        const int c = stencil_cell(c0, s, C.nbins, DIM);
+        const int cnt = C.count(j, c);
+        const unsigned short *mem = C.members(j, c);
+        for (int q = 0; q < cnt; ++q) {
+            const int k = mem[q];
+            if (k == n) continue;
+            T rk[DIM];
+            P.get(k, j, rk);
+            T d0 = mindist(rk[0], pj[0], p);
+            T r2 = d0 * d0;
+            if (DIM > 1) {
+                T d1 = mindist(rk[1], pj[1], p);
+                r2 += d1 * d1;
+            }
+            if (r2 > p.h2) continue;
+            T rkn[DIM];
+            P.get(k, jn, rkn);
+            T e0 = mindist(rkn[0], pj1[0], p);
+            T s2 = e0 * e0;
+            if (DIM > 1) {
+                T e1 = mindist(rkn[1], pj1[1], p);
+                s2 += e1 * e1;
+            }
+            acc += (T(-0.5) * p.tau) * (vpair_r2(r2, p) + vpair_r2(s2, p));
+        }
+    }
+    return acc;
+}
+
+// The same link sum without cell lists: every k != n is tested against the
+// radius.  With nbins >= 3 the 9 stencil cells around pj hold every point
+// within h of pj exactly once, so this enumerates the same neighbour set; for
+// nbins < 3 the stencil repeats cells and the reference counts a candidate
+// once per repeat, which `mult` reproduces.
+template <typename T, int DIM>
+__device__ T pair_link_all(const Path<T, DIM> &P, int N, int nbins, const Prm<T> &p, int n, int j, const T *pj,
+                           const T *pj1)
+{
+    const int M = P.M;
+    const int jn = (j + 1 == M) ? 0 : j + 1;
+    const int ns = (DIM == 1) ? 3 : 9;
+    const int c0 = (nbins < 3) ? cell_of<T, DIM>(pj, p, nbins) : 0;
+    T acc = T(0);
+    for (int k = 0; k < N; ++k) {
+        if (k == n) continue;
+        T rk[DIM];
+        P.get(k, j, rk);
+        T d0 = mindist(rk[0], pj[0], p);
+        T r2 = d0 * d0;
+        if (DIM > 1) {
+            T d1 = mindist(rk[1], pj[1], p);
+            r2 += d1 * d1;
+        }
+        if (r2 > p.h2) continue;
+        int mult = 1;
+        if (nbins < 3) {
+            const int ck = cell_of<T, DIM>(rk, p, nbins);
+            mult = 0;
+            for (int s = 0; s < ns; ++s) mult += (stencil_cell(c0, s, nbins, DIM) == ck);
+        }
+        T rkn[DIM];
+        P.get(k, jn, rkn);
+        T e0 = mindist(rkn[0], pj1[0], p);
+        T s2 = e0 * e0;
+        if (DIM > 1) {
+            T e1 = mindist(rkn[1], pj1[1], p);
+            s2 += e1 * e1;
+        }
+        acc += T(mult) * ((T(-0.5) * p.tau) * (vpair_r2(r2, p) + vpair_r2(s2, p)));
+    }
+    return acc;
+}
+
+// Move particle n's bead at slice j from its stored cell to the cell of r.
+template <typename T, int DIM>
+__device__ __forceinline__ bool cells_move(const Cells &C, const Prm<T> &p, int n, int j, const T *r)
+{
+    const int nc = cell_of<T, DIM>(r, p, C.nbins);
+    int &b = C.bin[(long long)n * C.M + j];
+    const int oc = b;
+    if (nc == oc) return true;
+    if (oc >= 0) {
+        unsigned short *om = C.members(j, oc);
+        int &ocnt = C.count(j, oc);
+        for (int q = 0; q < ocnt; ++q) {
+            if (om[q] == n) {
+                om[q] = om[ocnt - 1];
+                ocnt -= 1;
+                break;
+            }
+        }
+    }
+    int &ncnt = C.count(j, nc);
+    if (ncnt >= C.cap) {
+        b = -1; // overflow: reported through the status word
+        return false;
+    }
+    C.members(j, nc)[ncnt] = (unsigned short)n;
+    ncnt += 1;
+    b = nc;
+    return true;
+}
+
+} // namespace pimc

@@ -1,0 +1,1119 @@
+// C ABI of the MI355X sampling path (declarations and reference citations in
+// include/pimc_hip.h).  Host side: chain layout in HBM, tiles for the LDS
+// sweep kernel, launch sequencing, estimator blocks, error codes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hook_kernels.hpp"
+#include "pimc_hip.h"
+#include "sweep_kernels.hpp"
+
+using namespace pimc;
+
+namespace {
+
+struct HipError {
+    hipError_t e;
+    const char *what;
+};
+
+#define HIPCHK(call)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) throw HipError{e_, #call};                                                               \
+    } while (0)
+
+template <typename T> T *dalloc(size_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    return reinterpret_cast<T *>(p);
+}
+
+void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+struct Launch {
+    hipEvent_t a, b;
+};
+
+} // namespace
+
+struct pimc_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    pimc_options opt{};
+    int prec = PIMC_FP32;
+    int dim = 2;
+    int n_chains = 0;
+    std::vector<pimc_chain_desc> desc;
+    std::vector<ChainDev> ch;
+    std::vector<int> is_whole;   // chain -> 1 if run by the whole-chain kernel
+    std::vector<int> whole_ids;  // whole chains
+    std::vector<int> whole_cells_ids;  // whole chains with cell lists
+    std::vector<int> whole_brute_ids;  // whole chains with brute-force neighbour sums
+    std::vector<TileDev> tiles;
+    std::vector<int> part_chain, part_idx;
+    long long n_pos = 0, n_cell_cnt = 0, n_bins = 0;
+    int n_part = 0;
+    int maxl = 8;
+    int threads_tile = 256, threads_whole = 256;
+    size_t lds_tile = 0, lds_whole = 0;
+    int64_t moves_per_sweep = 0, levy_tries_per_sweep = 0, com_tries_per_sweep = 0;
+    uint32_t sweep = 0;
+    int nblk = 0, nblk_cap = 0;
+    // device
+    ChainDev *d_ch = nullptr;
+    TileDev *d_tiles = nullptr;
+    int *d_pchain = nullptr, *d_pidx = nullptr;
+    void *d_pos = nullptr;
+    double *d_part_acc = nullptr;
+    int *d_placc = nullptr, *d_pcacc = nullptr;
+    double *d_step = nullptr;
+    long long *d_cnt = nullptr;
+    double *d_chain_meas = nullptr;
+    double *d_blocks = nullptr;
+    unsigned long long *d_radial = nullptr, *d_cart = nullptr;
+    int *d_whole = nullptr, *d_whole_cells = nullptr, *d_whole_brute = nullptr, *d_is_whole = nullptr;
+    int *d_cell_cnt = nullptr, *d_bins = nullptr;
+    unsigned short *d_cell_mem = nullptr;
+    int *d_status = nullptr;
+    double *d_work = nullptr, *d_io = nullptr;
+    size_t work_elems = 0, io_elems = 0;
+    std::vector<Launch> events;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    long long cart_per_chain = 0;
+};
+
+namespace {
+
+int set_err(pimc_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_err(pimc_ctx *c, const HipError &h)
+{
+    return set_err(c, PIMC_ERR_HIP, std::string(h.what) + ": " + hipGetErrorString(h.e));
+}
+
+size_t tsize(const pimc_ctx *c) { return c->prec == PIMC_FP64 ? sizeof(double) : sizeof(float); }
+
+int next_pow2_maxl(int ell)
+{
+    int m = 4;
+    while (m < ell) m <<= 1;
+    return m;
+}
+
+// ---- kernel dispatch over (T, DIM, MAXL) ----
+template <typename T, int DIM, int MAXL> void launch_tiles_t(pimc_ctx *c, const SweepArgs &a)
+{
+    auto k = k_sweep_tiles<T, DIM, MAXL>;
+    if (c->lds_tile > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_tile));
+    hipLaunchKernelGGL(k, dim3((unsigned)c->tiles.size()), dim3(c->threads_tile), c->lds_tile, c->stream, a);
+    HIPCHK(hipGetLastError());
+}
+
+template <typename T, int DIM> void launch_tiles_d(pimc_ctx *c, const SweepArgs &a)
+{
+    switch (c->maxl) {
+    case 4: launch_tiles_t<T, DIM, 4>(c, a); break;
+    case 8: launch_tiles_t<T, DIM, 8>(c, a); break;
+    case 16: launch_tiles_t<T, DIM, 16>(c, a); break;
+    default: launch_tiles_t<T, DIM, 32>(c, a); break;
+    }
+}
+
+void launch_tiles(pimc_ctx *c, const SweepArgs &a)
+{
+    if (c->prec == PIMC_FP64) {
+        if (c->dim == 1) launch_tiles_d<double, 1>(c, a);
+        else launch_tiles_d<double, 2>(c, a);
+    } else {
+        if (c->dim == 1) launch_tiles_d<float, 1>(c, a);
+        else launch_tiles_d<float, 2>(c, a);
+    }
+}
+
+template <typename T, int DIM, bool CELLS> void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &ids, const int *d_ids)
+{
+    if (ids.empty()) return;
+    a.whole = d_ids;
+    auto k = k_sweep_whole<T, DIM, 32, CELLS>;
+    if (c->lds_whole > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_whole));
+    hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(c->threads_whole), c->lds_whole, c->stream, a);
+    HIPCHK(hipGetLastError());
+}
+
+template <typename T, int DIM> void launch_whole_d(pimc_ctx *c, const SweepArgs &a)
+{
+    launch_whole_t<T, DIM, true>(c, a, c->whole_cells_ids, c->d_whole_cells);
+    launch_whole_t<T, DIM, false>(c, a, c->whole_brute_ids, c->d_whole_brute);
+}
+
+void launch_whole(pimc_ctx *c, const SweepArgs &a)
+{
+    if (c->prec == PIMC_FP64) {
+        if (c->dim == 1) launch_whole_d<double, 1>(c, a);
+        else launch_whole_d<double, 2>(c, a);
+    } else {
+        if (c->dim == 1) launch_whole_d<float, 1>(c, a);
+        else launch_whole_d<float, 2>(c, a);
+    }
+}
+
+void build_cells(pimc_ctx *c)
+{
+    if (c->whole_cells_ids.empty()) return;
+    int maxM = 0;
+    for (int id : c->whole_cells_ids) maxM = std::max(maxM, c->ch[id].M);
+    dim3 grid((maxM + 63) / 64, (unsigned)c->whole_cells_ids.size());
+#define BC(T, D)                                                                                                       \
+    hipLaunchKernelGGL((k_build_cells<T, D>), grid, dim3(64), 0, c->stream, c->d_ch, c->d_whole_cells,                 \
+                       (int)c->whole_cells_ids.size(), c->d_pos, c->d_cell_cnt, c->d_cell_mem, c->d_bins, c->d_status)
+    if (c->prec == PIMC_FP64) {
+        if (c->dim == 1) BC(double, 1);
+        else BC(double, 2);
+    } else {
+        if (c->dim == 1) BC(float, 1);
+        else BC(float, 2);
+    }
+#undef BC
+    HIPCHK(hipGetLastError());
+}
+
+SweepArgs sweep_args(pimc_ctx *c)
+{
+    SweepArgs a{};
+    a.chains = c->d_ch;
+    a.tiles = c->d_tiles;
+    a.part_chain = c->d_pchain;
+    a.part_idx = c->d_pidx;
+    a.pos = c->d_pos;
+    a.part_acc = c->d_part_acc;
+    a.part_lacc = c->d_placc;
+    a.part_cacc = c->d_pcacc;
+    a.com_step = c->d_step;
+    a.radial = c->d_radial;
+    a.cart = c->d_cart;
+    a.radial_bins = c->opt.radial_bins;
+    a.radial_winv = c->opt.radial_bins > 0 ? c->opt.radial_bins / c->opt.radial_rmax : 0.0;
+    a.cart_bins = c->opt.cart_bins;
+    a.seed = c->opt.seed;
+    a.cell_cnt = c->d_cell_cnt;
+    a.cell_mem = c->d_cell_mem;
+    a.bins = c->d_bins;
+    a.chain_meas = c->d_chain_meas;
+    a.status = c->d_status;
+    return a;
+}
+
+void grow_blocks(pimc_ctx *c, int need)
+{
+    if (need <= c->nblk_cap) return;
+    int cap = std::max(need, std::max(64, c->nblk_cap * 2));
+    double *nb = dalloc<double>((size_t)cap * c->n_chains * kNQ);
+    if (c->nblk > 0)
+        HIPCHK(hipMemcpyAsync(nb, c->d_blocks, sizeof(double) * (size_t)c->nblk * c->n_chains * kNQ,
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c->d_blocks);
+    c->d_blocks = nb;
+    c->nblk_cap = cap;
+}
+
+int check_chain(pimc_ctx *c, int chain)
+{
+    if (!c) return PIMC_ERR_ARG;
+    if (chain < 0 || chain >= c->n_chains) return set_err(c, PIMC_ERR_ARG, "chain index out of range");
+    return PIMC_OK;
+}
+
+int check_status(pimc_ctx *c)
+{
+    int st = 0;
+    HIPCHK(hipMemcpy(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    if (st) return set_err(c, PIMC_ERR_CAPACITY, "a cell list overflowed its capacity; raise r_a or lower the density");
+    return PIMC_OK;
+}
+
+void ensure_io(pimc_ctx *c, size_t n)
+{
+    if (n <= c->io_elems) return;
+    dfree(c->d_io);
+    c->d_io = dalloc<double>(n);
+    c->io_elems = n;
+}
+
+HookArgs hook_args(pimc_ctx *c, int chain)
+{
+    HookArgs h{};
+    h.chains = c->d_ch;
+    h.pos = c->d_pos;
+    h.cell_cnt = c->d_cell_cnt;
+    h.cell_mem = c->d_cell_mem;
+    h.bins = c->d_bins;
+    h.status = c->d_status;
+    h.chain = chain;
+    h.work = c->d_work;
+    h.use_cells = (c->ch[chain].ncell > 0) ? 1 : 0;
+    return h;
+}
+
+// Launch a hook kernel templated on <T, DIM, CELLS>.
+#define HOOK_DISPATCH(KERNEL, grid, block, lds, ...)                                                                   \
+    do {                                                                                                               \
+        const bool cells_ = h.use_cells != 0;                                                                          \
+        if (c->prec == PIMC_FP64) {                                                                                    \
+            if (c->dim == 1) {                                                                                         \
+                if (cells_) hipLaunchKernelGGL((KERNEL<double, 1, true>), grid, block, lds, c->stream, __VA_ARGS__);   \
+                else hipLaunchKernelGGL((KERNEL<double, 1, false>), grid, block, lds, c->stream, __VA_ARGS__);         \
+            } else {                                                                                                   \
+                if (cells_) hipLaunchKernelGGL((KERNEL<double, 2, true>), grid, block, lds, c->stream, __VA_ARGS__);   \
+                else hipLaunchKernelGGL((KERNEL<double, 2, false>), grid, block, lds, c->stream, __VA_ARGS__);         \
+            }                                                                                                          \
+        } else {                                                                                                       \
+            if (c->dim == 1) {                                                                                         \
+                if (cells_) hipLaunchKernelGGL((KERNEL<float, 1, true>), grid, block, lds, c->stream, __VA_ARGS__);    \
+                else hipLaunchKernelGGL((KERNEL<float, 1, false>), grid, block, lds, c->stream, __VA_ARGS__);          \
+            } else {                                                                                                   \
+                if (cells_) hipLaunchKernelGGL((KERNEL<float, 2, true>), grid, block, lds, c->stream, __VA_ARGS__);    \
+                else hipLaunchKernelGGL((KERNEL<float, 2, false>), grid, block, lds, c->stream, __VA_ARGS__);          \
+            }                                                                                                          \
+        }                                                                                                              \
+        HIPCHK(hipGetLastError());                                                                                     \
+    } while (0)
+
+int validate(const pimc_chain_desc &d, std::string &why)
+{
+    if (d.dim != 1 && d.dim != 2) return why = "dim must be 1 or 2", -1;
+    if (d.N < 1 || d.N > 65535) return why = "N must be in [1, 65535]", -1;
+    if (d.M < 4) return why = "M must be >= 4", -1;
+    if (!(d.beta > 0) || !(d.lambda > 0) || !(d.L > 0)) return why = "beta, lambda and L must be positive", -1;
+    if (d.potential < PIMC_POT_NONE || d.potential > PIMC_POT_LATTICE) return why = "unknown potential", -1;
+    if (d.pair < PIMC_PAIR_NONE || d.pair > PIMC_PAIR_GAUSSIAN) return why = "unknown pair potential", -1;
+    if (d.pair == PIMC_PAIR_GAUSSIAN && !(d.sigma > 0)) return why = "sigma must be positive", -1;
+    if (d.potential == PIMC_POT_LATTICE && !(d.a > 0)) return why = "lattice constant a must be positive", -1;
+    if (d.ell != 0 && (d.ell < 2 || d.ell > 32 || d.ell > d.M)) return why = "ell must be in [2, min(32, M)]", -1;
+    if (!(d.com_step >= 0)) return why = "com_step must be >= 0", -1;
+    return 0;
+}
+
+} // namespace
+
+// ============================================================================
+extern "C" {
+
+const char *pimc_last_error(const pimc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pimc_device_info(char *name, int32_t len, int32_t *cu_count)
+{
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess) return PIMC_ERR_HIP;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return PIMC_ERR_HIP;
+    if (name && len > 0) {
+        std::snprintf(name, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    return PIMC_OK;
+}
+
+int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains, const pimc_options *opt)
+{
+    if (!out || !chains || n_chains < 1 || !opt) return PIMC_ERR_ARG;
+    *out = nullptr;
+    pimc_ctx *c = new (std::nothrow) pimc_ctx();
+    if (!c) return PIMC_ERR_NOMEM;
+    std::string why;
+    for (int i = 0; i < n_chains; ++i) {
+        if (validate(chains[i], why) != 0) {
+            c->err = "chain " + std::to_string(i) + ": " + why;
+            *out = c;
+            return PIMC_ERR_ARG;
+        }
+        if (chains[i].dim != chains[0].dim) {
+            c->err = "all chains of a context share one dimension";
+            *out = c;
+            return PIMC_ERR_ARG;
+        }
+    }
+    if (opt->precision != PIMC_FP32 && opt->precision != PIMC_FP64) {
+        c->err = "precision must be PIMC_FP32 or PIMC_FP64";
+        *out = c;
+        return PIMC_ERR_ARG;
+    }
+    if (opt->radial_bins < 0 || opt->cart_bins < 0 || (opt->radial_bins > 0 && !(opt->radial_rmax > 0))) {
+        c->err = "bad histogram options";
+        *out = c;
+        return PIMC_ERR_ARG;
+    }
+    *out = c;
+    try {
+        c->opt = *opt;
+        if (c->opt.sweeps_per_launch <= 0) c->opt.sweeps_per_launch = 50;
+        c->prec = opt->precision;
+        c->dim = chains[0].dim;
+        c->n_chains = n_chains;
+        c->desc.assign(chains, chains + n_chains);
+        HIPCHK(hipGetDevice(&c->dev));
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&c->ev0));
+        HIPCHK(hipEventCreate(&c->ev1));
+        int cu = 256;
+        {
+            hipDeviceProp_t p;
+            HIPCHK(hipGetDeviceProperties(&p, c->dev));
+            cu = p.multiProcessorCount;
+        }
+        (void)cu;
+
+        // ---- chain table; whole-chain kernel for pair chains (and all chains with derivatives)
+        c->ch.resize(n_chains);
+        c->is_whole.assign(n_chains, 0);
+        int ellmax = 2;
+        for (int i = 0; i < n_chains; ++i) {
+            const pimc_chain_desc &d = chains[i];
+            ChainDev &h = c->ch[i];
+            std::memset(&h, 0, sizeof(h));
+            h.beta = d.beta;
+            h.tau = d.beta / d.M;
+            h.lambda = d.lambda;
+            h.L = d.L;
+            h.omega = d.omega;
+            h.V0 = d.V0;
+            h.a = d.a > 0 ? d.a : 1.0;
+            h.g = d.g;
+            h.sigma = d.sigma > 0 ? d.sigma : 1.0;
+            h.N = d.N;
+            h.M = d.M;
+            h.dim = d.dim;
+            h.pot = d.potential;
+            h.pair = d.pair;
+            h.pair_in_accept = d.pair_in_accept;
+            h.ell = d.ell ? d.ell : std::min(8, d.M);
+            h.nseg = (d.M + h.ell - 1) / h.ell;
+            const double ra = d.r_a > 0 ? d.r_a : d.L / 4.0;                // src/system.jl:22-24
+            h.nbins = std::max(1, (int)std::floor((2.0 * d.L) / ra));     // src/system.jl:81
+            h.h = (2.0 * d.L) / h.nbins;                                  // src/nearest_neighbours.jl:127
+            ellmax = std::max(ellmax, h.ell);
+            c->is_whole[i] = (d.pair != PIMC_PAIR_NONE || opt->derivatives) ? 1 : 0;
+        }
+        c->maxl = next_pow2_maxl(ellmax);
+        if (c->maxl > 32) throw HipError{hipErrorInvalidValue, "ell > 32"};
+
+        // ---- memory order: tile chains sorted by (M, ell), then whole chains
+        std::vector<int> order(n_chains);
+        for (int i = 0; i < n_chains; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            if (c->is_whole[x] != c->is_whole[y]) return c->is_whole[x] < c->is_whole[y];
+            if (c->ch[x].M != c->ch[y].M) return c->ch[x].M > c->ch[y].M;
+            return c->ch[x].ell < c->ch[y].ell;
+        });
+        long long off = 0, celloff = 0, binoff = 0;
+        int pbase = 0;
+        for (int id : order) {
+            ChainDev &h = c->ch[id];
+            h.pos_off = off;
+            off += (long long)h.N * h.dim * h.M;
+            h.part_base = pbase;
+            pbase += h.N;
+            if (c->is_whole[id]) {
+                c->whole_ids.push_back(id);
+                // cell lists when they beat the all-pairs scan and stay small
+                const long long ncell = (h.dim == 1) ? h.nbins : (long long)h.nbins * h.nbins;
+                const double mean = (double)h.N / (double)ncell;
+                int cap = (int)std::min<long long>(h.N, (long long)std::ceil(4.0 * mean + 16.0));
+                const long long bytes = (long long)h.M * ncell * (cap * 2 + 4);
+                const bool cells = h.pair != PIMC_PAIR_NONE && h.N > 96 && h.nbins >= 3 && bytes < (256ll << 20);
+                if (cells) {
+                    h.ncell = (int)ncell;
+                    h.cap = cap;
+                    h.cell_off = celloff;
+                    celloff += (long long)h.M * ncell;
+                    h.bin_off = binoff;
+                    binoff += (long long)h.N * h.M;
+                    c->whole_cells_ids.push_back(id);
+                } else {
+                    h.ncell = 0;
+                    h.cap = 0;
+                    c->whole_brute_ids.push_back(id);
+                }
+            }
+        }
+        c->n_pos = off;
+        c->n_part = pbase;
+        c->n_cell_cnt = celloff;
+        c->n_bins = binoff;
+
+        // ---- particle table and tiles of the LDS kernel
+        c->part_chain.assign(c->n_part, 0);
+        c->part_idx.assign(c->n_part, 0);
+        for (int i = 0; i < n_chains; ++i)
+            for (int p = 0; p < c->ch[i].N; ++p) {
+                c->part_chain[c->ch[i].part_base + p] = i;
+                c->part_idx[c->ch[i].part_base + p] = p;
+            }
+        const int ts = (int)tsize(c);
+        c->threads_tile = opt->block_threads > 0 ? opt->block_threads : 256;
+        if (c->threads_tile % 64 != 0 || c->threads_tile > 1024) throw HipError{hipErrorInvalidValue, "block_threads must be a multiple of 64, <= 1024"};
+        int tile_beads = opt->tile_beads > 0 ? opt->tile_beads : (ts == 4 ? 4096 : 2048);
+        const int max_items = kItemsPerThread * c->threads_tile;
+        size_t lds_max = 0;
+        for (int id : order) {
+            if (c->is_whole[id]) continue;
+            const ChainDev &h = c->ch[id];
+            if (h.nseg > max_items) throw HipError{hipErrorInvalidValue, "a particle has more segments than a workgroup holds; raise block_threads or ell"};
+            for (int p = 0; p < h.N; ++p) {
+                const int gq = h.part_base + p;
+                bool fresh = c->tiles.empty();
+                if (!fresh) {
+                    TileDev &t = c->tiles.back();
+                    fresh = t.M != h.M || t.ell != h.ell || (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
+                }
+                if (fresh) {
+                    TileDev t{};
+                    t.pos_off = h.pos_off + (long long)p * h.dim * h.M;
+                    t.q0 = gq;
+                    t.np = 0;
+                    t.M = h.M;
+                    t.ell = h.ell;
+                    t.nseg = h.nseg;
+                    c->tiles.push_back(t);
+                }
+                TileDev &t = c->tiles.back();
+                t.np += 1;
+                t.items = t.np * t.nseg;
+            }
+        }
+        for (const TileDev &t : c->tiles) lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.M, t.items, ts));
+        c->lds_tile = lds_max;
+        if (lds_max > 160 * 1024) throw HipError{hipErrorInvalidValue, "tile exceeds LDS; lower tile_beads"};
+        // whole-chain kernel
+        int maxMw = 0;
+        for (int id : c->whole_ids) maxMw = std::max(maxMw, c->ch[id].M);
+        c->threads_whole = 256;
+        if (maxMw > 0) {
+            c->lds_whole = whole_lds_bytes(maxMw, c->dim, ts, c->threads_whole);
+            if (c->lds_whole > 160 * 1024) throw HipError{hipErrorInvalidValue, "M too large for the whole-chain kernel"};
+        }
+        // schedule counts per sweep
+        for (int i = 0; i < n_chains; ++i) {
+            const ChainDev &h = c->ch[i];
+            int64_t per = 0, segs = 0;
+            for (int k = 0; k < h.nseg; ++k) {
+                const int m = std::min(h.ell, h.M - k * h.ell);
+                if (m >= 2) {
+                    per += m - 1;
+                    ++segs;
+                }
+            }
+            c->levy_tries_per_sweep += (int64_t)h.N * segs;
+            c->com_tries_per_sweep += h.N;
+            c->moves_per_sweep += (int64_t)h.N * (per + h.M);
+        }
+
+        // ---- device allocations
+        c->d_ch = dalloc<ChainDev>(n_chains);
+        HIPCHK(hipMemcpy(c->d_ch, c->ch.data(), sizeof(ChainDev) * n_chains, hipMemcpyHostToDevice));
+        c->d_tiles = dalloc<TileDev>(c->tiles.size());
+        if (!c->tiles.empty())
+            HIPCHK(hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice));
+        c->d_pchain = dalloc<int>(c->n_part);
+        c->d_pidx = dalloc<int>(c->n_part);
+        HIPCHK(hipMemcpy(c->d_pchain, c->part_chain.data(), sizeof(int) * c->n_part, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_pidx, c->part_idx.data(), sizeof(int) * c->n_part, hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&c->d_pos, std::max<size_t>(1, (size_t)c->n_pos * ts)));
+        HIPCHK(hipMemset(c->d_pos, 0, std::max<size_t>(1, (size_t)c->n_pos * ts)));
+        c->d_part_acc = dalloc<double>((size_t)c->n_part * 4);
+        c->d_placc = dalloc<int>(c->n_part);
+        c->d_pcacc = dalloc<int>(c->n_part);
+        HIPCHK(hipMemset(c->d_placc, 0, sizeof(int) * c->n_part));
+        HIPCHK(hipMemset(c->d_pcacc, 0, sizeof(int) * c->n_part));
+        c->d_step = dalloc<double>(n_chains);
+        {
+            std::vector<double> st(n_chains);
+            for (int i = 0; i < n_chains; ++i) st[i] = chains[i].com_step;
+            HIPCHK(hipMemcpy(c->d_step, st.data(), sizeof(double) * n_chains, hipMemcpyHostToDevice));
+        }
+        c->d_cnt = dalloc<long long>((size_t)n_chains * 2);
+        HIPCHK(hipMemset(c->d_cnt, 0, sizeof(long long) * n_chains * 2));
+        c->d_chain_meas = dalloc<double>((size_t)n_chains * (kNQ + 2));
+        HIPCHK(hipMemset(c->d_chain_meas, 0, sizeof(double) * n_chains * (kNQ + 2)));
+        c->d_is_whole = dalloc<int>(n_chains);
+        HIPCHK(hipMemcpy(c->d_is_whole, c->is_whole.data(), sizeof(int) * n_chains, hipMemcpyHostToDevice));
+        c->d_whole = dalloc<int>(c->whole_ids.size());
+        c->d_whole_cells = dalloc<int>(c->whole_cells_ids.size());
+        c->d_whole_brute = dalloc<int>(c->whole_brute_ids.size());
+        if (!c->whole_ids.empty())
+            HIPCHK(hipMemcpy(c->d_whole, c->whole_ids.data(), sizeof(int) * c->whole_ids.size(), hipMemcpyHostToDevice));
+        if (!c->whole_cells_ids.empty())
+            HIPCHK(hipMemcpy(c->d_whole_cells, c->whole_cells_ids.data(), sizeof(int) * c->whole_cells_ids.size(), hipMemcpyHostToDevice));
+        if (!c->whole_brute_ids.empty())
+            HIPCHK(hipMemcpy(c->d_whole_brute, c->whole_brute_ids.data(), sizeof(int) * c->whole_brute_ids.size(), hipMemcpyHostToDevice));
+        long long mem_slots = 0;
+        for (int id : c->whole_cells_ids) {
+            ChainDev &h = c->ch[id];
+            h.mem_off = mem_slots;
+            mem_slots += (long long)h.M * h.ncell * h.cap;
+        }
+        HIPCHK(hipMemcpy(c->d_ch, c->ch.data(), sizeof(ChainDev) * n_chains, hipMemcpyHostToDevice));
+        c->d_cell_cnt = dalloc<int>((size_t)c->n_cell_cnt);
+        c->d_cell_mem = dalloc<unsigned short>((size_t)mem_slots);
+        c->d_bins = dalloc<int>((size_t)c->n_bins);
+        c->d_status = dalloc<int>(1);
+        HIPCHK(hipMemset(c->d_status, 0, sizeof(int)));
+        int maxM = 0;
+        for (int i = 0; i < n_chains; ++i) maxM = std::max(maxM, c->ch[i].M);
+        c->work_elems = (size_t)(maxM + 1) * c->dim + 16;
+        c->d_work = dalloc<double>(c->work_elems);
+        if (opt->radial_bins > 0) {
+            c->d_radial = dalloc<unsigned long long>((size_t)n_chains * opt->radial_bins);
+            HIPCHK(hipMemset(c->d_radial, 0, sizeof(unsigned long long) * n_chains * opt->radial_bins));
+        }
+        if (opt->cart_bins > 0) {
+            c->cart_per_chain = c->dim == 1 ? opt->cart_bins : (long long)opt->cart_bins * opt->cart_bins;
+            c->d_cart = dalloc<unsigned long long>((size_t)n_chains * c->cart_per_chain);
+            HIPCHK(hipMemset(c->d_cart, 0, sizeof(unsigned long long) * n_chains * c->cart_per_chain));
+        }
+        grow_blocks(c, 64);
+        HIPCHK(hipDeviceSynchronize());
+    } catch (const HipError &h) {
+        return hip_err(c, h);
+    } catch (const std::bad_alloc &) {
+        return set_err(c, PIMC_ERR_NOMEM, "host allocation failed");
+    }
+    return PIMC_OK;
+}
+
+int pimc_destroy(pimc_ctx *c)
+{
+    if (!c) return PIMC_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->d_ch, c->d_tiles, c->d_pchain, c->d_pidx, c->d_pos, c->d_part_acc, c->d_placc, c->d_pcacc,
+                    c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
+                    c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
+                    c->d_status, c->d_work, c->d_io};
+    for (void *p : ptrs) dfree(p);
+    for (auto &e : c->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PIMC_OK;
+}
+
+int pimc_set_paths(pimc_ctx *c, int32_t chain, const double *r)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!r) return set_err(c, PIMC_ERR_ARG, "null path buffer");
+    const ChainDev &h = c->ch[chain];
+    const size_t n = (size_t)h.N * h.dim * h.M;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(r[i]) || r[i] < -h.L || r[i] >= h.L)
+            return set_err(c, PIMC_ERR_ARG, "bead outside the box [-L, L) or not finite");
+    try {
+        if (c->prec == PIMC_FP64) {
+            HIPCHK(hipMemcpy((double *)c->d_pos + h.pos_off, r, n * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(n);
+            for (size_t i = 0; i < n; ++i) {
+                float v = (float)r[i];
+                if (v >= (float)h.L) v = std::nextafter((float)h.L, -INFINITY);
+                f[i] = v;
+            }
+            HIPCHK(hipMemcpy((float *)c->d_pos + h.pos_off, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        }
+        if (h.ncell > 0) {
+            build_cells(c);
+            HIPCHK(hipStreamSynchronize(c->stream));
+            return check_status(c);
+        }
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_get_paths(pimc_ctx *c, int32_t chain, double *r)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!r) return set_err(c, PIMC_ERR_ARG, "null path buffer");
+    const ChainDev &h = c->ch[chain];
+    const size_t n = (size_t)h.N * h.dim * h.M;
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->prec == PIMC_FP64) {
+            HIPCHK(hipMemcpy(r, (double *)c->d_pos + h.pos_off, n * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> f(n);
+            HIPCHK(hipMemcpy(f.data(), (float *)c->d_pos + h.pos_off, n * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < n; ++i) r[i] = f[i];
+        }
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_get_com_step(pimc_ctx *c, int32_t chain, double *step)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(step, c->d_step + chain, sizeof(double), hipMemcpyDeviceToHost));
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_set_com_step(pimc_ctx *c, int32_t chain, double step)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!(step >= 0)) return set_err(c, PIMC_ERR_ARG, "step must be >= 0");
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(c->d_step + chain, &step, sizeof(double), hipMemcpyHostToDevice));
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
+{
+    if (!c) return PIMC_ERR_ARG;
+    if (n_sweeps < 0) return set_err(c, PIMC_ERR_ARG, "n_sweeps < 0");
+    try {
+        std::vector<long long> before((size_t)c->n_chains * 2);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(before.data(), c->d_cnt, sizeof(long long) * before.size(), hipMemcpyDeviceToHost));
+        const int K = c->opt.sweeps_per_launch;
+        const int nl = (n_sweeps + K - 1) / K;
+        if (measure) grow_blocks(c, c->nblk + nl);
+        while ((int)c->events.size() < nl) {
+            Launch l;
+            HIPCHK(hipEventCreate(&l.a));
+            HIPCHK(hipEventCreate(&l.b));
+            c->events.push_back(l);
+        }
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        int left = n_sweeps, li = 0;
+        while (left > 0) {
+            const int k = std::min(K, left);
+            SweepArgs a = sweep_args(c);
+            a.sweep0 = c->sweep;
+            a.nsweeps = k;
+            a.measure = measure ? 1 : 0;
+            HIPCHK(hipEventRecord(c->events[li].a, c->stream));
+            if (!c->tiles.empty()) launch_tiles(c, a);
+            if (!c->whole_ids.empty()) launch_whole(c, a);
+            HIPCHK(hipEventRecord(c->events[li].b, c->stream));
+            PostArgs pa{};
+            pa.chains = c->d_ch;
+            pa.n_chains = c->n_chains;
+            pa.is_whole = c->d_is_whole;
+            pa.part_acc = c->d_part_acc;
+            pa.part_lacc = c->d_placc;
+            pa.part_cacc = c->d_pcacc;
+            pa.chain_meas = c->d_chain_meas;
+            pa.com_step = c->d_step;
+            pa.counters = c->d_cnt;
+            pa.blocks = c->d_blocks;
+            pa.blk = c->nblk;
+            pa.measure = measure ? 1 : 0;
+            pa.nsweeps = k;
+            hipLaunchKernelGGL(k_post, dim3((c->n_chains + 127) / 128), dim3(128), 0, c->stream, pa);
+            HIPCHK(hipGetLastError());
+            if (measure) c->nblk++;
+            c->sweep += (uint32_t)k;
+            left -= k;
+            ++li;
+        }
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int rc = check_status(c);
+        if (out) {
+            std::memset(out, 0, sizeof(*out));
+            std::vector<long long> after(before.size());
+            HIPCHK(hipMemcpy(after.data(), c->d_cnt, sizeof(long long) * after.size(), hipMemcpyDeviceToHost));
+            for (int i = 0; i < c->n_chains; ++i) {
+                out->levy_acc += after[i * 2] - before[i * 2];
+                out->com_acc += after[i * 2 + 1] - before[i * 2 + 1];
+            }
+            out->sweeps = n_sweeps;
+            out->bead_moves = c->moves_per_sweep * n_sweeps;
+            out->levy_tries = c->levy_tries_per_sweep * n_sweeps;
+            out->com_tries = c->com_tries_per_sweep * n_sweeps;
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            out->wall_ms = ms;
+            double kms = 0.0;
+            for (int i = 0; i < li; ++i) {
+                float t = 0.f;
+                HIPCHK(hipEventElapsedTime(&t, c->events[i].a, c->events[i].b));
+                kms += t;
+            }
+            out->kernel_ms = kms;
+            out->kernel_launches = li;
+        }
+        return rc;
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+}
+
+int pimc_get_blocks(pimc_ctx *c, int32_t chain, double *E, double *Ev, double *Epair, int64_t cap, int64_t *n_blocks)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> q((size_t)c->nblk * kNQ);
+        for (int b = 0; b < c->nblk; ++b)
+            HIPCHK(hipMemcpy(&q[(size_t)b * kNQ], c->d_blocks + ((size_t)b * c->n_chains + chain) * kNQ,
+                             sizeof(double) * kNQ, hipMemcpyDeviceToHost));
+        if (n_blocks) *n_blocks = c->nblk;
+        for (int b = 0; b < c->nblk && b < cap; ++b) {
+            const double *x = &q[(size_t)b * kNQ];
+            const double n = x[qN] > 0 ? x[qN] : 1.0;
+            if (E) E[b] = x[qE] / n;
+            if (Ev) Ev[b] = x[qEv] / n;
+            if (Epair) Epair[b] = x[qEpair] / n;
+        }
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_get_estimates(pimc_ctx *c, int32_t chain, pimc_estimates *out)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!out) return set_err(c, PIMC_ERR_ARG, "null output");
+    std::memset(out, 0, sizeof(*out));
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const int nb = c->nblk;
+        std::vector<double> q((size_t)nb * kNQ);
+        for (int b = 0; b < nb; ++b)
+            HIPCHK(hipMemcpy(&q[(size_t)b * kNQ], c->d_blocks + ((size_t)b * c->n_chains + chain) * kNQ,
+                             sizeof(double) * kNQ, hipMemcpyDeviceToHost));
+        out->n_blocks = nb;
+        if (nb == 0) return PIMC_OK;
+        // totals and block means
+        std::vector<double> tot(kNQ, 0.0);
+        for (int b = 0; b < nb; ++b)
+            for (int k = 0; k < kNQ; ++k) tot[k] += q[(size_t)b * kNQ + k];
+        const double n = tot[qN];
+        out->n_meas = (int64_t)n;
+        if (n <= 0) return PIMC_OK;
+        auto mean_err = [&](int k, double &m, double &e) {
+            m = tot[k] / n;
+            e = 0.0;
+            if (nb < 2) return;
+            double s = 0.0;
+            for (int b = 0; b < nb; ++b) {
+                const double nbk = q[(size_t)b * kNQ + qN];
+                const double x = nbk > 0 ? q[(size_t)b * kNQ + k] / nbk : m;
+                s += (x - m) * (x - m);
+            }
+            e = std::sqrt(s / (nb - 1) / nb);
+        };
+        mean_err(qE, out->E, out->E_err);
+        mean_err(qEv, out->Ev, out->Ev_err);
+        mean_err(qEpair, out->Epair, out->Epair_err);
+        out->dSdomega = tot[qSw] / n;
+        out->dSdg = tot[qSg] / n;
+        // reweighting derivatives with jackknife errors over blocks:
+        // d<E>/dx = <dE/dx> - (<E dS/dx> - <E><dS/dx>)
+        const bool whole = tot[qWhole] > 0;
+        if (whole) {
+            auto deriv = [&](const std::vector<double> &t, int kdE, int kES, int kS) {
+                const double m = t[qN];
+                return t[kdE] / m - (t[kES] / m - (t[qE] / m) * (t[kS] / m));
+            };
+            out->dE_domega = deriv(tot, qdEw, qESw, qSw);
+            out->dE_dg = deriv(tot, qdEg, qESg, qSg);
+            if (nb >= 2) {
+                double s1 = 0.0, s2 = 0.0;
+                std::vector<double> loo(kNQ);
+                for (int b = 0; b < nb; ++b) {
+                    for (int k = 0; k < kNQ; ++k) loo[k] = tot[k] - q[(size_t)b * kNQ + k];
+                    if (loo[qN] <= 0) continue;
+                    const double a1 = deriv(loo, qdEw, qESw, qSw) - out->dE_domega;
+                    const double a2 = deriv(loo, qdEg, qESg, qSg) - out->dE_dg;
+                    s1 += a1 * a1;
+                    s2 += a2 * a2;
+                }
+                out->dE_domega_err = std::sqrt((double)(nb - 1) / nb * s1);
+                out->dE_dg_err = std::sqrt((double)(nb - 1) / nb * s2);
+            }
+        } else {
+            out->dE_domega = NAN;
+            out->dE_dg = NAN;
+        }
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_get_radial(pimc_ctx *c, int32_t chain, double *hist, int64_t *ndata)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!c->d_radial) return set_err(c, PIMC_ERR_STATE, "radial histogram not enabled");
+    try {
+        const int nb = c->opt.radial_bins;
+        std::vector<unsigned long long> h(nb);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(h.data(), c->d_radial + (size_t)chain * nb, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nb; ++i) hist[i] = (double)h[i];
+        double nm = 0.0;
+        for (int b = 0; b < c->nblk; ++b) {
+            double x = 0;
+            HIPCHK(hipMemcpy(&x, c->d_blocks + ((size_t)b * c->n_chains + chain) * kNQ + qN, sizeof(double), hipMemcpyDeviceToHost));
+            nm += x;
+        }
+        if (ndata) *ndata = (int64_t)nm * c->ch[chain].M;
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_get_cartesian(pimc_ctx *c, int32_t chain, double *hist, int64_t *ndata)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!c->d_cart) return set_err(c, PIMC_ERR_STATE, "Cartesian histogram not enabled");
+    try {
+        const long long nb = c->cart_per_chain;
+        std::vector<unsigned long long> h((size_t)nb);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(h.data(), c->d_cart + (size_t)chain * nb, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost));
+        for (long long i = 0; i < nb; ++i) hist[i] = (double)h[i];
+        double nm = 0.0;
+        for (int b = 0; b < c->nblk; ++b) {
+            double x = 0;
+            HIPCHK(hipMemcpy(&x, c->d_blocks + ((size_t)b * c->n_chains + chain) * kNQ + qN, sizeof(double), hipMemcpyDeviceToHost));
+            nm += x;
+        }
+        if (ndata) *ndata = (int64_t)nm * c->ch[chain].M;
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_reset_estimators(pimc_ctx *c)
+{
+    if (!c) return PIMC_ERR_ARG;
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->nblk = 0;
+        if (c->d_radial) HIPCHK(hipMemset(c->d_radial, 0, sizeof(unsigned long long) * c->n_chains * c->opt.radial_bins));
+        if (c->d_cart) HIPCHK(hipMemset(c->d_cart, 0, sizeof(unsigned long long) * c->n_chains * c->cart_per_chain));
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+// ---------------------------------------------------------------- hooks
+static int hook_move(pimc_ctx *c, int chain, int n, int j0, int m, const double *in, int nin, int op, double u,
+                     double *r_out, double *w_out, int32_t *accepted, bool com)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    const ChainDev &h = c->ch[chain];
+    if (n < 0 || n >= h.N) return set_err(c, PIMC_ERR_ARG, "particle out of range");
+    if (!com && (j0 < 0 || j0 >= h.M || m < 2 || m > h.M)) return set_err(c, PIMC_ERR_ARG, "bad j0 or m");
+    if (!in) return set_err(c, PIMC_ERR_ARG, "null input");
+    try {
+        const int rows = com ? h.M : m + 1;
+        const size_t nout = (size_t)rows * h.dim + 5;
+        ensure_io(c, nout + (size_t)nin + 8);
+        double *d_in = c->d_io + nout;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(d_in, in, sizeof(double) * nin, hipMemcpyHostToDevice));
+        HookArgs h = hook_args(c, chain);
+        h.n = n;
+        h.j0 = j0;
+        h.m = m;
+        h.op = op;
+        h.in = d_in;
+        h.u = u;
+        h.out = c->d_io;
+        if (com) HOOK_DISPATCH(k_hook_com, dim3(1), dim3(64), 0, h);
+        else HOOK_DISPATCH(k_hook_reshape, dim3(1), dim3(64), 0, h);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> o(nout);
+        HIPCHK(hipMemcpy(o.data(), c->d_io, sizeof(double) * nout, hipMemcpyDeviceToHost));
+        if (r_out) std::memcpy(r_out, o.data(), sizeof(double) * (size_t)rows * c->dim);
+        if (w_out) std::memcpy(w_out, o.data() + (size_t)rows * c->dim, sizeof(double) * 4);
+        if (accepted) *accepted = o[(size_t)rows * c->dim + 4] != 0.0;
+        return check_status(c);
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+}
+
+int pimc_levy_propose(pimc_ctx *c, int32_t chain, int32_t n, int32_t j0, int32_t m, const double *normals,
+                      double *r_out, double *w_out)
+{
+    if (!c) return PIMC_ERR_ARG;
+    return hook_move(c, chain, n, j0, m, normals, (m - 1) * c->dim, 0, 0.0, r_out, w_out, nullptr, false);
+}
+
+int pimc_reshape_linear(pimc_ctx *c, int32_t chain, int32_t n, int32_t j0, int32_t m, const double *normals,
+                        double u, int32_t *accepted)
+{
+    if (!c) return PIMC_ERR_ARG;
+    return hook_move(c, chain, n, j0, m, normals, (m - 1) * c->dim, 1, u, nullptr, nullptr, accepted, false);
+}
+
+int pimc_com_propose(pimc_ctx *c, int32_t chain, int32_t n, const double *d, double *r_out, double *w_out)
+{
+    if (!c) return PIMC_ERR_ARG;
+    return hook_move(c, chain, n, 0, 0, d, c->dim, 0, 0.0, r_out, w_out, nullptr, true);
+}
+
+int pimc_center_of_mass(pimc_ctx *c, int32_t chain, int32_t n, const double *d, double u, int32_t *accepted)
+{
+    if (!c) return PIMC_ERR_ARG;
+    return hook_move(c, chain, n, 0, 0, d, c->dim, 1, u, nullptr, nullptr, accepted, true);
+}
+
+int pimc_interaction(pimc_ctx *c, int32_t chain, int32_t n, int32_t j, double *w)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    const ChainDev &hc = c->ch[chain];
+    if (n < 0 || n >= hc.N || j < 0 || j >= hc.M) return set_err(c, PIMC_ERR_ARG, "bad particle or slice");
+    try {
+        ensure_io(c, 8);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HookArgs h = hook_args(c, chain);
+        h.n = n;
+        h.j0 = j;
+        h.out = c->d_io;
+        HOOK_DISPATCH(k_hook_interaction, dim3(1), dim3(64), 0, h);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(w, c->d_io, sizeof(double), hipMemcpyDeviceToHost));
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_energy(pimc_ctx *c, int32_t chain, double *out)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    try {
+        ensure_io(c, 8);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HookArgs h = hook_args(c, chain);
+        h.out = c->d_io;
+        HOOK_DISPATCH(k_hook_energy, dim3(1), dim3(256), 256 * sizeof(double), h);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(out, c->d_io, sizeof(double) * 4, hipMemcpyDeviceToHost));
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_density(pimc_ctx *c, int32_t chain, int32_t nbins, double *dens, int64_t *ndata)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (nbins < 1 || !dens) return set_err(c, PIMC_ERR_ARG, "bad histogram");
+    const ChainDev &hc = c->ch[chain];
+    const size_t nh = hc.dim == 1 ? (size_t)nbins : (size_t)nbins * nbins;
+    unsigned long long *d_h = nullptr;
+    try {
+        d_h = dalloc<unsigned long long>(nh);
+        HIPCHK(hipMemset(d_h, 0, sizeof(unsigned long long) * nh));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HookArgs h = hook_args(c, chain);
+        const int nb = std::max(1, std::min(1024, (hc.N * hc.M + 255) / 256));
+        if (c->prec == PIMC_FP64) {
+            if (c->dim == 1) hipLaunchKernelGGL((k_hook_density<double, 1>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h);
+            else hipLaunchKernelGGL((k_hook_density<double, 2>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h);
+        } else {
+            if (c->dim == 1) hipLaunchKernelGGL((k_hook_density<float, 1>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h);
+            else hipLaunchKernelGGL((k_hook_density<float, 2>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<unsigned long long> hh(nh);
+        HIPCHK(hipMemcpy(hh.data(), d_h, sizeof(unsigned long long) * nh, hipMemcpyDeviceToHost));
+        dfree(d_h);
+        d_h = nullptr;
+        for (size_t i = 0; i < nh; ++i) dens[i] += (double)hh[i];
+        if (ndata) *ndata += hc.M;
+    } catch (const HipError &e) {
+        dfree(d_h);
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_adjoint(pimc_ctx *c, int32_t chain, double *dS_dr, double *dS_dparams, double *S_out)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    const ChainDev &hc = c->ch[chain];
+    const size_t nb = (size_t)hc.N * hc.dim * hc.M;
+    try {
+        ensure_io(c, nb + 8);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HookArgs h = hook_args(c, chain);
+        h.out = c->d_io;
+        HOOK_DISPATCH(k_hook_adjoint, dim3(1), dim3(256), 256 * sizeof(double), h);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> o(nb + 3);
+        HIPCHK(hipMemcpy(o.data(), c->d_io, sizeof(double) * (nb + 3), hipMemcpyDeviceToHost));
+        if (dS_dr) std::memcpy(dS_dr, o.data(), sizeof(double) * nb);
+        if (dS_dparams) {
+            dS_dparams[0] = o[nb + 1];
+            dS_dparams[1] = o[nb + 2];
+        }
+        if (S_out) *S_out = o[nb];
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_action(pimc_ctx *c, int32_t chain, double *S_out)
+{
+    return pimc_adjoint(c, chain, nullptr, nullptr, S_out);
+}
+
+} // extern "C"

@@ -1,0 +1,856 @@
+// Batched sweep kernels.
+//
+// A sweep of one chain is, per particle, staging segments of length `ell`
+// tiling the ring at a Philox-drawn offset (ReshapeLinear, src/updates/
+// reshape.jl:31-91, with levy!, src/updates/helper.jl:118-139), followed by
+// one centre-of-mass try per particle (SingleCenterOfMass, src/updates/
+// com.jl:136-224).  Segments that share only end beads touch disjoint
+// interior beads, so the segments of a ring are independent given their ends.
+//
+//  * k_sweep_tiles: chains without pair action.  Particles are then
+//    independent, so a workgroup owns a tile of particles (of one or several
+//    chains with equal M), stages the tile in LDS, runs K sweeps there and
+//    writes it back once.  One thread per (particle, segment).
+//  * k_sweep_whole: chains with a pair action (and chains whose estimators
+//    need whole-chain products).  A workgroup owns a chain in HBM; particles
+//    are updated one after another, the segments (and links) of the current
+//    particle in parallel, pair sums through per-slice cell lists.
+#pragma once
+#include "physics.hpp"
+
+namespace pimc {
+
+constexpr int kItemsPerThread = 4; // register slots of the estimator sums per thread
+constexpr int kNQ = 12;            // per-chain per-block estimator quantities
+
+// block accumulator layout (doubles), per (block, chain)
+enum BlockQ {
+    qN = 0,      // measurements
+    qE = 1,      // sum E (thermodynamic + pair)
+    qEv = 2,     // sum E_virial (+ pair virial)
+    qEpair = 3,  // sum pair energy
+    qSw = 4,     // sum dS/d omega
+    qSg = 5,     // sum dS/d g
+    qESw = 6,    // sum E dS/d omega
+    qESg = 7,    // sum E dS/d g
+    qdEw = 8,    // sum dE/d omega
+    qdEg = 9,    // sum dE/d g
+    qE2 = 10,    // sum E^2
+    qWhole = 11, // 1 if the products above were measured on the whole chain
+};
+
+struct TileDev {
+    long long pos_off; // element offset of the tile's first particle
+    int q0, np, M, nseg;
+    int ell, items, pad0, pad1;
+};
+
+struct SweepArgs {
+    const ChainDev *chains;
+    const TileDev *tiles;
+    const int *part_chain; // global particle -> chain
+    const int *part_idx;   // global particle -> index within its chain
+    void *pos;
+    double *part_acc;      // per particle, per launch: sum link^2, sum V, sum r.gradV, measurements
+    int *part_lacc;        // per particle, per launch: accepted staging segments
+    int *part_cacc;        // per particle, per launch: accepted COM moves
+    const double *com_step;
+    unsigned long long *radial;
+    unsigned long long *cart;
+    double radial_winv;
+    int radial_bins, cart_bins;
+    uint64_t seed;
+    unsigned sweep0;
+    int nsweeps, measure;
+    // whole-chain kernel
+    const int *whole;
+    int *cell_cnt;
+    unsigned short *cell_mem;
+    int *bins;
+    double *chain_meas;    // per chain, per launch: kNQ sums
+    int *status;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// LDS layout of k_sweep_tiles; the host sizes the launch with the same function.
+__host__ __device__ inline size_t tile_lds_bytes(int np, int dim, int M, int items, int tsize)
+{
+    size_t posb = (size_t)np * dim * M * tsize;
+    size_t redb = (size_t)items * 3 * sizeof(double);
+    size_t o = align16(posb > redb ? posb : redb);
+    o += 4 * align16((size_t)np * 4);     // off, flag, lacc, cacc
+    o += align16((size_t)np * dim * tsize); // COM displacement
+    o += align16((size_t)np * tsize);       // COM uniform
+    o += align16((size_t)items * tsize);    // COM partial sums
+    return o;
+}
+
+// One staging segment of particle base `xp` (LDS or HBM), start slice
+// `start`, m links.  Returns true if accepted (beads written back).
+template <typename T, int DIM, int MAXL>
+__device__ __forceinline__ bool levy_segment(T *xp, int M, int start, int m, const Prm<T> &P, uint64_t seed,
+                                             uint32_t item0, uint32_t sweep, uint32_t chain)
+{
+    int jm = start + m;
+    if (jm >= M) jm -= M;
+    T cur[DIM], em[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        cur[d] = xp[d * M + start];
+        em[d] = shift_end(cur[d], xp[d * M + jm], P);
+    }
+    T nb[MAXL - 1][DIM];
+    T dv = T(0);
+#pragma unroll
+    for (int i = 1; i < MAXL; ++i) {
+        if (i < m) {
+            int b = start + i;
+            if (b >= M) b -= M;
+            const U4 x = draw(seed, item0 + (uint32_t)b, sweep, chain, kLevy);
+            T n0, n1;
+            box_muller(x.x, x.y, n0, n1);
+            const T alpha = bridge_alpha<T>(m, i);
+            const T sd = bridge_sd(alpha, P);
+            cur[0] = bridge_step(cur[0], em[0], alpha, n0, sd);
+            nb[i - 1][0] = teleport(cur[0], P);
+            if (DIM > 1) {
+                cur[1] = bridge_step(cur[1], em[1], alpha, n1, sd);
+                nb[i - 1][1] = teleport(cur[1], P);
+            }
+            T old[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) old[d] = xp[d * M + b];
+            dv += potV<T, DIM>(nb[i - 1], P) - potV<T, DIM>(old, P);
+        }
+    }
+    const T lw = -P.tau * dv;
+    const U4 xa = draw(seed, item0 + (uint32_t)start, sweep, chain, kLevyAccept);
+    const bool acc = metropolis(ex(lw), Uniform<T>::get(xa.x));
+    if (acc) {
+#pragma unroll
+        for (int i = 1; i < MAXL; ++i) {
+            if (i < m) {
+                int b = start + i;
+                if (b >= M) b -= M;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xp[d * M + b] = nb[i - 1][d];
+            }
+        }
+    }
+    return acc;
+}
+
+template <typename T, int DIM, int MAXL>
+__global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const TileDev tl = a.tiles[blockIdx.x];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int np = tl.np, M = tl.M, nseg = tl.nseg, ell = tl.ell, items = tl.items;
+    const int nel = np * DIM * M;
+
+    T *sp = reinterpret_cast<T *>(smem);
+    size_t posb = (size_t)nel * sizeof(T), redb = (size_t)items * 3 * sizeof(double);
+    size_t o = align16(posb > redb ? posb : redb);
+    int *s_off = reinterpret_cast<int *>(smem + o);
+    o += align16((size_t)np * 4);
+    int *s_flag = reinterpret_cast<int *>(smem + o);
+    o += align16((size_t)np * 4);
+    int *s_lacc = reinterpret_cast<int *>(smem + o);
+    o += align16((size_t)np * 4);
+    int *s_cacc = reinterpret_cast<int *>(smem + o);
+    o += align16((size_t)np * 4);
+    T *s_d = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)np * DIM * sizeof(T));
+    T *s_u = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)np * sizeof(T));
+    T *s_red = reinterpret_cast<T *>(smem + o);
+
+    T *gpos = reinterpret_cast<T *>(a.pos) + tl.pos_off;
+    for (int i = tid; i < nel; i += nt) sp[i] = gpos[i];
+    for (int q = tid; q < np; q += nt) {
+        s_lacc[q] = 0;
+        s_cacc[q] = 0;
+    }
+    double mL[kItemsPerThread], mV[kItemsPerThread], mR[kItemsPerThread];
+#pragma unroll
+    for (int r = 0; r < kItemsPerThread; ++r) mL[r] = mV[r] = mR[r] = 0.0;
+    int nmeas = 0;
+    __syncthreads();
+
+    for (int s = 0; s < a.nsweeps; ++s) {
+        const uint32_t sw = a.sweep0 + (uint32_t)s;
+        // per-particle draws of this sweep: staging offset, COM displacement and uniform
+        for (int q = tid; q < np; q += nt) {
+            const int gq = tl.q0 + q;
+            const int c = a.part_chain[gq], p = a.part_idx[gq];
+            const U4 x = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kOffset);
+            s_off[q] = (int)(((uint64_t)x.x * (uint64_t)ell) >> 32);
+            const U4 y = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kCom);
+            const T step2 = (T)a.com_step[c] * T(2);
+            s_d[q * DIM + 0] = step2 * (Uniform<T>::get(y.x) - T(0.5));
+            if (DIM > 1) s_d[q * DIM + 1] = step2 * (Uniform<T>::get(y.y) - T(0.5));
+            s_u[q] = Uniform<T>::get(y.z);
+        }
+        __syncthreads();
+
+        // staging segments
+        for (int it = tid; it < items; it += nt) {
+            const int q = it / nseg, k = it - q * nseg;
+            const int m = min(ell, M - k * ell);
+            if (m < 2) continue;
+            const int gq = tl.q0 + q;
+            const int c = a.part_chain[gq], p = a.part_idx[gq];
+            const Prm<T> P = Prm<T>::make(a.chains[c]);
+            int start = s_off[q] + k * ell;
+            if (start >= M) start -= M;
+            if (levy_segment<T, DIM, MAXL>(sp + (size_t)q * DIM * M, M, start, m, P, a.seed, (uint32_t)(p * M), sw,
+                                           (uint32_t)c))
+                atomicAdd(&s_lacc[q], 1);
+        }
+        __syncthreads();
+
+        // centre of mass: partial Delta V per chunk of `ell` beads
+        for (int it = tid; it < items; it += nt) {
+            const int q = it / nseg, k = it - q * nseg;
+            const int j0 = k * ell, j1 = min(M, j0 + ell);
+            const int c = a.part_chain[tl.q0 + q];
+            const Prm<T> P = Prm<T>::make(a.chains[c]);
+            const T *xp = sp + (size_t)q * DIM * M;
+            T d[DIM];
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) d[e] = s_d[q * DIM + e];
+            T dv = T(0);
+            for (int j = j0; j < j1; ++j) {
+                T ro[DIM], rn[DIM];
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) {
+                    ro[e] = xp[e * M + j];
+                    rn[e] = teleport(ro[e] + d[e], P);
+                }
+                dv += potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P);
+            }
+            s_red[it] = dv;
+        }
+        __syncthreads();
+        for (int q = tid; q < np; q += nt) {
+            T sum = T(0);
+            for (int k = 0; k < nseg; ++k) sum += s_red[q * nseg + k];
+            const int c = a.part_chain[tl.q0 + q];
+            const T lw = -(T)a.chains[c].tau * sum;
+            const int acc = metropolis(ex(lw), s_u[q]) ? 1 : 0;
+            s_flag[q] = acc;
+            s_cacc[q] += acc;
+        }
+        __syncthreads();
+        for (int it = tid; it < items; it += nt) {
+            const int q = it / nseg, k = it - q * nseg;
+            if (!s_flag[q]) continue;
+            const int j0 = k * ell, j1 = min(M, j0 + ell);
+            const int c = a.part_chain[tl.q0 + q];
+            const Prm<T> P = Prm<T>::make(a.chains[c]);
+            T *xp = sp + (size_t)q * DIM * M;
+            for (int j = j0; j < j1; ++j) {
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) xp[e * M + j] = teleport(xp[e * M + j] + s_d[q * DIM + e], P);
+            }
+        }
+        __syncthreads();
+
+        // estimators (src/measurement.jl:45-55, 92-122), FP64 sums per chunk
+        if (a.measure) {
+#pragma unroll
+            for (int r = 0; r < kItemsPerThread; ++r) {
+                const int it = tid + r * nt;
+                if (it < items) {
+                    const int q = it / nseg, k = it - q * nseg;
+                    const int j0 = k * ell, j1 = min(M, j0 + ell);
+                    const int c = a.part_chain[tl.q0 + q];
+                    const ChainDev &ch = a.chains[c];
+                    const Prm<T> P = Prm<T>::make(ch);
+                    const T *xp = sp + (size_t)q * DIM * M;
+                    double l = 0.0, v = 0.0, g = 0.0;
+                    for (int j = j0; j < j1; ++j) {
+                        const int jn = (j + 1 == M) ? 0 : j + 1;
+                        T r0[DIM];
+                        T dd = T(0);
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) {
+                            r0[e] = xp[e * M + j];
+                            const T dx = mindist(r0[e], xp[e * M + jn], P);
+                            dd += dx * dx;
+                        }
+                        l += (double)dd;
+                        v += (double)potV<T, DIM>(r0, P);
+                        g += (double)rgradV<T, DIM>(r0, P);
+                        if (a.radial_bins > 0) {
+                            T rr = r0[0] * r0[0];
+                            if (DIM > 1) rr += r0[1] * r0[1];
+                            const double qb = (double)sq(rr) * a.radial_winv;
+                            if (qb < (double)a.radial_bins)
+                                atomicAdd(&a.radial[(size_t)c * a.radial_bins + (int)qb], 1ull);
+                        }
+                        if (a.cart_bins > 0) {
+                            // the reference's Density: floor((r + L)/bin) used as a 1-based index
+                            const double bw = (2.0 * ch.L) / a.cart_bins;
+                            long long idx = 0, stride = 1;
+                            bool ok = true;
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) {
+                                const long long ib = (long long)floor(((double)r0[e] + ch.L) / bw);
+                                ok = ok && ib > 0 && ib < a.cart_bins + 1;
+                                idx += (ib - 1) * stride;
+                                stride *= a.cart_bins;
+                            }
+                            if (ok) atomicAdd(&a.cart[(size_t)c * stride + idx], 1ull);
+                        }
+                    }
+                    mL[r] += l;
+                    mV[r] += v;
+                    mR[r] += g;
+                }
+            }
+            ++nmeas;
+        }
+    }
+
+    __syncthreads();
+    for (int i = tid; i < nel; i += nt) gpos[i] = sp[i];
+    __syncthreads();
+    if (a.measure) {
+        double *red3 = reinterpret_cast<double *>(smem);
+#pragma unroll
+        for (int r = 0; r < kItemsPerThread; ++r) {
+            const int it = tid + r * nt;
+            if (it < items) {
+                red3[it * 3 + 0] = mL[r];
+                red3[it * 3 + 1] = mV[r];
+                red3[it * 3 + 2] = mR[r];
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < np; q += nt) {
+            double l = 0.0, v = 0.0, g = 0.0;
+            for (int k = 0; k < nseg; ++k) {
+                l += red3[(q * nseg + k) * 3 + 0];
+                v += red3[(q * nseg + k) * 3 + 1];
+                g += red3[(q * nseg + k) * 3 + 2];
+            }
+            double *pa = a.part_acc + (size_t)(tl.q0 + q) * 4;
+            pa[0] = l;
+            pa[1] = v;
+            pa[2] = g;
+            pa[3] = (double)nmeas;
+        }
+    }
+    for (int q = tid; q < np; q += nt) {
+        a.part_lacc[tl.q0 + q] = s_lacc[q];
+        a.part_cacc[tl.q0 + q] = s_cacc[q];
+    }
+}
+
+// Deterministic block sum (fixed tree for a given blockDim).  Every thread
+// gets the result.  `buf` holds blockDim doubles.
+__device__ inline double block_sum(double v, double *buf)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    buf[tid] = v;
+    __syncthreads();
+    for (int w = 1; w < nt; w <<= 1) {
+        double add = 0.0;
+        const bool take = ((tid & (2 * w - 1)) == 0) && (tid + w < nt);
+        if (take) add = buf[tid + w];
+        __syncthreads();
+        if (take) buf[tid] += add;
+        __syncthreads();
+    }
+    const double r = buf[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename T, int DIM, bool CELLS>
+__device__ __forceinline__ T link_sum(const Path<T, DIM> &path, const Cells &C, int N, int nbins, const Prm<T> &P,
+                                      int n, int j, const T *pj, const T *pj1)
+{
+    if (CELLS) return pair_link<T, DIM>(path, C, P, n, j, pj, pj1);
+    return pair_link_all<T, DIM>(path, N, nbins, P, n, j, pj, pj1);
+}
+
+// Pair energy of bead (i, j): S_U share (each unordered pair counted from
+// both ends with weight 1/2) and its r . grad S_U share.
+template <typename T, int DIM, bool CELLS>
+__device__ inline void pair_energy_bead(const Path<T, DIM> &path, const Cells &C, int N, int nbins, const Prm<T> &P,
+                                        double inv_s2, int i, int j, double &su, double &rg)
+{
+    const int M = path.M;
+    const int jn = (j + 1 == M) ? 0 : j + 1;
+    T ri[DIM], rin[DIM];
+    path.get(i, j, ri);
+    path.get(i, jn, rin);
+    su = 0.0;
+    rg = 0.0;
+    auto visit = [&](int k, int mult) {
+        T rk[DIM];
+        path.get(k, j, rk);
+        T d0 = mindist(rk[0], ri[0], P);
+        T r2 = d0 * d0;
+        if (DIM > 1) {
+            T d1 = mindist(rk[1], ri[1], P);
+            r2 += d1 * d1;
+        }
+        if (r2 > P.h2) return;
+        T rkn[DIM];
+        path.get(k, jn, rkn);
+        T e0 = mindist(rkn[0], rin[0], P);
+        T s2 = e0 * e0;
+        if (DIM > 1) {
+            T e1 = mindist(rkn[1], rin[1], P);
+            s2 += e1 * e1;
+        }
+        const double va = (double)vpair_r2(r2, P), vb = (double)vpair_r2(s2, P);
+        const double w = 0.5 * (0.5 * (double)P.tau) * mult;
+        su += w * (va + vb);
+        rg += w * (-((double)r2 * inv_s2) * va - ((double)s2 * inv_s2) * vb);
+    };
+    if (CELLS) {
+        const int c0 = cell_of<T, DIM>(ri, P, C.nbins);
+        const int ns = (DIM == 1) ? 3 : 9;
+        for (int s = 0; s < ns; ++s) {
+            const int c = stencil_cell(c0, s, C.nbins, DIM);
+            const int cnt = C.count(j, c);
+            const unsigned short *mem = C.members(j, c);
+            for (int q = 0; q < cnt; ++q)
+                if (mem[q] != i) visit(mem[q], 1);
+        }
+    } else {
+        const int ns = (DIM == 1) ? 3 : 9;
+        const int c0 = (nbins < 3) ? cell_of<T, DIM>(ri, P, nbins) : 0;
+        for (int k = 0; k < N; ++k) {
+            if (k == i) continue;
+            int mult = 1;
+            if (nbins < 3) {
+                T rk[DIM];
+                path.get(k, j, rk);
+                const int ck = cell_of<T, DIM>(rk, P, nbins);
+                mult = 0;
+                for (int s = 0; s < ns; ++s) mult += (stencil_cell(c0, s, nbins, DIM) == ck);
+                if (!mult) continue;
+            }
+            visit(k, mult);
+        }
+    }
+}
+
+// Per-measurement whole-chain sums -> the kNQ quantities.
+struct ChainMeas {
+    double link, V, rg, su, rgp, r2w;
+};
+
+template <typename T, int DIM, bool CELLS>
+__device__ inline ChainMeas measure_chain(const Path<T, DIM> &path, const Cells &C, const ChainDev &ch, const Prm<T> &P,
+                                          double *buf, unsigned long long *radial, int radial_bins, double radial_winv,
+                                          unsigned long long *cart, int cart_bins, int c)
+{
+    const int N = ch.N, M = ch.M;
+    const double inv_s2 = (ch.pair != kPairNone) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
+    double l = 0.0, v = 0.0, g = 0.0, su = 0.0, rgp = 0.0, r2w = 0.0;
+    for (int b = threadIdx.x; b < N * M; b += blockDim.x) {
+        const int i = b / M, j = b - i * M;
+        const int jn = (j + 1 == M) ? 0 : j + 1;
+        T r0[DIM], r1[DIM];
+        path.get(i, j, r0);
+        path.get(i, jn, r1);
+        T dd = T(0);
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+            const T dx = mindist(r0[e], r1[e], P);
+            dd += dx * dx;
+        }
+        l += (double)dd;
+        v += (double)potV<T, DIM>(r0, P);
+        g += (double)rgradV<T, DIM>(r0, P);
+        if (ch.pot == kPotHarmonic) r2w += (double)dVdomega<T, DIM>(r0, ch.omega);
+        if (ch.pair != kPairNone) {
+            double a, bb;
+            pair_energy_bead<T, DIM, CELLS>(path, C, N, ch.nbins, P, inv_s2, i, j, a, bb);
+            su += a;
+            rgp += bb;
+        }
+        if (radial_bins > 0) {
+            T rr = r0[0] * r0[0];
+            if (DIM > 1) rr += r0[1] * r0[1];
+            const double qb = (double)sq(rr) * radial_winv;
+            if (qb < (double)radial_bins) atomicAdd(&radial[(size_t)c * radial_bins + (int)qb], 1ull);
+        }
+        if (cart_bins > 0) {
+            const double bw = (2.0 * ch.L) / cart_bins;
+            long long idx = 0, stride = 1;
+            bool ok = true;
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) {
+                const long long ib = (long long)floor(((double)r0[e] + ch.L) / bw);
+                ok = ok && ib > 0 && ib < cart_bins + 1;
+                idx += (ib - 1) * stride;
+                stride *= cart_bins;
+            }
+            if (ok) atomicAdd(&cart[(size_t)c * stride + idx], 1ull);
+        }
+    }
+    ChainMeas r;
+    r.link = block_sum(l, buf);
+    r.V = block_sum(v, buf);
+    r.rg = block_sum(g, buf);
+    r.su = block_sum(su, buf);
+    r.rgp = block_sum(rgp, buf);
+    r.r2w = block_sum(r2w, buf);
+    return r;
+}
+
+// Fold one measurement of a whole chain into the kNQ sums.
+__device__ inline void fold_meas(const ChainMeas &m, const ChainDev &ch, double *q)
+{
+    const double tau = ch.tau, M = ch.M;
+    const double Epair = m.su / ch.beta;
+    const double E = ((double)(ch.dim * ch.N) / (2.0 * tau) - m.link / (((4.0 * ch.lambda) * (tau * tau)) * M)) +
+                     m.V / M + Epair;
+    const double Ev = m.rg / (2.0 * M) + m.V / M + Epair + m.rgp / (2.0 * ch.beta);
+    const double Sw = tau * m.r2w;
+    const double Sg = (ch.g != 0.0) ? m.su / ch.g : 0.0;
+    const double dEw = m.r2w / M;
+    const double dEg = (ch.g != 0.0) ? Epair / ch.g : 0.0;
+    q[qN] += 1.0;
+    q[qE] += E;
+    q[qEv] += Ev;
+    q[qEpair] += Epair;
+    q[qSw] += Sw;
+    q[qSg] += Sg;
+    q[qESw] += E * Sw;
+    q[qESg] += E * Sg;
+    q[qdEw] += dEw;
+    q[qdEg] += dEg;
+    q[qE2] += E * E;
+    q[qWhole] = 1.0;
+}
+
+// LDS of k_sweep_whole: new beads (M x DIM T), Delta V per segment/slice (M
+// T), Delta lnU per link (M T), accept flags (M ints), reduction buffer.
+__host__ __device__ inline size_t whole_lds_bytes(int M, int dim, int tsize, int nt)
+{
+    size_t o = align16((size_t)M * dim * tsize);
+    o += align16((size_t)M * tsize);
+    o += align16((size_t)M * tsize);
+    o += align16((size_t)M * 4);
+    o += align16((size_t)nt * sizeof(double));
+    return o;
+}
+
+template <typename T, int DIM, int MAXL, bool CELLS>
+__global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int c = a.whole[blockIdx.x];
+    const ChainDev ch = a.chains[c];
+    const Prm<T> P = Prm<T>::make(ch);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int N = ch.N, M = ch.M, ell = ch.ell, nseg = ch.nseg;
+    const Path<T, DIM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, M};
+    const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off,
+                      M, ch.ncell, ch.cap, ch.nbins};
+    const bool use_pair = (ch.pair != kPairNone) && ch.pair_in_accept;
+
+    size_t o = 0;
+    T *s_new = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)M * DIM * sizeof(T));
+    T *s_dv = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)M * sizeof(T));
+    T *s_du = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)M * sizeof(T));
+    int *s_flag = reinterpret_cast<int *>(smem + o);
+    o += align16((size_t)M * 4);
+    double *s_buf = reinterpret_cast<double *>(smem + o);
+
+    double q[kNQ];
+#pragma unroll
+    for (int k = 0; k < kNQ; ++k) q[k] = 0.0;
+    int lacc = 0, cacc = 0, overflow = 0;
+
+    for (int s = 0; s < a.nsweeps; ++s) {
+        const uint32_t sw = a.sweep0 + (uint32_t)s;
+        // ---- staging: particles in order, segments of one particle in parallel
+        for (int p = 0; p < N; ++p) {
+            const U4 xo = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kOffset);
+            const int off = (int)(((uint64_t)xo.x * (uint64_t)ell) >> 32);
+            for (int k = tid; k < nseg; k += nt) {
+                const int m = min(ell, M - k * ell);
+                s_dv[k] = T(0);
+                if (m < 2) continue;
+                int start = off + k * ell;
+                if (start >= M) start -= M;
+                int jm = start + m;
+                if (jm >= M) jm -= M;
+                T cur[DIM], em[DIM];
+                path.get(p, start, cur);
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], path.at(p, e, jm), P);
+                T dv = T(0);
+                for (int i = 1; i < m; ++i) {
+                    int b = start + i;
+                    if (b >= M) b -= M;
+                    const U4 x = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c, kLevy);
+                    T n0, n1;
+                    box_muller(x.x, x.y, n0, n1);
+                    const T alpha = bridge_alpha<T>(m, i);
+                    const T sd = bridge_sd(alpha, P);
+                    T rn[DIM], ro[DIM];
+                    cur[0] = bridge_step(cur[0], em[0], alpha, n0, sd);
+                    rn[0] = teleport(cur[0], P);
+                    if (DIM > 1) {
+                        cur[1] = bridge_step(cur[1], em[1], alpha, n1, sd);
+                        rn[1] = teleport(cur[1], P);
+                    }
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) s_new[b * DIM + e] = rn[e];
+                    path.get(p, b, ro);
+                    dv += potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P);
+                }
+                s_dv[k] = dv;
+            }
+            __syncthreads();
+            if (use_pair) {
+                for (int j = tid; j < M; j += nt) {
+                    int rel = j - off;
+                    if (rel < 0) rel += M;
+                    const int k = rel / ell, t = rel - k * ell;
+                    const int m = min(ell, M - k * ell);
+                    s_du[j] = T(0);
+                    if (m < 2) continue;
+                    const int jn = (j + 1 == M) ? 0 : j + 1;
+                    T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
+                    path.get(p, j, o0);
+                    path.get(p, jn, o1);
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) {
+                        n0[e] = (t == 0) ? o0[e] : s_new[j * DIM + e];
+                        n1[e] = (t + 1 == m) ? o1[e] : s_new[jn * DIM + e];
+                    }
+                    s_du[j] = link_sum<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, n0, n1) -
+                              link_sum<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, o0, o1);
+                }
+                __syncthreads();
+            }
+            for (int k = tid; k < nseg; k += nt) {
+                const int m = min(ell, M - k * ell);
+                s_flag[k] = 0;
+                if (m < 2) continue;
+                int start = off + k * ell;
+                if (start >= M) start -= M;
+                T lw = -P.tau * s_dv[k];
+                if (use_pair) {
+                    for (int t = 0; t < m; ++t) {
+                        int j = start + t;
+                        if (j >= M) j -= M;
+                        lw += s_du[j];
+                    }
+                }
+                const U4 xa = draw(a.seed, (uint32_t)(p * M + start), sw, (uint32_t)c, kLevyAccept);
+                const int acc = metropolis(ex(lw), Uniform<T>::get(xa.x)) ? 1 : 0;
+                s_flag[k] = acc;
+                lacc += acc;
+            }
+            __syncthreads();
+            for (int j = tid; j < M; j += nt) {
+                int rel = j - off;
+                if (rel < 0) rel += M;
+                const int k = rel / ell, t = rel - k * ell;
+                const int m = min(ell, M - k * ell);
+                if (m >= 2 && t >= 1 && s_flag[k]) {
+                    T rn[DIM];
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) rn[e] = s_new[j * DIM + e];
+                    path.put(p, j, rn);
+                    if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn);
+                }
+            }
+            __syncthreads();
+        }
+        // ---- centre of mass: one particle after another
+        for (int p = 0; p < N; ++p) {
+            const U4 y = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kCom);
+            const T step2 = (T)a.com_step[c] * T(2);
+            T d[DIM];
+            d[0] = step2 * (Uniform<T>::get(y.x) - T(0.5));
+            if (DIM > 1) d[1] = step2 * (Uniform<T>::get(y.y) - T(0.5));
+            double part = 0.0;
+            for (int j = tid; j < M; j += nt) {
+                T ro[DIM], rn[DIM];
+                path.get(p, j, ro);
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) {
+                    rn[e] = teleport(ro[e] + d[e], P);
+                    s_new[j * DIM + e] = rn[e];
+                }
+                part += (double)(potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P));
+            }
+            __syncthreads();
+            double du = 0.0;
+            if (use_pair) {
+                for (int j = tid; j < M; j += nt) {
+                    const int jn = (j + 1 == M) ? 0 : j + 1;
+                    T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
+                    path.get(p, j, o0);
+                    path.get(p, jn, o1);
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) {
+                        n0[e] = s_new[j * DIM + e];
+                        n1[e] = s_new[jn * DIM + e];
+                    }
+                    du += (double)(link_sum<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, n0, n1) -
+                                   link_sum<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, o0, o1));
+                }
+            }
+            const double dvs = block_sum(part, s_buf);
+            const double dus = use_pair ? block_sum(du, s_buf) : 0.0;
+            const T lw = -P.tau * (T)dvs + (T)dus;
+            const bool acc = metropolis(ex(lw), Uniform<T>::get(y.z));
+            cacc += (acc && tid == 0) ? 1 : 0;
+            if (acc) {
+                for (int j = tid; j < M; j += nt) {
+                    T rn[DIM];
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) rn[e] = s_new[j * DIM + e];
+                    path.put(p, j, rn);
+                    if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn);
+                }
+            }
+            __syncthreads();
+        }
+        if (a.measure) {
+            const ChainMeas m = measure_chain<T, DIM, CELLS>(path, cells, ch, P, s_buf, a.radial, a.radial_bins,
+                                                              a.radial_winv, a.cart, a.cart_bins, c);
+            fold_meas(m, ch, q);
+        }
+    }
+    // per-launch outputs (thread sums of accept counts reduced deterministically)
+    const double la = block_sum((double)lacc, s_buf);
+    const double ca = block_sum((double)cacc, s_buf);
+    if (tid == 0) {
+        double *out = a.chain_meas + (size_t)c * (kNQ + 2);
+        for (int k = 0; k < kNQ; ++k) out[k] = q[k];
+        out[kNQ] = la;
+        out[kNQ + 1] = ca;
+    }
+    if (overflow) atomicOr(a.status, 1);
+}
+
+// After each launch: fold per-particle (tile kernel) or per-chain (whole
+// kernel) launch outputs into the cumulative counters and, when measuring,
+// into block `blk`; while thermalising, adapt the COM step per chain with the
+// rule of adjust!(::Step) (src/updates/helper.jl:22-32) on this launch's
+// acceptance.
+struct PostArgs {
+    const ChainDev *chains;
+    int n_chains;
+    const int *is_whole;
+    const double *part_acc;
+    const int *part_lacc, *part_cacc;
+    const double *chain_meas;
+    double *com_step;
+    long long *counters; // per chain: levy accepted, COM accepted
+    double *blocks;      // [blk][chain][kNQ]
+    int blk, measure, nsweeps;
+};
+
+__global__ void k_post(PostArgs a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n_chains) return;
+    const ChainDev &ch = a.chains[c];
+    long long la = 0, ca = 0;
+    double q[kNQ];
+    for (int k = 0; k < kNQ; ++k) q[k] = 0.0;
+    if (a.is_whole[c]) {
+        const double *src = a.chain_meas + (size_t)c * (kNQ + 2);
+        for (int k = 0; k < kNQ; ++k) q[k] = src[k];
+        la = (long long)src[kNQ];
+        ca = (long long)src[kNQ + 1];
+    } else {
+        double l = 0.0, v = 0.0, g = 0.0, nm = 0.0;
+        for (int p = 0; p < ch.N; ++p) {
+            const int gq = ch.part_base + p;
+            la += a.part_lacc[gq];
+            ca += a.part_cacc[gq];
+            if (a.measure) {
+                l += a.part_acc[(size_t)gq * 4 + 0];
+                v += a.part_acc[(size_t)gq * 4 + 1];
+                g += a.part_acc[(size_t)gq * 4 + 2];
+                nm = a.part_acc[(size_t)gq * 4 + 3];
+            }
+        }
+        if (a.measure) {
+            const double tau = ch.tau, M = ch.M;
+            q[qN] = nm;
+            q[qE] = nm * ((double)(ch.dim * ch.N) / (2.0 * tau)) - l / (((4.0 * ch.lambda) * (tau * tau)) * M) + v / M;
+            q[qEv] = g / (2.0 * M) + v / M;
+            if (ch.pot == kPotHarmonic && ch.omega != 0.0) {
+                // sum omega r^2 = 2 sum V / omega for V = omega^2 r^2 / 2
+                q[qSw] = tau * 2.0 * v / ch.omega;
+                q[qdEw] = 2.0 * v / (ch.omega * M);
+            }
+        }
+    }
+    a.counters[(size_t)c * 2 + 0] += la;
+    a.counters[(size_t)c * 2 + 1] += ca;
+    if (a.measure) {
+        double *b = a.blocks + ((size_t)a.blk * a.n_chains + c) * kNQ;
+        for (int k = 0; k < kNQ; ++k) b[k] = q[k];
+    } else {
+        const long long tries = (long long)ch.N * a.nsweeps;
+        double step = a.com_step[c];
+        if (tries > 0) {
+            const double acc = (double)ca / (double)tries;
+            if (acc < 0.4) step *= 0.9;
+            else if (acc > 0.6) step *= 1.1;
+            step = fmax(0.1, fmin(ch.L / 2.0, step));
+        }
+        a.com_step[c] = step;
+    }
+}
+
+// Rebuild the cell lists of the whole chains from their positions (init_nn,
+// src/system.jl:80-91; psort!, src/nearest_neighbours.jl:40-52): one thread
+// per (chain, slice), particles in index order.
+template <typename T, int DIM>
+__global__ void k_build_cells(const ChainDev *chains, const int *whole, int n_whole, const void *pos, int *cell_cnt,
+                              unsigned short *cell_mem, int *bins, int *status)
+{
+    const int w = blockIdx.y;
+    if (w >= n_whole) return;
+    const int c = whole[w];
+    const ChainDev ch = chains[c];
+    if (ch.ncell <= 0) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ch.M) return;
+    const Prm<T> P = Prm<T>::make(ch);
+    const Path<T, DIM> path{const_cast<T *>(reinterpret_cast<const T *>(pos)) + ch.pos_off, ch.M};
+    const Cells C{cell_cnt + ch.cell_off, cell_mem + ch.mem_off, bins + ch.bin_off, ch.M,
+                  ch.ncell, ch.cap, ch.nbins};
+    for (int cc = 0; cc < ch.ncell; ++cc) C.count(j, cc) = 0;
+    for (int n = 0; n < ch.N; ++n) {
+        T r[DIM];
+        path.get(n, j, r);
+        const int cell = cell_of<T, DIM>(r, P, ch.nbins);
+        int &cnt = C.count(j, cell);
+        if (cnt < ch.cap) {
+            C.members(j, cell)[cnt] = (unsigned short)n;
+            cnt += 1;
+            C.bin[(long long)n * ch.M + j] = cell;
+        } else {
+            C.bin[(long long)n * ch.M + j] = -1;
+            atomicOr(status, 1);
+        }
+    }
+}
+
+} // namespace pimc
