@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Throughput of the PIMC sampling path on one MI355X.
+
+One step = one sweep of every chain of the batched trapped ideal-gas workload
+(4096 chains, N ~ U[4,64], M ~ U{32,64,128,256}, beta ~ U[0.5,4], omega = 1,
+FP32 paths and action differences, FP64 estimators): per particle, staging
+segments of length 8 tiling the ring at a random offset, one centre-of-mass
+try, then the energy and radial-density estimators.  Inputs are resident in
+HBM before the timed region.  The metric counts proposed beads: m-1 per
+staging segment of m links, M per centre-of-mass try.
+
+Printed (one JSON line): the batched value, the plain single-chain GPU form
+(N=10, M=64, beta=1, FP64), the energy check against the closed-form
+primitive-action energies, the roofline of the sweep kernel and the CPU
+baseline (the oracle's FP64 restatement of the reference schedule on host
+cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "pimc.jl_amd"), os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_MOVE_FP32 = 16  # read + write of one 2D FP32 bead
+BYTES_PER_MOVE_FP64 = 32
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--chains", type=int, default=4096)
+    ap.add_argument("--spl", type=int, default=50, help="sweeps per launch (one estimator block)")
+    ap.add_argument("--plain-steps", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-beads", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def energy_check(batch, wl):
+    from analytic import energy_boltzmann_primitive
+    z = []
+    for c, d in enumerate(wl.descs):
+        e = batch.estimates(c)
+        exact = energy_boltzmann_primitive(d.N, d.M, d.beta)
+        if e.E_err > 0:
+            z.append((e.E - exact) / e.E_err)
+    z = np.array(z)
+    return {
+        "chains": int(len(z)),
+        "frac_within_2sigma": float(np.mean(np.abs(z) < 2.0)) if len(z) else None,
+        "chi2_per_chain": float(np.mean(z ** 2)) if len(z) else None,
+        "mean_pull": float(np.mean(z)) if len(z) else None,
+        "reference": "closed-form primitive-action energy per chain (normal modes of the discrete ring)",
+    }
+
+
+def cpu_baseline(wl, seconds: float):
+    """The oracle's FP64 restatement of run!(s, n, [(1, SingleCenterOfMass(s, 1.0)),
+    (1, ReshapeLinear(s, 8))]) on one chain per host core, chains taken from
+    the same batched workload.  ctypes releases the GIL, so threads run in parallel."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.lib()
+    cores = max(1, min(16, os.cpu_count() or 1))
+    systems = []
+    for c in range(cores):
+        d = wl.descs[c % len(wl.descs)]
+        s = O.System(wl.init[c % len(wl.init)], d.lam, d.L, d.beta, O.make_pot(O.POT_HARMONIC, d.omega))
+        systems.append((s, O.RefRunner(s, cap=8, step=1.0, seed=1000 + c)))
+    # each thread runs chunks of attempts until the shared deadline
+    for s, r in systems:
+        r.run(2000)  # thermalise the adaptive cap and step a little
+        r.st.bead_moves = 0
+    attempts = [0] * len(systems)
+    t0 = time.perf_counter()
+    deadline = t0 + seconds
+
+    def worker(i, r):
+        while time.perf_counter() < deadline:
+            r.run(2000)
+            attempts[i] += 2000
+
+    th = [threading.Thread(target=worker, args=(i, r)) for i, (_, r) in enumerate(systems)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    dt = time.perf_counter() - t0
+    moves = sum(r.st.bead_moves for _, r in systems)
+    iters = int(np.mean(attempts))
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": moves / dt,
+        "unit": "bead-moves/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"{cores} chains of the batched workload (one per thread), {iters} attempts each of the "
+                   f"reference schedule (COM + ReshapeLinear cap 8, adaptive), FP64 C restatement, {dt:.1f} s; "
+                   f"CPU: {cpu_model}"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of chains is design-only "
+                                   "(see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    from pimc import _capi as A
+    from pimc.batch import Batch, trapped_batch, trapped_plain
+
+    name, cus = A.device_info()
+    log(f"device {name}, {cus} CUs")
+
+    # ---- batched workload
+    spl = args.spl
+    wl = trapped_batch(n_chains=args.chains, seed=2, precision=A.FP32)
+    t = time.perf_counter()
+    batch = Batch(wl, sweeps_per_launch=spl, seed=20240601, tile_beads=args.tile_beads, block_threads=args.threads)
+    log(f"batched context: {len(wl.descs)} chains, {sum(d.N * d.M for d in wl.descs)} beads, "
+        f"setup {time.perf_counter() - t:.1f} s")
+    warm = max(spl, (args.warmup // spl) * spl)
+    steps = max(spl, (args.steps // spl) * spl)
+    batch.run(warm, measure=False)
+    log(f"warmup {warm} sweeps done")
+    t0 = time.perf_counter()
+    cnt = batch.run(steps, measure=True)
+    wall = time.perf_counter() - t0
+    log(f"timed {steps} sweeps: wall {wall:.3f} s, device {cnt.wall_ms:.1f} ms, kernels {cnt.kernel_ms:.1f} ms")
+    value = cnt.bead_moves / wall
+    ms_per_step = 1000.0 * wall / steps
+    launches = max(1, cnt.kernel_launches)
+    avg_launch_s = cnt.kernel_ms / 1000.0 / launches
+    bytes_per_launch = BYTES_PER_MOVE_FP32 * cnt.bead_moves / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic_trapped_batch.json")
+    if os.path.exists(tp):
+        try:
+            with open(tp) as f:
+                tj = json.load(f)
+            if tj.get("sweeps_per_launch") == spl and tj.get("chains") == args.chains:
+                traffic = tj.get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    echeck = energy_check(batch, wl)
+    acc = {"levy": cnt.levy_acc / max(1, cnt.levy_tries), "com": cnt.com_acc / max(1, cnt.com_tries)}
+    batch.close()
+
+    # ---- plain single-chain form
+    pw = trapped_plain(seed=1)
+    plain = Batch(pw, sweeps_per_launch=spl, seed=7)
+    plain.run(max(spl, 500 // spl * spl), measure=False)
+    psteps = max(spl, (args.plain_steps // spl) * spl)
+    t0 = time.perf_counter()
+    pc = plain.run(psteps, measure=True)
+    pwall = time.perf_counter() - t0
+    plain_value = pc.bead_moves / pwall
+    pe = plain.estimates(0)
+    from analytic import energy_boltzmann_primitive
+    pex = energy_boltzmann_primitive(10, 64, 1.0)
+    plain.close()
+    log(f"plain: {plain_value:.3e} bead-moves/s, E = {pe.E:.4f} +- {pe.E_err:.4f} (exact {pex:.4f})")
+
+    cpu = None if args.no_cpu else cpu_baseline(wl, args.cpu_seconds)
+    if cpu:
+        log(f"cpu baseline {cpu['value']:.3e} bead-moves/s on {cpu['cores']} cores")
+
+    out = {
+        "metric": "bead-moves/sec at 1 GPU, batched vs plain, E within 2σ of FP64 ref; % HBM peak",
+        "value": value,
+        "unit": "bead-moves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "trapped_batch: 4096 independent trapped 2D ideal-boson chains, N~U[4,64], "
+                        "M~U{32,64,128,256}, beta~U[0.5,4], omega=1, FP32 paths, FP64 estimators",
+            "chains": len(wl.descs),
+            "beads": int(sum(d.N * d.M for d in wl.descs)),
+            "staging_length": 8,
+            "sweeps_per_launch": spl,
+            "bead_moves_per_sweep": int(cnt.bead_moves // steps),
+            "parallelism": "single GPU, one workgroup per tile of particles",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+        "plain_gpu": {
+            "workload": "trapped_plain: one chain N=10, M=64, beta=1, omega=1, FP64",
+            "value": plain_value,
+            "unit": "bead-moves/s",
+            "E": pe.E, "E_err": pe.E_err, "E_exact_primitive": pex,
+        },
+        "batched_vs_plain": value / plain_value,
+        "batched_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "energy_check": echeck,
+        "acceptance": acc,
+        "kernel_avg_launch_ms": avg_launch_s * 1000.0,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

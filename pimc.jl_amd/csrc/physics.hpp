@@ -32,7 +32,8 @@ struct ChainDev {
     int cap, ncell, part_base, nseg;
 };
 
-// The chain constants converted to the arithmetic type.
+// The chain constants converted to the arithmetic type (built once per chain
+// on the host, so the kernels load them instead of dividing per item).
 template <typename T> struct Prm {
     T tau, lambda, L, twoL, inv2L;
     T halfw2, w2;        // omega^2 / 2, omega^2
@@ -40,7 +41,7 @@ template <typename T> struct Prm {
     T g, inv2s2, h2;     // pair strength, 1/(2 sigma^2), cut-off radius^2
     int pot, pair;
 
-    __device__ __forceinline__ static Prm make(const ChainDev &c)
+    __host__ __device__ __forceinline__ static Prm make(const ChainDev &c)
     {
         Prm p;
         p.tau = (T)c.tau;

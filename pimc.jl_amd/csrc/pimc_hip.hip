@@ -75,6 +75,7 @@ struct pimc_ctx {
     int nblk = 0, nblk_cap = 0;
     // device
     ChainDev *d_ch = nullptr;
+    void *d_prm = nullptr;
     TileDev *d_tiles = nullptr;
     int *d_pchain = nullptr, *d_pidx = nullptr;
     void *d_pos = nullptr;
@@ -94,6 +95,8 @@ struct pimc_ctx {
     std::vector<Launch> events;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     long long cart_per_chain = 0;
+    std::vector<double> hblocks; // host copy of the estimator blocks
+    int hblk_n = -1;             // blocks held in hblocks (-1: stale)
 };
 
 namespace {
@@ -199,6 +202,7 @@ SweepArgs sweep_args(pimc_ctx *c)
 {
     SweepArgs a{};
     a.chains = c->d_ch;
+    a.prm = c->d_prm;
     a.tiles = c->d_tiles;
     a.part_chain = c->d_pchain;
     a.part_idx = c->d_pidx;
@@ -233,6 +237,19 @@ void grow_blocks(pimc_ctx *c, int need)
     dfree(c->d_blocks);
     c->d_blocks = nb;
     c->nblk_cap = cap;
+}
+
+// Block b of one chain from the host copy, refreshed after each run.
+const double *host_block(pimc_ctx *c, int b, int chain)
+{
+    if (c->hblk_n != c->nblk) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->hblocks.resize((size_t)c->nblk * c->n_chains * kNQ);
+        if (c->nblk > 0)
+            HIPCHK(hipMemcpy(c->hblocks.data(), c->d_blocks, sizeof(double) * c->hblocks.size(), hipMemcpyDeviceToHost));
+        c->hblk_n = c->nblk;
+    }
+    return &c->hblocks[((size_t)b * c->n_chains + chain) * kNQ];
 }
 
 int check_chain(pimc_ctx *c, int chain)
@@ -499,7 +516,12 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 t.items = t.np * t.nseg;
             }
         }
-        for (const TileDev &t : c->tiles) lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.M, t.items, ts));
+        for (TileDev &t : c->tiles) {
+            t.nslot = 1;
+            for (int q = 1; q < t.np; ++q)
+                if (c->part_chain[t.q0 + q] != c->part_chain[t.q0 + q - 1]) ++t.nslot;
+            lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.M, t.items, ts, t.nslot, opt->radial_bins));
+        }
         c->lds_tile = lds_max;
         if (lds_max > 160 * 1024) throw HipError{hipErrorInvalidValue, "tile exceeds LDS; lower tile_beads"};
         // whole-chain kernel
@@ -529,6 +551,17 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         // ---- device allocations
         c->d_ch = dalloc<ChainDev>(n_chains);
         HIPCHK(hipMemcpy(c->d_ch, c->ch.data(), sizeof(ChainDev) * n_chains, hipMemcpyHostToDevice));
+        if (c->prec == PIMC_FP64) {
+            std::vector<Prm<double>> pr(n_chains);
+            for (int i = 0; i < n_chains; ++i) pr[i] = Prm<double>::make(c->ch[i]);
+            c->d_prm = dalloc<Prm<double>>(n_chains);
+            HIPCHK(hipMemcpy(c->d_prm, pr.data(), sizeof(Prm<double>) * n_chains, hipMemcpyHostToDevice));
+        } else {
+            std::vector<Prm<float>> pr(n_chains);
+            for (int i = 0; i < n_chains; ++i) pr[i] = Prm<float>::make(c->ch[i]);
+            c->d_prm = dalloc<Prm<float>>(n_chains);
+            HIPCHK(hipMemcpy(c->d_prm, pr.data(), sizeof(Prm<float>) * n_chains, hipMemcpyHostToDevice));
+        }
         c->d_tiles = dalloc<TileDev>(c->tiles.size());
         if (!c->tiles.empty())
             HIPCHK(hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice));
@@ -603,7 +636,7 @@ int pimc_destroy(pimc_ctx *c)
 {
     if (!c) return PIMC_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *ptrs[] = {c->d_ch, c->d_tiles, c->d_pchain, c->d_pidx, c->d_pos, c->d_part_acc, c->d_placc, c->d_pcacc,
+    void *ptrs[] = {c->d_ch, c->d_prm, c->d_tiles, c->d_pchain, c->d_pidx, c->d_pos, c->d_part_acc, c->d_placc, c->d_pcacc,
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
                     c->d_status, c->d_work, c->d_io};
@@ -711,6 +744,7 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
         HIPCHK(hipMemcpy(before.data(), c->d_cnt, sizeof(long long) * before.size(), hipMemcpyDeviceToHost));
         const int K = c->opt.sweeps_per_launch;
         const int nl = (n_sweeps + K - 1) / K;
+        c->hblk_n = -1;
         if (measure) grow_blocks(c, c->nblk + nl);
         while ((int)c->events.size() < nl) {
             Launch l;
@@ -792,8 +826,7 @@ int pimc_get_blocks(pimc_ctx *c, int32_t chain, double *E, double *Ev, double *E
         HIPCHK(hipStreamSynchronize(c->stream));
         std::vector<double> q((size_t)c->nblk * kNQ);
         for (int b = 0; b < c->nblk; ++b)
-            HIPCHK(hipMemcpy(&q[(size_t)b * kNQ], c->d_blocks + ((size_t)b * c->n_chains + chain) * kNQ,
-                             sizeof(double) * kNQ, hipMemcpyDeviceToHost));
+            std::memcpy(&q[(size_t)b * kNQ], host_block(c, b, chain), sizeof(double) * kNQ);
         if (n_blocks) *n_blocks = c->nblk;
         for (int b = 0; b < c->nblk && b < cap; ++b) {
             const double *x = &q[(size_t)b * kNQ];
@@ -819,8 +852,7 @@ int pimc_get_estimates(pimc_ctx *c, int32_t chain, pimc_estimates *out)
         const int nb = c->nblk;
         std::vector<double> q((size_t)nb * kNQ);
         for (int b = 0; b < nb; ++b)
-            HIPCHK(hipMemcpy(&q[(size_t)b * kNQ], c->d_blocks + ((size_t)b * c->n_chains + chain) * kNQ,
-                             sizeof(double) * kNQ, hipMemcpyDeviceToHost));
+            std::memcpy(&q[(size_t)b * kNQ], host_block(c, b, chain), sizeof(double) * kNQ);
         out->n_blocks = nb;
         if (nb == 0) return PIMC_OK;
         // totals and block means
@@ -894,9 +926,7 @@ int pimc_get_radial(pimc_ctx *c, int32_t chain, double *hist, int64_t *ndata)
         for (int i = 0; i < nb; ++i) hist[i] = (double)h[i];
         double nm = 0.0;
         for (int b = 0; b < c->nblk; ++b) {
-            double x = 0;
-            HIPCHK(hipMemcpy(&x, c->d_blocks + ((size_t)b * c->n_chains + chain) * kNQ + qN, sizeof(double), hipMemcpyDeviceToHost));
-            nm += x;
+            nm += host_block(c, b, chain)[qN];
         }
         if (ndata) *ndata = (int64_t)nm * c->ch[chain].M;
     } catch (const HipError &e) {
@@ -918,9 +948,7 @@ int pimc_get_cartesian(pimc_ctx *c, int32_t chain, double *hist, int64_t *ndata)
         for (long long i = 0; i < nb; ++i) hist[i] = (double)h[i];
         double nm = 0.0;
         for (int b = 0; b < c->nblk; ++b) {
-            double x = 0;
-            HIPCHK(hipMemcpy(&x, c->d_blocks + ((size_t)b * c->n_chains + chain) * kNQ + qN, sizeof(double), hipMemcpyDeviceToHost));
-            nm += x;
+            nm += host_block(c, b, chain)[qN];
         }
         if (ndata) *ndata = (int64_t)nm * c->ch[chain].M;
     } catch (const HipError &e) {
@@ -935,6 +963,7 @@ int pimc_reset_estimators(pimc_ctx *c)
     try {
         HIPCHK(hipStreamSynchronize(c->stream));
         c->nblk = 0;
+        c->hblk_n = -1;
         if (c->d_radial) HIPCHK(hipMemset(c->d_radial, 0, sizeof(unsigned long long) * c->n_chains * c->opt.radial_bins));
         if (c->d_cart) HIPCHK(hipMemset(c->d_cart, 0, sizeof(unsigned long long) * c->n_chains * c->cart_per_chain));
     } catch (const HipError &e) {

@@ -42,11 +42,12 @@ enum BlockQ {
 struct TileDev {
     long long pos_off; // element offset of the tile's first particle
     int q0, np, M, nseg;
-    int ell, items, pad0, pad1;
+    int ell, items, nslot, pad1; // nslot: distinct chains in the tile
 };
 
 struct SweepArgs {
     const ChainDev *chains;
+    const void *prm;       // Prm<T> per chain
     const TileDev *tiles;
     const int *part_chain; // global particle -> chain
     const int *part_idx;   // global particle -> index within its chain
@@ -74,15 +75,16 @@ struct SweepArgs {
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // LDS layout of k_sweep_tiles; the host sizes the launch with the same function.
-__host__ __device__ inline size_t tile_lds_bytes(int np, int dim, int M, int items, int tsize)
+__host__ __device__ inline size_t tile_lds_bytes(int np, int dim, int M, int items, int tsize, int nslot, int rbins)
 {
     size_t posb = (size_t)np * dim * M * tsize;
     size_t redb = (size_t)items * 3 * sizeof(double);
     size_t o = align16(posb > redb ? posb : redb);
-    o += 4 * align16((size_t)np * 4);     // off, flag, lacc, cacc
+    o += 5 * align16((size_t)np * 4);       // off, flag, lacc, cacc, chain slot
     o += align16((size_t)np * dim * tsize); // COM displacement
     o += align16((size_t)np * tsize);       // COM uniform
     o += align16((size_t)items * tsize);    // COM partial sums
+    o += align16((size_t)nslot * rbins * 4); // radial histogram per chain of the tile
     return o;
 }
 
@@ -146,9 +148,11 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const TileDev tl = a.tiles[blockIdx.x];
+    const Prm<T> *prm = reinterpret_cast<const Prm<T> *>(a.prm);
     const int tid = threadIdx.x, nt = blockDim.x;
     const int np = tl.np, M = tl.M, nseg = tl.nseg, ell = tl.ell, items = tl.items;
     const int nel = np * DIM * M;
+    const int rbins = a.radial_bins;
 
     T *sp = reinterpret_cast<T *>(smem);
     size_t posb = (size_t)nel * sizeof(T), redb = (size_t)items * 3 * sizeof(double);
@@ -161,17 +165,30 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
     o += align16((size_t)np * 4);
     int *s_cacc = reinterpret_cast<int *>(smem + o);
     o += align16((size_t)np * 4);
+    int *s_slot = reinterpret_cast<int *>(smem + o);
+    o += align16((size_t)np * 4);
     T *s_d = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)np * DIM * sizeof(T));
     T *s_u = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)np * sizeof(T));
     T *s_red = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)items * sizeof(T));
+    unsigned *s_hist = reinterpret_cast<unsigned *>(smem + o);
 
     T *gpos = reinterpret_cast<T *>(a.pos) + tl.pos_off;
     for (int i = tid; i < nel; i += nt) sp[i] = gpos[i];
     for (int q = tid; q < np; q += nt) {
         s_lacc[q] = 0;
         s_cacc[q] = 0;
+    }
+    for (int i = tid; i < tl.nslot * rbins; i += nt) s_hist[i] = 0u;
+    if (tid == 0) {
+        // chains are contiguous in the particle order: slot = chain boundaries before q
+        int slot = 0;
+        for (int q = 0; q < np; ++q) {
+            if (q > 0 && a.part_chain[tl.q0 + q] != a.part_chain[tl.q0 + q - 1]) ++slot;
+            s_slot[q] = slot;
+        }
     }
     double mL[kItemsPerThread], mV[kItemsPerThread], mR[kItemsPerThread];
 #pragma unroll
@@ -202,7 +219,7 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
             if (m < 2) continue;
             const int gq = tl.q0 + q;
             const int c = a.part_chain[gq], p = a.part_idx[gq];
-            const Prm<T> P = Prm<T>::make(a.chains[c]);
+            const Prm<T> P = prm[c];
             int start = s_off[q] + k * ell;
             if (start >= M) start -= M;
             if (levy_segment<T, DIM, MAXL>(sp + (size_t)q * DIM * M, M, start, m, P, a.seed, (uint32_t)(p * M), sw,
@@ -215,8 +232,7 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
         for (int it = tid; it < items; it += nt) {
             const int q = it / nseg, k = it - q * nseg;
             const int j0 = k * ell, j1 = min(M, j0 + ell);
-            const int c = a.part_chain[tl.q0 + q];
-            const Prm<T> P = Prm<T>::make(a.chains[c]);
+            const Prm<T> P = prm[a.part_chain[tl.q0 + q]];
             const T *xp = sp + (size_t)q * DIM * M;
             T d[DIM];
 #pragma unroll
@@ -237,8 +253,7 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
         for (int q = tid; q < np; q += nt) {
             T sum = T(0);
             for (int k = 0; k < nseg; ++k) sum += s_red[q * nseg + k];
-            const int c = a.part_chain[tl.q0 + q];
-            const T lw = -(T)a.chains[c].tau * sum;
+            const T lw = -prm[a.part_chain[tl.q0 + q]].tau * sum;
             const int acc = metropolis(ex(lw), s_u[q]) ? 1 : 0;
             s_flag[q] = acc;
             s_cacc[q] += acc;
@@ -248,8 +263,7 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
             const int q = it / nseg, k = it - q * nseg;
             if (!s_flag[q]) continue;
             const int j0 = k * ell, j1 = min(M, j0 + ell);
-            const int c = a.part_chain[tl.q0 + q];
-            const Prm<T> P = Prm<T>::make(a.chains[c]);
+            const Prm<T> P = prm[a.part_chain[tl.q0 + q]];
             T *xp = sp + (size_t)q * DIM * M;
             for (int j = j0; j < j1; ++j) {
 #pragma unroll
@@ -258,7 +272,8 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
         }
         __syncthreads();
 
-        // estimators (src/measurement.jl:45-55, 92-122), FP64 sums per chunk
+        // estimators (src/measurement.jl:45-55, 92-122): T sums over a chunk
+        // of `ell` beads, FP64 accumulation across chunks, sweeps and particles
         if (a.measure) {
 #pragma unroll
             for (int r = 0; r < kItemsPerThread; ++r) {
@@ -267,10 +282,10 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
                     const int q = it / nseg, k = it - q * nseg;
                     const int j0 = k * ell, j1 = min(M, j0 + ell);
                     const int c = a.part_chain[tl.q0 + q];
-                    const ChainDev &ch = a.chains[c];
-                    const Prm<T> P = Prm<T>::make(ch);
+                    const Prm<T> P = prm[c];
                     const T *xp = sp + (size_t)q * DIM * M;
-                    double l = 0.0, v = 0.0, g = 0.0;
+                    unsigned *hist = s_hist + s_slot[q] * rbins;
+                    T l = T(0), v = T(0), g = T(0);
                     for (int j = j0; j < j1; ++j) {
                         const int jn = (j + 1 == M) ? 0 : j + 1;
                         T r0[DIM];
@@ -281,24 +296,23 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
                             const T dx = mindist(r0[e], xp[e * M + jn], P);
                             dd += dx * dx;
                         }
-                        l += (double)dd;
-                        v += (double)potV<T, DIM>(r0, P);
-                        g += (double)rgradV<T, DIM>(r0, P);
-                        if (a.radial_bins > 0) {
+                        l += dd;
+                        v += potV<T, DIM>(r0, P);
+                        g += rgradV<T, DIM>(r0, P);
+                        if (rbins > 0) {
                             T rr = r0[0] * r0[0];
                             if (DIM > 1) rr += r0[1] * r0[1];
-                            const double qb = (double)sq(rr) * a.radial_winv;
-                            if (qb < (double)a.radial_bins)
-                                atomicAdd(&a.radial[(size_t)c * a.radial_bins + (int)qb], 1ull);
+                            const T qb = sq(rr) * (T)a.radial_winv;
+                            if (qb < (T)rbins) atomicAdd(&hist[(int)qb], 1u);
                         }
                         if (a.cart_bins > 0) {
                             // the reference's Density: floor((r + L)/bin) used as a 1-based index
-                            const double bw = (2.0 * ch.L) / a.cart_bins;
+                            const double bw = (2.0 * a.chains[c].L) / a.cart_bins;
                             long long idx = 0, stride = 1;
                             bool ok = true;
 #pragma unroll
                             for (int e = 0; e < DIM; ++e) {
-                                const long long ib = (long long)floor(((double)r0[e] + ch.L) / bw);
+                                const long long ib = (long long)floor(((double)r0[e] + a.chains[c].L) / bw);
                                 ok = ok && ib > 0 && ib < a.cart_bins + 1;
                                 idx += (ib - 1) * stride;
                                 stride *= a.cart_bins;
@@ -306,9 +320,9 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
                             if (ok) atomicAdd(&a.cart[(size_t)c * stride + idx], 1ull);
                         }
                     }
-                    mL[r] += l;
-                    mV[r] += v;
-                    mR[r] += g;
+                    mL[r] += (double)l;
+                    mV[r] += (double)v;
+                    mR[r] += (double)g;
                 }
             }
             ++nmeas;
@@ -317,6 +331,18 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
 
     __syncthreads();
     for (int i = tid; i < nel; i += nt) gpos[i] = sp[i];
+    // radial histogram of the launch -> global counts (one atomic per non-empty bin)
+    if (a.measure && rbins > 0) {
+        for (int i = tid; i < tl.nslot * rbins; i += nt) {
+            const unsigned cnt = s_hist[i];
+            if (cnt == 0u) continue;
+            const int slot = i / rbins, b = i - slot * rbins;
+            int q = 0;
+            while (q + 1 < np && s_slot[q] != slot) ++q;
+            const int c = a.part_chain[tl.q0 + q];
+            atomicAdd(&a.radial[(size_t)c * rbins + b], (unsigned long long)cnt);
+        }
+    }
     __syncthreads();
     if (a.measure) {
         double *red3 = reinterpret_cast<double *>(smem);
