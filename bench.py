@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-beads", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size kernel for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -139,6 +142,17 @@ def main():
 
     name, cus = A.device_info()
     log(f"device {name}, {cus} CUs")
+    if args.calib_copy_mib > 0:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        nbytes = ctypes.c_size_t(args.calib_copy_mib << 20)
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        ok = hip.hipMalloc(ctypes.byref(src), nbytes) == 0 and hip.hipMalloc(ctypes.byref(dst), nbytes) == 0
+        ok = ok and hip.hipMemset(src, 1, nbytes) == 0 and hip.hipDeviceSynchronize() == 0
+        ok = ok and hip.hipMemcpy(dst, src, nbytes, 3) == 0 and hip.hipDeviceSynchronize() == 0  # device to device
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        log(f"calibration copy of {args.calib_copy_mib} MiB {'done' if ok else 'FAILED'}")
 
     # ---- batched workload
     spl = args.spl

@@ -899,9 +899,12 @@ void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, d
             }
             for (int i = 1; i < m; ++i) {
                 int b = (start + i) % M;
-                keyed(seed, (uint32_t)(p * M + b), sweep, chain, PUR_LEVY, x);
+                /* one Philox block serves the interior beads (i, i+1), i odd,
+                 * keyed by the slice of bead i: words 0,1 for bead i, 2,3 for i+1 */
+                if (i & 1) keyed(seed, (uint32_t)(p * M + b), sweep, chain, PUR_LEVY, x);
                 double n0, n1;
-                o_box_muller(x[0], x[1], &n0, &n1);
+                if (i & 1) o_box_muller(x[0], x[1], &n0, &n1);
+                else o_box_muller(x[2], x[3], &n0, &n1);
                 nrm[(i - 1) * dim + 0] = n0;
                 if (dim > 1) nrm[(i - 1) * dim + 1] = n1;
             }

@@ -13,7 +13,8 @@
 namespace pimc {
 
 enum Purpose : uint32_t {
-    kLevy = 1,       // normals of one bridge bead, item = particle * M + slice
+    kLevy = 1,       // normals of the interior beads (i, i+1), i odd, of a staging segment,
+                     // item = particle * M + slice of bead i: words x, y for bead i, z, w for i + 1
     kLevyAccept = 2, // Metropolis uniform of one staging segment, item = particle * M + start slice
     kCom = 3,        // displacement (words 0..dim-1) and Metropolis uniform (word 2), item = particle
     kOffset = 4,     // staging offset of one particle in one sweep, item = particle
@@ -73,7 +74,7 @@ __device__ __forceinline__ void box_muller(uint32_t x0, uint32_t x1, float &n0, 
 {
     const float u0 = u01f(x0), u1 = u01f(x1);
     // v_log_f32 is log2; v_sin/v_cos take revolutions, so cos(2 pi u1) needs no scaling.
-    const float rad = __builtin_sqrtf(-2.0f * 0.69314718055994530942f * __builtin_amdgcn_logf(u0));
+    const float rad = __builtin_amdgcn_sqrtf(-2.0f * 0.69314718055994530942f * __builtin_amdgcn_logf(u0));
     n0 = rad * __builtin_amdgcn_cosf(u1);
     n1 = rad * __builtin_amdgcn_sinf(u1);
 }

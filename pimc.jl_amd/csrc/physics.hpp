@@ -71,7 +71,7 @@ __device__ __forceinline__ float spi(float x) { return sinpif(x); }
 __device__ __forceinline__ double ex(double x) { return exp(x); }
 __device__ __forceinline__ float ex(float x) { return __expf(x); }
 __device__ __forceinline__ double sq(double x) { return sqrt(x); }
-__device__ __forceinline__ float sq(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); } // v_sqrt_f32, 1 ulp
 
 // src/propagator.jl:30-32, teleport(x) = (x + L) - floor(x / 2L + 1/2) 2L - L
 template <typename T> __device__ __forceinline__ T teleport(T x, const Prm<T> &p)
@@ -87,15 +87,15 @@ template <typename T> __device__ __forceinline__ T mindist(T x1, T x2, const Prm
     return a < dx ? a : dx;
 }
 
-// External potential V(r) (the `potential` closure of System, src/system.jl:129-131).
-template <typename T, int DIM> __device__ __forceinline__ T potV(const T *r, const Prm<T> &p)
+// External potential V(r) (the `potential` closure of System, src/system.jl:129-131),
+// kind fixed at compile time (the tile kernel is instantiated per kind).
+template <typename T, int DIM, int POT> __device__ __forceinline__ T potVk(const T *r, const Prm<T> &p)
 {
-    if (p.pot == kPotHarmonic) {
+    if constexpr (POT == kPotHarmonic) {
         T rr = r[0] * r[0];
         if (DIM > 1) rr += r[1] * r[1];
         return p.halfw2 * rr;
-    }
-    if (p.pot == kPotLattice) {
+    } else if constexpr (POT == kPotLattice) {
         T c0 = cpi(r[0] * p.inva);
         T v = c0 * c0;
         if (DIM > 1) {
@@ -103,25 +103,42 @@ template <typename T, int DIM> __device__ __forceinline__ T potV(const T *r, con
             v += c1 * c1;
         }
         return p.V0 * v;
+    } else {
+        return T(0);
     }
-    return T(0);
 }
 
 // r . grad V (the virial term of src/measurement.jl:107)
-template <typename T, int DIM> __device__ __forceinline__ T rgradV(const T *r, const Prm<T> &p)
+template <typename T, int DIM, int POT> __device__ __forceinline__ T rgradVk(const T *r, const Prm<T> &p)
 {
-    if (p.pot == kPotHarmonic) {
+    if constexpr (POT == kPotHarmonic) {
         T rr = r[0] * r[0];
         if (DIM > 1) rr += r[1] * r[1];
         return p.w2 * rr;
-    }
-    if (p.pot == kPotLattice) {
+    } else if constexpr (POT == kPotLattice) {
         // d/dx V0 cos^2(pi x / a) = -V0 (pi/a) sin(2 pi x / a)
         const T pia = T(3.14159265358979323846) * p.inva;
         T s = r[0] * spi(T(2) * r[0] * p.inva);
         if (DIM > 1) s += r[1] * spi(T(2) * r[1] * p.inva);
         return -p.V0 * pia * s;
+    } else {
+        return T(0);
     }
+}
+
+// Runtime-dispatched forms for kernels whose workgroup owns one chain (the
+// kind is then uniform and the branch is scalar).
+template <typename T, int DIM> __device__ __forceinline__ T potV(const T *r, const Prm<T> &p)
+{
+    if (p.pot == kPotHarmonic) return potVk<T, DIM, kPotHarmonic>(r, p);
+    if (p.pot == kPotLattice) return potVk<T, DIM, kPotLattice>(r, p);
+    return T(0);
+}
+
+template <typename T, int DIM> __device__ __forceinline__ T rgradV(const T *r, const Prm<T> &p)
+{
+    if (p.pot == kPotHarmonic) return rgradVk<T, DIM, kPotHarmonic>(r, p);
+    if (p.pot == kPotLattice) return rgradVk<T, DIM, kPotLattice>(r, p);
     return T(0);
 }
 
@@ -174,6 +191,11 @@ template <typename T> __device__ __forceinline__ T shift_end(T e0, T em, const P
 // i of a segment of m links, alpha = (m - i) / (m - i + 1), from the
 // (untelported) previous bead `prev` towards the end bead `em`.
 template <typename T> __device__ __forceinline__ T bridge_alpha(int m, int i) { return T(m - i) / T(m - i + 1); }
+// FP32: the hardware reciprocal (1 ulp) instead of the correctly rounded divide.
+template <> __device__ __forceinline__ float bridge_alpha<float>(int m, int i)
+{
+    return float(m - i) * __builtin_amdgcn_rcpf(float(m - i + 1));
+}
 
 template <typename T> __device__ __forceinline__ T bridge_sd(T alpha, const Prm<T> &p)
 {

@@ -47,6 +47,11 @@ struct Launch {
     hipEvent_t a, b;
 };
 
+// A run of consecutive tiles whose chains share one external potential (one launch).
+struct TileGroup {
+    int pot, first, count;
+};
+
 } // namespace
 
 struct pimc_ctx {
@@ -64,6 +69,7 @@ struct pimc_ctx {
     std::vector<int> whole_cells_ids;  // whole chains with cell lists
     std::vector<int> whole_brute_ids;  // whole chains with brute-force neighbour sums
     std::vector<TileDev> tiles;
+    std::vector<TileGroup> tile_groups;
     std::vector<int> part_chain, part_idx;
     long long n_pos = 0, n_cell_cnt = 0, n_bins = 0;
     int n_part = 0;
@@ -116,24 +122,33 @@ size_t tsize(const pimc_ctx *c) { return c->prec == PIMC_FP64 ? sizeof(double) :
 
 int next_pow2_maxl(int ell)
 {
-    int m = 4;
+    int m = 8;
     while (m < ell) m <<= 1;
     return m;
 }
 
 // ---- kernel dispatch over (T, DIM, MAXL) ----
+template <typename T, int DIM, int MAXL, int POT> void launch_tiles_p(pimc_ctx *c, SweepArgs a, const TileGroup &g)
+{
+    auto k = k_sweep_tiles<T, DIM, MAXL, POT>;
+    if (c->lds_tile > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_tile));
+    a.tile0 = g.first;
+    hipLaunchKernelGGL(k, dim3((unsigned)g.count), dim3(c->threads_tile), c->lds_tile, c->stream, a);
+    HIPCHK(hipGetLastError());
+}
+
 template <typename T, int DIM, int MAXL> void launch_tiles_t(pimc_ctx *c, const SweepArgs &a)
 {
-    auto k = k_sweep_tiles<T, DIM, MAXL>;
-    if (c->lds_tile > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_tile));
-    hipLaunchKernelGGL(k, dim3((unsigned)c->tiles.size()), dim3(c->threads_tile), c->lds_tile, c->stream, a);
-    HIPCHK(hipGetLastError());
+    for (const TileGroup &g : c->tile_groups) {
+        if (g.pot == PIMC_POT_HARMONIC) launch_tiles_p<T, DIM, MAXL, kPotHarmonic>(c, a, g);
+        else if (g.pot == PIMC_POT_LATTICE) launch_tiles_p<T, DIM, MAXL, kPotLattice>(c, a, g);
+        else launch_tiles_p<T, DIM, MAXL, kPotNone>(c, a, g);
+    }
 }
 
 template <typename T, int DIM> void launch_tiles_d(pimc_ctx *c, const SweepArgs &a)
 {
     switch (c->maxl) {
-    case 4: launch_tiles_t<T, DIM, 4>(c, a); break;
     case 8: launch_tiles_t<T, DIM, 8>(c, a); break;
     case 16: launch_tiles_t<T, DIM, 16>(c, a); break;
     default: launch_tiles_t<T, DIM, 32>(c, a); break;
@@ -437,6 +452,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         for (int i = 0; i < n_chains; ++i) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
             if (c->is_whole[x] != c->is_whole[y]) return c->is_whole[x] < c->is_whole[y];
+            if (c->ch[x].pot != c->ch[y].pot) return c->ch[x].pot < c->ch[y].pot;
             if (c->ch[x].M != c->ch[y].M) return c->ch[x].M > c->ch[y].M;
             return c->ch[x].ell < c->ch[y].ell;
         });
@@ -499,7 +515,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 bool fresh = c->tiles.empty();
                 if (!fresh) {
                     TileDev &t = c->tiles.back();
-                    fresh = t.M != h.M || t.ell != h.ell || (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
+                    fresh = t.M != h.M || t.ell != h.ell || c->tile_groups.back().pot != h.pot ||
+                            (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
                 }
                 if (fresh) {
                     TileDev t{};
@@ -509,6 +526,9 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                     t.M = h.M;
                     t.ell = h.ell;
                     t.nseg = h.nseg;
+                    if (c->tile_groups.empty() || c->tile_groups.back().pot != h.pot)
+                        c->tile_groups.push_back(TileGroup{h.pot, (int)c->tiles.size(), 0});
+                    c->tile_groups.back().count += 1;
                     c->tiles.push_back(t);
                 }
                 TileDev &t = c->tiles.back();
@@ -520,7 +540,9 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             t.nslot = 1;
             for (int q = 1; q < t.np; ++q)
                 if (c->part_chain[t.q0 + q] != c->part_chain[t.q0 + q - 1]) ++t.nslot;
-            lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.M, t.items, ts, t.nslot, opt->radial_bins));
+            t.sh = 0;
+            while ((2 << t.sh) <= t.ell) ++t.sh; // floor(log2(ell))
+            lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.M, t.sh, t.items, ts, t.nslot, opt->radial_bins));
         }
         c->lds_tile = lds_max;
         if (lds_max > 160 * 1024) throw HipError{hipErrorInvalidValue, "tile exceeds LDS; lower tile_beads"};

@@ -42,8 +42,11 @@ enum BlockQ {
 struct TileDev {
     long long pos_off; // element offset of the tile's first particle
     int q0, np, M, nseg;
-    int ell, items, nslot, pad1; // nslot: distinct chains in the tile
+    int ell, items, nslot, sh; // nslot: distinct chains in the tile; sh: LDS skew shift
 };
+
+// Padded LDS row length of M slices skewed by j >> sh.
+__host__ __device__ inline int tile_row_len(int M, int sh) { return M + (M >> sh) + 1; }
 
 struct SweepArgs {
     const ChainDev *chains;
@@ -63,6 +66,7 @@ struct SweepArgs {
     uint64_t seed;
     unsigned sweep0;
     int nsweeps, measure;
+    int tile0;             // first tile of this launch (tiles are grouped by potential kind)
     // whole-chain kernel
     const int *whole;
     int *cell_cnt;
@@ -75,9 +79,10 @@ struct SweepArgs {
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // LDS layout of k_sweep_tiles; the host sizes the launch with the same function.
-__host__ __device__ inline size_t tile_lds_bytes(int np, int dim, int M, int items, int tsize, int nslot, int rbins)
+__host__ __device__ inline size_t tile_lds_bytes(int np, int dim, int M, int sh, int items, int tsize, int nslot,
+                                                 int rbins)
 {
-    size_t posb = (size_t)np * dim * M * tsize;
+    size_t posb = (size_t)np * dim * tile_row_len(M, sh) * tsize;
     size_t redb = (size_t)items * 3 * sizeof(double);
     size_t o = align16(posb > redb ? posb : redb);
     o += 5 * align16((size_t)np * 4);       // off, flag, lacc, cacc, chain slot
@@ -88,30 +93,46 @@ __host__ __device__ inline size_t tile_lds_bytes(int np, int dim, int M, int ite
     return o;
 }
 
-// One staging segment of particle base `xp` (LDS or HBM), start slice
-// `start`, m links.  Returns true if accepted (beads written back).
-template <typename T, int DIM, int MAXL>
-__device__ __forceinline__ bool levy_segment(T *xp, int M, int start, int m, const Prm<T> &P, uint64_t seed,
-                                             uint32_t item0, uint32_t sweep, uint32_t chain)
+// Slice j of coordinate row d of one particle in LDS.  Rows are padded to
+// Mp = M + (M >> sh) + 1 and slice j sits at j + (j >> sh): the threads of a
+// wave walk the segments of a ring, which start `ell` slices apart, so with
+// sh = floor(log2(ell)) neighbouring lanes hit addresses ell + 1 apart (an
+// odd stride for even ell) instead of ell apart, which spreads them over the
+// LDS banks.
+struct LdsRow {
+    int Mp, sh;
+    __device__ __forceinline__ int at(int d, int j) const { return d * Mp + j + (j >> sh); }
+};
+
+// One staging segment of a particle held in LDS, start slice `start`, m
+// links (levy!, src/updates/helper.jl:118-139, then the Metropolis test of
+// src/updates/reshape.jl:66-80 on the potential link actions).  Returns true
+// if accepted (interior beads written back).
+template <typename T, int DIM, int MAXL, int POT>
+__device__ __forceinline__ bool levy_segment(T *xp, const LdsRow &R, int M, int start, int m, const Prm<T> &P,
+                                             uint64_t seed, uint32_t item0, uint32_t sweep, uint32_t chain)
 {
     int jm = start + m;
     if (jm >= M) jm -= M;
     T cur[DIM], em[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        cur[d] = xp[d * M + start];
-        em[d] = shift_end(cur[d], xp[d * M + jm], P);
+        cur[d] = xp[R.at(d, start)];
+        em[d] = shift_end(cur[d], xp[R.at(d, jm)], P);
     }
     T nb[MAXL - 1][DIM];
     T dv = T(0);
+    U4 xr{0u, 0u, 0u, 0u};
 #pragma unroll
     for (int i = 1; i < MAXL; ++i) {
         if (i < m) {
             int b = start + i;
             if (b >= M) b -= M;
-            const U4 x = draw(seed, item0 + (uint32_t)b, sweep, chain, kLevy);
+            // one Philox block per interior bead pair (i, i+1), i odd
+            if (i & 1) xr = draw(seed, item0 + (uint32_t)b, sweep, chain, kLevy);
             T n0, n1;
-            box_muller(x.x, x.y, n0, n1);
+            if (i & 1) box_muller(xr.x, xr.y, n0, n1);
+            else box_muller(xr.z, xr.w, n0, n1);
             const T alpha = bridge_alpha<T>(m, i);
             const T sd = bridge_sd(alpha, P);
             cur[0] = bridge_step(cur[0], em[0], alpha, n0, sd);
@@ -122,8 +143,8 @@ __device__ __forceinline__ bool levy_segment(T *xp, int M, int start, int m, con
             }
             T old[DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) old[d] = xp[d * M + b];
-            dv += potV<T, DIM>(nb[i - 1], P) - potV<T, DIM>(old, P);
+            for (int d = 0; d < DIM; ++d) old[d] = xp[R.at(d, b)];
+            dv += potVk<T, DIM, POT>(nb[i - 1], P) - potVk<T, DIM, POT>(old, P);
         }
     }
     const T lw = -P.tau * dv;
@@ -136,26 +157,38 @@ __device__ __forceinline__ bool levy_segment(T *xp, int M, int start, int m, con
                 int b = start + i;
                 if (b >= M) b -= M;
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) xp[d * M + b] = nb[i - 1][d];
+                for (int d = 0; d < DIM; ++d) xp[R.at(d, b)] = nb[i - 1][d];
             }
         }
     }
     return acc;
 }
 
-template <typename T, int DIM, int MAXL>
+template <typename T, int DIM, int MAXL, int POT>
 __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const TileDev tl = a.tiles[blockIdx.x];
+    const TileDev tl = a.tiles[a.tile0 + blockIdx.x];
     const Prm<T> *prm = reinterpret_cast<const Prm<T> *>(a.prm);
     const int tid = threadIdx.x, nt = blockDim.x;
     const int np = tl.np, M = tl.M, nseg = tl.nseg, ell = tl.ell, items = tl.items;
-    const int nel = np * DIM * M;
+    const LdsRow R{tile_row_len(M, tl.sh), tl.sh};
+    const int pstride = DIM * R.Mp; // LDS elements per particle
     const int rbins = a.radial_bins;
+    // item it = (particle q, segment k) with k fastest; thread tid visits it = tid + r nt,
+    // so (q, k) advance by (nt / nseg, nt % nseg) per step (no divide in the loops)
+    const int dq = nt / nseg, dk = nt - (nt / nseg) * nseg;
+    const int q_first = tid / nseg, k_first = tid - (tid / nseg) * nseg;
+    int slot_q[kItemsPerThread], slot_k[kItemsPerThread];
+#pragma unroll
+    for (int r = 0; r < kItemsPerThread; ++r) {
+        const int it = tid + r * nt;
+        slot_q[r] = it / nseg;
+        slot_k[r] = it - slot_q[r] * nseg;
+    }
 
     T *sp = reinterpret_cast<T *>(smem);
-    size_t posb = (size_t)nel * sizeof(T), redb = (size_t)items * 3 * sizeof(double);
+    size_t posb = (size_t)np * pstride * sizeof(T), redb = (size_t)items * 3 * sizeof(double);
     size_t o = align16(posb > redb ? posb : redb);
     int *s_off = reinterpret_cast<int *>(smem + o);
     o += align16((size_t)np * 4);
@@ -175,8 +208,13 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
     o += align16((size_t)items * sizeof(T));
     unsigned *s_hist = reinterpret_cast<unsigned *>(smem + o);
 
+    // HBM rows (particle, coordinate) of M slices -> padded, skewed LDS rows
     T *gpos = reinterpret_cast<T *>(a.pos) + tl.pos_off;
-    for (int i = tid; i < nel; i += nt) sp[i] = gpos[i];
+    const int nrow = np * DIM;
+    for (int i = tid; i < nrow * M; i += nt) {
+        const int row = i / M, j = i - row * M;
+        sp[row * R.Mp + j + (j >> R.sh)] = gpos[i];
+    }
     for (int q = tid; q < np; q += nt) {
         s_lacc[q] = 0;
         s_cacc[q] = 0;
@@ -213,8 +251,8 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
         __syncthreads();
 
         // staging segments
-        for (int it = tid; it < items; it += nt) {
-            const int q = it / nseg, k = it - q * nseg;
+        for (int it = tid, q = q_first, k = k_first; it < items;
+             it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk) {
             const int m = min(ell, M - k * ell);
             if (m < 2) continue;
             const int gq = tl.q0 + q;
@@ -222,18 +260,18 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
             const Prm<T> P = prm[c];
             int start = s_off[q] + k * ell;
             if (start >= M) start -= M;
-            if (levy_segment<T, DIM, MAXL>(sp + (size_t)q * DIM * M, M, start, m, P, a.seed, (uint32_t)(p * M), sw,
+            if (levy_segment<T, DIM, MAXL, POT>(sp + (size_t)q * pstride, R, M, start, m, P, a.seed, (uint32_t)(p * M), sw,
                                            (uint32_t)c))
                 atomicAdd(&s_lacc[q], 1);
         }
         __syncthreads();
 
         // centre of mass: partial Delta V per chunk of `ell` beads
-        for (int it = tid; it < items; it += nt) {
-            const int q = it / nseg, k = it - q * nseg;
+        for (int it = tid, q = q_first, k = k_first; it < items;
+             it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk) {
             const int j0 = k * ell, j1 = min(M, j0 + ell);
             const Prm<T> P = prm[a.part_chain[tl.q0 + q]];
-            const T *xp = sp + (size_t)q * DIM * M;
+            const T *xp = sp + (size_t)q * pstride;
             T d[DIM];
 #pragma unroll
             for (int e = 0; e < DIM; ++e) d[e] = s_d[q * DIM + e];
@@ -242,10 +280,10 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
                 T ro[DIM], rn[DIM];
 #pragma unroll
                 for (int e = 0; e < DIM; ++e) {
-                    ro[e] = xp[e * M + j];
+                    ro[e] = xp[R.at(e, j)];
                     rn[e] = teleport(ro[e] + d[e], P);
                 }
-                dv += potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P);
+                dv += potVk<T, DIM, POT>(rn, P) - potVk<T, DIM, POT>(ro, P);
             }
             s_red[it] = dv;
         }
@@ -259,15 +297,16 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
             s_cacc[q] += acc;
         }
         __syncthreads();
-        for (int it = tid; it < items; it += nt) {
-            const int q = it / nseg, k = it - q * nseg;
+
+        for (int it = tid, q = q_first, k = k_first; it < items;
+             it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk) {
             if (!s_flag[q]) continue;
             const int j0 = k * ell, j1 = min(M, j0 + ell);
             const Prm<T> P = prm[a.part_chain[tl.q0 + q]];
-            T *xp = sp + (size_t)q * DIM * M;
+            T *xp = sp + (size_t)q * pstride;
             for (int j = j0; j < j1; ++j) {
 #pragma unroll
-                for (int e = 0; e < DIM; ++e) xp[e * M + j] = teleport(xp[e * M + j] + s_d[q * DIM + e], P);
+                for (int e = 0; e < DIM; ++e) xp[R.at(e, j)] = teleport(xp[R.at(e, j)] + s_d[q * DIM + e], P);
             }
         }
         __syncthreads();
@@ -279,26 +318,29 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
             for (int r = 0; r < kItemsPerThread; ++r) {
                 const int it = tid + r * nt;
                 if (it < items) {
-                    const int q = it / nseg, k = it - q * nseg;
+                    const int q = slot_q[r], k = slot_k[r];
                     const int j0 = k * ell, j1 = min(M, j0 + ell);
                     const int c = a.part_chain[tl.q0 + q];
                     const Prm<T> P = prm[c];
-                    const T *xp = sp + (size_t)q * DIM * M;
+                    const T *xp = sp + (size_t)q * pstride;
                     unsigned *hist = s_hist + s_slot[q] * rbins;
                     T l = T(0), v = T(0), g = T(0);
+                    T r0[DIM];
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) r0[e] = xp[R.at(e, j0)];
                     for (int j = j0; j < j1; ++j) {
                         const int jn = (j + 1 == M) ? 0 : j + 1;
-                        T r0[DIM];
+                        T r1[DIM];
                         T dd = T(0);
 #pragma unroll
                         for (int e = 0; e < DIM; ++e) {
-                            r0[e] = xp[e * M + j];
-                            const T dx = mindist(r0[e], xp[e * M + jn], P);
+                            r1[e] = xp[R.at(e, jn)];
+                            const T dx = mindist(r0[e], r1[e], P);
                             dd += dx * dx;
                         }
                         l += dd;
-                        v += potV<T, DIM>(r0, P);
-                        g += rgradV<T, DIM>(r0, P);
+                        v += potVk<T, DIM, POT>(r0, P);
+                        g += rgradVk<T, DIM, POT>(r0, P);
                         if (rbins > 0) {
                             T rr = r0[0] * r0[0];
                             if (DIM > 1) rr += r0[1] * r0[1];
@@ -319,6 +361,8 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
                             }
                             if (ok) atomicAdd(&a.cart[(size_t)c * stride + idx], 1ull);
                         }
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) r0[e] = r1[e];
                     }
                     mL[r] += (double)l;
                     mV[r] += (double)v;
@@ -328,9 +372,12 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
             ++nmeas;
         }
     }
-
     __syncthreads();
-    for (int i = tid; i < nel; i += nt) gpos[i] = sp[i];
+
+    for (int i = tid; i < nrow * M; i += nt) {
+        const int row = i / M, j = i - row * M;
+        gpos[i] = sp[row * R.Mp + j + (j >> R.sh)];
+    }
     // radial histogram of the launch -> global counts (one atomic per non-empty bin)
     if (a.measure && rbins > 0) {
         for (int i = tid; i < tl.nslot * rbins; i += nt) {
@@ -375,7 +422,6 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
         a.part_cacc[tl.q0 + q] = s_cacc[q];
     }
 }
-
 // Deterministic block sum (fixed tree for a given blockDim).  Every thread
 // gets the result.  `buf` holds blockDim doubles.
 __device__ inline double block_sum(double v, double *buf)
@@ -621,12 +667,14 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
 #pragma unroll
                 for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], path.at(p, e, jm), P);
                 T dv = T(0);
+                U4 xr{0u, 0u, 0u, 0u};
                 for (int i = 1; i < m; ++i) {
                     int b = start + i;
                     if (b >= M) b -= M;
-                    const U4 x = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c, kLevy);
+                    if (i & 1) xr = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c, kLevy);
                     T n0, n1;
-                    box_muller(x.x, x.y, n0, n1);
+                    if (i & 1) box_muller(xr.x, xr.y, n0, n1);
+                    else box_muller(xr.z, xr.w, n0, n1);
                     const T alpha = bridge_alpha<T>(m, i);
                     const T sd = bridge_sd(alpha, P);
                     T rn[DIM], ro[DIM];

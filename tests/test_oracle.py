@@ -1,0 +1,293 @@
+"""Pins of the FP64 oracle (CPU).
+
+The reference is Julia and cannot run here, and its tests hold no vectors for
+the sampling path (its only numeric golden value is the N=1 oscillator energy
+of examples/energy_2d_harmonically_trapped_bose_gas.jl:28).  Each oracle
+function is therefore pinned by something the mathematics fixes: published
+known-answer vectors, closed forms, brute force on small inputs, invariants,
+chosen so that a dropped term, a wrong sign or index fails.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import analytic as AN
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def blockerr(x, nb=40):
+    x = np.asarray(x, float)
+    b = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
+    return b.mean(), b.std(ddof=1) / math.sqrt(nb)
+
+
+def harmonic_sys(N, M, beta, L=100.0, seed=0, omega=1.0, scale=1.0):
+    r = np.random.default_rng(seed).normal(0, scale, size=(N, 2, M))
+    return O.System(r, 0.5, L, beta, O.make_pot(O.POT_HARMONIC, omega))
+
+
+# ---------------------------------------------------------------- RNG
+def test_philox_known_answers():
+    kat = json.load(open(os.path.join(GOLD, "philox_kat.json")))["vectors"]
+    for v in kat:
+        out = O.philox([int(x, 16) for x in v["ctr"]], [int(x, 16) for x in v["key"]])
+        assert [f"{x:08x}" for x in out] == v["out"]
+
+
+def test_box_muller_closed_form_and_moments():
+    # u0 = 1/2 exactly when x0 = 2^31 - 1/2 is not representable; take x0 = 2^31:
+    # u0 = (2^31 + 0.5) 2^-32, u1 = 0.25 + 0.5/2^32 -> n0 = rad cos(pi/2 + eps)
+    n0, n1 = O.box_muller(2 ** 31, 2 ** 30)
+    u0 = (2 ** 31 + 0.5) / 2 ** 32
+    u1 = (2 ** 30 + 0.5) / 2 ** 32
+    rad = math.sqrt(-2 * math.log(u0))
+    assert n0 == pytest.approx(rad * math.cos(2 * math.pi * u1), abs=1e-15)
+    assert n1 == pytest.approx(rad * math.sin(2 * math.pi * u1), abs=1e-15)
+    rng = np.random.default_rng(1)
+    xs = rng.integers(0, 2 ** 32, size=(20000, 2), dtype=np.uint64)
+    z = np.array([O.box_muller(int(a), int(b)) for a, b in xs]).ravel()
+    assert abs(z.mean()) < 4 / math.sqrt(z.size)
+    assert abs(z.var() - 1.0) < 4 * math.sqrt(2 / z.size)
+
+
+# ---------------------------------------------------------------- primitives
+def test_distance_and_teleport():
+    L = 3.0
+    assert O.distance(0.9 * L, -0.9 * L, L) == pytest.approx(0.2 * L)
+    assert O.distance(1.0, 1.0, L) == 0.0
+    rng = np.random.default_rng(2)
+    for a, b in rng.uniform(-L, L, size=(200, 2)):
+        d = O.distance(a, b, L)
+        assert d == pytest.approx(O.distance(b, a, L))
+        assert 0.0 <= d <= L + 1e-12
+        # equals the minimum over periodic images
+        assert d == pytest.approx(min(abs(a - b + k * 2 * L) for k in (-1, 0, 1)), abs=1e-12)
+    assert O.teleport(L, L) == -L
+    assert O.teleport(-L, L) == -L
+    for x in rng.uniform(-L, L, size=100):
+        for k in (-2, -1, 1, 3):
+            assert O.teleport(x + 2 * L * k, L) == pytest.approx(x, abs=1e-12)
+        assert O.teleport(x, L) == pytest.approx(x, abs=1e-15)
+
+
+def test_levy_one_interior_bead_by_hand():
+    tau, lam, L = 0.1, 0.5, 50.0
+    r = np.array([[1.0, -2.0], [0.0, 0.0], [3.0, 4.0]])
+    xi = np.array([0.7, -1.3])
+    out = O.levy(r, tau, L, lam, xi)
+    # m = 1 interior bead: alpha = 1/2, sd = sqrt(2 lambda tau / 2)
+    expect = 0.5 * r[0] + 0.5 * r[2] + xi * math.sqrt(lam * tau)
+    assert np.allclose(out[1], expect, rtol=0, atol=1e-15)
+    assert np.allclose(out[[0, 2]], r[[0, 2]])
+
+
+def test_levy_end_shift_across_boundary():
+    tau, lam, L = 0.1, 0.5, 2.0
+    r = np.array([[1.8, 0.0], [0.0, 0.0], [-1.8, 0.0]])
+    out = O.levy(r, tau, L, lam, np.zeros(2))
+    # the far end moves by +2L to 2.2, the midpoint 2.0 wraps to -2.0
+    assert out[1, 0] == pytest.approx(-2.0)
+    assert out[2, 0] == pytest.approx(-1.8)
+    # sign(0) = 0: an end more than L away from a start at 0 is not shifted
+    r = np.array([[0.0, 0.0], [0.0, 0.0], [2.5, 0.0]])
+    out = O.levy(r, tau, L, lam, np.zeros(2))
+    assert out[1, 0] == pytest.approx(O.teleport(1.25, L))
+
+
+def test_levy_bridge_moments():
+    """Bead k of a K-interval bridge: mean r0 + k/K (rK - r0), variance 2 lambda tau k (K-k)/K."""
+    tau, lam, L, K = 0.05, 0.5, 100.0, 6
+    r = np.zeros((K + 1, 1))
+    r[0, 0], r[K, 0] = -1.0, 2.0
+    rng = np.random.default_rng(3)
+    n = 20000
+    out = np.array([O.levy(r, tau, L, lam, rng.standard_normal(K - 1))[:, 0] for _ in range(n)])
+    for k in range(1, K):
+        mean = -1.0 + k / K * 3.0
+        var = 2 * lam * tau * k * (K - k) / K
+        assert abs(out[:, k].mean() - mean) < 4 * math.sqrt(var / n)
+        assert abs(out[:, k].var() - var) < 4 * var * math.sqrt(2 / n)
+
+
+# ---------------------------------------------------------------- moves vs brute force
+def test_reshape_delta_equals_full_action_difference():
+    s = harmonic_sys(5, 24, 2.0, seed=4, scale=1.3)
+    rng = np.random.default_rng(4)
+    for _ in range(40):
+        n, j0, m = int(rng.integers(5)), int(rng.integers(24)), int(rng.integers(2, 12))
+        before = s.total_lnV()
+        acc, w = s.reshape_apply(n, j0, m, rng.standard_normal((m - 1, 2)), 0.0)
+        assert acc  # u = 0 accepts every move
+        after = s.total_lnV()
+        scale = abs(w.wV_updated) + abs(w.wV_initial)
+        assert (w.wV_updated - w.wV_initial) == pytest.approx(after - before, abs=1e-12 * scale + 1e-13)
+
+
+def test_com_keeps_kinetic_action_and_matches_potential_difference():
+    s = harmonic_sys(3, 20, 1.0, L=5.0, seed=5, scale=2.0)
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        n = int(rng.integers(3))
+        d = rng.uniform(-3, 3, size=2)
+        k0, v0 = s.total_lnK(), s.total_lnV()
+        acc, w = s.com_apply(n, d, 0.0)
+        assert acc
+        assert s.total_lnK() == pytest.approx(k0, rel=1e-10)  # relative positions unchanged, minimum image
+        assert (w.wV_updated - w.wV_initial) == pytest.approx(s.total_lnV() - v0, abs=1e-10)
+
+
+def pair_sys(N, L, M, ra, seed, g=1.0, sigma=0.5):
+    r = np.random.default_rng(seed).uniform(-L, L, size=(N, 2, M))
+    return O.System(r, 0.5, L, 2.0, O.make_pot(), O.make_pair(O.PAIR_GAUSSIAN, g, sigma), ra, True)
+
+
+def brute_neighbours(s, target, j, exclude, nbins):
+    """Independent restatement of the 9-cell stencil with multiplicities."""
+    r = s.r
+    w = 2 * s._s.L / nbins
+    def cell(p):
+        return tuple(int(math.floor((p[d] + s._s.L) / w)) for d in range(2))
+    ct = cell(target)
+    out = []
+    for k in range(s.N):
+        if k == exclude:
+            continue
+        pk = r[k, :, j]
+        ck = cell(pk)
+        mult = sum(1 for dx in (-1, 0, 1) for dy in (-1, 0, 1)
+                   if ((ct[0] + dx) % nbins, (ct[1] + dy) % nbins) == ck)
+        dist = math.sqrt(sum(O.distance(pk[d], target[d], s._s.L) ** 2 for d in range(2)))
+        if mult and dist <= s.h:
+            out += [k] * mult
+    return sorted(out)
+
+
+@pytest.mark.parametrize("N,L,ra", [(40, 4.0, 2.0), (30, 1.5, 1.5), (25, 2.0, 3.0), (60, 6.0, 1.0)])
+def test_find_nns_against_brute_force(N, L, ra):
+    s = pair_sys(N, L, 4, ra, seed=6)
+    rng = np.random.default_rng(6)
+    for _ in range(30):
+        n, j = int(rng.integers(N)), int(rng.integers(4))
+        t = s.r[n, :, j]
+        assert sorted(s.find_nns(t, j, n)) == brute_neighbours(s, t, j, n, s.nbins)
+
+
+def test_pair_sums_against_brute_force_total():
+    s = pair_sys(50, 4.0, 6, 2.0, seed=7)   # nbins = 4: no repeated cells
+    tot = sum(s.interaction_old(n, j) for n in range(50) for j in range(6))
+    assert tot == pytest.approx(2 * s.total_lnU(), rel=1e-12)
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        n, j0, m = int(rng.integers(50)), int(rng.integers(6)), int(rng.integers(2, 5))
+        before = s.total_lnU()
+        acc, w = s.reshape_apply(n, j0, m, rng.standard_normal((m - 1, 2)), 0.0)
+        assert acc
+        assert (w.wU_updated - w.wU_initial) == pytest.approx(s.total_lnU() - before, abs=1e-12)
+    for _ in range(10):
+        n = int(rng.integers(50))
+        before = s.total_lnU()
+        acc, w = s.com_apply(n, rng.uniform(-1, 1, size=2), 0.0)
+        assert acc
+        assert (w.wU_updated - w.wU_initial) == pytest.approx(s.total_lnU() - before, abs=1e-12)
+
+
+def test_pair_energy_matches_total_action():
+    s = pair_sys(40, 4.0, 6, 2.0, seed=8)
+    Ep, _ = s.pair_energy()
+    # E_pair = <S_U> / beta with S_U = -sum lnU
+    assert Ep == pytest.approx(-s.total_lnU() / 2.0, rel=1e-12)
+
+
+# ---------------------------------------------------------------- estimators
+def test_density_bin_convention():
+    L, nb = 2.0, 4
+    w = 2 * L / nb
+    # beads at the centres of bins (bx, by), 0-based; the reference stores bin b at index b-1
+    # and drops b = 0 (src/measurement.jl:48-50)
+    pts = [(0, 0), (1, 2), (3, 3), (2, 0), (1, 1)]
+    r = np.zeros((1, 2, len(pts)))
+    for j, (bx, by) in enumerate(pts):
+        r[0, :, j] = (-L + (bx + 0.5) * w, -L + (by + 0.5) * w)
+    s = O.System(r, 0.5, L, 1.0, O.make_pot())
+    dens, nd = s.density(nb)
+    assert nd == len(pts)
+    expect = np.zeros((nb, nb))
+    for bx, by in pts:
+        if bx > 0 and by > 0:
+            expect[bx - 1, by - 1] += 1
+    assert np.array_equal(dens, expect)
+    assert dens.sum() == 3  # (0,0) and (2,0) dropped
+
+
+def test_radial_histogram():
+    s = harmonic_sys(6, 16, 1.0, seed=9)
+    h, nd = s.radial(8, 3.0)
+    rr = np.sqrt((s.r ** 2).sum(axis=1)).ravel()
+    expect, _ = np.histogram(rr, bins=8, range=(0, 3.0))
+    assert np.array_equal(h, expect)
+    assert nd == 16
+
+
+def test_reference_example_energy():
+    """examples/energy_2d_harmonically_trapped_bose_gas.jl: N=1, M=5, beta=1,
+    COM + ReshapeLinear(s, 2).  The sampled thermodynamic and virial energies
+    reach the M=5 primitive-action value (the example's `exact`, 2.16395...,
+    is the continuum limit it approaches as M grows)."""
+    gold = json.load(open(os.path.join(GOLD, "analytic_energies.json")))
+    assert AN.energy_boltzmann_continuum(1, 1.0) == pytest.approx(gold["reference_example_exact"], rel=1e-14)
+    assert AN.energy_boltzmann_primitive(1, 4096, 1.0) == pytest.approx(gold["reference_example_exact"], rel=1e-6)
+    s = harmonic_sys(1, 5, 1.0, seed=10)
+    run = O.RefRunner(s, cap=2, step=1.0, seed=3)
+    run.run(20000)
+    E, Ev = run.run(400000, ncycle=10, measure=True)
+    exact5 = AN.energy_boltzmann_primitive(1, 5, 1.0)
+    m, e = blockerr(E)
+    assert abs(m - exact5) < 3.5 * e, (m, e, exact5)
+    m, e = blockerr(Ev)
+    assert abs(m - exact5) < 3.5 * e + 0.01, (m, e, exact5)
+
+
+def test_sweep_schedule_energy_and_x2():
+    """The sweep schedule of the kernels samples the primitive-action
+    distribution: <E>, <E_virial> and <x^2> against the normal-mode closed form."""
+    N, M, beta = 4, 16, 2.0
+    s = harmonic_sys(N, M, beta, seed=11)
+    E, Ev, x2 = [], [], []
+    for sw in range(30000):
+        s.sweep(0, sw, 77, 4, 1.0)
+        if sw >= 500:
+            e = s.energy()
+            E.append(e[0])
+            Ev.append(e[1])
+            x2.append(np.mean(s.r ** 2))
+    for vals, exact in ((E, AN.energy_boltzmann_primitive(N, M, beta)),
+                        (Ev, AN.energy_boltzmann_primitive(N, M, beta)),
+                        (x2, AN.x2_primitive(M, beta))):
+        m, e = blockerr(vals)
+        assert abs(m - exact) < 3.5 * e, (m, e, exact)
+
+
+def test_reference_schedule_counts_and_adaptation():
+    s = harmonic_sys(3, 32, 1.0, seed=12)
+    run = O.RefRunner(s, cap=8, step=1.0, seed=4)
+    run.run(5000)
+    st = run.st
+    assert st.levy_tries + st.com_tries == 5000
+    assert st.bead_moves >= st.com_tries * 32 + st.levy_tries
+    assert 2 <= st.cap <= 30 and 0.1 <= st.step <= 50.0
+
+
+def test_sweep_bead_move_count_and_step_rule():
+    # M = 10, ell = 4: segments of 4, 4, 2 links -> 3 + 3 + 1 interior beads, + M for COM
+    assert O.sweep_bead_moves(1, 10, 4) == 7 + 10
+    assert O.sweep_bead_moves(3, 64, 8) == 3 * (8 * 7 + 64)
+    assert O.adapt_step(1.0, 1, 10, 100.0) == pytest.approx(0.9)
+    assert O.adapt_step(1.0, 9, 10, 100.0) == pytest.approx(1.1)
+    assert O.adapt_step(1.0, 5, 10, 100.0) == 1.0
+    assert O.adapt_step(0.1, 0, 10, 100.0) == 0.1
+    assert O.adapt_step(49.0, 10, 10, 100.0) == 50.0
