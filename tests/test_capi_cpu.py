@@ -1,0 +1,90 @@
+"""CPU checks of the C ABI and of the host-side mirror (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from pimc import _capi as A
+from pimc.potentials import GaussianPair, HarmonicTrap, SquareLattice
+from pimc.system import chain_desc
+from pimc.updates import Counter, NumbOfSlices, Step, acceptance, adjust, queue
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pimc_hip.h")).read()
+    return sorted(set(re.findall(r"\b(pimc_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(A.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(A.EXPORTS)
+
+
+def test_struct_layouts_match_header():
+    # field counts and sizes of the ABI structs (8-byte aligned doubles after int32 groups)
+    assert ctypes.sizeof(A.ChainDesc) == 8 * 4 + 10 * 8
+    assert ctypes.sizeof(A.Options) == 4 * 4 + 8 + 4 * 4 + 8
+    assert ctypes.sizeof(A.Counters) == 6 * 8 + 2 * 8 + 8
+    assert ctypes.sizeof(A.Estimates) == 2 * 8 + 12 * 8
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("dim", 3, "dim"), ("N", 0, "N"), ("M", 2, "M"), ("beta", -1.0, "beta"), ("ell", 1, "ell"),
+    ("ell", 40, "ell"), ("potential", 7, "potential"), ("sigma", 0.0, "sigma"),
+])
+def test_invalid_descriptors_are_rejected_before_any_device_call(field, value, msg):
+    d = chain_desc(HarmonicTrap(1.0), dim=2, M=32, N=4, L=10.0, beta=1.0, lam=0.5, pair=GaussianPair(1.0, 0.5))
+    setattr(d, field, value)
+    opt = A.Options()
+    opt.precision = A.FP32
+    with pytest.raises(A.PimcError) as e:
+        A.Context([d], opt)
+    assert "PIMC_ERR_ARG" in str(e.value)
+    assert msg in str(e.value)
+
+
+def test_invalid_options_are_rejected():
+    d = chain_desc(SquareLattice(4.0, 1.0), dim=2, M=32, N=4, L=10.0, beta=1.0, lam=0.5)
+    opt = A.Options()
+    opt.precision = 5
+    with pytest.raises(A.PimcError):
+        A.Context([d], opt)
+    opt.precision = A.FP64
+    opt.radial_bins = 16
+    opt.radial_rmax = 0.0
+    with pytest.raises(A.PimcError):
+        A.Context([d], opt)
+
+
+def test_counter_and_adjust_follow_the_reference():
+    # queue! keeps a sliding window of `range` flags (src/simulation.jl:3-10)
+    c = Counter(adj=10, range=4)
+    for f in (1, 0, 1, 1, 1, 1):
+        queue(c, bool(f))
+    assert c.tries == 6 and list(c.queue) == [1, 1, 1, 1] and acceptance(c.queue) == 1.0
+    # adjust!(::Step) x0.9 / x1.1 within [minstep, maxstep] (src/updates/helper.jl:22-32)
+    s = Step(1.0, 0.1, 2.0, 0.4, 0.6)
+    adjust(s, 0.3)
+    assert s.size == pytest.approx(0.9)
+    adjust(s, 0.7)
+    assert s.size == pytest.approx(0.99)
+    adjust(s, 0.5)
+    assert s.size == pytest.approx(0.99)
+    s.size = 1.95
+    adjust(s, 0.9)
+    assert s.size == 2.0
+    # adjust!(::NumbOfSlices) -1 / +1 within [minslices, maxslices] (:42-52)
+    n = NumbOfSlices(3, 2, 4, 0.6, 0.8)
+    adjust(n, 0.9)
+    adjust(n, 0.9)
+    assert n.m == 4
+    for _ in range(5):
+        adjust(n, 0.1)
+    assert n.m == 2
