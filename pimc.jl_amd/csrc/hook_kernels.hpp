@@ -217,8 +217,8 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_energy(HookArg
         a.out[0] = ((double)(ch.dim * N) / (2.0 * tau) - (1.0 / (((4.0 * ch.lambda) * (tau * tau)) * M)) * link) +
                    (1.0 / (2.0 * M)) * pot;
         a.out[1] = (1.0 / (2.0 * M)) * vkin + (1.0 / (2.0 * M)) * pot;
-        a.out[2] = su / ch.beta;
-        a.out[3] = su / ch.beta + rgp / (2.0 * ch.beta);
+        a.out[2] = ch.g * su / ch.beta;
+        a.out[3] = ch.g * (su / ch.beta + rgp / (2.0 * ch.beta));
     }
 }
 
@@ -297,7 +297,7 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_adjoint(HookAr
                         x[e] = dx;
                         x2 += dx * dx;
                     }
-                    const T v = vpair_r2(x2, P);
+                    const T v = P.g * ex(-x2 * P.inv2s2);
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) g[e] += T(mult) * ht * (-(x[e] * inv_s2) * v);
                     if (side == 0) {
@@ -312,7 +312,8 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_adjoint(HookAr
                             T e1 = mindist(rkn[1], rnn[1], P);
                             s2 += e1 * e1;
                         }
-                        SU += 0.5 * (double)(T(mult) * ht * (vpair_r2(r2, P) + vpair_r2(s2, P)));
+                        // per unit coupling: S_U = g * SU, dS/dg = SU
+                        SU += 0.5 * (double)(T(mult) * ht * (ex(-r2 * P.inv2s2) + ex(-s2 * P.inv2s2)));
                     }
                 };
                 if (CELLS) {
@@ -351,9 +352,9 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_adjoint(HookAr
     SU = block_sum(SU, buf);
     if (threadIdx.x == 0) {
         double *tail = a.out + (size_t)N * DIM * M;
-        tail[0] = S + SU;                          // S_pot
-        tail[1] = Sw;                              // dS/d omega
-        tail[2] = (ch.g != 0.0) ? SU / ch.g : 0.0; // dS/d g
+        tail[0] = S + ch.g * SU; // S_pot
+        tail[1] = Sw;            // dS/d omega
+        tail[2] = SU;            // dS/d g
     }
 }
 

@@ -450,8 +450,9 @@ __device__ __forceinline__ T link_sum(const Path<T, DIM> &path, const Cells &C, 
     return pair_link_all<T, DIM>(path, N, nbins, P, n, j, pj, pj1);
 }
 
-// Pair energy of bead (i, j): S_U share (each unordered pair counted from
-// both ends with weight 1/2) and its r . grad S_U share.
+// Pair energy of bead (i, j) per unit coupling g: its share of S_U / g
+// (each unordered pair counted from both ends with weight 1/2) and of
+// (r . grad S_U) / g.  Kept free of g so that dS/dg is defined at g = 0.
 template <typename T, int DIM, bool CELLS>
 __device__ inline void pair_energy_bead(const Path<T, DIM> &path, const Cells &C, int N, int nbins, const Prm<T> &P,
                                         double inv_s2, int i, int j, double &su, double &rg)
@@ -481,10 +482,10 @@ __device__ inline void pair_energy_bead(const Path<T, DIM> &path, const Cells &C
             T e1 = mindist(rkn[1], rin[1], P);
             s2 += e1 * e1;
         }
-        const double va = (double)vpair_r2(r2, P), vb = (double)vpair_r2(s2, P);
+        const double ea = (double)ex(-r2 * P.inv2s2), eb = (double)ex(-s2 * P.inv2s2);
         const double w = 0.5 * (0.5 * (double)P.tau) * mult;
-        su += w * (va + vb);
-        rg += w * (-((double)r2 * inv_s2) * va - ((double)s2 * inv_s2) * vb);
+        su += w * (ea + eb);
+        rg += w * (-((double)r2 * inv_s2) * ea - ((double)s2 * inv_s2) * eb);
     };
     if (CELLS) {
         const int c0 = cell_of<T, DIM>(ri, P, C.nbins);
@@ -584,14 +585,14 @@ __device__ inline ChainMeas measure_chain(const Path<T, DIM> &path, const Cells 
 __device__ inline void fold_meas(const ChainMeas &m, const ChainDev &ch, double *q)
 {
     const double tau = ch.tau, M = ch.M;
-    const double Epair = m.su / ch.beta;
+    const double Epair = ch.g * m.su / ch.beta;
     const double E = ((double)(ch.dim * ch.N) / (2.0 * tau) - m.link / (((4.0 * ch.lambda) * (tau * tau)) * M)) +
                      m.V / M + Epair;
-    const double Ev = m.rg / (2.0 * M) + m.V / M + Epair + m.rgp / (2.0 * ch.beta);
+    const double Ev = m.rg / (2.0 * M) + m.V / M + Epair + ch.g * m.rgp / (2.0 * ch.beta);
     const double Sw = tau * m.r2w;
-    const double Sg = (ch.g != 0.0) ? m.su / ch.g : 0.0;
+    const double Sg = m.su;
     const double dEw = m.r2w / M;
-    const double dEg = (ch.g != 0.0) ? Epair / ch.g : 0.0;
+    const double dEg = m.su / ch.beta;
     q[qN] += 1.0;
     q[qE] += E;
     q[qEv] += Ev;
