@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-beads", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--workload", default="trapped_batch",
+                    choices=["trapped_batch", "trapped_plain", "gauss_box", "trapped_gauss", "lattice"],
+                    help="workload of the timed sweeps (the default is the headline batched form)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -131,6 +134,65 @@ def cpu_baseline(wl, seconds: float):
     }
 
 
+WORKLOAD_TEXT = {
+    "trapped_plain": "one trapped 2D ideal-boson chain, N=10, M=64, beta=1, omega=1, FP64",
+    "gauss_box": "256 chains of N=256 bosons in a periodic 16x16 box, Gaussian pair g=1 sigma=0.5 (cut-off 4 sigma, "
+                 "cell lists), M=128, beta=2, FP32 pair arithmetic, FP64 energy sums",
+    "trapped_gauss": "512 chains of N=64 trapped 2D bosons with Gaussian pair g=0.5 sigma=0.5, M=256, beta=2, "
+                     "FP32, reweighting sums for d<E>/d omega and d<E>/d g",
+    "lattice": "64 chains of N=400 bosons on a 20x20 periodic square lattice V0=4 a=1, Gaussian pair g=1 "
+               "sigma=0.3, M=512, beta=4, FP32, FP64 256x256 density histogram and energies",
+}
+
+
+def other_workload(args, A):
+    """Throughput of one of the other workloads (no CPU leg)."""
+    from pimc import batch as B
+    build = {"trapped_plain": B.trapped_plain, "gauss_box": B.gauss_box, "trapped_gauss": B.trapped_gauss,
+             "lattice": B.lattice}[args.workload]
+    wl = build()
+    spl = args.spl
+    bt = Batch = B.Batch
+    t = time.perf_counter()
+    b = Batch(wl, sweeps_per_launch=spl, seed=99, tile_beads=args.tile_beads, block_threads=args.threads)
+    log(f"{args.workload}: {len(wl.descs)} chains, setup {time.perf_counter() - t:.1f} s")
+    warm = max(spl, (args.warmup // spl) * spl)
+    steps = max(spl, (args.steps // spl) * spl)
+    b.run(warm, measure=False)
+    t0 = time.perf_counter()
+    cnt = b.run(steps, measure=True)
+    wall = time.perf_counter() - t0
+    value = cnt.bead_moves / wall
+    launches = max(1, cnt.kernel_launches)
+    bpm = BYTES_PER_MOVE_FP64 if wl.precision == A.FP64 else BYTES_PER_MOVE_FP32
+    achieved = bpm * cnt.bead_moves / launches / (cnt.kernel_ms / 1000.0 / launches) / 1e9
+    est = [b.estimates(c) for c in range(len(wl.descs))]
+    E = np.array([e.E for e in est])
+    Ev = np.array([e.Ev for e in est])
+    out = {
+        "metric": "bead-moves/sec at 1 GPU, batched vs plain, E within 2σ of FP64 ref; % HBM peak",
+        "value": value, "unit": "bead-moves/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": 1000.0 * wall / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if wl.precision == A.FP64 else "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {WORKLOAD_TEXT[args.workload]}", "chains": len(wl.descs),
+                   "sweeps_per_launch": spl, "bead_moves_per_sweep": int(cnt.bead_moves // steps)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "energy": {"E_mean_over_chains": float(E.mean()), "E_sem": float(E.std(ddof=1) / math.sqrt(len(E)))
+                   if len(E) > 1 else float(est[0].E_err),
+                   "Ev_mean_over_chains": float(Ev.mean()),
+                   "Epair_mean_over_chains": float(np.mean([e.Epair for e in est]))},
+        "acceptance": {"levy": cnt.levy_acc / max(1, cnt.levy_tries), "com": cnt.com_acc / max(1, cnt.com_tries)},
+        "kernel_avg_launch_ms": cnt.kernel_ms / launches,
+    }
+    if wl.derivatives:
+        out["derivatives"] = {"dE_domega_mean": float(np.mean([e.dE_domega for e in est])),
+                              "dE_dg_mean": float(np.mean([e.dE_dg for e in est]))}
+    b.close()
+    del bt
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -153,6 +215,10 @@ def main():
         hip.hipFree(src)
         hip.hipFree(dst)
         log(f"calibration copy of {args.calib_copy_mib} MiB {'done' if ok else 'FAILED'}")
+
+    if args.workload != "trapped_batch":
+        print(json.dumps(other_workload(args, A)), flush=True)
+        return
 
     # ---- batched workload
     spl = args.spl

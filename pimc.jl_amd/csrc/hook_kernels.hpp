@@ -106,7 +106,7 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_reshape(HookAr
                 path.put(n, j, rp + t * DIM);
                 if (CELLS) ok = cells_move<T, DIM>(C, P, n, j, rp + t * DIM) && ok;
             }
-            if (!ok) atomicOr(a.status, 1);
+            if (!ok) atomicOr(a.status + a.chain, 1);
         }
     }
 }
@@ -154,7 +154,7 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_com(HookArgs a
                 path.put(n, j, rp + j * DIM);
                 if (CELLS) ok = cells_move<T, DIM>(C, P, n, j, rp + j * DIM) && ok;
             }
-            if (!ok) atomicOr(a.status, 1);
+            if (!ok) atomicOr(a.status + a.chain, 1);
         }
     }
 }

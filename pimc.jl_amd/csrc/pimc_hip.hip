@@ -193,7 +193,8 @@ void launch_whole(pimc_ctx *c, const SweepArgs &a)
     }
 }
 
-void build_cells(pimc_ctx *c)
+// Rebuild the cell lists of one chain (only >= 0) or of all cell-list chains.
+void build_cells(pimc_ctx *c, int only)
 {
     if (c->whole_cells_ids.empty()) return;
     int maxM = 0;
@@ -201,7 +202,7 @@ void build_cells(pimc_ctx *c)
     dim3 grid((maxM + 63) / 64, (unsigned)c->whole_cells_ids.size());
 #define BC(T, D)                                                                                                       \
     hipLaunchKernelGGL((k_build_cells<T, D>), grid, dim3(64), 0, c->stream, c->d_ch, c->d_whole_cells,                 \
-                       (int)c->whole_cells_ids.size(), c->d_pos, c->d_cell_cnt, c->d_cell_mem, c->d_bins, c->d_status)
+                       (int)c->whole_cells_ids.size(), c->d_pos, c->d_cell_cnt, c->d_cell_mem, c->d_bins, c->d_status, only)
     if (c->prec == PIMC_FP64) {
         if (c->dim == 1) BC(double, 1);
         else BC(double, 2);
@@ -274,11 +275,15 @@ int check_chain(pimc_ctx *c, int chain)
     return PIMC_OK;
 }
 
+// Cell-list overflow flags are per chain and stay set until the chain's paths are set again.
 int check_status(pimc_ctx *c)
 {
-    int st = 0;
-    HIPCHK(hipMemcpy(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
-    if (st) return set_err(c, PIMC_ERR_CAPACITY, "a cell list overflowed its capacity; raise r_a or lower the density");
+    std::vector<int> st(c->n_chains);
+    HIPCHK(hipMemcpy(st.data(), c->d_status, sizeof(int) * c->n_chains, hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->n_chains; ++i)
+        if (st[i])
+            return set_err(c, PIMC_ERR_CAPACITY, "chain " + std::to_string(i) +
+                                                     ": a cell list overflowed its capacity; raise r_a or lower the density");
     return PIMC_OK;
 }
 
@@ -629,8 +634,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         c->d_cell_cnt = dalloc<int>((size_t)c->n_cell_cnt);
         c->d_cell_mem = dalloc<unsigned short>((size_t)mem_slots);
         c->d_bins = dalloc<int>((size_t)c->n_bins);
-        c->d_status = dalloc<int>(1);
-        HIPCHK(hipMemset(c->d_status, 0, sizeof(int)));
+        c->d_status = dalloc<int>(n_chains);
+        HIPCHK(hipMemset(c->d_status, 0, sizeof(int) * n_chains));
         int maxM = 0;
         for (int i = 0; i < n_chains; ++i) maxM = std::max(maxM, c->ch[i].M);
         c->work_elems = (size_t)(maxM + 1) * c->dim + 16;
@@ -697,7 +702,8 @@ int pimc_set_paths(pimc_ctx *c, int32_t chain, const double *r)
             HIPCHK(hipMemcpy((float *)c->d_pos + h.pos_off, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
         }
         if (h.ncell > 0) {
-            build_cells(c);
+            HIPCHK(hipMemset(c->d_status + chain, 0, sizeof(int)));
+            build_cells(c, chain);
             HIPCHK(hipStreamSynchronize(c->stream));
             return check_status(c);
         }

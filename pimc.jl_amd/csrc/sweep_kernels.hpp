@@ -73,7 +73,7 @@ struct SweepArgs {
     unsigned short *cell_mem;
     int *bins;
     double *chain_meas;    // per chain, per launch: kNQ sums
-    int *status;
+    int *status;           // per chain: cell-list overflow flag
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -816,7 +816,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
         out[kNQ] = la;
         out[kNQ + 1] = ca;
     }
-    if (overflow) atomicOr(a.status, 1);
+    if (overflow) atomicOr(a.status + c, 1);
 }
 
 // After each launch: fold per-particle (tile kernel) or per-chain (whole
@@ -898,11 +898,12 @@ __global__ void k_post(PostArgs a)
 // per (chain, slice), particles in index order.
 template <typename T, int DIM>
 __global__ void k_build_cells(const ChainDev *chains, const int *whole, int n_whole, const void *pos, int *cell_cnt,
-                              unsigned short *cell_mem, int *bins, int *status)
+                              unsigned short *cell_mem, int *bins, int *status, int only)
 {
     const int w = blockIdx.y;
     if (w >= n_whole) return;
     const int c = whole[w];
+    if (only >= 0 && c != only) return;
     const ChainDev ch = chains[c];
     if (ch.ncell <= 0) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -923,7 +924,7 @@ __global__ void k_build_cells(const ChainDev *chains, const int *whole, int n_wh
             C.bin[(long long)n * ch.M + j] = cell;
         } else {
             C.bin[(long long)n * ch.M + j] = -1;
-            atomicOr(status, 1);
+            atomicOr(status + c, 1);
         }
     }
 }
