@@ -158,7 +158,7 @@ def other_workload(args, A):
     log(f"{args.workload}: {len(wl.descs)} chains, setup {time.perf_counter() - t:.1f} s")
     warm = max(spl, (args.warmup // spl) * spl)
     steps = max(spl, (args.steps // spl) * spl)
-    b.run(warm, measure=False)
+    cw = b.run(warm, measure=False)
     t0 = time.perf_counter()
     cnt = b.run(steps, measure=True)
     wall = time.perf_counter() - t0
@@ -184,6 +184,7 @@ def other_workload(args, A):
                    "Epair_mean_over_chains": float(np.mean([e.Epair for e in est]))},
         "acceptance": {"levy": cnt.levy_acc / max(1, cnt.levy_tries), "com": cnt.com_acc / max(1, cnt.com_tries)},
         "kernel_avg_launch_ms": cnt.kernel_ms / launches,
+        "kernel_ms_per_sweep": {"thermalising": cw.kernel_ms / max(1, warm), "measuring": cnt.kernel_ms / steps},
     }
     if wl.derivatives:
         out["derivatives"] = {"dE_domega_mean": float(np.mean([e.dE_domega for e in est])),

@@ -29,8 +29,8 @@ template <typename T, int DIM> __device__ inline Cells hook_cells(const HookArgs
                  ch.ncell, ch.cap, ch.nbins};
 }
 
-template <typename T, int DIM, bool CELLS>
-__device__ inline T hook_link(const Path<T, DIM> &path, const Cells &C, const ChainDev &ch, const Prm<T> &P, int n,
+template <typename T, int DIM, bool CELLS, typename PathT>
+__device__ inline T hook_link(const PathT &path, const Cells &C, const ChainDev &ch, const Prm<T> &P, int n,
                               int j, const T *pj, const T *pj1)
 {
     if (ch.pair == kPairNone) return T(0);
@@ -40,13 +40,13 @@ __device__ inline T hook_link(const Path<T, DIM> &path, const Cells &C, const Ch
 // ReshapeLinear on particle n, start slice j0, m links (src/updates/
 // reshape.jl:31-91): the bridge of levy! with the injected normals, then the
 // link sums w_initial / w_updated of the potential and pair actions.
-template <typename T, int DIM, bool CELLS> __global__ void k_hook_reshape(HookArgs a)
+template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_reshape(HookArgs a)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const ChainDev ch = a.chains[a.chain];
     const Prm<T> P = Prm<T>::make(ch);
     const int M = ch.M, m = a.m, n = a.n, j0 = a.j0, nrows = m + 1;
-    const Path<T, DIM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, M};
+    const Path<T, DIM, SM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M, ch.N};
     const Cells C = hook_cells<T, DIM>(a, ch);
     T *rp = reinterpret_cast<T *>(a.work); // (m+1) x DIM, row-major per bead
     const int jm = (j0 + m) % M;
@@ -112,13 +112,13 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_reshape(HookAr
 }
 
 // SingleCenterOfMass with displacement in[0..dim) (src/updates/com.jl:136-224).
-template <typename T, int DIM, bool CELLS> __global__ void k_hook_com(HookArgs a)
+template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_com(HookArgs a)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const ChainDev ch = a.chains[a.chain];
     const Prm<T> P = Prm<T>::make(ch);
     const int M = ch.M, n = a.n;
-    const Path<T, DIM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, M};
+    const Path<T, DIM, SM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M, ch.N};
     const Cells C = hook_cells<T, DIM>(a, ch);
     T *rp = reinterpret_cast<T *>(a.work); // M x DIM row-major
     for (int j = 0; j < M; ++j)
@@ -160,12 +160,12 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_com(HookArgs a
 }
 
 // interaction_action! value for particle n at slice j (src/updates/helper.jl:306-324)
-template <typename T, int DIM, bool CELLS> __global__ void k_hook_interaction(HookArgs a)
+template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_interaction(HookArgs a)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const ChainDev ch = a.chains[a.chain];
     const Prm<T> P = Prm<T>::make(ch);
-    const Path<T, DIM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M};
+    const Path<T, DIM, SM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M, ch.N};
     const Cells C = hook_cells<T, DIM>(a, ch);
     T r0[DIM], r1[DIM];
     path.get(a.n, a.j0, r0);
@@ -174,14 +174,14 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_interaction(Ho
 }
 
 // Energy (src/measurement.jl:92-122) plus the pair energy, one workgroup.
-template <typename T, int DIM, bool CELLS> __global__ void k_hook_energy(HookArgs a)
+template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_energy(HookArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     double *buf = reinterpret_cast<double *>(smem);
     const ChainDev ch = a.chains[a.chain];
     const Prm<T> P = Prm<T>::make(ch);
     const int N = ch.N, M = ch.M;
-    const Path<T, DIM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, M};
+    const Path<T, DIM, SM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M, ch.N};
     const Cells C = hook_cells<T, DIM>(a, ch);
     const double inv_s2 = (ch.pair != kPairNone) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
     double link = 0.0, pot = 0.0, vkin = 0.0, su = 0.0, rgp = 0.0;
@@ -223,10 +223,10 @@ template <typename T, int DIM, bool CELLS> __global__ void k_hook_energy(HookArg
 }
 
 // Density (src/measurement.jl:45-55): counts into a uint64 histogram.
-template <typename T, int DIM> __global__ void k_hook_density(HookArgs a, int nbins, unsigned long long *hist)
+template <typename T, int DIM, bool SM> __global__ void k_hook_density(HookArgs a, int nbins, unsigned long long *hist)
 {
     const ChainDev ch = a.chains[a.chain];
-    const Path<T, DIM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M};
+    const Path<T, DIM, SM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M, ch.N};
     const double bw = (2.0 * ch.L) / nbins;
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < ch.N * ch.M; b += gridDim.x * blockDim.x) {
         const int i = b / ch.M, j = b - i * ch.M;
@@ -248,14 +248,14 @@ template <typename T, int DIM> __global__ void k_hook_density(HookArgs a, int nb
 // terms of links j (neighbour set at slice j) and j-1 (set at slice j-1);
 // distances are recomputed, nothing is stored.  Block sums give S_pot,
 // dS/domega and dS/dg.
-template <typename T, int DIM, bool CELLS> __global__ void k_hook_adjoint(HookArgs a)
+template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_adjoint(HookArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     double *buf = reinterpret_cast<double *>(smem);
     const ChainDev ch = a.chains[a.chain];
     const Prm<T> P = Prm<T>::make(ch);
     const int N = ch.N, M = ch.M;
-    const Path<T, DIM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, M};
+    const Path<T, DIM, SM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M, ch.N};
     const Cells C = hook_cells<T, DIM>(a, ch);
     const T inv_s2 = (ch.pair != kPairNone) ? T(1.0 / (ch.sigma * ch.sigma)) : T(0);
     const T ht = T(0.5) * P.tau;

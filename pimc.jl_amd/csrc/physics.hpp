@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "philox.hpp"
 
 namespace pimc {
@@ -30,6 +32,7 @@ struct ChainDev {
     int N, M, dim, pot;
     int pair, pair_in_accept, ell, nbins;
     int cap, ncell, part_base, nseg;
+    int sm, pad0, pad1, pad2; // sm: positions stored slice-major (chains with a pair action)
 };
 
 // The chain constants converted to the arithmetic type (built once per chain
@@ -235,15 +238,31 @@ __device__ __forceinline__ int stencil_cell(int c, int s, int nbins, int dim)
     return cx + nbins * cy;
 }
 
-// Accessor for one chain's positions (global or LDS) with the reference layout.
-template <typename T, int DIM> struct Path {
+// Accessor for one chain's positions.  Particle-major (SM = false): the
+// reference's layout, particle n, coordinate d, slice j at (n*DIM + d)*M + j,
+// used by the tile kernel (a ring is contiguous).  Slice-major (SM = true):
+// (j*N + n)*DIM + d, used for chains with a pair action, where the beads of
+// all particles at one slice are gathered together and one bead is one
+// vector load.
+template <typename T, int DIM, bool SM = false> struct Path {
     T *base;
-    int M;
-    __device__ __forceinline__ T &at(int n, int d, int j) const { return base[((long long)n * DIM + d) * M + j]; }
+    int M, N;
+    __device__ __forceinline__ T &at(int n, int d, int j) const
+    {
+        if constexpr (SM) return base[((long long)j * N + n) * DIM + d];
+        else return base[((long long)n * DIM + d) * M + j];
+    }
     __device__ __forceinline__ void get(int n, int j, T *r) const
     {
-        r[0] = at(n, 0, j);
-        if (DIM > 1) r[1] = at(n, 1, j);
+        if constexpr (SM && DIM == 2) {
+            using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+            const V2 v = *reinterpret_cast<const V2 *>(&at(n, 0, j));
+            r[0] = v.x;
+            r[1] = v.y;
+        } else {
+            r[0] = at(n, 0, j);
+            if (DIM > 1) r[1] = at(n, 1, j);
+        }
     }
     __device__ __forceinline__ void put(int n, int j, const T *r) const
     {
@@ -273,9 +292,8 @@ struct Cells {
 // (find_nns, src/nearest_neighbours.jl:106-154), each contributing
 // lnU = -tau/2 (v(|r_jk|) + v(|r_j+1,k|)) (src/updates/helper.jl:306-366).
 // Returns the sum of lnU (a log weight, <= 0 for a repulsion).
-template <typename T, int DIM>
-__device__ T pair_link(const Path<T, DIM> &P, const Cells &C, const Prm<T> &p, int n, int j, const T *pj,
-                       const T *pj1)
+template <typename T, int DIM, typename PathT>
+__device__ T pair_link(const PathT &P, const Cells &C, const Prm<T> &p, int n, int j, const T *pj, const T *pj1)
 {
     const int M = P.M;
     const int jn = (j + 1 == M) ? 0 : j + 1;
@@ -317,8 +335,8 @@ __device__ T pair_link(const Path<T, DIM> &P, const Cells &C, const Prm<T> &p, i
 // within h of pj exactly once, so this enumerates the same neighbour set; for
 // nbins < 3 the stencil repeats cells and the reference counts a candidate
 // once per repeat, which `mult` reproduces.
-template <typename T, int DIM>
-__device__ T pair_link_all(const Path<T, DIM> &P, int N, int nbins, const Prm<T> &p, int n, int j, const T *pj,
+template <typename T, int DIM, typename PathT>
+__device__ T pair_link_all(const PathT &P, int N, int nbins, const Prm<T> &p, int n, int j, const T *pj,
                            const T *pj1)
 {
     const int M = P.M;

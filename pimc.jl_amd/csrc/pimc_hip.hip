@@ -170,7 +170,13 @@ template <typename T, int DIM, bool CELLS> void launch_whole_t(pimc_ctx *c, Swee
 {
     if (ids.empty()) return;
     a.whole = d_ids;
-    auto k = k_sweep_whole<T, DIM, 32, CELLS>;
+    int maxM = 0;
+    for (int id : ids) maxM = std::max(maxM, c->ch[id].M);
+    // lanes per link: as many as the workgroup holds for one pass over the links (2..16)
+    int G = 1;
+    while (G < 16 && 2 * G * maxM <= c->threads_whole) G <<= 1;
+    a.link_group = std::max(2, G);
+    auto k = k_sweep_whole<T, DIM, CELLS>;
     if (c->lds_whole > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_whole));
     hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(c->threads_whole), c->lds_whole, c->stream, a);
     HIPCHK(hipGetLastError());
@@ -310,26 +316,24 @@ HookArgs hook_args(pimc_ctx *c, int chain)
     return h;
 }
 
-// Launch a hook kernel templated on <T, DIM, CELLS>.
+// Launch a hook kernel templated on <T, DIM, CELLS, SM>: cell lists exist only
+// for slice-major (pair) chains, so three layouts occur.
+#define HOOK_T(KERNEL, T, D, grid, block, lds, ...)                                                                   \
+    do {                                                                                                               \
+        if (cells_) hipLaunchKernelGGL((KERNEL<T, D, true, true>), grid, block, lds, c->stream, __VA_ARGS__);          \
+        else if (sm_) hipLaunchKernelGGL((KERNEL<T, D, false, true>), grid, block, lds, c->stream, __VA_ARGS__);       \
+        else hipLaunchKernelGGL((KERNEL<T, D, false, false>), grid, block, lds, c->stream, __VA_ARGS__);               \
+    } while (0)
 #define HOOK_DISPATCH(KERNEL, grid, block, lds, ...)                                                                   \
     do {                                                                                                               \
         const bool cells_ = h.use_cells != 0;                                                                          \
+        const bool sm_ = c->ch[h.chain].sm != 0;                                                                       \
         if (c->prec == PIMC_FP64) {                                                                                    \
-            if (c->dim == 1) {                                                                                         \
-                if (cells_) hipLaunchKernelGGL((KERNEL<double, 1, true>), grid, block, lds, c->stream, __VA_ARGS__);   \
-                else hipLaunchKernelGGL((KERNEL<double, 1, false>), grid, block, lds, c->stream, __VA_ARGS__);         \
-            } else {                                                                                                   \
-                if (cells_) hipLaunchKernelGGL((KERNEL<double, 2, true>), grid, block, lds, c->stream, __VA_ARGS__);   \
-                else hipLaunchKernelGGL((KERNEL<double, 2, false>), grid, block, lds, c->stream, __VA_ARGS__);         \
-            }                                                                                                          \
+            if (c->dim == 1) HOOK_T(KERNEL, double, 1, grid, block, lds, __VA_ARGS__);                                 \
+            else HOOK_T(KERNEL, double, 2, grid, block, lds, __VA_ARGS__);                                             \
         } else {                                                                                                       \
-            if (c->dim == 1) {                                                                                         \
-                if (cells_) hipLaunchKernelGGL((KERNEL<float, 1, true>), grid, block, lds, c->stream, __VA_ARGS__);    \
-                else hipLaunchKernelGGL((KERNEL<float, 1, false>), grid, block, lds, c->stream, __VA_ARGS__);          \
-            } else {                                                                                                   \
-                if (cells_) hipLaunchKernelGGL((KERNEL<float, 2, true>), grid, block, lds, c->stream, __VA_ARGS__);    \
-                else hipLaunchKernelGGL((KERNEL<float, 2, false>), grid, block, lds, c->stream, __VA_ARGS__);          \
-            }                                                                                                          \
+            if (c->dim == 1) HOOK_T(KERNEL, float, 1, grid, block, lds, __VA_ARGS__);                                  \
+            else HOOK_T(KERNEL, float, 2, grid, block, lds, __VA_ARGS__);                                              \
         }                                                                                                              \
         HIPCHK(hipGetLastError());                                                                                     \
     } while (0)
@@ -470,6 +474,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             h.part_base = pbase;
             pbase += h.N;
             if (c->is_whole[id]) {
+                h.sm = 1; // slice-major: the beads of one slice together
                 c->whole_ids.push_back(id);
                 // cell lists when they beat the all-pairs scan and stay small
                 const long long ncell = (h.dim == 1) ? h.nbins : (long long)h.nbins * h.nbins;
@@ -554,9 +559,9 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         // whole-chain kernel
         int maxMw = 0;
         for (int id : c->whole_ids) maxMw = std::max(maxMw, c->ch[id].M);
-        c->threads_whole = 256;
+        c->threads_whole = 1024;
         if (maxMw > 0) {
-            c->lds_whole = whole_lds_bytes(maxMw, c->dim, ts, c->threads_whole);
+            c->lds_whole = whole_lds_bytes(maxMw, c->dim, ts);
             if (c->lds_whole > 160 * 1024) throw HipError{hipErrorInvalidValue, "M too large for the whole-chain kernel"};
         }
         // schedule counts per sweep
@@ -690,15 +695,25 @@ int pimc_set_paths(pimc_ctx *c, int32_t chain, const double *r)
         if (!std::isfinite(r[i]) || r[i] < -h.L || r[i] >= h.L)
             return set_err(c, PIMC_ERR_ARG, "bead outside the box [-L, L) or not finite");
     try {
+        // caller layout N x dim x M (Particle.r blocks) -> device layout of the chain
+        auto dst_index = [&](size_t n_, int d_, int j_) -> size_t {
+            return h.sm ? ((size_t)j_ * h.N + n_) * h.dim + d_ : ((size_t)n_ * h.dim + d_) * h.M + j_;
+        };
         if (c->prec == PIMC_FP64) {
-            HIPCHK(hipMemcpy((double *)c->d_pos + h.pos_off, r, n * sizeof(double), hipMemcpyHostToDevice));
+            std::vector<double> f(n);
+            for (int p = 0; p < h.N; ++p)
+                for (int d = 0; d < h.dim; ++d)
+                    for (int j = 0; j < h.M; ++j) f[dst_index(p, d, j)] = r[((size_t)p * h.dim + d) * h.M + j];
+            HIPCHK(hipMemcpy((double *)c->d_pos + h.pos_off, f.data(), n * sizeof(double), hipMemcpyHostToDevice));
         } else {
             std::vector<float> f(n);
-            for (size_t i = 0; i < n; ++i) {
-                float v = (float)r[i];
-                if (v >= (float)h.L) v = std::nextafter((float)h.L, -INFINITY);
-                f[i] = v;
-            }
+            for (int p = 0; p < h.N; ++p)
+                for (int d = 0; d < h.dim; ++d)
+                    for (int j = 0; j < h.M; ++j) {
+                        float v = (float)r[((size_t)p * h.dim + d) * h.M + j];
+                        if (v >= (float)h.L) v = std::nextafter((float)h.L, -INFINITY);
+                        f[dst_index(p, d, j)] = v;
+                    }
             HIPCHK(hipMemcpy((float *)c->d_pos + h.pos_off, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
         }
         if (h.ncell > 0) {
@@ -722,12 +737,21 @@ int pimc_get_paths(pimc_ctx *c, int32_t chain, double *r)
     const size_t n = (size_t)h.N * h.dim * h.M;
     try {
         HIPCHK(hipStreamSynchronize(c->stream));
+        auto src_index = [&](size_t n_, int d_, int j_) -> size_t {
+            return h.sm ? ((size_t)j_ * h.N + n_) * h.dim + d_ : ((size_t)n_ * h.dim + d_) * h.M + j_;
+        };
         if (c->prec == PIMC_FP64) {
-            HIPCHK(hipMemcpy(r, (double *)c->d_pos + h.pos_off, n * sizeof(double), hipMemcpyDeviceToHost));
+            std::vector<double> f(n);
+            HIPCHK(hipMemcpy(f.data(), (double *)c->d_pos + h.pos_off, n * sizeof(double), hipMemcpyDeviceToHost));
+            for (int p = 0; p < h.N; ++p)
+                for (int d = 0; d < h.dim; ++d)
+                    for (int j = 0; j < h.M; ++j) r[((size_t)p * h.dim + d) * h.M + j] = f[src_index(p, d, j)];
         } else {
             std::vector<float> f(n);
             HIPCHK(hipMemcpy(f.data(), (float *)c->d_pos + h.pos_off, n * sizeof(float), hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < n; ++i) r[i] = f[i];
+            for (int p = 0; p < h.N; ++p)
+                for (int d = 0; d < h.dim; ++d)
+                    for (int j = 0; j < h.M; ++j) r[((size_t)p * h.dim + d) * h.M + j] = f[src_index(p, d, j)];
         }
     } catch (const HipError &e) {
         return hip_err(c, e);
@@ -1119,13 +1143,16 @@ int pimc_density(pimc_ctx *c, int32_t chain, int32_t nbins, double *dens, int64_
         HIPCHK(hipStreamSynchronize(c->stream));
         HookArgs h = hook_args(c, chain);
         const int nb = std::max(1, std::min(1024, (hc.N * hc.M + 255) / 256));
+#define DENS(T, D, S) hipLaunchKernelGGL((k_hook_density<T, D, S>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h)
+        const bool sm = hc.sm != 0;
         if (c->prec == PIMC_FP64) {
-            if (c->dim == 1) hipLaunchKernelGGL((k_hook_density<double, 1>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h);
-            else hipLaunchKernelGGL((k_hook_density<double, 2>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h);
+            if (c->dim == 1) { if (sm) DENS(double, 1, true); else DENS(double, 1, false); }
+            else { if (sm) DENS(double, 2, true); else DENS(double, 2, false); }
         } else {
-            if (c->dim == 1) hipLaunchKernelGGL((k_hook_density<float, 1>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h);
-            else hipLaunchKernelGGL((k_hook_density<float, 2>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h);
+            if (c->dim == 1) { if (sm) DENS(float, 1, true); else DENS(float, 1, false); }
+            else { if (sm) DENS(float, 2, true); else DENS(float, 2, false); }
         }
+#undef DENS
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         std::vector<unsigned long long> hh(nh);

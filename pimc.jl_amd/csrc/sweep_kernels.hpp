@@ -67,6 +67,7 @@ struct SweepArgs {
     unsigned sweep0;
     int nsweeps, measure;
     int tile0;             // first tile of this launch (tiles are grouped by potential kind)
+    int link_group;        // lanes per link in the whole-chain kernel (power of two <= 64)
     // whole-chain kernel
     const int *whole;
     int *cell_cnt;
@@ -422,40 +423,113 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
         a.part_cacc[tl.q0 + q] = s_cacc[q];
     }
 }
-// Deterministic block sum (fixed tree for a given blockDim).  Every thread
-// gets the result.  `buf` holds blockDim doubles.
-__device__ inline double block_sum(double v, double *buf)
+// Deterministic workgroup sum: butterfly within each wave (lane 0's order is
+// fixed), wave totals through LDS, then every thread adds the wave totals in
+// wave order.  `wbuf` holds 16 doubles.  Every thread gets the result.
+__device__ inline double block_sum(double v, double *wbuf)
 {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    buf[tid] = v;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) wbuf[wave] = v;
     __syncthreads();
-    for (int w = 1; w < nt; w <<= 1) {
-        double add = 0.0;
-        const bool take = ((tid & (2 * w - 1)) == 0) && (tid + w < nt);
-        if (take) add = buf[tid + w];
-        __syncthreads();
-        if (take) buf[tid] += add;
-        __syncthreads();
-    }
-    const double r = buf[0];
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += wbuf[w];
     __syncthreads();
-    return r;
+    return s;
 }
 
-template <typename T, int DIM, bool CELLS>
-__device__ __forceinline__ T link_sum(const Path<T, DIM> &path, const Cells &C, int N, int nbins, const Prm<T> &P,
-                                      int n, int j, const T *pj, const T *pj1)
+// Sum over the lanes of an aligned group of G lanes (G a power of two <= 64).
+template <typename T> __device__ __forceinline__ T group_sum(T v, int G)
 {
-    if (CELLS) return pair_link<T, DIM>(path, C, P, n, j, pj, pj1);
-    return pair_link_all<T, DIM>(path, N, nbins, P, n, j, pj, pj1);
+    for (int off = G >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// This lane's share of the pair link sum of particle n over link j -> j+1,
+// with its beads there replaced by pj, pj1 (see pair_link): the candidates of
+// the 9 stencil cells (or of all particles) are split over the G lanes of a
+// group, lane g taking candidates g, g + G, ...
+template <typename T, int DIM, bool CELLS, typename PathT>
+__device__ T link_part(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P, int n, int j,
+                       const T *pj, const T *pj1, int g, int G)
+{
+    const int M = path.M;
+    const int jn = (j + 1 == M) ? 0 : j + 1;
+    constexpr int NS = (DIM == 1) ? 3 : 9;
+    T acc = T(0);
+    auto pair_term = [&](int k, T mult) {
+        T rk[DIM];
+        path.get(k, j, rk);
+        T d0 = mindist(rk[0], pj[0], P);
+        T r2 = d0 * d0;
+        if (DIM > 1) {
+            T d1 = mindist(rk[1], pj[1], P);
+            r2 += d1 * d1;
+        }
+        if (r2 > P.h2) return;
+        T rkn[DIM];
+        path.get(k, jn, rkn);
+        T e0 = mindist(rkn[0], pj1[0], P);
+        T s2 = e0 * e0;
+        if (DIM > 1) {
+            T e1 = mindist(rkn[1], pj1[1], P);
+            s2 += e1 * e1;
+        }
+        acc += mult * ((T(-0.5) * P.tau) * (vpair_r2(r2, P) + vpair_r2(s2, P)));
+    };
+    if constexpr (CELLS) {
+        const int c0 = cell_of<T, DIM>(pj, P, C.nbins);
+        int cs[NS], cnt[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            cs[s] = stencil_cell(c0, s, C.nbins, DIM);
+            cnt[s] = C.count(j, cs[s]);
+        }
+        int s = 0, base = 0;
+        for (int idx = g;; idx += G) {
+            while (s < NS && idx >= base + cnt[s]) {
+                base += cnt[s];
+                ++s;
+            }
+            if (s >= NS) break;
+            const int k = C.members(j, cs[s])[idx - base];
+            if (k != n) pair_term(k, T(1));
+        }
+    } else {
+        const int c0 = (nbins < 3) ? cell_of<T, DIM>(pj, P, nbins) : 0;
+        for (int k = g; k < N; k += G) {
+            if (k == n) continue;
+            T mult = T(1);
+            if (nbins < 3) {
+                T rk[DIM];
+                path.get(k, j, rk);
+                const int ck = cell_of<T, DIM>(rk, P, nbins);
+                int mm = 0;
+                for (int s = 0; s < NS; ++s) mm += (stencil_cell(c0, s, nbins, DIM) == ck);
+                if (!mm) continue;
+                mult = T(mm);
+            }
+            pair_term(k, mult);
+        }
+    }
+    return acc;
+}
+
+template <typename T, int DIM, bool CELLS, typename PathT>
+__device__ __forceinline__ T link_sum(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P, int n,
+                                      int j, const T *pj, const T *pj1)
+{
+    return link_part<T, DIM, CELLS>(path, C, N, nbins, P, n, j, pj, pj1, 0, 1);
 }
 
 // Pair energy of bead (i, j) per unit coupling g: its share of S_U / g
 // (each unordered pair counted from both ends with weight 1/2) and of
 // (r . grad S_U) / g.  Kept free of g so that dS/dg is defined at g = 0.
-template <typename T, int DIM, bool CELLS>
-__device__ inline void pair_energy_bead(const Path<T, DIM> &path, const Cells &C, int N, int nbins, const Prm<T> &P,
-                                        double inv_s2, int i, int j, double &su, double &rg)
+template <typename T, int DIM, bool CELLS, typename PathT>
+__device__ inline void pair_energy_bead(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P,
+                                        double inv_s2, int i, int j, double &su, double &rg, int g = 0, int G = 1)
 {
     const int M = path.M;
     const int jn = (j + 1 == M) ? 0 : j + 1;
@@ -487,20 +561,29 @@ __device__ inline void pair_energy_bead(const Path<T, DIM> &path, const Cells &C
         su += w * (ea + eb);
         rg += w * (-((double)r2 * inv_s2) * ea - ((double)s2 * inv_s2) * eb);
     };
-    if (CELLS) {
+    constexpr int NS = (DIM == 1) ? 3 : 9;
+    // candidates split over the G lanes of a group: lane g takes g, g + G, ...
+    if constexpr (CELLS) {
         const int c0 = cell_of<T, DIM>(ri, P, C.nbins);
-        const int ns = (DIM == 1) ? 3 : 9;
-        for (int s = 0; s < ns; ++s) {
-            const int c = stencil_cell(c0, s, C.nbins, DIM);
-            const int cnt = C.count(j, c);
-            const unsigned short *mem = C.members(j, c);
-            for (int q = 0; q < cnt; ++q)
-                if (mem[q] != i) visit(mem[q], 1);
+        int cs[NS], cnt[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            cs[s] = stencil_cell(c0, s, C.nbins, DIM);
+            cnt[s] = C.count(j, cs[s]);
+        }
+        int s = 0, base = 0;
+        for (int idx = g;; idx += G) {
+            while (s < NS && idx >= base + cnt[s]) {
+                base += cnt[s];
+                ++s;
+            }
+            if (s >= NS) break;
+            const int k = C.members(j, cs[s])[idx - base];
+            if (k != i) visit(k, 1);
         }
     } else {
-        const int ns = (DIM == 1) ? 3 : 9;
         const int c0 = (nbins < 3) ? cell_of<T, DIM>(ri, P, nbins) : 0;
-        for (int k = 0; k < N; ++k) {
+        for (int k = g; k < N; k += G) {
             if (k == i) continue;
             int mult = 1;
             if (nbins < 3) {
@@ -508,7 +591,7 @@ __device__ inline void pair_energy_bead(const Path<T, DIM> &path, const Cells &C
                 path.get(k, j, rk);
                 const int ck = cell_of<T, DIM>(rk, P, nbins);
                 mult = 0;
-                for (int s = 0; s < ns; ++s) mult += (stencil_cell(c0, s, nbins, DIM) == ck);
+                for (int s = 0; s < NS; ++s) mult += (stencil_cell(c0, s, nbins, DIM) == ck);
                 if (!mult) continue;
             }
             visit(k, mult);
@@ -521,16 +604,27 @@ struct ChainMeas {
     double link, V, rg, su, rgp, r2w;
 };
 
-template <typename T, int DIM, bool CELLS>
-__device__ inline ChainMeas measure_chain(const Path<T, DIM> &path, const Cells &C, const ChainDev &ch, const Prm<T> &P,
-                                          double *buf, unsigned long long *radial, int radial_bins, double radial_winv,
-                                          unsigned long long *cart, int cart_bins, int c)
+template <typename T, int DIM, bool CELLS, typename PathT>
+__device__ inline ChainMeas measure_chain(const PathT &path, const Cells &C, const ChainDev &ch, const Prm<T> &P,
+                                          double *wbuf, unsigned long long *radial, int radial_bins,
+                                          double radial_winv, unsigned long long *cart, int cart_bins, int c,
+                                          int G = 1)
 {
     const int N = ch.N, M = ch.M;
     const double inv_s2 = (ch.pair != kPairNone) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
     double l = 0.0, v = 0.0, g = 0.0, su = 0.0, rgp = 0.0, r2w = 0.0;
-    for (int b = threadIdx.x; b < N * M; b += blockDim.x) {
-        const int i = b / M, j = b - i * M;
+    // bead b = (slice j, particle i) per group of G lanes; the pair scan is split over
+    // the lanes, the single-bead terms are taken by lane 0 of the group
+    const int lg = threadIdx.x & (G - 1), ngroups = blockDim.x / G;
+    for (int b = threadIdx.x / G; b < N * M; b += ngroups) {
+        const int j = b / N, i = b - j * N;
+        if (ch.pair != kPairNone) {
+            double a, bb;
+            pair_energy_bead<T, DIM, CELLS>(path, C, N, ch.nbins, P, inv_s2, i, j, a, bb, lg, G);
+            su += a;
+            rgp += bb;
+        }
+        if (lg != 0) continue;
         const int jn = (j + 1 == M) ? 0 : j + 1;
         T r0[DIM], r1[DIM];
         path.get(i, j, r0);
@@ -545,12 +639,6 @@ __device__ inline ChainMeas measure_chain(const Path<T, DIM> &path, const Cells 
         v += (double)potV<T, DIM>(r0, P);
         g += (double)rgradV<T, DIM>(r0, P);
         if (ch.pot == kPotHarmonic) r2w += (double)dVdomega<T, DIM>(r0, ch.omega);
-        if (ch.pair != kPairNone) {
-            double a, bb;
-            pair_energy_bead<T, DIM, CELLS>(path, C, N, ch.nbins, P, inv_s2, i, j, a, bb);
-            su += a;
-            rgp += bb;
-        }
         if (radial_bins > 0) {
             T rr = r0[0] * r0[0];
             if (DIM > 1) rr += r0[1] * r0[1];
@@ -572,12 +660,12 @@ __device__ inline ChainMeas measure_chain(const Path<T, DIM> &path, const Cells 
         }
     }
     ChainMeas r;
-    r.link = block_sum(l, buf);
-    r.V = block_sum(v, buf);
-    r.rg = block_sum(g, buf);
-    r.su = block_sum(su, buf);
-    r.rgp = block_sum(rgp, buf);
-    r.r2w = block_sum(r2w, buf);
+    r.link = block_sum(l, wbuf);
+    r.V = block_sum(v, wbuf);
+    r.rg = block_sum(g, wbuf);
+    r.su = block_sum(su, wbuf);
+    r.rgp = block_sum(rgp, wbuf);
+    r.r2w = block_sum(r2w, wbuf);
     return r;
 }
 
@@ -607,34 +695,48 @@ __device__ inline void fold_meas(const ChainMeas &m, const ChainDev &ch, double 
     q[qWhole] = 1.0;
 }
 
-// LDS of k_sweep_whole: new beads (M x DIM T), Delta V per segment/slice (M
-// T), Delta lnU per link (M T), accept flags (M ints), reduction buffer.
-__host__ __device__ inline size_t whole_lds_bytes(int M, int dim, int tsize, int nt)
+// LDS of k_sweep_whole: two buffers of new beads (M x DIM T each), Delta V
+// per segment (M T), Delta lnU per link (M T), accept flags (M ints), two
+// sets of 16 wave partials (double), 16 doubles for block sums.
+__host__ __device__ inline size_t whole_lds_bytes(int M, int dim, int tsize)
 {
-    size_t o = align16((size_t)M * dim * tsize);
+    size_t o = 2 * align16((size_t)M * dim * tsize);
     o += align16((size_t)M * tsize);
     o += align16((size_t)M * tsize);
     o += align16((size_t)M * 4);
-    o += align16((size_t)nt * sizeof(double));
+    o += align16(32 * sizeof(double));
+    o += align16(16 * sizeof(double));
     return o;
 }
 
-template <typename T, int DIM, int MAXL, bool CELLS>
+// One workgroup per chain with a pair action; positions slice-major in HBM.
+// Staging: for each particle in order, (A) one thread per segment builds the
+// bridge into LDS, (B) each link's old and new pair sums are split over a
+// group of G lanes and reduced by shuffles, (C) one thread per segment
+// decides, (D) one thread per slice commits beads and cell lists.  New beads
+// are double-buffered by particle parity, so a particle costs three barriers.
+// Centre of mass: new beads per slice, the link sums over all slices, one
+// workgroup reduction, a uniform decision and the commit: two barriers.
+template <typename T, int DIM, bool CELLS>
 __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = a.whole[blockIdx.x];
     const ChainDev ch = a.chains[c];
-    const Prm<T> P = Prm<T>::make(ch);
+    const Prm<T> P = reinterpret_cast<const Prm<T> *>(a.prm)[c];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int N = ch.N, M = ch.M, ell = ch.ell, nseg = ch.nseg;
-    const Path<T, DIM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, M};
-    const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off,
-                      M, ch.ncell, ch.cap, ch.nbins};
+    const Path<T, DIM, true> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, M, N};
+    const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, M, ch.ncell, ch.cap,
+                      ch.nbins};
     const bool use_pair = (ch.pair != kPairNone) && ch.pair_in_accept;
+    const int G = a.link_group, ngroups = nt / G, g = tid & (G - 1), grp = tid / G;
+    const int lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
 
     size_t o = 0;
-    T *s_new = reinterpret_cast<T *>(smem + o);
+    T *s_new0 = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)M * DIM * sizeof(T));
+    T *s_new1 = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)M * DIM * sizeof(T));
     T *s_dv = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)M * sizeof(T));
@@ -642,6 +744,8 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     o += align16((size_t)M * sizeof(T));
     int *s_flag = reinterpret_cast<int *>(smem + o);
     o += align16((size_t)M * 4);
+    double *s_wsum = reinterpret_cast<double *>(smem + o);
+    o += align16(32 * sizeof(double));
     double *s_buf = reinterpret_cast<double *>(smem + o);
 
     double q[kNQ];
@@ -651,90 +755,105 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
 
     for (int s = 0; s < a.nsweeps; ++s) {
         const uint32_t sw = a.sweep0 + (uint32_t)s;
-        // ---- staging: particles in order, segments of one particle in parallel
+        // ---- staging: particles in order, segments (and links) of one particle in parallel
         for (int p = 0; p < N; ++p) {
+            T *buf = (p & 1) ? s_new1 : s_new0;
             const U4 xo = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kOffset);
             const int off = (int)(((uint64_t)xo.x * (uint64_t)ell) >> 32);
+            // (A) bridges
             for (int k = tid; k < nseg; k += nt) {
                 const int m = min(ell, M - k * ell);
-                s_dv[k] = T(0);
-                if (m < 2) continue;
-                int start = off + k * ell;
-                if (start >= M) start -= M;
-                int jm = start + m;
-                if (jm >= M) jm -= M;
-                T cur[DIM], em[DIM];
-                path.get(p, start, cur);
-#pragma unroll
-                for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], path.at(p, e, jm), P);
                 T dv = T(0);
-                U4 xr{0u, 0u, 0u, 0u};
-                for (int i = 1; i < m; ++i) {
-                    int b = start + i;
-                    if (b >= M) b -= M;
-                    if (i & 1) xr = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c, kLevy);
-                    T n0, n1;
-                    if (i & 1) box_muller(xr.x, xr.y, n0, n1);
-                    else box_muller(xr.z, xr.w, n0, n1);
-                    const T alpha = bridge_alpha<T>(m, i);
-                    const T sd = bridge_sd(alpha, P);
-                    T rn[DIM], ro[DIM];
-                    cur[0] = bridge_step(cur[0], em[0], alpha, n0, sd);
-                    rn[0] = teleport(cur[0], P);
-                    if (DIM > 1) {
-                        cur[1] = bridge_step(cur[1], em[1], alpha, n1, sd);
-                        rn[1] = teleport(cur[1], P);
-                    }
+                if (m >= 2) {
+                    int start = off + k * ell;
+                    if (start >= M) start -= M;
+                    int jm = start + m;
+                    if (jm >= M) jm -= M;
+                    T cur[DIM], em[DIM];
+                    path.get(p, start, cur);
+                    path.get(p, jm, em);
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) s_new[b * DIM + e] = rn[e];
-                    path.get(p, b, ro);
-                    dv += potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P);
+                    for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
+                    U4 xr{0u, 0u, 0u, 0u};
+                    for (int i = 1; i < m; ++i) {
+                        int b = start + i;
+                        if (b >= M) b -= M;
+                        if (i & 1) xr = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c, kLevy);
+                        T n0, n1;
+                        if (i & 1) box_muller(xr.x, xr.y, n0, n1);
+                        else box_muller(xr.z, xr.w, n0, n1);
+                        const T alpha = bridge_alpha<T>(m, i);
+                        const T sd = bridge_sd(alpha, P);
+                        T rn[DIM], ro[DIM];
+                        cur[0] = bridge_step(cur[0], em[0], alpha, n0, sd);
+                        rn[0] = teleport(cur[0], P);
+                        if (DIM > 1) {
+                            cur[1] = bridge_step(cur[1], em[1], alpha, n1, sd);
+                            rn[1] = teleport(cur[1], P);
+                        }
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) buf[b * DIM + e] = rn[e];
+                        path.get(p, b, ro);
+                        dv += potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P);
+                    }
                 }
                 s_dv[k] = dv;
             }
             __syncthreads();
+            // (B) pair link differences, G lanes per link
             if (use_pair) {
-                for (int j = tid; j < M; j += nt) {
-                    int rel = j - off;
-                    if (rel < 0) rel += M;
-                    const int k = rel / ell, t = rel - k * ell;
-                    const int m = min(ell, M - k * ell);
-                    s_du[j] = T(0);
-                    if (m < 2) continue;
-                    const int jn = (j + 1 == M) ? 0 : j + 1;
-                    T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
-                    path.get(p, j, o0);
-                    path.get(p, jn, o1);
+                for (int j0 = 0; j0 < M; j0 += ngroups) {
+                    const int j = j0 + grp;
+                    T part = T(0);
+                    int t = 0, m = 0;
+                    if (j < M) {
+                        int rel = j - off;
+                        if (rel < 0) rel += M;
+                        const int k = rel / ell;
+                        t = rel - k * ell;
+                        m = min(ell, M - k * ell);
+                        if (m >= 2) {
+                            const int jn = (j + 1 == M) ? 0 : j + 1;
+                            T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
+                            path.get(p, j, o0);
+                            path.get(p, jn, o1);
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) {
-                        n0[e] = (t == 0) ? o0[e] : s_new[j * DIM + e];
-                        n1[e] = (t + 1 == m) ? o1[e] : s_new[jn * DIM + e];
+                            for (int e = 0; e < DIM; ++e) {
+                                n0[e] = (t == 0) ? o0[e] : buf[j * DIM + e];
+                                n1[e] = (t + 1 == m) ? o1[e] : buf[jn * DIM + e];
+                            }
+                            part = link_part<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, n0, n1, g, G) -
+                                   link_part<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, o0, o1, g, G);
+                        }
                     }
-                    s_du[j] = link_sum<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, n0, n1) -
-                              link_sum<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, o0, o1);
+                    part = group_sum(part, G);
+                    if (j < M && g == 0) s_du[j] = part;
                 }
                 __syncthreads();
             }
+            // (C) decisions
             for (int k = tid; k < nseg; k += nt) {
                 const int m = min(ell, M - k * ell);
-                s_flag[k] = 0;
-                if (m < 2) continue;
-                int start = off + k * ell;
-                if (start >= M) start -= M;
-                T lw = -P.tau * s_dv[k];
-                if (use_pair) {
-                    for (int t = 0; t < m; ++t) {
-                        int j = start + t;
-                        if (j >= M) j -= M;
-                        lw += s_du[j];
+                int acc = 0;
+                if (m >= 2) {
+                    int start = off + k * ell;
+                    if (start >= M) start -= M;
+                    T lw = -P.tau * s_dv[k];
+                    if (use_pair) {
+                        for (int t = 0; t < m; ++t) {
+                            int j = start + t;
+                            if (j >= M) j -= M;
+                            lw += s_du[j];
+                        }
                     }
+                    const U4 xa = draw(a.seed, (uint32_t)(p * M + start), sw, (uint32_t)c, kLevyAccept);
+                    acc = metropolis(ex(lw), Uniform<T>::get(xa.x)) ? 1 : 0;
                 }
-                const U4 xa = draw(a.seed, (uint32_t)(p * M + start), sw, (uint32_t)c, kLevyAccept);
-                const int acc = metropolis(ex(lw), Uniform<T>::get(xa.x)) ? 1 : 0;
                 s_flag[k] = acc;
                 lacc += acc;
             }
             __syncthreads();
+            // (D) commit interior beads of accepted segments
             for (int j = tid; j < M; j += nt) {
                 int rel = j - off;
                 if (rel < 0) rel += M;
@@ -743,71 +862,80 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                 if (m >= 2 && t >= 1 && s_flag[k]) {
                     T rn[DIM];
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) rn[e] = s_new[j * DIM + e];
+                    for (int e = 0; e < DIM; ++e) rn[e] = buf[j * DIM + e];
                     path.put(p, j, rn);
                     if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn);
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
         // ---- centre of mass: one particle after another
         for (int p = 0; p < N; ++p) {
+            T *buf = (p & 1) ? s_new1 : s_new0;
             const U4 y = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kCom);
             const T step2 = (T)a.com_step[c] * T(2);
             T d[DIM];
             d[0] = step2 * (Uniform<T>::get(y.x) - T(0.5));
             if (DIM > 1) d[1] = step2 * (Uniform<T>::get(y.y) - T(0.5));
-            double part = 0.0;
+            T part = T(0);
             for (int j = tid; j < M; j += nt) {
                 T ro[DIM], rn[DIM];
                 path.get(p, j, ro);
 #pragma unroll
                 for (int e = 0; e < DIM; ++e) {
                     rn[e] = teleport(ro[e] + d[e], P);
-                    s_new[j * DIM + e] = rn[e];
+                    buf[j * DIM + e] = rn[e];
                 }
-                part += (double)(potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P));
+                part += -P.tau * (potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P));
             }
             __syncthreads();
-            double du = 0.0;
             if (use_pair) {
-                for (int j = tid; j < M; j += nt) {
-                    const int jn = (j + 1 == M) ? 0 : j + 1;
-                    T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
-                    path.get(p, j, o0);
-                    path.get(p, jn, o1);
+                for (int j0 = 0; j0 < M; j0 += ngroups) {
+                    const int j = j0 + grp;
+                    if (j < M) {
+                        const int jn = (j + 1 == M) ? 0 : j + 1;
+                        T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
+                        path.get(p, j, o0);
+                        path.get(p, jn, o1);
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) {
-                        n0[e] = s_new[j * DIM + e];
-                        n1[e] = s_new[jn * DIM + e];
+                        for (int e = 0; e < DIM; ++e) {
+                            n0[e] = buf[j * DIM + e];
+                            n1[e] = buf[jn * DIM + e];
+                        }
+                        part += link_part<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, n0, n1, g, G) -
+                                link_part<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, o0, o1, g, G);
                     }
-                    du += (double)(link_sum<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, n0, n1) -
-                                   link_sum<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, o0, o1));
                 }
             }
-            const double dvs = block_sum(part, s_buf);
-            const double dus = use_pair ? block_sum(du, s_buf) : 0.0;
-            const T lw = -P.tau * (T)dvs + (T)dus;
-            const bool acc = metropolis(ex(lw), Uniform<T>::get(y.z));
+            // workgroup sum of the log-weight change: waves, then wave totals in order
+            double x = (double)part;
+#pragma unroll
+            for (int offs = 32; offs >= 1; offs >>= 1) x += __shfl_xor(x, offs, 64);
+            double *ws = s_wsum + (p & 1) * 16;
+            if (lane == 0) ws[wave] = x;
+            __syncthreads();
+            double lw = 0.0;
+            for (int w = 0; w < nw; ++w) lw += ws[w];
+            const bool acc = metropolis(ex((T)lw), Uniform<T>::get(y.z));
             cacc += (acc && tid == 0) ? 1 : 0;
             if (acc) {
                 for (int j = tid; j < M; j += nt) {
                     T rn[DIM];
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) rn[e] = s_new[j * DIM + e];
+                    for (int e = 0; e < DIM; ++e) rn[e] = buf[j * DIM + e];
                     path.put(p, j, rn);
                     if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn);
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
         if (a.measure) {
             const ChainMeas m = measure_chain<T, DIM, CELLS>(path, cells, ch, P, s_buf, a.radial, a.radial_bins,
-                                                              a.radial_winv, a.cart, a.cart_bins, c);
+                                                              a.radial_winv, a.cart, a.cart_bins, c, 8);
             fold_meas(m, ch, q);
         }
     }
-    // per-launch outputs (thread sums of accept counts reduced deterministically)
+    // per-launch outputs (accept counts reduced deterministically)
     const double la = block_sum((double)lacc, s_buf);
     const double ca = block_sum((double)cacc, s_buf);
     if (tid == 0) {
@@ -909,7 +1037,7 @@ __global__ void k_build_cells(const ChainDev *chains, const int *whole, int n_wh
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ch.M) return;
     const Prm<T> P = Prm<T>::make(ch);
-    const Path<T, DIM> path{const_cast<T *>(reinterpret_cast<const T *>(pos)) + ch.pos_off, ch.M};
+    const Path<T, DIM, true> path{const_cast<T *>(reinterpret_cast<const T *>(pos)) + ch.pos_off, ch.M, ch.N};
     const Cells C{cell_cnt + ch.cell_off, cell_mem + ch.mem_off, bins + ch.bin_off, ch.M,
                   ch.ncell, ch.cap, ch.nbins};
     for (int cc = 0; cc < ch.ncell; ++cc) C.count(j, cc) = 0;
