@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-beads", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep staging length 8 and the default tile")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     ap.add_argument("--workload", default="trapped_batch",
                     choices=["trapped_batch", "trapped_plain", "gauss_box", "trapped_gauss", "lattice"],
                     help="workload of the timed sweeps (the default is the headline batched form)")
@@ -225,9 +227,15 @@ def main():
     spl = args.spl
     wl = trapped_batch(n_chains=args.chains, seed=2, precision=A.FP32)
     t = time.perf_counter()
-    batch = Batch(wl, sweeps_per_launch=spl, seed=20240601, tile_beads=args.tile_beads, block_threads=args.threads)
+    tune = not args.no_autotune and args.tile_beads == 0 and args.threads == 0
+    batch = Batch(wl, sweeps_per_launch=spl, seed=20240601, tile_beads=args.tile_beads, block_threads=args.threads,
+                  autotune=tune, use_graph=not args.no_graph)
     log(f"batched context: {len(wl.descs)} chains, {sum(d.N * d.M for d in wl.descs)} beads, "
         f"setup {time.perf_counter() - t:.1f} s")
+    if batch.tune:
+        ells, counts = np.unique(batch.tune.ell, return_counts=True)
+        log(f"autotune ({batch.tune.seconds:.1f} s): tile_beads {batch.tune.tile_beads}, threads "
+            f"{batch.tune.block_threads}, staging lengths {dict(zip(ells.tolist(), counts.tolist()))}")
     warm = max(spl, (args.warmup // spl) * spl)
     steps = max(spl, (args.steps // spl) * spl)
     batch.run(warm, measure=False)
@@ -258,7 +266,7 @@ def main():
 
     # ---- plain single-chain form
     pw = trapped_plain(seed=1)
-    plain = Batch(pw, sweeps_per_launch=spl, seed=7)
+    plain = Batch(pw, sweeps_per_launch=spl, seed=7, use_graph=not args.no_graph)
     plain.run(max(spl, 500 // spl * spl), measure=False)
     psteps = max(spl, (args.plain_steps // spl) * spl)
     t0 = time.perf_counter()
@@ -293,7 +301,11 @@ def main():
                         "M~U{32,64,128,256}, beta~U[0.5,4], omega=1, FP32 paths, FP64 estimators",
             "chains": len(wl.descs),
             "beads": int(sum(d.N * d.M for d in wl.descs)),
-            "staging_length": 8,
+            "staging_length": ({str(k): int(v) for k, v in zip(*np.unique(batch.tune.ell, return_counts=True))}
+                               if batch.tune else 8),
+            "tile_beads": batch.tune.tile_beads if batch.tune else (args.tile_beads or 4096),
+            "block_threads": batch.tune.block_threads if batch.tune else (args.threads or 256),
+            "hipgraph": not args.no_graph,
             "sweeps_per_launch": spl,
             "bead_moves_per_sweep": int(cnt.bead_moves // steps),
             "parallelism": "single GPU, one workgroup per tile of particles",

@@ -49,7 +49,7 @@ struct Launch {
 
 // A run of consecutive tiles whose chains share one external potential (one launch).
 struct TileGroup {
-    int pot, first, count;
+    int pot, maxl, first, count; // maxl: register slots of the bridge (8, 16 or 32)
 };
 
 } // namespace
@@ -101,6 +101,9 @@ struct pimc_ctx {
     std::vector<Launch> events;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     long long cart_per_chain = 0;
+    RunState *d_rs = nullptr;                   // device sweep/block counters
+    hipGraphExec_t graph[2] = {nullptr, nullptr}; // captured launch step (thermalising, measuring)
+    std::vector<int64_t> levy_tries_chain;      // staging tries per sweep, per chain
     std::vector<double> hblocks; // host copy of the estimator blocks
     int hblk_n = -1;             // blocks held in hblocks (-1: stale)
 };
@@ -137,21 +140,21 @@ template <typename T, int DIM, int MAXL, int POT> void launch_tiles_p(pimc_ctx *
     HIPCHK(hipGetLastError());
 }
 
-template <typename T, int DIM, int MAXL> void launch_tiles_t(pimc_ctx *c, const SweepArgs &a)
+template <typename T, int DIM, int MAXL> void launch_tiles_t(pimc_ctx *c, const SweepArgs &a, const TileGroup &g)
 {
-    for (const TileGroup &g : c->tile_groups) {
-        if (g.pot == PIMC_POT_HARMONIC) launch_tiles_p<T, DIM, MAXL, kPotHarmonic>(c, a, g);
-        else if (g.pot == PIMC_POT_LATTICE) launch_tiles_p<T, DIM, MAXL, kPotLattice>(c, a, g);
-        else launch_tiles_p<T, DIM, MAXL, kPotNone>(c, a, g);
-    }
+    if (g.pot == PIMC_POT_HARMONIC) launch_tiles_p<T, DIM, MAXL, kPotHarmonic>(c, a, g);
+    else if (g.pot == PIMC_POT_LATTICE) launch_tiles_p<T, DIM, MAXL, kPotLattice>(c, a, g);
+    else launch_tiles_p<T, DIM, MAXL, kPotNone>(c, a, g);
 }
 
 template <typename T, int DIM> void launch_tiles_d(pimc_ctx *c, const SweepArgs &a)
 {
-    switch (c->maxl) {
-    case 8: launch_tiles_t<T, DIM, 8>(c, a); break;
-    case 16: launch_tiles_t<T, DIM, 16>(c, a); break;
-    default: launch_tiles_t<T, DIM, 32>(c, a); break;
+    for (const TileGroup &g : c->tile_groups) {
+        switch (g.maxl) {
+        case 8: launch_tiles_t<T, DIM, 8>(c, a, g); break;
+        case 16: launch_tiles_t<T, DIM, 16>(c, a, g); break;
+        default: launch_tiles_t<T, DIM, 32>(c, a, g); break;
+        }
     }
 }
 
@@ -239,6 +242,7 @@ SweepArgs sweep_args(pimc_ctx *c)
     a.radial_winv = c->opt.radial_bins > 0 ? c->opt.radial_bins / c->opt.radial_rmax : 0.0;
     a.cart_bins = c->opt.cart_bins;
     a.seed = c->opt.seed;
+    a.rs = c->d_rs;
     a.cell_cnt = c->d_cell_cnt;
     a.cell_mem = c->d_cell_mem;
     a.bins = c->d_bins;
@@ -247,9 +251,10 @@ SweepArgs sweep_args(pimc_ctx *c)
     return a;
 }
 
-void grow_blocks(pimc_ctx *c, int need)
+// Ensure room for `need` blocks; returns true if the buffer was reallocated.
+bool grow_blocks(pimc_ctx *c, int need)
 {
-    if (need <= c->nblk_cap) return;
+    if (need <= c->nblk_cap) return false;
     int cap = std::max(need, std::max(64, c->nblk_cap * 2));
     double *nb = dalloc<double>((size_t)cap * c->n_chains * kNQ);
     if (c->nblk > 0)
@@ -259,6 +264,7 @@ void grow_blocks(pimc_ctx *c, int need)
     dfree(c->d_blocks);
     c->d_blocks = nb;
     c->nblk_cap = cap;
+    return true;
 }
 
 // Block b of one chain from the host copy, refreshed after each run.
@@ -337,6 +343,44 @@ HookArgs hook_args(pimc_ctx *c, int chain)
         }                                                                                                              \
         HIPCHK(hipGetLastError());                                                                                     \
     } while (0)
+
+// One launch step: the sweep kernels for k sweeps, the fold into counters and
+// blocks, and the advance of the device-side sweep/block counters.
+void launch_step(pimc_ctx *c, int k, int measure)
+{
+    SweepArgs a = sweep_args(c);
+    a.nsweeps = k;
+    a.measure = measure ? 1 : 0;
+    if (!c->tiles.empty()) launch_tiles(c, a);
+    if (!c->whole_ids.empty()) launch_whole(c, a);
+    PostArgs pa{};
+    pa.chains = c->d_ch;
+    pa.n_chains = c->n_chains;
+    pa.is_whole = c->d_is_whole;
+    pa.part_acc = c->d_part_acc;
+    pa.part_lacc = c->d_placc;
+    pa.part_cacc = c->d_pcacc;
+    pa.chain_meas = c->d_chain_meas;
+    pa.com_step = c->d_step;
+    pa.counters = c->d_cnt;
+    pa.blocks = c->d_blocks;
+    pa.rs = c->d_rs;
+    pa.measure = measure ? 1 : 0;
+    pa.nsweeps = k;
+    hipLaunchKernelGGL(k_post, dim3((c->n_chains + 127) / 128), dim3(128), 0, c->stream, pa);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_rs, k, measure ? 1 : 0);
+    HIPCHK(hipGetLastError());
+}
+
+void drop_graphs(pimc_ctx *c)
+{
+    for (auto &g : c->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
 
 int validate(const pimc_chain_desc &d, std::string &why)
 {
@@ -462,6 +506,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
             if (c->is_whole[x] != c->is_whole[y]) return c->is_whole[x] < c->is_whole[y];
             if (c->ch[x].pot != c->ch[y].pot) return c->ch[x].pot < c->ch[y].pot;
+            const int lx = next_pow2_maxl(c->ch[x].ell), ly = next_pow2_maxl(c->ch[y].ell);
+            if (lx != ly) return lx < ly;
             if (c->ch[x].M != c->ch[y].M) return c->ch[x].M > c->ch[y].M;
             return c->ch[x].ell < c->ch[y].ell;
         });
@@ -526,6 +572,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 if (!fresh) {
                     TileDev &t = c->tiles.back();
                     fresh = t.M != h.M || t.ell != h.ell || c->tile_groups.back().pot != h.pot ||
+                            c->tile_groups.back().maxl != next_pow2_maxl(h.ell) ||
                             (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
                 }
                 if (fresh) {
@@ -536,8 +583,9 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                     t.M = h.M;
                     t.ell = h.ell;
                     t.nseg = h.nseg;
-                    if (c->tile_groups.empty() || c->tile_groups.back().pot != h.pot)
-                        c->tile_groups.push_back(TileGroup{h.pot, (int)c->tiles.size(), 0});
+                    if (c->tile_groups.empty() || c->tile_groups.back().pot != h.pot ||
+                        c->tile_groups.back().maxl != next_pow2_maxl(h.ell))
+                        c->tile_groups.push_back(TileGroup{h.pot, next_pow2_maxl(h.ell), (int)c->tiles.size(), 0});
                     c->tile_groups.back().count += 1;
                     c->tiles.push_back(t);
                 }
@@ -576,6 +624,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 }
             }
             c->levy_tries_per_sweep += (int64_t)h.N * segs;
+            c->levy_tries_chain.push_back((int64_t)h.N * segs);
             c->com_tries_per_sweep += h.N;
             c->moves_per_sweep += (int64_t)h.N * (per + h.M);
         }
@@ -639,6 +688,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         c->d_cell_cnt = dalloc<int>((size_t)c->n_cell_cnt);
         c->d_cell_mem = dalloc<unsigned short>((size_t)mem_slots);
         c->d_bins = dalloc<int>((size_t)c->n_bins);
+        c->d_rs = dalloc<RunState>(1);
+        HIPCHK(hipMemset(c->d_rs, 0, sizeof(RunState)));
         c->d_status = dalloc<int>(n_chains);
         HIPCHK(hipMemset(c->d_status, 0, sizeof(int) * n_chains));
         int maxM = 0;
@@ -668,6 +719,8 @@ int pimc_destroy(pimc_ctx *c)
 {
     if (!c) return PIMC_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    dfree(c->d_rs);
     void *ptrs[] = {c->d_ch, c->d_prm, c->d_tiles, c->d_pchain, c->d_pidx, c->d_pos, c->d_part_acc, c->d_placc, c->d_pcacc,
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
@@ -797,7 +850,7 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
         const int K = c->opt.sweeps_per_launch;
         const int nl = (n_sweeps + K - 1) / K;
         c->hblk_n = -1;
-        if (measure) grow_blocks(c, c->nblk + nl);
+        if (measure && grow_blocks(c, c->nblk + nl)) drop_graphs(c); // the block buffer moved
         while ((int)c->events.size() < nl) {
             Launch l;
             HIPCHK(hipEventCreate(&l.a));
@@ -808,30 +861,23 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
         int left = n_sweeps, li = 0;
         while (left > 0) {
             const int k = std::min(K, left);
-            SweepArgs a = sweep_args(c);
-            a.sweep0 = c->sweep;
-            a.nsweeps = k;
-            a.measure = measure ? 1 : 0;
             HIPCHK(hipEventRecord(c->events[li].a, c->stream));
-            if (!c->tiles.empty()) launch_tiles(c, a);
-            if (!c->whole_ids.empty()) launch_whole(c, a);
+            hipGraphExec_t &ge = c->graph[measure ? 1 : 0];
+            if (c->opt.use_graph && k == K && ge) {
+                HIPCHK(hipGraphLaunch(ge, c->stream));
+            } else if (c->opt.use_graph && k == K && li > 0) {
+                // capture one full launch step once (after an eager step has run) and replay it
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                launch_step(c, k, measure);
+                HIPCHK(hipStreamEndCapture(c->stream, &g));
+                HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                HIPCHK(hipGraphDestroy(g));
+                HIPCHK(hipGraphLaunch(ge, c->stream));
+            } else {
+                launch_step(c, k, measure);
+            }
             HIPCHK(hipEventRecord(c->events[li].b, c->stream));
-            PostArgs pa{};
-            pa.chains = c->d_ch;
-            pa.n_chains = c->n_chains;
-            pa.is_whole = c->d_is_whole;
-            pa.part_acc = c->d_part_acc;
-            pa.part_lacc = c->d_placc;
-            pa.part_cacc = c->d_pcacc;
-            pa.chain_meas = c->d_chain_meas;
-            pa.com_step = c->d_step;
-            pa.counters = c->d_cnt;
-            pa.blocks = c->d_blocks;
-            pa.blk = c->nblk;
-            pa.measure = measure ? 1 : 0;
-            pa.nsweeps = k;
-            hipLaunchKernelGGL(k_post, dim3((c->n_chains + 127) / 128), dim3(128), 0, c->stream, pa);
-            HIPCHK(hipGetLastError());
             if (measure) c->nblk++;
             c->sweep += (uint32_t)k;
             left -= k;
@@ -868,6 +914,25 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
     } catch (const HipError &e) {
         return hip_err(c, e);
     }
+}
+
+int pimc_get_counters(pimc_ctx *c, int32_t chain, int64_t *out)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!out) return set_err(c, PIMC_ERR_ARG, "null output");
+    try {
+        long long acc[2];
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(acc, c->d_cnt + (size_t)chain * 2, sizeof(acc), hipMemcpyDeviceToHost));
+        out[0] = acc[0];
+        out[1] = c->levy_tries_chain[chain] * (int64_t)c->sweep;
+        out[2] = acc[1];
+        out[3] = (int64_t)c->ch[chain].N * (int64_t)c->sweep;
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
 }
 
 int pimc_get_blocks(pimc_ctx *c, int32_t chain, double *E, double *Ev, double *Epair, int64_t cap, int64_t *n_blocks)
@@ -914,17 +979,37 @@ int pimc_get_estimates(pimc_ctx *c, int32_t chain, pimc_estimates *out)
         const double n = tot[qN];
         out->n_meas = (int64_t)n;
         if (n <= 0) return PIMC_OK;
+        // Standard error by blocking (Flyvbjerg & Petersen, J. Chem. Phys. 91, 461 (1989)):
+        // the launch blocks are merged pairwise level by level, and the largest error over
+        // the levels that keep at least 8 blocks is kept, so correlations longer than one
+        // launch do not shrink the error bar.
         auto mean_err = [&](int k, double &m, double &e) {
             m = tot[k] / n;
             e = 0.0;
             if (nb < 2) return;
-            double s = 0.0;
+            std::vector<double> sum(nb), cnt(nb);
             for (int b = 0; b < nb; ++b) {
-                const double nbk = q[(size_t)b * kNQ + qN];
-                const double x = nbk > 0 ? q[(size_t)b * kNQ + k] / nbk : m;
-                s += (x - m) * (x - m);
+                sum[b] = q[(size_t)b * kNQ + k];
+                cnt[b] = q[(size_t)b * kNQ + qN];
             }
-            e = std::sqrt(s / (nb - 1) / nb);
+            int len = nb;
+            while (len >= 2) {
+                double s = 0.0;
+                int used = 0;
+                for (int b = 0; b < len; ++b) {
+                    if (cnt[b] <= 0) continue;
+                    const double x = sum[b] / cnt[b];
+                    s += (x - m) * (x - m);
+                    ++used;
+                }
+                if (used >= 2) e = std::max(e, std::sqrt(s / (used - 1) / used));
+                if (len / 2 < 8) break;
+                for (int b = 0; b < len / 2; ++b) {
+                    sum[b] = sum[2 * b] + sum[2 * b + 1];
+                    cnt[b] = cnt[2 * b] + cnt[2 * b + 1];
+                }
+                len /= 2;
+            }
         };
         mean_err(qE, out->E, out->E_err);
         mean_err(qEv, out->Ev, out->Ev_err);
@@ -1016,6 +1101,12 @@ int pimc_reset_estimators(pimc_ctx *c)
         HIPCHK(hipStreamSynchronize(c->stream));
         c->nblk = 0;
         c->hblk_n = -1;
+        {
+            RunState rs{};
+            HIPCHK(hipMemcpy(&rs, c->d_rs, sizeof(RunState), hipMemcpyDeviceToHost));
+            rs.blk = 0;
+            HIPCHK(hipMemcpy(c->d_rs, &rs, sizeof(RunState), hipMemcpyHostToDevice));
+        }
         if (c->d_radial) HIPCHK(hipMemset(c->d_radial, 0, sizeof(unsigned long long) * c->n_chains * c->opt.radial_bins));
         if (c->d_cart) HIPCHK(hipMemset(c->d_cart, 0, sizeof(unsigned long long) * c->n_chains * c->cart_per_chain));
     } catch (const HipError &e) {

@@ -39,6 +39,23 @@ enum BlockQ {
     qWhole = 11, // 1 if the products above were measured on the whole chain
 };
 
+// Sweep and block counters of a context, kept on the device and advanced by
+// k_advance after each launch, so the launch arguments never change and a
+// captured launch sequence can be replayed.
+struct RunState {
+    unsigned sweep;
+    int blk;
+    int pad0, pad1;
+};
+
+__global__ void k_advance(RunState *rs, int nsweeps, int measure)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        rs->sweep += (unsigned)nsweeps;
+        if (measure) rs->blk += 1;
+    }
+}
+
 struct TileDev {
     long long pos_off; // element offset of the tile's first particle
     int q0, np, M, nseg;
@@ -64,7 +81,7 @@ struct SweepArgs {
     double radial_winv;
     int radial_bins, cart_bins;
     uint64_t seed;
-    unsigned sweep0;
+    const struct RunState *rs; // first sweep of this launch (device side, so a captured graph replays)
     int nsweeps, measure;
     int tile0;             // first tile of this launch (tiles are grouped by potential kind)
     int link_group;        // lanes per link in the whole-chain kernel (power of two <= 64)
@@ -174,6 +191,7 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
     const int tid = threadIdx.x, nt = blockDim.x;
     const int np = tl.np, M = tl.M, nseg = tl.nseg, ell = tl.ell, items = tl.items;
     const LdsRow R{tile_row_len(M, tl.sh), tl.sh};
+    const uint32_t sweep0 = a.rs->sweep;
     const int pstride = DIM * R.Mp; // LDS elements per particle
     const int rbins = a.radial_bins;
     // item it = (particle q, segment k) with k fastest; thread tid visits it = tid + r nt,
@@ -236,7 +254,7 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
     __syncthreads();
 
     for (int s = 0; s < a.nsweeps; ++s) {
-        const uint32_t sw = a.sweep0 + (uint32_t)s;
+        const uint32_t sw = sweep0 + (uint32_t)s;
         // per-particle draws of this sweep: staging offset, COM displacement and uniform
         for (int q = tid; q < np; q += nt) {
             const int gq = tl.q0 + q;
@@ -730,6 +748,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, M, ch.ncell, ch.cap,
                       ch.nbins};
     const bool use_pair = (ch.pair != kPairNone) && ch.pair_in_accept;
+    const uint32_t sweep0 = a.rs->sweep;
     const int G = a.link_group, ngroups = nt / G, g = tid & (G - 1), grp = tid / G;
     const int lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
 
@@ -754,7 +773,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     int lacc = 0, cacc = 0, overflow = 0;
 
     for (int s = 0; s < a.nsweeps; ++s) {
-        const uint32_t sw = a.sweep0 + (uint32_t)s;
+        const uint32_t sw = sweep0 + (uint32_t)s;
         // ---- staging: particles in order, segments (and links) of one particle in parallel
         for (int p = 0; p < N; ++p) {
             T *buf = (p & 1) ? s_new1 : s_new0;
@@ -962,7 +981,8 @@ struct PostArgs {
     double *com_step;
     long long *counters; // per chain: levy accepted, COM accepted
     double *blocks;      // [blk][chain][kNQ]
-    int blk, measure, nsweeps;
+    const RunState *rs; // block index of this launch
+    int measure, nsweeps;
 };
 
 __global__ void k_post(PostArgs a)
@@ -1006,7 +1026,7 @@ __global__ void k_post(PostArgs a)
     a.counters[(size_t)c * 2 + 0] += la;
     a.counters[(size_t)c * 2 + 1] += ca;
     if (a.measure) {
-        double *b = a.blocks + ((size_t)a.blk * a.n_chains + c) * kNQ;
+        double *b = a.blocks + ((size_t)a.rs->blk * a.n_chains + c) * kNQ;
         for (int k = 0; k < kNQ; ++k) b[k] = q[k];
     } else {
         const long long tries = (long long)ch.N * a.nsweeps;

@@ -21,7 +21,7 @@ FP32, FP64 = 0, 1
 EXPORTS = [
     "pimc_create", "pimc_destroy", "pimc_last_error", "pimc_device_info",
     "pimc_set_paths", "pimc_get_paths", "pimc_get_com_step", "pimc_set_com_step",
-    "pimc_run", "pimc_get_estimates", "pimc_get_blocks", "pimc_get_radial", "pimc_get_cartesian",
+    "pimc_run", "pimc_get_estimates", "pimc_get_counters", "pimc_get_blocks", "pimc_get_radial", "pimc_get_cartesian",
     "pimc_reset_estimators", "pimc_levy_propose", "pimc_reshape_linear", "pimc_com_propose",
     "pimc_center_of_mass", "pimc_interaction", "pimc_energy", "pimc_density", "pimc_adjoint", "pimc_action",
 ]
@@ -96,6 +96,7 @@ def load(path: str = LIB_PATH):
     L.pimc_set_com_step.argtypes = [vp, C.c_int32, C.c_double]
     L.pimc_run.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(Counters)]
     L.pimc_get_estimates.argtypes = [vp, C.c_int32, C.POINTER(Estimates)]
+    L.pimc_get_counters.argtypes = [vp, C.c_int32, _I64]
     L.pimc_get_blocks.argtypes = [vp, C.c_int32, _D, _D, _D, C.c_int64, _I64]
     L.pimc_get_radial.argtypes = [vp, C.c_int32, _D, _I64]
     L.pimc_get_cartesian.argtypes = [vp, C.c_int32, _D, _I64]
@@ -183,6 +184,12 @@ class Context:
         e = Estimates()
         self.check(self.lib.pimc_get_estimates(self.h, chain, C.byref(e)), "pimc_get_estimates")
         return e
+
+    def counters(self, chain: int) -> np.ndarray:
+        """{staging accepted, staging tries, COM accepted, COM tries} of one chain."""
+        out = np.zeros(4, dtype=np.int64)
+        self.check(self.lib.pimc_get_counters(self.h, chain, out.ctypes.data_as(_I64)), "pimc_get_counters")
+        return out
 
     def blocks(self, chain: int):
         n = C.c_int64()

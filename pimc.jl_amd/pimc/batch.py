@@ -109,10 +109,23 @@ def lattice(n_chains: int = 64, seed: int = 5, precision: int = FP32, M: int = 5
 
 
 class Batch:
-    """Many independent chains in one context (the batched HIPSweep form)."""
+    """Many independent chains in one context (the batched HIPSweep form).
+
+    autotune=True picks the staging length per chain and the tile and
+    workgroup sizes first (pimc.autotune); use_graph=True replays the launch
+    sequence from a captured hipGraph."""
 
     def __init__(self, wl: Workload, sweeps_per_launch: int = 50, seed: int = 12345, tile_beads: int = 0,
-                 block_threads: int = 0):
+                 block_threads: int = 0, autotune: bool = False, use_graph: bool = False, tune_sweeps: int = 100):
+        import copy
+        descs = [copy.copy(d) for d in wl.descs]
+        self.tune = None
+        if autotune:
+            from .autotune import autotune as _tune
+            self.tune = _tune(wl, trial_sweeps=tune_sweeps)
+            for d, ell in zip(descs, self.tune.ell):
+                d.ell = ell
+            tile_beads, block_threads = self.tune.tile_beads, self.tune.block_threads
         opt = Options()
         opt.precision = wl.precision
         opt.sweeps_per_launch = sweeps_per_launch
@@ -122,9 +135,11 @@ class Batch:
         opt.derivatives = wl.derivatives
         opt.tile_beads = tile_beads
         opt.block_threads = block_threads
+        opt.use_graph = 1 if use_graph else 0
         opt.seed = seed
         self.wl = wl
-        self.ctx = Context(wl.descs, opt)
+        self.descs = descs
+        self.ctx = Context(descs, opt)
         for c, r in enumerate(wl.init):
             self.ctx.set_paths(c, r)
 

@@ -306,3 +306,54 @@ def test_batch_fp32_runs_and_counts():
     assert 0.3 < c2.levy_acc / c2.levy_tries <= 1.0
     e = b.estimates(0)
     assert e.n_blocks == 2 and np.isfinite(e.E)
+
+
+def test_graph_replay_matches_eager_launches():
+    """The captured launch step replayed from a hipGraph gives the same paths,
+    counters and estimator blocks as eager launches, replay after replay, with
+    the program's own copies and synchronisations in between."""
+    from pimc.batch import trapped_batch
+    wl = trapped_batch(n_chains=48, seed=21, precision=A.FP32)
+    ctxs = []
+    for g in (0, 1):
+        opt = A.Options()
+        opt.precision = A.FP32
+        opt.sweeps_per_launch = 5
+        opt.seed = 77
+        opt.radial_bins = 16
+        opt.radial_rmax = 4.0
+        opt.use_graph = g
+        ctx = A.Context(wl.descs, opt)
+        for c, r in enumerate(wl.init):
+            ctx.set_paths(c, r)
+        ctxs.append(ctx)
+    for rep in range(4):
+        measure = rep % 2 == 1
+        cnts = [ctx.run(15, measure=measure) for ctx in ctxs]
+        assert cnts[0].levy_acc == cnts[1].levy_acc and cnts[0].com_acc == cnts[1].com_acc, rep
+        for c in (0, 17, 47):
+            assert np.array_equal(ctxs[0].get_paths(c), ctxs[1].get_paths(c)), (rep, c)
+            assert np.array_equal(ctxs[0].counters(c), ctxs[1].counters(c))
+    for c in (0, 17, 47):
+        e0, e1 = ctxs[0].blocks(c), ctxs[1].blocks(c)
+        assert all(np.array_equal(x, y) for x, y in zip(e0, e1))
+        assert np.array_equal(ctxs[0].radial(c)[0], ctxs[1].radial(c)[0])
+    d = wl.descs[0]
+    segs = sum(1 for k in range(-(-d.M // d.ell)) if min(d.ell, d.M - k * d.ell) >= 2)
+    assert ctxs[0].counters(0)[1] == 60 * d.N * segs and ctxs[0].counters(0)[3] == 60 * d.N
+
+
+def test_cell_list_overflow_is_reported():
+    """All particles of a pair chain in one cell at every slice: set_paths
+    reports PIMC_ERR_CAPACITY for that chain instead of sampling with a
+    truncated list."""
+    pair = GaussianPair(1.0, 0.5)
+    d = chain_desc(NoPotential(), dim=2, M=8, N=200, L=8.0, beta=2.0, lam=0.5, pair=pair, ell=4)
+    ctx = make_ctx([d], A.FP32)
+    r = np.zeros((200, 2, 8)) + 0.1
+    with pytest.raises(A.PimcError) as e:
+        ctx.set_paths(0, r)
+    assert "CAPACITY" in str(e.value)
+    ok = np.random.default_rng(1).uniform(-8, 8, size=(200, 2, 8))
+    ctx.set_paths(0, ok)   # a valid configuration clears the flag
+    ctx.run(2, measure=False)
