@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--tile-beads", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true", help="keep staging length 8 and the default tile")
+    ap.add_argument("--ell", type=int, default=0,
+                    help="staging length of every chain (capped at M); with --tile-beads and --threads this "
+                         "fixes the configuration the autotuner would otherwise pick")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     ap.add_argument("--workload", default="trapped_batch",
                     choices=["trapped_batch", "trapped_plain", "gauss_box", "trapped_gauss", "lattice"],
@@ -226,8 +229,11 @@ def main():
     # ---- batched workload
     spl = args.spl
     wl = trapped_batch(n_chains=args.chains, seed=2, precision=A.FP32)
+    if args.ell > 0:
+        for d in wl.descs:
+            d.ell = min(args.ell, d.M)
     t = time.perf_counter()
-    tune = not args.no_autotune and args.tile_beads == 0 and args.threads == 0
+    tune = not args.no_autotune and args.tile_beads == 0 and args.threads == 0 and args.ell == 0
     batch = Batch(wl, sweeps_per_launch=spl, seed=20240601, tile_beads=args.tile_beads, block_threads=args.threads,
                   autotune=tune, use_graph=not args.no_graph)
     log(f"batched context: {len(wl.descs)} chains, {sum(d.N * d.M for d in wl.descs)} beads, "
@@ -256,7 +262,9 @@ def main():
         try:
             with open(tp) as f:
                 tj = json.load(f)
-            if tj.get("sweeps_per_launch") == spl and tj.get("chains") == args.chains:
+            if tj.get("sweeps_per_launch") == spl and tj.get("chains") == args.chains \
+                    and tj.get("staging_length") == (batch.tune.ell[0] if batch.tune and len(set(batch.tune.ell)) == 1
+                                                     else args.ell or None):
                 traffic = tj.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
@@ -302,7 +310,7 @@ def main():
             "chains": len(wl.descs),
             "beads": int(sum(d.N * d.M for d in wl.descs)),
             "staging_length": ({str(k): int(v) for k, v in zip(*np.unique(batch.tune.ell, return_counts=True))}
-                               if batch.tune else 8),
+                               if batch.tune else (args.ell or 8)),
             "tile_beads": batch.tune.tile_beads if batch.tune else (args.tile_beads or 4096),
             "block_threads": batch.tune.block_threads if batch.tune else (args.threads or 256),
             "hipgraph": not args.no_graph,

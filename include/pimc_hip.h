@@ -76,7 +76,7 @@ typedef struct pimc_options {
     int32_t cart_bins;         /* the reference's Cartesian Density per chain (0: off) */
     double radial_rmax;        /* radial histogram range [0, rmax) */
     int32_t tile_beads;        /* beads staged per workgroup (0: autotuned) */
-    int32_t block_threads;     /* workgroup size (0: autotuned) */
+    int32_t block_threads;     /* tile-kernel workgroup size, a multiple of 64 and <= 256 (0: 256) */
     int32_t use_graph;         /* replay launch sequences from a captured hipGraph */
     int32_t derivatives;       /* run every chain whole per workgroup and record the
                                   reweighting sums for d<E>/d omega, d<E>/d g */

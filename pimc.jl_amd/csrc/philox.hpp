@@ -24,6 +24,16 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
+// a ^ b ^ c; on the device one v_bitop3_b32 (truth table 0x96) instead of two XORs.
+__host__ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 __host__ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
 {
     constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
@@ -34,7 +44,7 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t
         const uint64_t p1 = (uint64_t)M1 * c.z;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        c = U4{xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0};
         k0 += W0;
         k1 += W1;
     }

@@ -42,6 +42,7 @@ template <typename T> struct Prm {
     T halfw2, w2;        // omega^2 / 2, omega^2
     T V0, inva;          // lattice depth, 1/a
     T g, inv2s2, h2;     // pair strength, 1/(2 sigma^2), cut-off radius^2
+    T sdl;               // sqrt(2 lambda tau): free-particle displacement of one link
     int pot, pair;
 
     __host__ __device__ __forceinline__ static Prm make(const ChainDev &c)
@@ -59,6 +60,7 @@ template <typename T> struct Prm {
         p.g = (T)c.g;
         p.inv2s2 = (T)(1.0 / (2.0 * c.sigma * c.sigma));
         p.h2 = (T)(c.h * c.h);
+        p.sdl = (T)sqrt(2.0 * c.lambda * c.tau);
         p.pot = c.pot;
         p.pair = c.pair;
         return p;
@@ -204,6 +206,28 @@ template <typename T> __device__ __forceinline__ T bridge_sd(T alpha, const Prm<
 {
     return sq(((T(2) * p.lambda) * alpha) * p.tau);
 }
+
+// Compile-time bridge coefficients of a segment of exactly MAXL links:
+// alpha_i = (MAXL - i)/(MAXL - i + 1) and sqrt(alpha_i), i = 1..MAXL-1, rounded
+// once from double (sd_i = sqrt(2 lambda tau) sqrt(alpha_i)).
+constexpr double csqrt(double x)
+{
+    double r = 1.0;
+    for (int k = 0; k < 60; ++k) r = 0.5 * (r + x / r);
+    return r;
+}
+template <int MAXL> struct BridgeConst {
+    float alpha[MAXL + 1] = {}, root[MAXL + 1] = {};
+    constexpr BridgeConst()
+    {
+        for (int i = 1; i < MAXL; ++i) {
+            const double al = double(MAXL - i) / double(MAXL - i + 1);
+            alpha[i] = float(al);
+            root[i] = float(csqrt(al));
+        }
+    }
+};
+template <int MAXL> constexpr BridgeConst<MAXL> kBridge{};
 
 template <typename T> __device__ __forceinline__ T bridge_step(T prev, T em, T alpha, T xi, T sd)
 {

@@ -558,7 +558,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             }
         const int ts = (int)tsize(c);
         c->threads_tile = opt->block_threads > 0 ? opt->block_threads : 256;
-        if (c->threads_tile % 64 != 0 || c->threads_tile > 1024) throw HipError{hipErrorInvalidValue, "block_threads must be a multiple of 64, <= 1024"};
+        if (c->threads_tile % 64 != 0 || c->threads_tile > kTileMaxThreads)
+            throw HipError{hipErrorInvalidValue, "block_threads must be a multiple of 64, <= 256"};
         int tile_beads = opt->tile_beads > 0 ? opt->tile_beads : (ts == 4 ? 4096 : 2048);
         const int max_items = kItemsPerThread * c->threads_tile;
         size_t lds_max = 0;

@@ -20,7 +20,8 @@
 
 namespace pimc {
 
-constexpr int kItemsPerThread = 4; // register slots of the estimator sums per thread
+constexpr int kItemsPerThread = 2; // register slots of the estimator sums per thread
+constexpr int kTileMaxThreads = 256; // workgroup size bound of the tile kernel
 constexpr int kNQ = 12;            // per-chain per-block estimator quantities
 
 // block accumulator layout (doubles), per (block, chain)
@@ -182,8 +183,76 @@ __device__ __forceinline__ bool levy_segment(T *xp, const LdsRow &R, int M, int 
     return acc;
 }
 
+// FP32 staging segment of exactly MAXL links: the draws, the bridge and the
+// Metropolis test of levy_segment, with two changes that only touch
+// rounding.  The bridge coefficients are compile-time constants (kBridge),
+// and the periodic wrap is applied only when a new bead of the segment left
+// [-L, L): inside the box teleport is the identity, and its FP32 evaluation
+// ((x + L) - ...) - L would only round x to the spacing of L.
+template <int DIM, int MAXL, int POT>
+__device__ __forceinline__ bool levy_full(float *xp, const LdsRow &R, int M, int start, const Prm<float> &P,
+                                          uint64_t seed, uint32_t item0, uint32_t sweep, uint32_t chain)
+{
+    int jm = start + MAXL;
+    if (jm >= M) jm -= M;
+    float cur[DIM], em[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        cur[d] = xp[R.at(d, start)];
+        em[d] = shift_end(cur[d], xp[R.at(d, jm)], P);
+    }
+    float nb[MAXL - 1][DIM];
+    float vn = 0.0f, vo = 0.0f, mx = 0.0f;
+    U4 xr{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int i = 1; i < MAXL; ++i) {
+        int b = start + i;
+        if (b >= M) b -= M;
+        if (i & 1) xr = draw(seed, item0 + (uint32_t)b, sweep, chain, kLevy);
+        float n[2];
+        if (i & 1) box_muller(xr.x, xr.y, n[0], n[1]);
+        else box_muller(xr.z, xr.w, n[0], n[1]);
+        const float al = kBridge<MAXL>.alpha[i], sd = P.sdl * kBridge<MAXL>.root[i];
+        float old[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            cur[d] = bridge_step(cur[d], em[d], al, n[d], sd);
+            nb[i - 1][d] = cur[d];
+            mx = fmaxf(mx, fabsf(cur[d]));
+            old[d] = xp[R.at(d, b)];
+        }
+        vn += potVk<float, DIM, POT>(nb[i - 1], P);
+        vo += potVk<float, DIM, POT>(old, P);
+    }
+    if (mx >= P.L) {
+        vn = 0.0f;
+#pragma unroll
+        for (int i = 1; i < MAXL; ++i) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) nb[i - 1][d] = teleport(nb[i - 1][d], P);
+            vn += potVk<float, DIM, POT>(nb[i - 1], P);
+        }
+    }
+    const float lw = -P.tau * (vn - vo);
+    const U4 xa = draw(seed, item0 + (uint32_t)start, sweep, chain, kLevyAccept);
+    const bool acc = metropolis(ex(lw), Uniform<float>::get(xa.x));
+    if (acc) {
+#pragma unroll
+        for (int i = 1; i < MAXL; ++i) {
+            int b = start + i;
+            if (b >= M) b -= M;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) xp[R.at(d, b)] = nb[i - 1][d];
+        }
+    }
+    return acc;
+}
+
+// At most 256 threads and at least 3 waves per SIMD: the register budget
+// (168 VGPRs) that keeps the MAXL = 16 bridge in registers without spilling
+// while LDS, not registers, limits how many tiles share a CU.
 template <typename T, int DIM, int MAXL, int POT>
-__global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
+__global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_per_eu(3))) k_sweep_tiles(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const TileDev tl = a.tiles[a.tile0 + blockIdx.x];
@@ -194,6 +263,10 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
     const uint32_t sweep0 = a.rs->sweep;
     const int pstride = DIM * R.Mp; // LDS elements per particle
     const int rbins = a.radial_bins;
+    constexpr bool kF32 = std::is_same<T, float>::value;
+    // FP32 fast paths when every segment is one skew block of the LDS rows
+    // (ell == MAXL, both powers of two, and ell divides M)
+    const bool aligned = kF32 && ell == MAXL && (M & (MAXL - 1)) == 0;
     // item it = (particle q, segment k) with k fastest; thread tid visits it = tid + r nt,
     // so (q, k) advance by (nt / nseg, nt % nseg) per step (no divide in the loops)
     const int dq = nt / nseg, dk = nt - (nt / nseg) * nseg;
@@ -279,21 +352,64 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
             const Prm<T> P = prm[c];
             int start = s_off[q] + k * ell;
             if (start >= M) start -= M;
-            if (levy_segment<T, DIM, MAXL, POT>(sp + (size_t)q * pstride, R, M, start, m, P, a.seed, (uint32_t)(p * M), sw,
-                                           (uint32_t)c))
-                atomicAdd(&s_lacc[q], 1);
+            bool acc;
+            if constexpr (kF32) {
+                if (m == MAXL)
+                    acc = levy_full<DIM, MAXL, POT>(sp + (size_t)q * pstride, R, M, start, P, a.seed,
+                                                    (uint32_t)(p * M), sw, (uint32_t)c);
+                else
+                    acc = levy_segment<T, DIM, MAXL, POT>(sp + (size_t)q * pstride, R, M, start, m, P, a.seed,
+                                                          (uint32_t)(p * M), sw, (uint32_t)c);
+            } else {
+                acc = levy_segment<T, DIM, MAXL, POT>(sp + (size_t)q * pstride, R, M, start, m, P, a.seed,
+                                                      (uint32_t)(p * M), sw, (uint32_t)c);
+            }
+            if (acc) atomicAdd(&s_lacc[q], 1);
         }
         __syncthreads();
 
         // centre of mass: partial Delta V per chunk of `ell` beads
-        for (int it = tid, q = q_first, k = k_first; it < items;
-             it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk) {
+        unsigned wrap = 0u; // bit r: chunk r of this thread has a moved bead outside the box
+        for (int it = tid, q = q_first, k = k_first, r = 0; it < items;
+             it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk, ++r) {
             const int j0 = k * ell, j1 = min(M, j0 + ell);
             const Prm<T> P = prm[a.part_chain[tl.q0 + q]];
             const T *xp = sp + (size_t)q * pstride;
             T d[DIM];
 #pragma unroll
             for (int e = 0; e < DIM; ++e) d[e] = s_d[q * DIM + e];
+            if constexpr (kF32) {
+                if (aligned) {
+                    // the chunk is one skew block: slice k MAXL + jj sits at k (MAXL + 1) + jj
+                    const T *x0 = xp + k * (MAXL + 1);
+                    T vn = T(0), vo = T(0), mx = T(0);
+#pragma unroll
+                    for (int jj = 0; jj < MAXL; ++jj) {
+                        T ro[DIM], rn[DIM];
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) {
+                            ro[e] = x0[e * R.Mp + jj];
+                            rn[e] = ro[e] + d[e];
+                            mx = fmaxf(mx, fabsf(rn[e]));
+                        }
+                        vn += potVk<T, DIM, POT>(rn, P);
+                        vo += potVk<T, DIM, POT>(ro, P);
+                    }
+                    if (mx >= P.L) {
+                        wrap |= 1u << r;
+                        vn = T(0);
+#pragma unroll
+                        for (int jj = 0; jj < MAXL; ++jj) {
+                            T rn[DIM];
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) rn[e] = teleport(x0[e * R.Mp + jj] + d[e], P);
+                            vn += potVk<T, DIM, POT>(rn, P);
+                        }
+                    }
+                    s_red[it] = vn - vo;
+                    continue;
+                }
+            }
             T dv = T(0);
             for (int j = j0; j < j1; ++j) {
                 T ro[DIM], rn[DIM];
@@ -317,12 +433,32 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
         }
         __syncthreads();
 
-        for (int it = tid, q = q_first, k = k_first; it < items;
-             it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk) {
+        for (int it = tid, q = q_first, k = k_first, r = 0; it < items;
+             it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk, ++r) {
             if (!s_flag[q]) continue;
             const int j0 = k * ell, j1 = min(M, j0 + ell);
             const Prm<T> P = prm[a.part_chain[tl.q0 + q]];
             T *xp = sp + (size_t)q * pstride;
+            if constexpr (kF32) {
+                if (aligned) {
+                    T *x0 = xp + k * (MAXL + 1);
+                    T d[DIM];
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) d[e] = s_d[q * DIM + e];
+                    if (!((wrap >> r) & 1u)) {
+#pragma unroll
+                        for (int jj = 0; jj < MAXL; ++jj)
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) x0[e * R.Mp + jj] += d[e];
+                    } else {
+#pragma unroll
+                        for (int jj = 0; jj < MAXL; ++jj)
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) x0[e * R.Mp + jj] = teleport(x0[e * R.Mp + jj] + d[e], P);
+                    }
+                    continue;
+                }
+            }
             for (int j = j0; j < j1; ++j) {
 #pragma unroll
                 for (int e = 0; e < DIM; ++e) xp[R.at(e, j)] = teleport(xp[R.at(e, j)] + s_d[q * DIM + e], P);
@@ -344,16 +480,12 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
                     const T *xp = sp + (size_t)q * pstride;
                     unsigned *hist = s_hist + s_slot[q] * rbins;
                     T l = T(0), v = T(0), g = T(0);
-                    T r0[DIM];
-#pragma unroll
-                    for (int e = 0; e < DIM; ++e) r0[e] = xp[R.at(e, j0)];
-                    for (int j = j0; j < j1; ++j) {
-                        const int jn = (j + 1 == M) ? 0 : j + 1;
-                        T r1[DIM];
+                    // link (r0 -> r1) of the kinetic sum, and the potential, virial and
+                    // histogram terms of bead r0
+                    auto bead = [&](const T *r0, const T *r1) {
                         T dd = T(0);
 #pragma unroll
                         for (int e = 0; e < DIM; ++e) {
-                            r1[e] = xp[R.at(e, jn)];
                             const T dx = mindist(r0[e], r1[e], P);
                             dd += dx * dx;
                         }
@@ -380,8 +512,42 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs a)
                             }
                             if (ok) atomicAdd(&a.cart[(size_t)c * stride + idx], 1ull);
                         }
+                    };
+                    bool done = false;
+                    if constexpr (kF32) {
+                        if (aligned && a.cart_bins == 0) {
+                            // one skew block; the link out of its last bead goes to slice
+                            // (k + 1) MAXL, or to slice 0 from the last block
+                            const T *x0 = xp + k * (MAXL + 1);
+                            const T *xn = (k + 1 == nseg) ? xp : x0 + (MAXL + 1);
+                            T r0[DIM];
 #pragma unroll
-                        for (int e = 0; e < DIM; ++e) r0[e] = r1[e];
+                            for (int e = 0; e < DIM; ++e) r0[e] = x0[e * R.Mp];
+#pragma unroll
+                            for (int jj = 0; jj < MAXL; ++jj) {
+                                T r1[DIM];
+#pragma unroll
+                                for (int e = 0; e < DIM; ++e) r1[e] = (jj + 1 < MAXL) ? x0[e * R.Mp + jj + 1] : xn[e * R.Mp];
+                                bead(r0, r1);
+#pragma unroll
+                                for (int e = 0; e < DIM; ++e) r0[e] = r1[e];
+                            }
+                            done = true;
+                        }
+                    }
+                    if (!done) {
+                        T r0[DIM];
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) r0[e] = xp[R.at(e, j0)];
+                        for (int j = j0; j < j1; ++j) {
+                            const int jn = (j + 1 == M) ? 0 : j + 1;
+                            T r1[DIM];
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) r1[e] = xp[R.at(e, jn)];
+                            bead(r0, r1);
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) r0[e] = r1[e];
+                        }
                     }
                     mL[r] += (double)l;
                     mV[r] += (double)v;

@@ -274,6 +274,50 @@ def test_sweep_trajectory_fp64(measure):
             assert np.allclose(Ev, blocks_v, rtol=1e-10, atol=1e-8)
 
 
+@pytest.mark.parametrize("case", ["trap", "lattice_box"])
+def test_sweep_trajectory_fp32(case):
+    """FP32 tile kernel == oracle sweep schedule up to FP32 rounding.
+
+    Covers both FP32 code paths of the tile kernel: segments that are one
+    skew block of the LDS rows (ell a power of two dividing M: compile-time
+    bridge constants, wrap only for beads outside the box) and the general
+    segments (ell = 6, M = 40), in a wide trap and in a periodic lattice box
+    of half-width 1.5 where beads leave the box and are wrapped."""
+    rng = np.random.default_rng(23)
+    descs, inits = [], []
+    if case == "trap":
+        for N, M, beta, ell in [(5, 32, 1.0, 8), (4, 64, 2.0, 16), (3, 40, 0.7, 6), (6, 24, 3.0, 4)]:
+            descs.append(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=N, L=100.0, beta=beta, lam=0.5, ell=ell,
+                                    com_step=1.0))
+            inits.append(rand_paths(rng, N, M, 100.0))
+    else:
+        for N, M, ell in [(4, 32, 8), (3, 64, 16), (3, 40, 6)]:
+            descs.append(chain_desc(SquareLattice(1.0, 1.0), dim=2, M=M, N=N, L=1.5, beta=2.0, lam=0.5, ell=ell,
+                                    com_step=1.0))
+            inits.append(rng.uniform(-1.5, 1.5, size=(N, 2, M)))
+    inits = [r.astype(np.float32).astype(np.float64) for r in inits]  # the FP32 start the kernel sees
+    seed, n_sweeps, spl = 7, 2, 2
+    ctx = make_ctx(descs, A.FP32, seed=seed, spl=spl)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    cnt = ctx.run(n_sweeps, measure=True)
+    ref = sweep_oracle(descs, inits, seed, n_sweeps, spl, measure=True)
+    assert cnt.levy_acc == sum(x[1] for x in ref) and cnt.com_acc == sum(x[2] for x in ref)
+    wrapped = 0
+    for c, d in enumerate(descs):
+        diff = ctx.get_paths(c) - ref[c][0]
+        wrapped += int(np.sum(np.abs(diff) > d.L))
+        diff -= 2 * d.L * np.round(diff / (2 * d.L))  # a bead on the box edge may sit at -L in one, +L in the other
+        assert np.max(np.abs(diff)) < 2e-4 * max(1.0, d.L / 10), (c, np.max(np.abs(diff)))
+        E, Ev, _ = ctx.blocks(c)
+        Eo = np.mean([e[0] for e in ref[c][4]])
+        assert abs(E[0] - Eo) <= 1e-4 * max(1.0, abs(Eo)), (c, E[0], Eo)
+    assert wrapped <= 2
+    if case == "lattice_box":
+        moved_out = sum(int(np.sum(np.abs(r0 - x[0]) > 1.5)) for r0, x in zip(inits, ref))
+        assert moved_out > 0  # the wrap branch ran
+
+
 @pytest.mark.parametrize("N,L,ra", [(110, 5.0, 2.0), (30, 3.0, 1.5)])
 def test_sweep_trajectory_pair_fp64(N, L, ra):
     """Whole-chain kernel (cell lists for N > 96, all-pairs scan otherwise) == oracle, FP64."""
