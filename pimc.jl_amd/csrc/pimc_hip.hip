@@ -575,6 +575,12 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                     fresh = t.M != h.M || t.ell != h.ell || c->tile_groups.back().pot != h.pot ||
                             c->tile_groups.back().maxl != next_pow2_maxl(h.ell) ||
                             (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
+                    if (!fresh) {
+                        // keep every tile within the LDS share of 4 workgroups per CU
+                        const int slots = t.nslot + (p == 0 ? 1 : 0);
+                        fresh = tile_lds_bytes(t.np + 1, c->dim, t.pst, (t.np + 1) * h.nseg, ts, slots,
+                                               opt->radial_bins) > kTileLdsBudget;
+                    }
                 }
                 if (fresh) {
                     TileDev t{};
@@ -584,6 +590,15 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                     t.M = h.M;
                     t.ell = h.ell;
                     t.nseg = h.nseg;
+                    t.sh = 0;
+                    while ((2 << t.sh) <= t.ell) ++t.sh; // floor(log2(ell))
+                    t.pst = tile_particle_stride(t.M, t.ell, t.sh, c->dim);
+                    t.nslot = 0;
+                    {
+                        const int maxl = next_pow2_maxl(h.ell);
+                        t.grouped = ts == 4 && h.ell == maxl && h.M % maxl == 0 && h.nseg <= 64 &&
+                                    (h.nseg & (h.nseg - 1)) == 0;
+                    }
                     if (c->tile_groups.empty() || c->tile_groups.back().pot != h.pot ||
                         c->tile_groups.back().maxl != next_pow2_maxl(h.ell))
                         c->tile_groups.push_back(TileGroup{h.pot, next_pow2_maxl(h.ell), (int)c->tiles.size(), 0});
@@ -591,17 +606,17 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                     c->tiles.push_back(t);
                 }
                 TileDev &t = c->tiles.back();
+                if (t.np == 0 || p == 0) t.nslot += 1; // chains are contiguous in the particle order
                 t.np += 1;
                 t.items = t.np * t.nseg;
             }
         }
         for (TileDev &t : c->tiles) {
-            t.nslot = 1;
+            int nslot = 1;
             for (int q = 1; q < t.np; ++q)
-                if (c->part_chain[t.q0 + q] != c->part_chain[t.q0 + q - 1]) ++t.nslot;
-            t.sh = 0;
-            while ((2 << t.sh) <= t.ell) ++t.sh; // floor(log2(ell))
-            lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.M, t.sh, t.items, ts, t.nslot, opt->radial_bins));
+                if (c->part_chain[t.q0 + q] != c->part_chain[t.q0 + q - 1]) ++nslot;
+            if (nslot != t.nslot) throw HipError{hipErrorUnknown, "tile chain-slot count mismatch"};
+            lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.pst, t.items, ts, t.nslot, opt->radial_bins));
         }
         c->lds_tile = lds_max;
         if (lds_max > 160 * 1024) throw HipError{hipErrorInvalidValue, "tile exceeds LDS; lower tile_beads"};

@@ -22,6 +22,7 @@ namespace pimc {
 
 constexpr int kItemsPerThread = 2; // register slots of the estimator sums per thread
 constexpr int kTileMaxThreads = 256; // workgroup size bound of the tile kernel
+constexpr size_t kTileLdsBudget = 40 * 1024; // LDS of one tile: 4 workgroups share a CU's 160 KiB
 constexpr int kNQ = 12;            // per-chain per-block estimator quantities
 
 // block accumulator layout (doubles), per (block, chain)
@@ -61,10 +62,42 @@ struct TileDev {
     long long pos_off; // element offset of the tile's first particle
     int q0, np, M, nseg;
     int ell, items, nslot, sh; // nslot: distinct chains in the tile; sh: LDS skew shift
+    int pst, grouped;          // pst: LDS elements per particle (dim rows plus padding);
+                               // grouped: FP32 tile whose particles' segments are aligned
+                               // lane groups (ell = MAXL a power of two dividing M, nseg a
+                               // power of two <= 64)
 };
 
 // Padded LDS row length of M slices skewed by j >> sh.
 __host__ __device__ inline int tile_row_len(int M, int sh) { return M + (M >> sh) + 1; }
+
+// LDS elements per particle: its dim rows, padded so that the 64 lanes of a
+// wave, which start on the segments k = 0..nseg-1 of consecutive particles,
+// hit distinct banks (checked for 32 and for 64 four-byte banks).
+inline int tile_particle_stride(int M, int ell, int sh, int dim)
+{
+    const int nseg = (M + ell - 1) / ell, base = dim * tile_row_len(M, sh);
+    auto conflicts = [&](int ps) {
+        int bad = 0;
+        for (int banks = 32; banks <= 64; banks *= 2)
+            for (int h = 0; h < 64; h += banks) {
+                unsigned long long seen = 0ull;
+                for (int lane = h; lane < h + banks; ++lane) {
+                    const int q = lane / nseg, k = lane % nseg, j = k * ell;
+                    const int b = (q * ps + j + (j >> sh)) % banks;
+                    if (seen >> b & 1ull) ++bad;
+                    seen |= 1ull << b;
+                }
+            }
+        return bad;
+    };
+    int best = base, least = conflicts(base);
+    for (int pad = 1; pad < 64 && least > 0; ++pad) {
+        const int n = conflicts(base + pad);
+        if (n < least) least = n, best = base + pad;
+    }
+    return best;
+}
 
 struct SweepArgs {
     const ChainDev *chains;
@@ -98,16 +131,15 @@ struct SweepArgs {
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // LDS layout of k_sweep_tiles; the host sizes the launch with the same function.
-__host__ __device__ inline size_t tile_lds_bytes(int np, int dim, int M, int sh, int items, int tsize, int nslot,
-                                                 int rbins)
+__host__ __device__ inline size_t tile_lds_bytes(int np, int dim, int pst, int items, int tsize, int nslot, int rbins)
 {
-    size_t posb = (size_t)np * dim * tile_row_len(M, sh) * tsize;
-    size_t redb = (size_t)items * 3 * sizeof(double);
+    const size_t posb = (size_t)np * pst * tsize;
+    const size_t redb = (size_t)items * 3 * sizeof(double); // end-of-launch estimator sums, over the positions
     size_t o = align16(posb > redb ? posb : redb);
-    o += 5 * align16((size_t)np * 4);       // off, flag, lacc, cacc, chain slot
-    o += align16((size_t)np * dim * tsize); // COM displacement
-    o += align16((size_t)np * tsize);       // COM uniform
-    o += align16((size_t)items * tsize);    // COM partial sums
+    o += 5 * align16((size_t)np * 4);        // off, flag, lacc, cacc, chain slot
+    o += align16((size_t)np * dim * tsize);  // COM displacement
+    o += align16((size_t)np * tsize);        // COM uniform
+    o += align16((size_t)items * tsize);     // COM partial sums
     o += align16((size_t)nslot * rbins * 4); // radial histogram per chain of the tile
     return o;
 }
@@ -183,6 +215,41 @@ __device__ __forceinline__ bool levy_segment(T *xp, const LdsRow &R, int M, int 
     return acc;
 }
 
+// Sum over the lanes of an aligned group of G lanes (G a power of two <= 64).
+template <typename T> __device__ __forceinline__ T group_sum(T v, int G)
+{
+    for (int off = G >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// The value of another lane of the same row of 16, chosen by a DPP control.
+template <int CTRL> __device__ __forceinline__ float dpp_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_lane(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// group_sum for groups of G <= 64 lanes (G uniform): butterfly partners by
+// DPP within rows of 16 (lane ^ 1, lane ^ 2, mirror in 8, mirror in 16), then
+// shuffles across rows.  Each step adds the same two numbers in both partner
+// lanes, so every lane of a group ends with the same bits.
+template <typename T> __device__ __forceinline__ T group_sum_dpp(T v, int G)
+{
+    if (G >= 2) v += dpp_lane<0xB1>(v);  // quad_perm [1,0,3,2]
+    if (G >= 4) v += dpp_lane<0x4E>(v);  // quad_perm [2,3,0,1]
+    if (G >= 8) v += dpp_lane<0x141>(v); // row_half_mirror
+    if (G >= 16) v += dpp_lane<0x140>(v); // row_mirror
+    if (G >= 32) v += __shfl_xor(v, 16, 64);
+    if (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 // FP32 staging segment of exactly MAXL links: the draws, the bridge and the
 // Metropolis test of levy_segment, with two changes that only touch
 // rounding.  The bridge coefficients are compile-time constants (kBridge),
@@ -248,11 +315,11 @@ __device__ __forceinline__ bool levy_full(float *xp, const LdsRow &R, int M, int
     return acc;
 }
 
-// At most 256 threads and at least 3 waves per SIMD: the register budget
-// (168 VGPRs) that keeps the MAXL = 16 bridge in registers without spilling
-// while LDS, not registers, limits how many tiles share a CU.
+// At most 256 threads; FP32 at 4 waves per SIMD (128 VGPRs: with tiles of
+// at most 40 KiB of LDS, four workgroups share a CU), FP64 at 2 (its values
+// take register pairs).
 template <typename T, int DIM, int MAXL, int POT>
-__global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_per_eu(3))) k_sweep_tiles(SweepArgs a)
+__global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) k_sweep_tiles(SweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const TileDev tl = a.tiles[a.tile0 + blockIdx.x];
@@ -261,12 +328,14 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
     const int np = tl.np, M = tl.M, nseg = tl.nseg, ell = tl.ell, items = tl.items;
     const LdsRow R{tile_row_len(M, tl.sh), tl.sh};
     const uint32_t sweep0 = a.rs->sweep;
-    const int pstride = DIM * R.Mp; // LDS elements per particle
+    const int pstride = tl.pst; // LDS elements per particle
     const int rbins = a.radial_bins;
     constexpr bool kF32 = std::is_same<T, float>::value;
     // FP32 fast paths when every segment is one skew block of the LDS rows
     // (ell == MAXL, both powers of two, and ell divides M)
     const bool aligned = kF32 && ell == MAXL && (M & (MAXL - 1)) == 0;
+    // ... and a particle's segments form aligned lane groups of a wave (set by the host)
+    const bool grouped = kF32 && tl.grouped;
     // item it = (particle q, segment k) with k fastest; thread tid visits it = tid + r nt,
     // so (q, k) advance by (nt / nseg, nt % nseg) per step (no divide in the loops)
     const int dq = nt / nseg, dk = nt - (nt / nseg) * nseg;
@@ -280,7 +349,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
     }
 
     T *sp = reinterpret_cast<T *>(smem);
-    size_t posb = (size_t)np * pstride * sizeof(T), redb = (size_t)items * 3 * sizeof(double);
+    const size_t posb = (size_t)np * pstride * sizeof(T), redb = (size_t)items * 3 * sizeof(double);
     size_t o = align16(posb > redb ? posb : redb);
     int *s_off = reinterpret_cast<int *>(smem + o);
     o += align16((size_t)np * 4);
@@ -304,8 +373,8 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
     T *gpos = reinterpret_cast<T *>(a.pos) + tl.pos_off;
     const int nrow = np * DIM;
     for (int i = tid; i < nrow * M; i += nt) {
-        const int row = i / M, j = i - row * M;
-        sp[row * R.Mp + j + (j >> R.sh)] = gpos[i];
+        const int row = i / M, j = i - row * M, q = row / DIM, d = row - q * DIM;
+        sp[q * pstride + d * R.Mp + j + (j >> R.sh)] = gpos[i];
     }
     for (int q = tid; q < np; q += nt) {
         s_lacc[q] = 0;
@@ -406,7 +475,31 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
                             vn += potVk<T, DIM, POT>(rn, P);
                         }
                     }
-                    s_red[it] = vn - vo;
+                    if (!grouped) {
+                        s_red[it] = vn - vo;
+                        continue;
+                    }
+                    // the nseg chunks of particle q are an aligned lane group of this
+                    // wave: every lane gets the same total (each butterfly step adds
+                    // the same two numbers in both partner lanes), decides, and moves
+                    // its own chunk
+                    const T tot = group_sum_dpp(vn - vo, nseg);
+                    const bool acc = metropolis(ex(-P.tau * tot), s_u[q]);
+                    if (acc && k == 0) s_cacc[q] += 1;
+                    if (acc) {
+                        T *xw = sp + (size_t)q * pstride + k * (MAXL + 1);
+                        if (!((wrap >> r) & 1u)) {
+#pragma unroll
+                            for (int jj = 0; jj < MAXL; ++jj)
+#pragma unroll
+                                for (int e = 0; e < DIM; ++e) xw[e * R.Mp + jj] += d[e];
+                        } else {
+#pragma unroll
+                            for (int jj = 0; jj < MAXL; ++jj)
+#pragma unroll
+                                for (int e = 0; e < DIM; ++e) xw[e * R.Mp + jj] = teleport(xw[e * R.Mp + jj] + d[e], P);
+                        }
+                    }
                     continue;
                 }
             }
@@ -422,6 +515,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
             }
             s_red[it] = dv;
         }
+        if (!grouped) {
         __syncthreads();
         for (int q = tid; q < np; q += nt) {
             T sum = T(0);
@@ -464,6 +558,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
                 for (int e = 0; e < DIM; ++e) xp[R.at(e, j)] = teleport(xp[R.at(e, j)] + s_d[q * DIM + e], P);
             }
         }
+        } // !grouped
         __syncthreads();
 
         // estimators (src/measurement.jl:45-55, 92-122): T sums over a chunk
@@ -560,8 +655,8 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
     __syncthreads();
 
     for (int i = tid; i < nrow * M; i += nt) {
-        const int row = i / M, j = i - row * M;
-        gpos[i] = sp[row * R.Mp + j + (j >> R.sh)];
+        const int row = i / M, j = i - row * M, q = row / DIM, d = row - q * DIM;
+        gpos[i] = sp[q * pstride + d * R.Mp + j + (j >> R.sh)];
     }
     // radial histogram of the launch -> global counts (one atomic per non-empty bin)
     if (a.measure && rbins > 0) {
@@ -624,12 +719,6 @@ __device__ inline double block_sum(double v, double *wbuf)
     return s;
 }
 
-// Sum over the lanes of an aligned group of G lanes (G a power of two <= 64).
-template <typename T> __device__ __forceinline__ T group_sum(T v, int G)
-{
-    for (int off = G >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // This lane's share of the pair link sum of particle n over link j -> j+1,
 // with its beads there replaced by pj, pj1 (see pair_link): the candidates of
