@@ -20,7 +20,7 @@
 
 namespace pimc {
 
-constexpr int kItemsPerThread = 2; // register slots of the estimator sums per thread
+constexpr int kItemsPerThread = 1; // register slots of the estimator sums per thread
 constexpr int kTileMaxThreads = 256; // workgroup size bound of the tile kernel
 constexpr size_t kTileLdsBudget = 40 * 1024; // LDS of one tile: 4 workgroups share a CU's 160 KiB
 constexpr int kNQ = 12;            // per-chain per-block estimator quantities
