@@ -256,7 +256,9 @@ def main():
     avg_launch_s = cnt.kernel_ms / 1000.0 / launches
     bytes_per_launch = BYTES_PER_MOVE_FP32 * cnt.bead_moves / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = None
+    # HBM bytes and VALU counters of one launch of this configuration, from the committed
+    # rocprofv3 runs (counter collection serialises the kernels, so it is not done here)
+    traffic, valu = None, None
     tp = os.path.join(ROOT, "profiles", "traffic_trapped_batch.json")
     if os.path.exists(tp):
         try:
@@ -266,8 +268,15 @@ def main():
                     and tj.get("staging_length") == (batch.tune.ell[0] if batch.tune and len(set(batch.tune.ell)) == 1
                                                      else args.ell or None):
                 traffic = tj.get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
+                issue_ms = tj["valu_wave_instructions_per_launch"] * 4 / 1024 / 2.4e9 * 1e3
+                valu = {"bound": "valu",
+                        "wave_instructions_per_launch": tj["valu_wave_instructions_per_launch"],
+                        "issue_ms_per_launch": issue_ms,
+                        "frac_of_launch": issue_ms / (avg_launch_s * 1e3),
+                        "busy_pct_counter": tj.get("valu_busy_pct"),
+                        "note": "4 cycles per wave64 VALU instruction on each of 1024 SIMDs at 2.4 GHz"}
+        except (OSError, ValueError, KeyError):
+            traffic, valu = None, None
     echeck = energy_check(batch, wl)
     acc = {"levy": cnt.levy_acc / max(1, cnt.levy_tries), "com": cnt.com_acc / max(1, cnt.com_tries)}
     batch.close()
@@ -326,6 +335,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
         },
+        "valu_roofline": valu,
         "cpu_baseline": cpu,
         "plain_gpu": {
             "workload": "trapped_plain: one chain N=10, M=64, beta=1, omega=1, FP64",
