@@ -407,14 +407,31 @@ __device__ __forceinline__ bool cells_move(const Cells &C, const Prm<T> &p, int 
     const int oc = b;
     if (nc == oc) return true;
     if (oc >= 0) {
+        // find n in the old cell's list, read in 16-byte chunks (cap is a multiple
+        // of 8 and every list starts 16-byte aligned) issued before any is tested
         unsigned short *om = C.members(j, oc);
         int &ocnt = C.count(j, oc);
-        for (int q = 0; q < ocnt; ++q) {
-            if (om[q] == n) {
-                om[q] = om[ocnt - 1];
-                ocnt -= 1;
-                break;
+        const int cnt = ocnt;
+        int at = -1;
+        for (int q0 = 0; q0 < C.cap && at < 0; q0 += 32) {
+            uint4 w[4];
+#pragma unroll
+            for (int h = 0; h < 4; ++h)
+                w[h] = (q0 + 8 * h < C.cap) ? reinterpret_cast<const uint4 *>(om + q0)[h] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const unsigned ww[4] = {w[h].x, w[h].y, w[h].z, w[h].w};
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int qi = q0 + 8 * h + t;
+                    const unsigned v = (ww[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+                    if (at < 0 && qi < cnt && v == (unsigned)n) at = qi;
+                }
             }
+        }
+        if (at >= 0) {
+            om[at] = om[cnt - 1];
+            ocnt = cnt - 1;
         }
     }
     int &ncnt = C.count(j, nc);

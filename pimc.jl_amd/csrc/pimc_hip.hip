@@ -68,6 +68,7 @@ struct pimc_ctx {
     std::vector<int> whole_ids;  // whole chains
     std::vector<int> whole_cells_ids;  // whole chains with cell lists
     std::vector<int> whole_brute_ids;  // whole chains with brute-force neighbour sums
+    std::vector<int> whole_lds_ids;    // ... whose path fits in LDS next to the kernel's buffers
     std::vector<TileDev> tiles;
     std::vector<TileGroup> tile_groups;
     std::vector<int> part_chain, part_idx;
@@ -93,6 +94,7 @@ struct pimc_ctx {
     double *d_blocks = nullptr;
     unsigned long long *d_radial = nullptr, *d_cart = nullptr;
     int *d_whole = nullptr, *d_whole_cells = nullptr, *d_whole_brute = nullptr, *d_is_whole = nullptr;
+    int *d_whole_lds = nullptr;
     int *d_cell_cnt = nullptr, *d_bins = nullptr;
     unsigned short *d_cell_mem = nullptr;
     int *d_status = nullptr;
@@ -169,26 +171,32 @@ void launch_tiles(pimc_ctx *c, const SweepArgs &a)
     }
 }
 
-template <typename T, int DIM, bool CELLS> void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &ids, const int *d_ids)
+template <typename T, int DIM, bool CELLS, bool LDSR>
+void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &ids, const int *d_ids)
 {
     if (ids.empty()) return;
     a.whole = d_ids;
     int maxM = 0;
-    for (int id : ids) maxM = std::max(maxM, c->ch[id].M);
+    size_t lds = c->lds_whole;
+    for (int id : ids) {
+        maxM = std::max(maxM, c->ch[id].M);
+        if (LDSR) lds = std::max(lds, whole_lds_bytes(c->ch[id].M, DIM, sizeof(T)) + whole_path_bytes(c->ch[id], sizeof(T)));
+    }
     // lanes per link: as many as the workgroup holds for one pass over the links (2..16)
     int G = 1;
     while (G < 16 && 2 * G * maxM <= c->threads_whole) G <<= 1;
     a.link_group = std::max(2, G);
-    auto k = k_sweep_whole<T, DIM, CELLS>;
-    if (c->lds_whole > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_whole));
-    hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(c->threads_whole), c->lds_whole, c->stream, a);
+    auto k = k_sweep_whole<T, DIM, CELLS, LDSR>;
+    if (lds > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(c->threads_whole), lds, c->stream, a);
     HIPCHK(hipGetLastError());
 }
 
 template <typename T, int DIM> void launch_whole_d(pimc_ctx *c, const SweepArgs &a)
 {
-    launch_whole_t<T, DIM, true>(c, a, c->whole_cells_ids, c->d_whole_cells);
-    launch_whole_t<T, DIM, false>(c, a, c->whole_brute_ids, c->d_whole_brute);
+    launch_whole_t<T, DIM, true, false>(c, a, c->whole_cells_ids, c->d_whole_cells);
+    launch_whole_t<T, DIM, false, false>(c, a, c->whole_brute_ids, c->d_whole_brute);
+    launch_whole_t<T, DIM, false, true>(c, a, c->whole_lds_ids, c->d_whole_lds);
 }
 
 void launch_whole(pimc_ctx *c, const SweepArgs &a)
@@ -526,6 +534,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 const long long ncell = (h.dim == 1) ? h.nbins : (long long)h.nbins * h.nbins;
                 const double mean = (double)h.N / (double)ncell;
                 int cap = (int)std::min<long long>(h.N, (long long)std::ceil(4.0 * mean + 16.0));
+                cap = (cap + 7) & ~7; // whole 16-byte chunks per member list
                 const long long bytes = (long long)h.M * ncell * (cap * 2 + 4);
                 const bool cells = h.pair != PIMC_PAIR_NONE && h.N > 96 && h.nbins >= 3 && bytes < (256ll << 20);
                 if (cells) {
@@ -628,6 +637,16 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             c->lds_whole = whole_lds_bytes(maxMw, c->dim, ts);
             if (c->lds_whole > 160 * 1024) throw HipError{hipErrorInvalidValue, "M too large for the whole-chain kernel"};
         }
+        // all-pairs chains whose positions fit in LDS next to the buffers keep them there
+        {
+            std::vector<int> keep;
+            for (int id : c->whole_brute_ids) {
+                const ChainDev &h = c->ch[id];
+                if (whole_lds_bytes(h.M, c->dim, ts) + whole_path_bytes(h, ts) <= 160 * 1024) c->whole_lds_ids.push_back(id);
+                else keep.push_back(id);
+            }
+            c->whole_brute_ids.swap(keep);
+        }
         // schedule counts per sweep
         for (int i = 0; i < n_chains; ++i) {
             const ChainDev &h = c->ch[i];
@@ -688,6 +707,9 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         c->d_whole = dalloc<int>(c->whole_ids.size());
         c->d_whole_cells = dalloc<int>(c->whole_cells_ids.size());
         c->d_whole_brute = dalloc<int>(c->whole_brute_ids.size());
+        c->d_whole_lds = dalloc<int>(c->whole_lds_ids.size());
+        if (!c->whole_lds_ids.empty())
+            HIPCHK(hipMemcpy(c->d_whole_lds, c->whole_lds_ids.data(), sizeof(int) * c->whole_lds_ids.size(), hipMemcpyHostToDevice));
         if (!c->whole_ids.empty())
             HIPCHK(hipMemcpy(c->d_whole, c->whole_ids.data(), sizeof(int) * c->whole_ids.size(), hipMemcpyHostToDevice));
         if (!c->whole_cells_ids.empty())
@@ -740,7 +762,7 @@ int pimc_destroy(pimc_ctx *c)
     void *ptrs[] = {c->d_ch, c->d_prm, c->d_tiles, c->d_pchain, c->d_pidx, c->d_pos, c->d_part_acc, c->d_placc, c->d_pcacc,
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
-                    c->d_status, c->d_work, c->d_io};
+                    c->d_status, c->d_work, c->d_io, c->d_whole_lds};
     for (void *p : ptrs) dfree(p);
     for (auto &e : c->events) {
         (void)hipEventDestroy(e.a);

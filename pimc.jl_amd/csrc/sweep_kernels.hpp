@@ -720,74 +720,191 @@ __device__ inline double block_sum(double v, double *wbuf)
 }
 
 
+// Squared periodic distance of two beads.
+template <typename T, int DIM> __device__ __forceinline__ T dist2(const T *x, const T *y, const Prm<T> &P)
+{
+    const T d0 = mindist(x[0], y[0], P);
+    T r2 = d0 * d0;
+    if (DIM > 1) {
+        const T d1 = mindist(x[1], y[1], P);
+        r2 += d1 * d1;
+    }
+    return r2;
+}
+
+// Neighbour scan of the pair sums (whole-chain kernel, hooks, estimators).
+// The candidates of query point q at slice j are the particles of the 9
+// stencil cells around q, in the stencil order of bin_neighbors and the
+// member order of each cell (CELLS), or all particles in index order, each
+// counted as often as its cell appears in the stencil when nbins < 3; a
+// candidate is a neighbour when its periodic distance to q is <= h (the
+// neighbour set of find_nns, src/nearest_neighbours.jl:106-154).  Candidates
+// are split over the G lanes of a group: lane g takes g, g + G, ...  For NQ
+// query points at once (the new and the old bead of a move) a lane loads
+// the ids, then the beads at slice j, of R candidates before it tests any,
+// then the beads at slice jn of the neighbours only.  A query whose cell is
+// query 0's shares query 0's list and loads.  Each query sees its candidates
+// in the order of a one-at-a-time scan.  term(u, rkn, r2, mult) is called for
+// every neighbour k != n of query u, with r2 = |r_kj - q_u|^2 and rkn = r_k,jn.
+template <typename T, int DIM, bool CELLS, int NQ, int R, typename PathT, typename F>
+__device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P,
+                                           int n, int j, int jn, const T (&q)[NQ][DIM], int g, int G, F &&term)
+{
+    constexpr int NS = (DIM == 1) ? 3 : 9;
+    if constexpr (CELLS) {
+        int c0[NQ], lst[NQ]; // lst[u]: the list query u reads (its own, or query 0's)
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+            c0[u] = cell_of<T, DIM>(q[u], P, C.nbins);
+            lst[u] = (u > 0 && c0[u] == c0[0]) ? 0 : u;
+        }
+        int cs[NQ][NS], pre[NQ][NS + 1], total = 0;
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+            pre[u][0] = 0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                cs[u][s] = stencil_cell(c0[u], s, C.nbins, DIM);
+                pre[u][s + 1] = (lst[u] == u) ? C.count(j, cs[u][s]) : 0;
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) pre[u][s + 1] += pre[u][s];
+            total = max(total, pre[u][NS]);
+        }
+        for (int i0 = g; i0 < total; i0 += G * R) {
+            int k[NQ][R];
+            T rk[NQ][R][DIM];
+#pragma unroll
+            for (int u = 0; u < NQ; ++u) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int idx = i0 + r * G;
+                    int cell = cs[u][0], base = 0;
+#pragma unroll
+                    for (int s = 1; s < NS; ++s) {
+                        if (idx >= pre[u][s]) {
+                            cell = cs[u][s];
+                            base = pre[u][s];
+                        }
+                    }
+                    k[u][r] = (idx < pre[u][NS]) ? (int)C.members(j, cell)[idx - base] : -1;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NQ; ++u)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (lst[u] == u) path.get(k[u][r] < 0 ? n : k[u][r], j, rk[u][r]);
+            T r2[NQ][R];
+            bool near[NQ][R];
+#pragma unroll
+            for (int u = 0; u < NQ; ++u) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int l = lst[u];
+                    const int kk = l == 0 ? k[0][r] : k[u][r];
+                    r2[u][r] = dist2<T, DIM>(l == 0 ? rk[0][r] : rk[u][r], q[u], P);
+                    near[u][r] = kk >= 0 && kk != n && r2[u][r] <= P.h2;
+                }
+            }
+            T rkn[NQ][R][DIM];
+#pragma unroll
+            for (int u = 0; u < NQ; ++u) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    bool need = false;
+#pragma unroll
+                    for (int v = 0; v < NQ; ++v) need = need || (lst[v] == u && near[v][r]);
+                    if (need) path.get(k[u][r], jn, rkn[u][r]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NQ; ++u) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (near[u][r]) term(u, lst[u] == 0 ? rkn[0][r] : rkn[u][r], r2[u][r], T(1));
+            }
+        }
+    } else {
+        int c0[NQ];
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) c0[u] = (nbins < 3) ? cell_of<T, DIM>(q[u], P, nbins) : 0;
+        for (int k0 = g; k0 < N; k0 += G * R) {
+            T rk[R][DIM], rkn[R][DIM], r2[NQ][R], mult[NQ][R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int kk = k0 + r * G;
+                path.get(kk < N ? kk : n, j, rk[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int kk = k0 + r * G;
+                const bool cand = kk < N && kk != n;
+                const int ck = (nbins < 3) ? cell_of<T, DIM>(rk[r], P, nbins) : 0;
+                bool need = false;
+#pragma unroll
+                for (int u = 0; u < NQ; ++u) {
+                    int mm = 1;
+                    if (nbins < 3) {
+                        mm = 0;
+                        for (int s = 0; s < NS; ++s) mm += (stencil_cell(c0[u], s, nbins, DIM) == ck);
+                    }
+                    r2[u][r] = dist2<T, DIM>(rk[r], q[u], P);
+                    mult[u][r] = (cand && mm > 0 && r2[u][r] <= P.h2) ? T(mm) : T(0);
+                    need = need || mult[u][r] > T(0);
+                }
+                if (need) path.get(kk, jn, rkn[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+#pragma unroll
+                for (int u = 0; u < NQ; ++u)
+                    if (mult[u][r] > T(0)) term(u, rkn[r], r2[u][r], mult[u][r]);
+            }
+        }
+    }
+}
+
 // This lane's share of the pair link sum of particle n over link j -> j+1,
-// with its beads there replaced by pj, pj1 (see pair_link): the candidates of
-// the 9 stencil cells (or of all particles) are split over the G lanes of a
-// group, lane g taking candidates g, g + G, ...
+// with its beads there replaced by pj, pj1 (see pair_link): lnU =
+// -tau/2 (v(r_j) + v(r_j+1)) for every neighbour found at slice j.
 template <typename T, int DIM, bool CELLS, typename PathT>
 __device__ T link_part(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P, int n, int j,
                        const T *pj, const T *pj1, int g, int G)
 {
-    const int M = path.M;
-    const int jn = (j + 1 == M) ? 0 : j + 1;
-    constexpr int NS = (DIM == 1) ? 3 : 9;
-    T acc = T(0);
-    auto pair_term = [&](int k, T mult) {
-        T rk[DIM];
-        path.get(k, j, rk);
-        T d0 = mindist(rk[0], pj[0], P);
-        T r2 = d0 * d0;
-        if (DIM > 1) {
-            T d1 = mindist(rk[1], pj[1], P);
-            r2 += d1 * d1;
-        }
-        if (r2 > P.h2) return;
-        T rkn[DIM];
-        path.get(k, jn, rkn);
-        T e0 = mindist(rkn[0], pj1[0], P);
-        T s2 = e0 * e0;
-        if (DIM > 1) {
-            T e1 = mindist(rkn[1], pj1[1], P);
-            s2 += e1 * e1;
-        }
-        acc += mult * ((T(-0.5) * P.tau) * (vpair_r2(r2, P) + vpair_r2(s2, P)));
-    };
-    if constexpr (CELLS) {
-        const int c0 = cell_of<T, DIM>(pj, P, C.nbins);
-        int cs[NS], cnt[NS];
+    const int jn = (j + 1 == path.M) ? 0 : j + 1;
+    T q[1][DIM];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            cs[s] = stencil_cell(c0, s, C.nbins, DIM);
-            cnt[s] = C.count(j, cs[s]);
-        }
-        int s = 0, base = 0;
-        for (int idx = g;; idx += G) {
-            while (s < NS && idx >= base + cnt[s]) {
-                base += cnt[s];
-                ++s;
-            }
-            if (s >= NS) break;
-            const int k = C.members(j, cs[s])[idx - base];
-            if (k != n) pair_term(k, T(1));
-        }
-    } else {
-        const int c0 = (nbins < 3) ? cell_of<T, DIM>(pj, P, nbins) : 0;
-        for (int k = g; k < N; k += G) {
-            if (k == n) continue;
-            T mult = T(1);
-            if (nbins < 3) {
-                T rk[DIM];
-                path.get(k, j, rk);
-                const int ck = cell_of<T, DIM>(rk, P, nbins);
-                int mm = 0;
-                for (int s = 0; s < NS; ++s) mm += (stencil_cell(c0, s, nbins, DIM) == ck);
-                if (!mm) continue;
-                mult = T(mm);
-            }
-            pair_term(k, mult);
-        }
-    }
+    for (int e = 0; e < DIM; ++e) q[0][e] = pj[e];
+    T acc = T(0);
+    scan_pairs<T, DIM, CELLS, 1, 4>(path, C, N, nbins, P, n, j, jn, q, g, G,
+                                    [&](int, const T *rkn, T r2, T mult) {
+                                        const T s2 = dist2<T, DIM>(rkn, pj1, P);
+                                        acc += mult * ((T(-0.5) * P.tau) * (vpair_r2(r2, P) + vpair_r2(s2, P)));
+                                    });
     return acc;
+}
+
+// This lane's share of (link sum with beads n0, n1) - (link sum with o0, o1)
+// for particle n over link j: both scans in one batched pass.
+template <typename T, int DIM, bool CELLS, int R, typename PathT>
+__device__ __forceinline__ T link_delta(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P, int n,
+                                        int j, int jn, const T *n0, const T *n1, const T *o0, const T *o1, int g,
+                                        int G)
+{
+    T q[2][DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) {
+        q[0][e] = n0[e];
+        q[1][e] = o0[e];
+    }
+    T acc[2] = {T(0), T(0)};
+    scan_pairs<T, DIM, CELLS, 2, R>(path, C, N, nbins, P, n, j, jn, q, g, G,
+                                    [&](int u, const T *rkn, T r2, T mult) {
+                                        const T s2 = dist2<T, DIM>(rkn, u == 0 ? n1 : o1, P);
+                                        acc[u] += mult * ((T(-0.5) * P.tau) * (vpair_r2(r2, P) + vpair_r2(s2, P)));
+                                    });
+    return acc[0] - acc[1];
 }
 
 template <typename T, int DIM, bool CELLS, typename PathT>
@@ -811,65 +928,17 @@ __device__ inline void pair_energy_bead(const PathT &path, const Cells &C, int N
     path.get(i, jn, rin);
     su = 0.0;
     rg = 0.0;
-    auto visit = [&](int k, int mult) {
-        T rk[DIM];
-        path.get(k, j, rk);
-        T d0 = mindist(rk[0], ri[0], P);
-        T r2 = d0 * d0;
-        if (DIM > 1) {
-            T d1 = mindist(rk[1], ri[1], P);
-            r2 += d1 * d1;
-        }
-        if (r2 > P.h2) return;
-        T rkn[DIM];
-        path.get(k, jn, rkn);
-        T e0 = mindist(rkn[0], rin[0], P);
-        T s2 = e0 * e0;
-        if (DIM > 1) {
-            T e1 = mindist(rkn[1], rin[1], P);
-            s2 += e1 * e1;
-        }
-        const double ea = (double)ex(-r2 * P.inv2s2), eb = (double)ex(-s2 * P.inv2s2);
-        const double w = 0.5 * (0.5 * (double)P.tau) * mult;
-        su += w * (ea + eb);
-        rg += w * (-((double)r2 * inv_s2) * ea - ((double)s2 * inv_s2) * eb);
-    };
-    constexpr int NS = (DIM == 1) ? 3 : 9;
-    // candidates split over the G lanes of a group: lane g takes g, g + G, ...
-    if constexpr (CELLS) {
-        const int c0 = cell_of<T, DIM>(ri, P, C.nbins);
-        int cs[NS], cnt[NS];
+    T q[1][DIM];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            cs[s] = stencil_cell(c0, s, C.nbins, DIM);
-            cnt[s] = C.count(j, cs[s]);
-        }
-        int s = 0, base = 0;
-        for (int idx = g;; idx += G) {
-            while (s < NS && idx >= base + cnt[s]) {
-                base += cnt[s];
-                ++s;
-            }
-            if (s >= NS) break;
-            const int k = C.members(j, cs[s])[idx - base];
-            if (k != i) visit(k, 1);
-        }
-    } else {
-        const int c0 = (nbins < 3) ? cell_of<T, DIM>(ri, P, nbins) : 0;
-        for (int k = g; k < N; k += G) {
-            if (k == i) continue;
-            int mult = 1;
-            if (nbins < 3) {
-                T rk[DIM];
-                path.get(k, j, rk);
-                const int ck = cell_of<T, DIM>(rk, P, nbins);
-                mult = 0;
-                for (int s = 0; s < NS; ++s) mult += (stencil_cell(c0, s, nbins, DIM) == ck);
-                if (!mult) continue;
-            }
-            visit(k, mult);
-        }
-    }
+    for (int e = 0; e < DIM; ++e) q[0][e] = ri[e];
+    scan_pairs<T, DIM, CELLS, 1, 4>(path, C, N, nbins, P, i, j, jn, q, g, G,
+                                    [&](int, const T *rkn, T r2, T mult) {
+                                        const T s2 = dist2<T, DIM>(rkn, rin, P);
+                                        const double ea = (double)ex(-r2 * P.inv2s2), eb = (double)ex(-s2 * P.inv2s2);
+                                        const double w = 0.5 * (0.5 * (double)P.tau) * (double)mult;
+                                        su += w * (ea + eb);
+                                        rg += w * (-((double)r2 * inv_s2) * ea - ((double)s2 * inv_s2) * eb);
+                                    });
 }
 
 // Per-measurement whole-chain sums -> the kNQ quantities.
@@ -968,51 +1037,72 @@ __device__ inline void fold_meas(const ChainMeas &m, const ChainDev &ch, double 
     q[qWhole] = 1.0;
 }
 
-// LDS of k_sweep_whole: two buffers of new beads (M x DIM T each), Delta V
-// per segment (M T), Delta lnU per link (M T), accept flags (M ints), two
-// sets of 16 wave partials (double), 16 doubles for block sums.
+// LDS of k_sweep_whole: two buffers of new beads and one of bridge normals
+// (M x DIM T each), Delta V per bead and Delta lnU per link (M T each), accept
+// flags (M ints), two sets of 16 wave partials (double), 16 doubles for block
+// sums, kNQ doubles of estimator sums.
 __host__ __device__ inline size_t whole_lds_bytes(int M, int dim, int tsize)
 {
-    size_t o = 2 * align16((size_t)M * dim * tsize);
-    o += align16((size_t)M * tsize);
-    o += align16((size_t)M * tsize);
+    size_t o = 3 * align16((size_t)M * dim * tsize);
+    o += 2 * align16((size_t)M * tsize);
     o += align16((size_t)M * 4);
     o += align16(32 * sizeof(double));
     o += align16(16 * sizeof(double));
+    o += align16(kNQ * sizeof(double));
     return o;
 }
 
-// One workgroup per chain with a pair action; positions slice-major in HBM.
-// Staging: for each particle in order, (A) one thread per segment builds the
-// bridge into LDS, (B) each link's old and new pair sums are split over a
-// group of G lanes and reduced by shuffles, (C) one thread per segment
-// decides, (D) one thread per slice commits beads and cell lists.  New beads
-// are double-buffered by particle parity, so a particle costs three barriers.
-// Centre of mass: new beads per slice, the link sums over all slices, one
-// workgroup reduction, a uniform decision and the commit: two barriers.
-template <typename T, int DIM, bool CELLS>
+// LDS that holds a whole chain's positions (chains whose path stays in LDS).
+__host__ __device__ inline size_t whole_path_bytes(const ChainDev &h, int tsize)
+{
+    return align16((size_t)h.N * h.M * h.dim * tsize);
+}
+
+// One workgroup per chain with a pair action; positions slice-major in HBM
+// (LDSR: copied into LDS for the launch when they fit, for all-pairs chains).
+// Staging, for each particle in order: (A1) one thread per interior bead pair
+// draws the bridge normals, (A2) one thread per segment runs the bridge
+// recursion into LDS, (B) a group of G lanes per link loads the old beads,
+// takes Delta V of the link's first bead and the link's new-minus-old pair
+// sum (batched neighbour scan), (C) one thread per segment decides, summing
+// in the bead and link order of the oracle, (D) one thread per slice commits
+// beads and cell lists.  New beads are double-buffered by particle parity.
+// Centre of mass, for each particle in order: the link groups form the moved
+// beads on the fly and add Delta V and the pair difference, one workgroup
+// reduction gives a uniform decision, and the commit: two barriers.
+template <typename T, int DIM, bool CELLS, bool LDSR>
 __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
 {
+    static_assert(!(CELLS && LDSR), "LDS-resident paths are for all-pairs chains");
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = a.whole[blockIdx.x];
     const ChainDev ch = a.chains[c];
     const Prm<T> P = reinterpret_cast<const Prm<T> *>(a.prm)[c];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int N = ch.N, M = ch.M, ell = ch.ell, nseg = ch.nseg;
-    const Path<T, DIM, true> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, M, N};
+    T *const gpath = reinterpret_cast<T *>(a.pos) + ch.pos_off;
+    T *const lpath = reinterpret_cast<T *>(smem + whole_lds_bytes(M, DIM, sizeof(T)));
+    if constexpr (LDSR) {
+        for (int i = tid; i < N * M * DIM; i += nt) lpath[i] = gpath[i];
+        __syncthreads();
+    }
+    const Path<T, DIM, true> path{LDSR ? lpath : gpath, M, N};
     const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, M, ch.ncell, ch.cap,
                       ch.nbins};
     const bool use_pair = (ch.pair != kPairNone) && ch.pair_in_accept;
     const uint32_t sweep0 = a.rs->sweep;
     const int G = a.link_group, ngroups = nt / G, g = tid & (G - 1), grp = tid / G;
     const int lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
+    constexpr int R = 2; // candidates per lane and query loaded together
 
     size_t o = 0;
     T *s_new0 = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)M * DIM * sizeof(T));
     T *s_new1 = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)M * DIM * sizeof(T));
-    T *s_dv = reinterpret_cast<T *>(smem + o);
+    T *s_nrm = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)M * DIM * sizeof(T));
+    T *s_dvb = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)M * sizeof(T));
     T *s_du = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)M * sizeof(T));
@@ -1021,11 +1111,12 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     double *s_wsum = reinterpret_cast<double *>(smem + o);
     o += align16(32 * sizeof(double));
     double *s_buf = reinterpret_cast<double *>(smem + o);
+    o += align16(16 * sizeof(double));
+    double *s_q = reinterpret_cast<double *>(smem + o); // estimator sums of the launch
 
-    double q[kNQ];
-#pragma unroll
-    for (int k = 0; k < kNQ; ++k) q[k] = 0.0;
+    if (tid < kNQ) s_q[tid] = 0.0;
     int lacc = 0, cacc = 0, overflow = 0;
+    const int hp = ell >> 1; // Philox blocks per segment: interior bead pairs (i, i + 1), i odd
 
     for (int s = 0; s < a.nsweeps; ++s) {
         const uint32_t sw = sweep0 + (uint32_t)s;
@@ -1034,10 +1125,30 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
             T *buf = (p & 1) ? s_new1 : s_new0;
             const U4 xo = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kOffset);
             const int off = (int)(((uint64_t)xo.x * (uint64_t)ell) >> 32);
-            // (A) bridges
+            // (A1) normals of the interior beads, by slice
+            for (int w = tid; w < nseg * hp; w += nt) {
+                const int k = w / hp, i = 2 * (w - k * hp) + 1;
+                const int m = min(ell, M - k * ell);
+                if (i < m) {
+                    int b = off + k * ell + i;
+                    if (b >= M) b -= M;
+                    const U4 xr = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c, kLevy);
+                    T n0, n1;
+                    box_muller(xr.x, xr.y, n0, n1);
+                    s_nrm[b * DIM] = n0;
+                    if (DIM > 1) s_nrm[b * DIM + 1] = n1;
+                    if (i + 1 < m) {
+                        const int b1 = (b + 1 == M) ? 0 : b + 1;
+                        box_muller(xr.z, xr.w, n0, n1);
+                        s_nrm[b1 * DIM] = n0;
+                        if (DIM > 1) s_nrm[b1 * DIM + 1] = n1;
+                    }
+                }
+            }
+            __syncthreads();
+            // (A2) bridges
             for (int k = tid; k < nseg; k += nt) {
                 const int m = min(ell, M - k * ell);
-                T dv = T(0);
                 if (m >= 2) {
                     int start = off + k * ell;
                     if (start >= M) start -= M;
@@ -1048,71 +1159,65 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                     path.get(p, jm, em);
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
-                    U4 xr{0u, 0u, 0u, 0u};
                     for (int i = 1; i < m; ++i) {
                         int b = start + i;
                         if (b >= M) b -= M;
-                        if (i & 1) xr = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c, kLevy);
-                        T n0, n1;
-                        if (i & 1) box_muller(xr.x, xr.y, n0, n1);
-                        else box_muller(xr.z, xr.w, n0, n1);
                         const T alpha = bridge_alpha<T>(m, i);
                         const T sd = bridge_sd(alpha, P);
-                        T rn[DIM], ro[DIM];
-                        cur[0] = bridge_step(cur[0], em[0], alpha, n0, sd);
-                        rn[0] = teleport(cur[0], P);
-                        if (DIM > 1) {
-                            cur[1] = bridge_step(cur[1], em[1], alpha, n1, sd);
-                            rn[1] = teleport(cur[1], P);
-                        }
 #pragma unroll
-                        for (int e = 0; e < DIM; ++e) buf[b * DIM + e] = rn[e];
-                        path.get(p, b, ro);
-                        dv += potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P);
+                        for (int e = 0; e < DIM; ++e) {
+                            cur[e] = bridge_step(cur[e], em[e], alpha, s_nrm[b * DIM + e], sd);
+                            buf[b * DIM + e] = teleport(cur[e], P);
+                        }
                     }
                 }
-                s_dv[k] = dv;
             }
             __syncthreads();
-            // (B) pair link differences, G lanes per link
-            if (use_pair) {
-                for (int j0 = 0; j0 < M; j0 += ngroups) {
-                    const int j = j0 + grp;
-                    T part = T(0);
-                    int t = 0, m = 0;
-                    if (j < M) {
-                        int rel = j - off;
-                        if (rel < 0) rel += M;
-                        const int k = rel / ell;
-                        t = rel - k * ell;
-                        m = min(ell, M - k * ell);
-                        if (m >= 2) {
-                            const int jn = (j + 1 == M) ? 0 : j + 1;
-                            T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
-                            path.get(p, j, o0);
-                            path.get(p, jn, o1);
+            // (B) per link j: Delta V of bead j (interior beads only) and the pair difference
+            for (int j0 = 0; j0 < M; j0 += ngroups) {
+                const int j = j0 + grp;
+                T part = T(0);
+                if (j < M) {
+                    int rel = j - off;
+                    if (rel < 0) rel += M;
+                    const int k = rel / ell, t = rel - k * ell;
+                    const int m = min(ell, M - k * ell);
+                    if (m >= 2) {
+                        const int jn = (j + 1 == M) ? 0 : j + 1;
+                        T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
+                        path.get(p, j, o0);
+                        path.get(p, jn, o1);
 #pragma unroll
-                            for (int e = 0; e < DIM; ++e) {
-                                n0[e] = (t == 0) ? o0[e] : buf[j * DIM + e];
-                                n1[e] = (t + 1 == m) ? o1[e] : buf[jn * DIM + e];
-                            }
-                            part = link_part<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, n0, n1, g, G) -
-                                   link_part<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, o0, o1, g, G);
+                        for (int e = 0; e < DIM; ++e) {
+                            n0[e] = (t == 0) ? o0[e] : buf[j * DIM + e];
+                            n1[e] = (t + 1 == m) ? o1[e] : buf[jn * DIM + e];
                         }
+                        if (g == 0) s_dvb[j] = (t >= 1) ? potV<T, DIM>(n0, P) - potV<T, DIM>(o0, P) : T(0);
+                        if (use_pair)
+                            part = link_delta<T, DIM, CELLS, R>(path, cells, N, ch.nbins, P, p, j, jn, n0, n1, o0, o1, g,
+                                                                G);
                     }
-                    part = group_sum(part, G);
+                }
+                if (use_pair) {
+                    part = group_sum_dpp(part, G);
                     if (j < M && g == 0) s_du[j] = part;
                 }
-                __syncthreads();
             }
-            // (C) decisions
+            __syncthreads();
+            // (C) decisions: Delta V over the interior beads, then the links, in order
             for (int k = tid; k < nseg; k += nt) {
                 const int m = min(ell, M - k * ell);
                 int acc = 0;
                 if (m >= 2) {
                     int start = off + k * ell;
                     if (start >= M) start -= M;
-                    T lw = -P.tau * s_dv[k];
+                    T dv = T(0);
+                    for (int i = 1; i < m; ++i) {
+                        int b = start + i;
+                        if (b >= M) b -= M;
+                        dv += s_dvb[b];
+                    }
+                    T lw = -P.tau * dv;
                     if (use_pair) {
                         for (int t = 0; t < m; ++t) {
                             int j = start + t;
@@ -1145,40 +1250,27 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
         __syncthreads();
         // ---- centre of mass: one particle after another
         for (int p = 0; p < N; ++p) {
-            T *buf = (p & 1) ? s_new1 : s_new0;
             const U4 y = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kCom);
             const T step2 = (T)a.com_step[c] * T(2);
             T d[DIM];
             d[0] = step2 * (Uniform<T>::get(y.x) - T(0.5));
             if (DIM > 1) d[1] = step2 * (Uniform<T>::get(y.y) - T(0.5));
             T part = T(0);
-            for (int j = tid; j < M; j += nt) {
-                T ro[DIM], rn[DIM];
-                path.get(p, j, ro);
+            for (int j0 = 0; j0 < M; j0 += ngroups) {
+                const int j = j0 + grp;
+                if (j < M) {
+                    const int jn = (j + 1 == M) ? 0 : j + 1;
+                    T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
+                    path.get(p, j, o0);
+                    path.get(p, jn, o1);
 #pragma unroll
-                for (int e = 0; e < DIM; ++e) {
-                    rn[e] = teleport(ro[e] + d[e], P);
-                    buf[j * DIM + e] = rn[e];
-                }
-                part += -P.tau * (potV<T, DIM>(rn, P) - potV<T, DIM>(ro, P));
-            }
-            __syncthreads();
-            if (use_pair) {
-                for (int j0 = 0; j0 < M; j0 += ngroups) {
-                    const int j = j0 + grp;
-                    if (j < M) {
-                        const int jn = (j + 1 == M) ? 0 : j + 1;
-                        T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
-                        path.get(p, j, o0);
-                        path.get(p, jn, o1);
-#pragma unroll
-                        for (int e = 0; e < DIM; ++e) {
-                            n0[e] = buf[j * DIM + e];
-                            n1[e] = buf[jn * DIM + e];
-                        }
-                        part += link_part<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, n0, n1, g, G) -
-                                link_part<T, DIM, CELLS>(path, cells, N, ch.nbins, P, p, j, o0, o1, g, G);
+                    for (int e = 0; e < DIM; ++e) {
+                        n0[e] = teleport(o0[e] + d[e], P);
+                        n1[e] = teleport(o1[e] + d[e], P);
                     }
+                    if (g == 0) part += -P.tau * (potV<T, DIM>(n0, P) - potV<T, DIM>(o0, P));
+                    if (use_pair)
+                        part += link_delta<T, DIM, CELLS, R>(path, cells, N, ch.nbins, P, p, j, jn, n0, n1, o0, o1, g, G);
                 }
             }
             // workgroup sum of the log-weight change: waves, then wave totals in order
@@ -1195,26 +1287,31 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
             if (acc) {
                 for (int j = tid; j < M; j += nt) {
                     T rn[DIM];
+                    path.get(p, j, rn);
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) rn[e] = buf[j * DIM + e];
+                    for (int e = 0; e < DIM; ++e) rn[e] = teleport(rn[e] + d[e], P);
                     path.put(p, j, rn);
                     if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn);
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
         if (a.measure) {
             const ChainMeas m = measure_chain<T, DIM, CELLS>(path, cells, ch, P, s_buf, a.radial, a.radial_bins,
                                                               a.radial_winv, a.cart, a.cart_bins, c, 8);
-            fold_meas(m, ch, q);
+            if (tid == 0) fold_meas(m, ch, s_q);
         }
+    }
+    if constexpr (LDSR) {
+        __syncthreads();
+        for (int i = tid; i < N * M * DIM; i += nt) gpath[i] = lpath[i];
     }
     // per-launch outputs (accept counts reduced deterministically)
     const double la = block_sum((double)lacc, s_buf);
     const double ca = block_sum((double)cacc, s_buf);
     if (tid == 0) {
         double *out = a.chain_meas + (size_t)c * (kNQ + 2);
-        for (int k = 0; k < kNQ; ++k) out[k] = q[k];
+        for (int k = 0; k < kNQ; ++k) out[k] = s_q[k];
         out[kNQ] = la;
         out[kNQ + 1] = ca;
     }
