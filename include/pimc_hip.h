@@ -129,7 +129,8 @@ int pimc_set_com_step(pimc_ctx *ctx, int32_t chain, double step);
 int pimc_run(pimc_ctx *ctx, int32_t n_sweeps, int32_t measure, pimc_counters *out);
 int pimc_get_estimates(pimc_ctx *ctx, int32_t chain, pimc_estimates *out);
 /* accept counters of one chain since pimc_create: {staging accepted, staging
- * tries, COM accepted, COM tries} (the Counter totals of src/updates/helper.jl:6-11) */
+ * tries, COM accepted, COM tries} (the Counter totals of src/updates/helper.jl:6-11);
+ * chain = -1 fills 4 values for every chain, in chain order */
 int pimc_get_counters(pimc_ctx *ctx, int32_t chain, int64_t *out);
 /* per-block means of one chain: E, Ev, Epair per block (each n_blocks long) */
 int pimc_get_blocks(pimc_ctx *ctx, int32_t chain, double *E, double *Ev, double *Epair, int64_t cap, int64_t *n_blocks);

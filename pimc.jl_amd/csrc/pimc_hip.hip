@@ -956,17 +956,25 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
 
 int pimc_get_counters(pimc_ctx *c, int32_t chain, int64_t *out)
 {
-    int rc = check_chain(c, chain);
-    if (rc) return rc;
+    if (!c) return PIMC_ERR_ARG;
+    if (chain != -1) {
+        int rc = check_chain(c, chain);
+        if (rc) return rc;
+    }
     if (!out) return set_err(c, PIMC_ERR_ARG, "null output");
     try {
-        long long acc[2];
+        const int n = (int)c->ch.size();
+        const int first = chain < 0 ? 0 : chain, count = chain < 0 ? n : 1;
+        std::vector<long long> acc((size_t)count * 2);
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipMemcpy(acc, c->d_cnt + (size_t)chain * 2, sizeof(acc), hipMemcpyDeviceToHost));
-        out[0] = acc[0];
-        out[1] = c->levy_tries_chain[chain] * (int64_t)c->sweep;
-        out[2] = acc[1];
-        out[3] = (int64_t)c->ch[chain].N * (int64_t)c->sweep;
+        HIPCHK(hipMemcpy(acc.data(), c->d_cnt + (size_t)first * 2, sizeof(long long) * acc.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < count; ++i) {
+            const int ch = first + i;
+            out[4 * i + 0] = acc[2 * i];
+            out[4 * i + 1] = c->levy_tries_chain[ch] * (int64_t)c->sweep;
+            out[4 * i + 2] = acc[2 * i + 1];
+            out[4 * i + 3] = (int64_t)c->ch[ch].N * (int64_t)c->sweep;
+        }
     } catch (const HipError &e) {
         return hip_err(c, e);
     }

@@ -191,6 +191,12 @@ class Context:
         self.check(self.lib.pimc_get_counters(self.h, chain, out.ctypes.data_as(_I64)), "pimc_get_counters")
         return out
 
+    def all_counters(self) -> np.ndarray:
+        """counters() of every chain, as an (n_chains, 4) array, in one copy."""
+        out = np.zeros((len(self.chains), 4), dtype=np.int64)
+        self.check(self.lib.pimc_get_counters(self.h, -1, out.ctypes.data_as(_I64)), "pimc_get_counters")
+        return out
+
     def blocks(self, chain: int):
         n = C.c_int64()
         self.check(self.lib.pimc_get_blocks(self.h, chain, None, None, None, 0, C.byref(n)), "pimc_get_blocks")

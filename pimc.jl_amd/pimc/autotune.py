@@ -70,9 +70,9 @@ def autotune(wl, trial_sweeps: int = 100, ells=(4, 8, 16, 32), tiles=(2048, 4096
                 d.ell = min(ell, d.M)
             ctx = _context(wl, trial, _options(wl, half))
             ctx.run(half, measure=False)
-            before = np.array([ctx.counters(c) for c in range(n)])
+            before = ctx.all_counters()
             ctx.run(half, measure=False)
-            after = np.array([ctx.counters(c) for c in range(n)])
+            after = ctx.all_counters()
             ctx.close()
             acc = (after[:, 0] - before[:, 0]) / np.maximum(1, after[:, 1] - before[:, 1])
             acc_table[ell] = float(acc.mean())

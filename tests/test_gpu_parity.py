@@ -274,6 +274,53 @@ def test_sweep_trajectory_fp64(measure):
             assert np.allclose(Ev, blocks_v, rtol=1e-10, atol=1e-8)
 
 
+@pytest.mark.parametrize("dim", [1, 2])
+def test_sweep_trajectory_edge_cases_fp64(dim):
+    """Tile kernel == oracle, FP64, at the edges of the schedule: one particle
+    with ell = M = 5 (one bridge around the whole ring, from a bead back to
+    itself), two-link segments (one interior bead), ell not dividing M, and 1D
+    chains.  (ell > M is rejected by pimc_create.)"""
+    rng = np.random.default_rng(31)
+    descs, inits = [], []
+    for N, M, ell in [(1, 5, 5), (2, 12, 2), (3, 16, 4), (4, 20, 6)]:
+        descs.append(chain_desc(HarmonicTrap(1.0), dim=dim, M=M, N=N, L=100.0, beta=1.0, lam=0.5, ell=ell,
+                                com_step=1.0))
+        inits.append(np.clip(rng.normal(0, 1.0, size=(N, dim, M)), -99.0, 99.0))
+    seed, n_sweeps, spl = 13, 6, 3
+    ctx = make_ctx(descs, A.FP64, seed=seed, spl=spl)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    cnt = ctx.run(n_sweeps, measure=True)
+    ref = sweep_oracle(descs, inits, seed, n_sweeps, spl, measure=True)
+    assert cnt.levy_acc == sum(x[1] for x in ref) and cnt.com_acc == sum(x[2] for x in ref)
+    assert cnt.bead_moves == n_sweeps * sum(O.sweep_bead_moves(d.N, d.M, d.ell) for d in descs)
+    for c, d in enumerate(descs):
+        assert np.max(np.abs(ctx.get_paths(c) - ref[c][0])) < 1e-9, c
+        E, Ev, _ = ctx.blocks(c)
+        Eo = np.array([e[0] for e in ref[c][4]])
+        assert np.allclose(E, [Eo[i:i + spl].mean() for i in range(0, n_sweeps, spl)], rtol=1e-10, atol=1e-8), c
+
+
+@pytest.mark.parametrize("N,L,ra", [(30, 3.0, 1.5), (110, 20.0, 2.0)])
+def test_sweep_trajectory_pair_1d_fp64(N, L, ra):
+    """Whole-chain kernel in 1D (all-pairs with the path in LDS for N = 30,
+    cell lists for N = 110) == oracle, FP64."""
+    rng = np.random.default_rng(37)
+    pair = GaussianPair(1.0, 0.5)
+    d = chain_desc(HarmonicTrap(0.3), dim=1, M=16, N=N, L=L, beta=2.0, lam=0.5, pair=pair, ell=8, r_a=ra,
+                   com_step=0.5)
+    r = rng.uniform(-L, L, size=(N, 1, 16))
+    seed, n_sweeps, spl = 6, 3, 3
+    ctx = make_ctx([d], A.FP64, seed=seed, spl=spl)
+    ctx.set_paths(0, r)
+    cnt = ctx.run(n_sweeps, measure=True)
+    ref = sweep_oracle([d], [r], seed, n_sweeps, spl, measure=True)
+    assert cnt.levy_acc == ref[0][1] and cnt.com_acc == ref[0][2]
+    assert np.max(np.abs(ctx.get_paths(0) - ref[0][0])) < 1e-9
+    E, Ev, Ep = ctx.blocks(0)
+    assert np.isfinite(E[0]) and np.isfinite(Ep[0])
+
+
 @pytest.mark.parametrize("case", ["trap", "lattice_box"])
 def test_sweep_trajectory_fp32(case):
     """FP32 tile kernel == oracle sweep schedule up to FP32 rounding.
@@ -350,6 +397,12 @@ def test_batch_fp32_runs_and_counts():
     assert 0.3 < c2.levy_acc / c2.levy_tries <= 1.0
     e = b.estimates(0)
     assert e.n_blocks == 2 and np.isfinite(e.E)
+    allc = b.ctx.all_counters()
+    assert allc.shape == (64, 4)
+    for c in (0, 17, 63):
+        assert np.array_equal(allc[c], b.ctx.counters(c))
+    tot = allc.sum(axis=0)
+    assert tot[0] == c1.levy_acc + c2.levy_acc and tot[2] == c1.com_acc + c2.com_acc
 
 
 def test_graph_replay_matches_eager_launches():
