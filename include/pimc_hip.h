@@ -81,6 +81,12 @@ typedef struct pimc_options {
     int32_t derivatives;       /* run every chain whole per workgroup and record the
                                   reweighting sums for d<E>/d omega, d<E>/d g */
     uint64_t seed;             /* Philox key */
+    int32_t measure_every;     /* estimators after every k-th sweep (0: 1), at global sweeps s with
+                                  (s + 1) % k == 0; the measurement cycle Ncycle of
+                                  measurement_Z_sector (src/measurement.jl:1-17), counted in sweeps.
+                                  sweeps_per_launch must be a multiple of it, and so must the sweep
+                                  count of a measuring pimc_run */
+    int32_t reserved;
 } pimc_options;
 
 typedef struct pimc_counters {

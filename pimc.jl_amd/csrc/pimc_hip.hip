@@ -359,6 +359,7 @@ void launch_step(pimc_ctx *c, int k, int measure)
     SweepArgs a = sweep_args(c);
     a.nsweeps = k;
     a.measure = measure ? 1 : 0;
+    a.measure_every = c->opt.measure_every;
     if (!c->tiles.empty()) launch_tiles(c, a);
     if (!c->whole_ids.empty()) launch_whole(c, a);
     PostArgs pa{};
@@ -454,10 +455,20 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         *out = c;
         return PIMC_ERR_ARG;
     }
+    {
+        const int K = opt->sweeps_per_launch > 0 ? opt->sweeps_per_launch : 50;
+        const int me = opt->measure_every > 0 ? opt->measure_every : 1;
+        if (K % me != 0) {
+            c->err = "sweeps_per_launch must be a multiple of measure_every";
+            *out = c;
+            return PIMC_ERR_ARG;
+        }
+    }
     *out = c;
     try {
         c->opt = *opt;
         if (c->opt.sweeps_per_launch <= 0) c->opt.sweeps_per_launch = 50;
+        if (c->opt.measure_every <= 0) c->opt.measure_every = 1;
         c->prec = opt->precision;
         c->dim = chains[0].dim;
         c->n_chains = n_chains;
@@ -881,6 +892,8 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
 {
     if (!c) return PIMC_ERR_ARG;
     if (n_sweeps < 0) return set_err(c, PIMC_ERR_ARG, "n_sweeps < 0");
+    if (measure && n_sweeps % c->opt.measure_every != 0)
+        return set_err(c, PIMC_ERR_ARG, "a measuring run must be a multiple of measure_every sweeps");
     try {
         std::vector<long long> before((size_t)c->n_chains * 2);
         HIPCHK(hipStreamSynchronize(c->stream));

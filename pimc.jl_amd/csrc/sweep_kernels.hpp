@@ -117,6 +117,7 @@ struct SweepArgs {
     uint64_t seed;
     const struct RunState *rs; // first sweep of this launch (device side, so a captured graph replays)
     int nsweeps, measure;
+    int measure_every; // estimators after the sweeps s with (s + 1) % measure_every == 0
     int tile0;             // first tile of this launch (tiles are grouped by potential kind)
     int link_group;        // lanes per link in the whole-chain kernel (power of two <= 64)
     // whole-chain kernel
@@ -563,7 +564,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
 
         // estimators (src/measurement.jl:45-55, 92-122): T sums over a chunk
         // of `ell` beads, FP64 accumulation across chunks, sweeps and particles
-        if (a.measure) {
+        if (a.measure && (sw + 1) % (uint32_t)a.measure_every == 0) {
 #pragma unroll
             for (int r = 0; r < kItemsPerThread; ++r) {
                 const int it = tid + r * nt;
@@ -1296,7 +1297,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
             }
             __syncthreads();
         }
-        if (a.measure) {
+        if (a.measure && (sw + 1) % (uint32_t)a.measure_every == 0) {
             const ChainMeas m = measure_chain<T, DIM, CELLS>(path, cells, ch, P, s_buf, a.radial, a.radial_bins,
                                                               a.radial_winv, a.cart, a.cart_bins, c, 8);
             if (tid == 0) fold_meas(m, ch, s_q);

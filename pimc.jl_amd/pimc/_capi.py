@@ -44,6 +44,7 @@ class Options(C.Structure):
         ("radial_bins", C.c_int32), ("cart_bins", C.c_int32), ("radial_rmax", C.c_double),
         ("tile_beads", C.c_int32), ("block_threads", C.c_int32),
         ("use_graph", C.c_int32), ("derivatives", C.c_int32), ("seed", C.c_uint64),
+        ("measure_every", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

@@ -113,10 +113,13 @@ class Batch:
 
     autotune=True picks the staging length per chain and the tile and
     workgroup sizes first (pimc.autotune); use_graph=True replays the launch
-    sequence from a captured hipGraph."""
+    sequence from a captured hipGraph; measure_every=k takes the estimators
+    after every k-th sweep (the reference's measurement cycle, counted in
+    sweeps)."""
 
     def __init__(self, wl: Workload, sweeps_per_launch: int = 50, seed: int = 12345, tile_beads: int = 0,
-                 block_threads: int = 0, autotune: bool = False, use_graph: bool = False, tune_sweeps: int = 100):
+                 block_threads: int = 0, autotune: bool = False, use_graph: bool = False, tune_sweeps: int = 100,
+                 measure_every: int = 1):
         import copy
         descs = [copy.copy(d) for d in wl.descs]
         self.tune = None
@@ -137,6 +140,7 @@ class Batch:
         opt.block_threads = block_threads
         opt.use_graph = 1 if use_graph else 0
         opt.seed = seed
+        opt.measure_every = measure_every
         self.wl = wl
         self.descs = descs
         self.ctx = Context(descs, opt)

@@ -213,14 +213,16 @@ def test_pair_moves_and_energy(precision):
     assert abs(Evp - Evpo) <= tol * max(1.0, abs(Evpo)) * 10
 
 
-def sweep_oracle(descs, inits, seed, n_sweeps, spl, measure=False):
+def sweep_oracle(descs, inits, seed, n_sweeps, spl, measure=False, measure_every=1, sweep0=0):
     """Run the oracle's sweep schedule for every chain, with the COM-step
-    adaptation between launches of `spl` sweeps when not measuring."""
+    adaptation between launches of `spl` sweeps when not measuring; when
+    measuring, the energies after the global sweeps s with (s + 1) %
+    measure_every == 0."""
     out = []
     for c, (d, r) in enumerate(zip(descs, inits)):
         s = oracle_sys(d, r)
         step = d.com_step
-        sweep = 0
+        sweep = sweep0
         acc_l = acc_c = 0
         left = n_sweeps
         Es = []
@@ -233,7 +235,7 @@ def sweep_oracle(descs, inits, seed, n_sweeps, spl, measure=False):
                 acc_c += st.com_acc
                 ca += st.com_acc
                 sweep += 1
-                if measure:
+                if measure and sweep % measure_every == 0:
                     Es.append(s.energy())
             if not measure:
                 step = O.adapt_step(step, ca, d.N * k, d.L)
@@ -272,6 +274,35 @@ def test_sweep_trajectory_fp64(measure):
             blocks_v = [Evo[i:i + spl].mean() for i in range(0, n_sweeps, spl)]
             assert np.allclose(E, blocks, rtol=1e-10, atol=1e-8)
             assert np.allclose(Ev, blocks_v, rtol=1e-10, atol=1e-8)
+
+
+def test_measure_every_fp64():
+    """measure_every = 3: the estimator blocks hold the energies after sweeps
+    3, 6, ... (global count), as in the oracle; the paths are unchanged."""
+    rng = np.random.default_rng(41)
+    descs, inits = [], []
+    for N, M, ell in [(3, 16, 4), (5, 24, 8)]:
+        descs.append(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=N, L=100.0, beta=1.0, lam=0.5, ell=ell,
+                                com_step=1.0))
+        inits.append(rand_paths(rng, N, M, 100.0))
+    opt = A.Options()
+    opt.precision, opt.sweeps_per_launch, opt.seed, opt.measure_every = A.FP64, 6, 19, 3
+    ctx = A.Context(descs, opt)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    ctx.run(12, measure=True)
+    ref = sweep_oracle(descs, inits, 19, 12, 6, measure=True, measure_every=3)
+    for c in range(len(descs)):
+        assert np.max(np.abs(ctx.get_paths(c) - ref[c][0])) < 1e-9
+        E, _, _ = ctx.blocks(c)
+        Eo = np.array([e[0] for e in ref[c][4]])  # 4 measurements: sweeps 3, 6 | 9, 12
+        assert len(Eo) == 4 and len(E) == 2
+        assert np.allclose(E, [Eo[:2].mean(), Eo[2:].mean()], rtol=1e-10, atol=1e-8)
+    with pytest.raises(A.PimcError):
+        ctx.run(4, measure=True)  # not a multiple of measure_every
+    opt.sweeps_per_launch = 5
+    with pytest.raises(A.PimcError):
+        A.Context(descs, opt)  # sweeps_per_launch not a multiple of measure_every
 
 
 @pytest.mark.parametrize("dim", [1, 2])
