@@ -124,6 +124,11 @@ int pimc_device_info(char *name, int32_t len, int32_t *cu_count);
  * particle an M x dim column-major matrix, src/system.jl:1-6). */
 int pimc_set_paths(pimc_ctx *ctx, int32_t chain, const double *r);
 int pimc_get_paths(pimc_ctx *ctx, int32_t chain, double *r);
+/* initial paths on the device, replacing init_world (src/system.jl:36-78) with
+ * a = 0: per particle the first bead uniform in [-L, L)^dim, the last bead equal
+ * to it and levy! over the M - 1 links in between; Philox-keyed by (seed,
+ * chain, particle).  chain = -1 initialises every chain.  Cell lists are rebuilt. */
+int pimc_init_paths(pimc_ctx *ctx, int32_t chain, uint64_t seed);
 int pimc_get_com_step(pimc_ctx *ctx, int32_t chain, double *step);
 int pimc_set_com_step(pimc_ctx *ctx, int32_t chain, double step);
 

@@ -77,6 +77,8 @@ def lib():
         u32p = C.POINTER(C.c_uint32)
         L.o_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.o_u01.argtypes = [C.c_uint32]; L.o_u01.restype = C.c_double
+        L.o_init_world.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, _D]
         L.o_box_muller.argtypes = [C.c_uint32, C.c_uint32, _D, _D]
         L.o_distance.argtypes = [C.c_double] * 3; L.o_distance.restype = C.c_double
         L.o_teleport.argtypes = [C.c_double] * 2; L.o_teleport.restype = C.c_double
@@ -329,6 +331,13 @@ class RefRunner:
 
 def sweep_bead_moves(N, M, ell):
     return lib().o_sweep_bead_moves(N, M, ell)
+
+
+def init_world(seed, chain, N, M, dim, L, tau, lam):
+    """o_init_world: N x dim x M initial paths (init_world with a = 0, Philox draws)."""
+    out = np.zeros((N, dim, M))
+    lib().o_init_world(seed, chain, N, M, dim, L, tau, lam, out.ctypes.data_as(_D))
+    return out
 
 
 def adapt_step(step, acc, tries, L):

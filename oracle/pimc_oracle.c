@@ -57,7 +57,7 @@ void o_box_muller(uint32_t x0, uint32_t x1, double *n0, double *n1)
 }
 
 /* draw purposes: the fourth counter word of every keyed draw */
-enum { PUR_LEVY = 1, PUR_LEVY_ACC = 2, PUR_COM = 3, PUR_OFFSET = 4 };
+enum { PUR_LEVY = 1, PUR_LEVY_ACC = 2, PUR_COM = 3, PUR_OFFSET = 4, PUR_INIT_START = 5, PUR_INIT_LEVY = 6 };
 
 static void keyed(uint64_t seed, uint32_t c0, uint32_t sweep, uint32_t chain, uint32_t purpose,
                   uint32_t out[4])
@@ -981,6 +981,44 @@ void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, d
         }
     }
     o_sys_refresh_V(s);
+    free(rp);
+    free(nrm);
+}
+
+/* ------------------------------------------------------------------------ */
+/* init_world (src/system.jl:36-78) with a = 0, Philox-keyed                 */
+/* ------------------------------------------------------------------------ */
+/* Each particle p: the first bead uniform in [-L, L)^dim (2L (u - 1/2)), the
+ * last bead equal to it, and levy! over the M - 1 links in between.  Draws:
+ * start = keyed(seed, p, 0, chain, PUR_INIT_START) words 0..dim-1; the
+ * normals of interior bead i (1 <= i <= M - 2) come in pairs (i, i+1), i
+ * odd, from keyed(seed, p*M + i, 0, chain, PUR_INIT_LEVY): Box-Muller of
+ * words (x, y) for bead i and (z, w) for bead i + 1.  out: N x dim x M. */
+void o_init_world(uint64_t seed, uint32_t chain, int N, int M, int dim, double L, double tau, double lambda,
+                  double *out)
+{
+    double *rp = (double *)malloc(sizeof(double) * (size_t)M * dim);
+    double *nrm = (double *)malloc(sizeof(double) * (size_t)(M > 2 ? M - 2 : 1) * dim);
+    for (int p = 0; p < N; ++p) {
+        uint32_t x[4];
+        keyed(seed, (uint32_t)p, 0u, chain, PUR_INIT_START, x);
+        for (int d = 0; d < dim; ++d) {
+            const double r0 = 2.0 * L * (o_u01(x[d]) - 0.5);
+            rp[d * M] = r0;
+            rp[d * M + (M - 1)] = r0;
+        }
+        for (int i = 1; i <= M - 2; ++i) {
+            double n0, n1;
+            if (i & 1) keyed(seed, (uint32_t)(p * M + i), 0u, chain, PUR_INIT_LEVY, x);
+            if (i & 1) o_box_muller(x[0], x[1], &n0, &n1);
+            else o_box_muller(x[2], x[3], &n0, &n1);
+            nrm[(i - 1) * dim] = n0;
+            if (dim > 1) nrm[(i - 1) * dim + 1] = n1;
+        }
+        o_levy(rp, M, dim, tau, L, lambda, nrm);
+        for (int d = 0; d < dim; ++d)
+            for (int j = 0; j < M; ++j) out[((size_t)p * dim + d) * M + j] = rp[d * M + j];
+    }
     free(rp);
     free(nrm);
 }

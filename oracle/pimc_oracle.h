@@ -155,6 +155,11 @@ double o_adapt_step(double step, int64_t acc, int64_t tries, double L);
 /* bead-moves of one sweep (Levy m-1 per segment, M per COM try) */
 int64_t o_sweep_bead_moves(int N, int M, int ell);
 
+/* init_world (src/system.jl:36-78) with a = 0 and Philox-keyed draws, the
+ * device initialisation pimc_init_paths restates this; out N x dim x M */
+void o_init_world(uint64_t seed, uint32_t chain, int N, int M, int dim, double L, double tau, double lambda,
+                  double *out);
+
 #ifdef __cplusplus
 }
 #endif

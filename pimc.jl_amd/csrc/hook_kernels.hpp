@@ -358,4 +358,56 @@ template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_adjoi
     }
 }
 
+// init_world (src/system.jl:36-78) with a = 0: each particle's first bead
+// uniform in [-L, L)^dim, the last bead equal to it, and levy! over the M - 1
+// links in between (the bridge arithmetic of levy_segment, teleport of every
+// row).  One thread per particle, FP64 arithmetic, stored in the chain's
+// layout; Philox purposes kInitStart and kInitLevy, sweep word 0.
+template <typename T, int DIM>
+__global__ void k_init_paths(const ChainDev *chains, const int *part_chain, const int *part_idx, int q0, int nq,
+                             void *pos, uint64_t seed)
+{
+    const int gq = q0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (gq >= q0 + nq) return;
+    const int c = part_chain[gq], p = part_idx[gq];
+    const ChainDev ch = chains[c];
+    const Prm<double> P = Prm<double>::make(ch);
+    const int M = ch.M, N = ch.N;
+    T *base = reinterpret_cast<T *>(pos) + ch.pos_off;
+    auto put = [&](int j, int d, double v) {
+        T x = (T)v;
+        if (x >= (T)ch.L) x = nextafter((T)ch.L, (T)(-ch.L)); // keep [-L, L) after rounding to T
+        if (ch.sm) base[((size_t)j * N + p) * DIM + d] = x;
+        else base[((size_t)p * DIM + d) * M + j] = x;
+    };
+    U4 x = draw(seed, (uint32_t)p, 0u, (uint32_t)c, kInitStart);
+    const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+    double r0[DIM], cur[DIM], em[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        r0[d] = 2.0 * ch.L * (u01d(w[d]) - 0.5);
+        cur[d] = r0[d];
+        em[d] = shift_end(r0[d], r0[d], P); // the ends coincide: no shift
+    }
+    const int K = M - 1; // links of the bridge
+    for (int i = 1; i < K; ++i) {
+        if (i & 1) x = draw(seed, (uint32_t)(p * M + i), 0u, (uint32_t)c, kInitLevy);
+        double n[2];
+        if (i & 1) box_muller(x.x, x.y, n[0], n[1]);
+        else box_muller(x.z, x.w, n[0], n[1]);
+        const double alpha = bridge_alpha<double>(K, i);
+        const double sd = bridge_sd(alpha, P);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            cur[d] = bridge_step(cur[d], em[d], alpha, n[d], sd);
+            put(i, d, teleport(cur[d], P));
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        put(0, d, teleport(r0[d], P));
+        put(M - 1, d, teleport(em[d], P));
+    }
+}
+
 } // namespace pimc

@@ -18,6 +18,8 @@ enum Purpose : uint32_t {
     kLevyAccept = 2, // Metropolis uniform of one staging segment, item = particle * M + start slice
     kCom = 3,        // displacement (words 0..dim-1) and Metropolis uniform (word 2), item = particle
     kOffset = 4,     // staging offset of one particle in one sweep, item = particle
+    kInitStart = 5,  // first bead of a particle's initial path (words 0..dim-1), item = particle
+    kInitLevy = 6,   // normals of the initial path, pairs (i, i+1) with i odd, item = particle * M + i
 };
 
 struct U4 {

@@ -830,6 +830,48 @@ int pimc_set_paths(pimc_ctx *c, int32_t chain, const double *r)
     return PIMC_OK;
 }
 
+int pimc_init_paths(pimc_ctx *c, int32_t chain, uint64_t seed)
+{
+    if (!c) return PIMC_ERR_ARG;
+    if (chain != -1) {
+        int rc = check_chain(c, chain);
+        if (rc) return rc;
+    }
+    try {
+        int q0 = 0, nq = (int)c->part_chain.size();
+        if (chain >= 0) {
+            q0 = c->ch[chain].part_base;
+            nq = c->ch[chain].N;
+        }
+        const int threads = 128, blocks = (nq + threads - 1) / threads;
+#define INIT(T, D)                                                                                                     \
+    hipLaunchKernelGGL((k_init_paths<T, D>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, c->d_ch, c->d_pchain, \
+                       c->d_pidx, q0, nq, c->d_pos, seed)
+        if (c->prec == PIMC_FP64) {
+            if (c->dim == 1) INIT(double, 1);
+            else INIT(double, 2);
+        } else {
+            if (c->dim == 1) INIT(float, 1);
+            else INIT(float, 2);
+        }
+#undef INIT
+        HIPCHK(hipGetLastError());
+        if (chain >= 0) {
+            if (c->ch[chain].ncell > 0) {
+                HIPCHK(hipMemsetAsync(c->d_status + chain, 0, sizeof(int), c->stream));
+                build_cells(c, chain);
+            }
+        } else if (!c->whole_cells_ids.empty()) {
+            HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int) * c->n_chains, c->stream));
+            build_cells(c, -1);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return check_status(c);
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+}
+
 int pimc_get_paths(pimc_ctx *c, int32_t chain, double *r)
 {
     int rc = check_chain(c, chain);

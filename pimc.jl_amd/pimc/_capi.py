@@ -20,7 +20,7 @@ FP32, FP64 = 0, 1
 
 EXPORTS = [
     "pimc_create", "pimc_destroy", "pimc_last_error", "pimc_device_info",
-    "pimc_set_paths", "pimc_get_paths", "pimc_get_com_step", "pimc_set_com_step",
+    "pimc_set_paths", "pimc_get_paths", "pimc_init_paths", "pimc_get_com_step", "pimc_set_com_step",
     "pimc_run", "pimc_get_estimates", "pimc_get_counters", "pimc_get_blocks", "pimc_get_radial", "pimc_get_cartesian",
     "pimc_reset_estimators", "pimc_levy_propose", "pimc_reshape_linear", "pimc_com_propose",
     "pimc_center_of_mass", "pimc_interaction", "pimc_energy", "pimc_density", "pimc_adjoint", "pimc_action",
@@ -93,6 +93,7 @@ def load(path: str = LIB_PATH):
     L.pimc_device_info.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32)]
     L.pimc_set_paths.argtypes = [vp, C.c_int32, _D]
     L.pimc_get_paths.argtypes = [vp, C.c_int32, _D]
+    L.pimc_init_paths.argtypes = [vp, C.c_int32, C.c_uint64]
     L.pimc_get_com_step.argtypes = [vp, C.c_int32, _D]
     L.pimc_set_com_step.argtypes = [vp, C.c_int32, C.c_double]
     L.pimc_run.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(Counters)]
@@ -156,6 +157,10 @@ class Context:
             pass
 
     # -- paths
+    def init_paths(self, chain: int = -1, seed: int = 0):
+        """init_world on the device for one chain (or every chain: -1)."""
+        self.check(self.lib.pimc_init_paths(self.h, chain, seed), "pimc_init_paths")
+
     def set_paths(self, chain: int, r: np.ndarray):
         d = self.chains[chain]
         r = np.ascontiguousarray(np.asarray(r, dtype=np.float64)).reshape(d.N, d.dim, d.M)

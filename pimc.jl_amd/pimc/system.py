@@ -109,8 +109,9 @@ class System:
         self.ctx = Context([desc], opt)
         self.nbins = max(1, int(math.floor(2 * L / (desc.r_a if desc.r_a > 0 else L / 4))))
         if r0 is None:
-            r0 = init_world(N, M, dim, L, self.tau, lam, self.rng)
-        self.ctx.set_paths(0, r0)
+            self.ctx.init_paths(0, int(self.rng.integers(0, 2 ** 63)))  # init_world on the device
+        else:
+            self.ctx.set_paths(0, r0)
 
     # paths in the C ABI layout (N, dim, M)
     @property

@@ -485,3 +485,30 @@ def test_cell_list_overflow_is_reported():
     ok = np.random.default_rng(1).uniform(-8, 8, size=(200, 2, 8))
     ctx.set_paths(0, ok)   # a valid configuration clears the flag
     ctx.run(2, measure=False)
+
+
+@pytest.mark.parametrize("precision", [A.FP64, A.FP32])
+def test_init_paths_matches_oracle(precision):
+    """pimc_init_paths == the oracle's init_world restatement for every chain:
+    tile chains (particle-major) and pair chains (slice-major, with cell lists
+    rebuilt afterwards, so a sweep runs)."""
+    pair = GaussianPair(1.0, 0.5)
+    descs = [harmonic_desc(N=5, M=16, L=6.0), harmonic_desc(N=3, M=24, beta=2.0, L=100.0),
+             chain_desc(NoPotential(), dim=2, M=16, N=120, L=6.0, beta=2.0, lam=0.5, pair=pair, ell=8, r_a=2.0),
+             chain_desc(NoPotential(), dim=2, M=12, N=20, L=3.0, beta=1.0, lam=0.5, pair=pair, ell=4, r_a=1.5)]
+    ctx = make_ctx(descs, precision, seed=3)
+    ctx.init_paths(-1, 99)
+    tol = 1e-12 if precision == A.FP64 else 1e-5
+    for c, d in enumerate(descs):
+        ref = O.init_world(99, c, d.N, d.M, d.dim, d.L, d.beta / d.M, d.lam)
+        got = ctx.get_paths(c)
+        diff = np.abs(got - ref)
+        diff = np.minimum(diff, 2 * d.L - diff)  # a bead at the wall may round to the other side in FP32
+        assert np.max(diff) <= tol * max(1.0, d.L), (c, np.max(diff))
+    before = ctx.get_paths(0)
+    ctx.init_paths(1, 5)  # one chain: the others keep their paths
+    assert np.array_equal(ctx.get_paths(0), before)
+    ref1 = O.init_world(5, 1, 3, 24, 2, 100.0, 2.0 / 24, 0.5)
+    assert np.max(np.abs(ctx.get_paths(1) - ref1)) <= tol * 100.0
+    cnt = ctx.run(4, measure=False)
+    assert cnt.levy_acc > 0

@@ -292,3 +292,35 @@ def test_sweep_bead_move_count_and_step_rule():
     assert O.adapt_step(1.0, 5, 10, 100.0) == 1.0
     assert O.adapt_step(0.1, 0, 10, 100.0) == 0.1
     assert O.adapt_step(49.0, 10, 10, 100.0) == 50.0
+
+
+def test_init_world_structure_and_keys():
+    """init_world (src/system.jl:36-78, a = 0): first bead == last bead, all in
+    [-L, L), reproducible for (seed, chain), different for another chain."""
+    r = O.init_world(11, 2, 6, 12, 2, 3.0, 0.2, 0.5)
+    assert r.shape == (6, 2, 12)
+    assert np.array_equal(r[:, :, 0], r[:, :, -1])
+    assert np.all(r >= -3.0) and np.all(r < 3.0)
+    assert np.array_equal(r, O.init_world(11, 2, 6, 12, 2, 3.0, 0.2, 0.5))
+    assert not np.allclose(r, O.init_world(11, 3, 6, 12, 2, 3.0, 0.2, 0.5))
+
+
+def test_init_world_statistics():
+    """Start beads uniform in the box (Kolmogorov-Smirnov), interior beads a
+    Brownian bridge from the start bead back to itself: mean 0 and variance
+    2 lambda tau i (K - i) / K per coordinate relative to the start, K = M - 1
+    links (a wide box, so no wrap)."""
+    from scipy import stats
+    N, M, L, tau, lam = 20000, 9, 1000.0, 0.3, 0.5
+    r = O.init_world(5, 0, N, M, 2, L, tau, lam)
+    start = r[:, :, 0].ravel()
+    assert stats.kstest((start + L) / (2 * L), "uniform").pvalue > 1e-3
+    K = M - 1
+    rel = r - r[:, :, :1]
+    rel -= 2 * L * np.round(rel / (2 * L))  # minimum image: paths that start near the wall wrap
+    for i in range(1, M - 1):
+        var = 2 * lam * tau * i * (K - i) / K
+        x = rel[:, :, i].ravel()
+        assert abs(x.mean()) < 5 * math.sqrt(var / x.size)
+        # variance of a variance estimate: 2 var^2 / n
+        assert abs(x.var() - var) < 5 * var * math.sqrt(2 / x.size), (i, x.var(), var)
