@@ -62,7 +62,8 @@ class ORefState(C.Structure):
 
 class OSweepStats(C.Structure):
     _fields_ = [("levy_acc", C.c_int64), ("levy_tries", C.c_int64), ("com_acc", C.c_int64),
-                ("com_tries", C.c_int64), ("bead_moves", C.c_int64)]
+                ("com_tries", C.c_int64), ("bead_moves", C.c_int64), ("swap_acc", C.c_int64),
+                ("swap_tries", C.c_int64)]
 
 
 _D = C.POINTER(C.c_double)
@@ -77,6 +78,8 @@ def lib():
         u32p = C.POINTER(C.c_uint32)
         L.o_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.o_u01.argtypes = [C.c_uint32]; L.o_u01.restype = C.c_double
+        L.o_sweep_perm.argtypes = [C.POINTER(OSys), C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_double,
+                                   C.c_int, C.c_int, C.c_int, C.POINTER(OSweepStats)]
         L.o_init_world.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                    C.c_double, _D]
         L.o_box_muller.argtypes = [C.c_uint32, C.c_uint32, _D, _D]
@@ -304,6 +307,21 @@ class System:
         st = OSweepStats()
         lib().o_sweep(self.ref, chain, sweep, seed, ell, com_step, C.byref(st))
         return st
+
+    def sweep_perm(self, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule=0):
+        """o_sweep_perm: the sweep with permutation cycles and n_swap swap tries."""
+        st = OSweepStats()
+        lib().o_sweep_perm(self.ref, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule, C.byref(st))
+        return st
+
+    @property
+    def next(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._s.next, shape=(self.N,)).copy()
+
+    @next.setter
+    def next(self, nxt):
+        a = np.ctypeslib.as_array(self._s.next, shape=(self.N,))
+        a[:] = np.asarray(nxt, dtype=np.int64)
 
 
 class RefRunner:

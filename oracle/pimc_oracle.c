@@ -57,7 +57,8 @@ void o_box_muller(uint32_t x0, uint32_t x1, double *n0, double *n1)
 }
 
 /* draw purposes: the fourth counter word of every keyed draw */
-enum { PUR_LEVY = 1, PUR_LEVY_ACC = 2, PUR_COM = 3, PUR_OFFSET = 4, PUR_INIT_START = 5, PUR_INIT_LEVY = 6 };
+enum { PUR_LEVY = 1, PUR_LEVY_ACC = 2, PUR_COM = 3, PUR_OFFSET = 4, PUR_INIT_START = 5, PUR_INIT_LEVY = 6,
+       PUR_SWAP = 7, PUR_SWAP_LEVY = 8 };
 
 static void keyed(uint64_t seed, uint32_t c0, uint32_t sweep, uint32_t chain, uint32_t purpose,
                   uint32_t out[4])
@@ -874,8 +875,14 @@ static double link_pair(const o_sys *s, int n, int j, const double *pj, const do
     return w;
 }
 
-void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
-             o_sweepstats *st)
+/* bead u of the polymer that starts at (p, slice 0): u < M on p, M <= u < 2M on next[p] */
+static inline int owner(const o_sys *s, int p, int u) { return u < s->M ? p : (int)s->next[p]; }
+
+static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int n_swap, int m, int rule,
+                    o_sweepstats *st);
+
+static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
+                       int n_swap, int swap_m, int rule, o_sweepstats *st)
 {
     const int M = s->M, dim = s->dim, N = s->N;
     const int nseg = (M + ell - 1) / ell;
@@ -891,11 +898,12 @@ void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, d
         for (int k = 0; k < nseg; ++k) {
             int m = (M - k * ell) < ell ? (M - k * ell) : ell;
             if (m < 2) continue;
-            int start = (off + k * ell) % M;
+            const int us = off + k * ell; /* window position of the start bead (< 2M) */
+            int start = us % M;
             const int nrows = m + 1;
             for (int d = 0; d < dim; ++d) {
-                rp[d * nrows + 0] = R(s, p, d, start);
-                rp[d * nrows + m] = R(s, p, d, (start + m) % M);
+                rp[d * nrows + 0] = R(s, owner(s, p, us), d, start);
+                rp[d * nrows + m] = R(s, owner(s, p, us + m), d, (start + m) % M);
             }
             for (int i = 1; i < m; ++i) {
                 int b = (start + i) % M;
@@ -917,7 +925,7 @@ void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, d
             double dv = 0.0, a[3], b[3];
             for (int i = 1; i < m; ++i) {
                 int j = (start + i) % M;
-                for (int d = 0; d < dim; ++d) { a[d] = rp[d * nrows + i]; b[d] = R(s, p, d, j); }
+                for (int d = 0; d < dim; ++d) { a[d] = rp[d * nrows + i]; b[d] = R(s, owner(s, p, us + i), d, j); }
                 dv += o_V(&s->pot, a, dim) - o_V(&s->pot, b, dim);
             }
             double lw = -s->tau * dv;
@@ -940,14 +948,15 @@ void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, d
             if (acc) {
                 st->levy_acc++;
                 for (int i = 1; i < m; ++i) {
-                    int j = (start + i) % M;
-                    for (int d = 0; d < dim; ++d) s->r[((size_t)p * dim + d) * M + j] = rp[d * nrows + i];
+                    int j = (start + i) % M, q = owner(s, p, us + i);
+                    for (int d = 0; d < dim; ++d) s->r[((size_t)q * dim + d) * M + j] = rp[d * nrows + i];
                 }
             }
         }
     }
-    /* COM phase */
+    /* COM phase: 1-cycles only (SingleCenterOfMass picks among next[i] == i) */
     for (int p = 0; p < N; ++p) {
+        if (s->next[p] != p) continue;
         keyed(seed, (uint32_t)p, sweep, chain, PUR_COM, x);
         double dsp[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) dsp[d] = (com_step * 2.0) * (o_u01(x[d]) - 0.5);
@@ -980,8 +989,157 @@ void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, d
                 for (int d = 0; d < dim; ++d) s->r[((size_t)p * dim + d) * M + j] = rp[d * M + j];
         }
     }
+    if (n_swap > 0) o_swaps(s, chain, sweep, seed, n_swap, swap_m, rule, st);
     o_sys_refresh_V(s);
     free(rp);
+    free(nrm);
+}
+
+void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
+             o_sweepstats *st)
+{
+    sweep_core(s, chain, sweep, seed, ell, com_step, 0, 0, 0, st);
+}
+
+void o_sweep_perm(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
+                  int swap_m, int rule, o_sweepstats *st)
+{
+    sweep_core(s, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule, st);
+}
+
+/* free-particle propagator over m links, up to its normalisation:
+ * exp(lnK) = exp(-|x - y|^2 / (4 lambda m tau)), minimum image (src/propagator.jl:16-19) */
+static double rho0(const o_sys *s, const double *x, const double *y, int m)
+{
+    double dd = 0.0;
+    for (int d = 0; d < s->dim; ++d) {
+        double r = o_distance(x[d], y[d], s->L);
+        dd += r * r;
+    }
+    return exp(-dd / ((4.0 * s->lambda) * ((double)m * s->tau)));
+}
+
+/* Swap tries of one sweep.  Try a: words of keyed(seed, a, sweep, chain,
+ * PUR_SWAP) give the slice j0 = floor(x0 M / 2^32), the first particle n1 =
+ * floor(x1 N / 2^32), the selection uniform u01(x2) and the Metropolis uniform
+ * u01(x3).  E_i is the bead m links after (i, j0) along i's cycle.  n2 is drawn
+ * with weights w_i = rho0(r_n1(j0), E_i) rho0(r_i(j0), E_n1) (the exchanged
+ * product of sampleparticles), scanning i in order; n2 == n1 rejects.  Paths:
+ * n1 from A = r_n1(j0) to D = E_n2 and n2 from B = r_n2(j0) to C = E_n1, by levy!
+ * with normals keyed(seed, (2a + w) M + i, sweep, chain, PUR_SWAP_LEVY) paired as
+ * in the staging move (w = 0 for n1, 1 for n2).  Delta lnW = -tau sum over the
+ * interior beads of both paths of (V' - V).  rule 0 multiplies exp(Delta lnW)
+ * by (1/W'(n1) + 1/W'(n2)) / (1/W(n1) + 1/W(n2)), W(n) = sum_i of n's selection
+ * weights (primed: after the swap): the ratio of the reverse and forward
+ * selection probabilities, either particle drawn first, which with free
+ * bridges gives detailed balance.  rule 1 is the reference's exp(Delta lnW).
+ * Accepted: new interiors written along the new cycles; for j0 + m < M the
+ * beads j0 + m .. M - 1 of n1 and n2 are exchanged; next[n1] <-> next[n2]. */
+static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int n_swap, int m, int rule,
+                    o_sweepstats *st)
+{
+    const int M = s->M, N = s->N, dim = s->dim;
+    double *E = (double *)malloc(sizeof(double) * (size_t)N * dim);
+    double *B0 = (double *)malloc(sizeof(double) * (size_t)N * dim);
+    double *rp1 = (double *)malloc(sizeof(double) * (size_t)(m + 1) * dim);
+    double *rp2 = (double *)malloc(sizeof(double) * (size_t)(m + 1) * dim);
+    double *nrm = (double *)malloc(sizeof(double) * (size_t)(m + 1) * dim);
+    uint32_t x[4];
+    for (int a = 0; a < n_swap; ++a) {
+        keyed(seed, (uint32_t)a, sweep, chain, PUR_SWAP, x);
+        const int j0 = (int)(((uint64_t)x[0] * (uint64_t)M) >> 32);
+        const int n1 = (int)(((uint64_t)x[1] * (uint64_t)N) >> 32);
+        const double usel = o_u01(x[2]), uacc = o_u01(x[3]);
+        st->swap_tries++;
+        st->bead_moves += 2 * (m - 1);
+        if (N < 2) continue;
+        const int jm = j0 + m, sm = jm % M;
+        for (int i = 0; i < N; ++i) {
+            const int oi = jm < M ? i : (int)s->next[i];
+            for (int d = 0; d < dim; ++d) {
+                E[i * dim + d] = R(s, oi, d, sm);
+                B0[i * dim + d] = R(s, i, d, j0);
+            }
+        }
+        const double *A = B0 + n1 * dim, *C = E + n1 * dim;
+        double W1 = 0.0;
+        for (int i = 0; i < N; ++i) W1 += rho0(s, A, E + i * dim, m) * rho0(s, B0 + i * dim, C, m);
+        const double target = usel * W1;
+        double cum = 0.0;
+        int n2 = N - 1;
+        for (int i = 0; i < N; ++i) {
+            cum += rho0(s, A, E + i * dim, m) * rho0(s, B0 + i * dim, C, m);
+            if (cum > target) { n2 = i; break; }
+        }
+        if (n2 == n1) continue;
+        const double *Bp = B0 + n2 * dim, *D = E + n2 * dim;
+        /* the two bridges */
+        for (int w = 0; w < 2; ++w) {
+            double *rp = w == 0 ? rp1 : rp2;
+            const double *st0 = w == 0 ? A : Bp, *en = w == 0 ? D : C;
+            const int nrows = m + 1;
+            for (int d = 0; d < dim; ++d) { rp[d * nrows] = st0[d]; rp[d * nrows + m] = en[d]; }
+            for (int i = 1; i < m; ++i) {
+                double n0, nn1;
+                if (i & 1) keyed(seed, (uint32_t)((2 * a + w) * M + i), sweep, chain, PUR_SWAP_LEVY, x);
+                if (i & 1) o_box_muller(x[0], x[1], &n0, &nn1);
+                else o_box_muller(x[2], x[3], &n0, &nn1);
+                nrm[(i - 1) * dim] = n0;
+                if (dim > 1) nrm[(i - 1) * dim + 1] = nn1;
+            }
+            o_levy(rp, nrows, dim, s->tau, s->L, s->lambda, nrm);
+            for (int d = 0; d < dim; ++d) { rp[d * nrows] = st0[d]; rp[d * nrows + m] = en[d]; }
+        }
+        /* Delta lnW: interior beads of both paths, new minus old along the old cycles */
+        double dv = 0.0, na[3], ob[3];
+        for (int w = 0; w < 2; ++w) {
+            const int nw = w == 0 ? n1 : n2;
+            const double *rp = w == 0 ? rp1 : rp2;
+            for (int i = 1; i < m; ++i) {
+                const int u = j0 + i, j = u % M;
+                for (int d = 0; d < dim; ++d) { na[d] = rp[d * (m + 1) + i]; ob[d] = R(s, owner(s, nw, u), d, j); }
+                dv += o_V(&s->pot, na, dim) - o_V(&s->pot, ob, dim);
+            }
+        }
+        double ratio = exp(-s->tau * dv);
+        if (rule == 0) {
+            /* selection normalisations before (E) and after (E' = E with n1, n2's ends exchanged) */
+            double W2 = 0.0, W1p = 0.0, W2p = 0.0;
+            for (int i = 0; i < N; ++i) {
+                const double *Ei = E + i * dim, *Bi = B0 + i * dim;
+                const double *Epi = i == n1 ? D : (i == n2 ? C : Ei);
+                W2 += rho0(s, Bp, Ei, m) * rho0(s, Bi, D, m);
+                W1p += rho0(s, A, Epi, m) * rho0(s, Bi, D, m);
+                W2p += rho0(s, Bp, Epi, m) * rho0(s, Bi, C, m);
+            }
+            ratio *= (1.0 / W1p + 1.0 / W2p) / (1.0 / W1 + 1.0 / W2);
+        }
+        if (!o_metropolis(ratio, uacc)) continue;
+        st->swap_acc++;
+        const int q1 = (int)s->next[n1], q2 = (int)s->next[n2];
+        for (int w = 0; w < 2; ++w) {
+            const int nw = w == 0 ? n1 : n2, qn = w == 0 ? q2 : q1; /* the new next of nw */
+            const double *rp = w == 0 ? rp1 : rp2;
+            for (int i = 1; i < m; ++i) {
+                const int u = j0 + i, j = u % M, q = u < M ? nw : qn;
+                for (int d = 0; d < dim; ++d) s->r[((size_t)q * dim + d) * M + j] = rp[d * (m + 1) + i];
+            }
+        }
+        if (jm < M) {
+            for (int j = jm; j < M; ++j)
+                for (int d = 0; d < dim; ++d) {
+                    double t = s->r[((size_t)n1 * dim + d) * M + j];
+                    s->r[((size_t)n1 * dim + d) * M + j] = s->r[((size_t)n2 * dim + d) * M + j];
+                    s->r[((size_t)n2 * dim + d) * M + j] = t;
+                }
+        }
+        s->next[n1] = q2;
+        s->next[n2] = q1;
+    }
+    free(E);
+    free(B0);
+    free(rp1);
+    free(rp2);
     free(nrm);
 }
 

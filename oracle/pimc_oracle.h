@@ -147,9 +147,19 @@ int  o_run_reference(o_sys *s, o_refstate *st, int64_t n_iter, uint64_t seed, in
  * at a Philox-drawn offset (particles in order), then one centre-of-mass try
  * per particle.  Draws are keyed by (chain, sweep, bead).  Returns the number
  * of accepted Levy segments and COM moves. */
-typedef struct { int64_t levy_acc, levy_tries, com_acc, com_tries, bead_moves; } o_sweepstats;
+typedef struct { int64_t levy_acc, levy_tries, com_acc, com_tries, bead_moves, swap_acc, swap_tries; } o_sweepstats;
 void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
              o_sweepstats *st);
+/* The sweep with permutations (chains without a pair action): the staging
+ * window of particle p runs along its permutation cycle (beads past slice
+ * M-1 belong to next[p]), the centre-of-mass try moves 1-cycles only
+ * (SingleCenterOfMass, src/updates/com.jl:136-160), and n_swap swap tries of
+ * m links follow (ReshapeSwapLinear, src/updates/reshape.jl:99-283, with
+ * sampleparticles, src/updates/helper.jl:224-267).  rule 0: acceptance with
+ * the heat-bath normalisations (detailed balance); rule 1: the reference's
+ * exp(w_updated - w_initial) only. */
+void o_sweep_perm(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
+                  int swap_m, int rule, o_sweepstats *st);
 /* the COM step adaptation applied between launches while thermalising */
 double o_adapt_step(double step, int64_t acc, int64_t tries, double L);
 /* bead-moves of one sweep (Levy m-1 per segment, M per COM try) */

@@ -80,6 +80,35 @@ def energy_boson_continuum(N: int, beta: float, dim: int = 2) -> float:
     return -(math.log(Z(N, beta + h)) - math.log(Z(N, beta - h))) / (2 * h)
 
 
+def _ln_ring(K: int, tau: float, omega: float, lam: float) -> float:
+    """ln of one coordinate's primitive-action partition function of a ring of
+    K beads at time step tau (kinetic normalisation per bead included, a
+    constant per bead dropped): -1/2 sum_k ln(2 - 2 cos(2 pi k/K) + 2 lam tau^2 omega^2)."""
+    k = np.arange(K)
+    return float(-0.5 * np.sum(np.log(2.0 - 2.0 * np.cos(2.0 * np.pi * k / K) + 2.0 * lam * tau * tau * omega ** 2)))
+
+
+def ln_z_bosons_primitive(N: int, M: int, beta: float, dim: int = 2, omega: float = 1.0, lam: float = 0.5) -> float:
+    """ln Z_N of N bosons in the trap at M slices: the canonical recursion
+    Z_n = (1/n) sum_k z_k Z_{n-k}, with z_k the d-dimensional partition
+    function of one k-cycle, a ring of k M beads at the same tau = beta/M."""
+    tau = beta / M
+    lnz = [0.0] + [dim * _ln_ring(k * M, tau, omega, lam) for k in range(1, N + 1)]
+    lnZ = [0.0]
+    for n in range(1, N + 1):
+        terms = np.array([lnz[k] + lnZ[n - k] for k in range(1, n + 1)])
+        mx = terms.max()
+        lnZ.append(float(mx + math.log(np.exp(terms - mx).sum()) - math.log(n)))
+    return lnZ[N]
+
+
+def energy_boson_primitive(N: int, M: int, beta: float, dim: int = 2, omega: float = 1.0, lam: float = 0.5,
+                           h: float = 1e-5) -> float:
+    """-d ln Z_N / d beta at fixed M (the thermodynamic estimator's mean)."""
+    return -(ln_z_bosons_primitive(N, M, beta + h, dim, omega, lam)
+             - ln_z_bosons_primitive(N, M, beta - h, dim, omega, lam)) / (2 * h)
+
+
 def radial_expected(N: int, M: int, nb: int, rmax: float, x2: float) -> np.ndarray:
     """Expected counts per radial bin and per measurement of N*M beads whose
     coordinates are i.i.d. Normal(0, x2) in 2D (Rayleigh distribution)."""

@@ -324,3 +324,61 @@ def test_init_world_statistics():
         assert abs(x.mean()) < 5 * math.sqrt(var / x.size)
         # variance of a variance estimate: 2 var^2 / n
         assert abs(x.var() - var) < 5 * var * math.sqrt(2 / x.size), (i, x.var(), var)
+
+
+def _oracle_trap(N, M, beta, seed=3):
+    rng = np.random.default_rng(seed)
+    r = rng.normal(0, 0.7, size=(N, 2, M))
+    return O.System(r, 0.5, 100.0, beta, O.make_pot(1, 1.0, 0.0, 1.0), None, 0.0, False)
+
+
+def _blocked(x):
+    x = np.asarray(x, float)
+    b, errs = x.copy(), []
+    while len(b) >= 16:
+        errs.append(b.std(ddof=1) / math.sqrt(len(b)))
+        b = 0.5 * (b[: len(b) // 2 * 2: 2] + b[1: len(b) // 2 * 2: 2])
+    return x.mean(), max(errs)
+
+
+def test_swap_exchange_sector_probability():
+    """Two trapped bosons: the fraction of configurations in the exchanged
+    sector is z_2 / (z_1^2 + z_2), z_k the partition function of one k-cycle
+    (a ring of k M beads).  The swap move with the heat-bath normalisations
+    (rule 0) samples it; the reference's acceptance exp(w_updated - w_initial)
+    (rule 1, src/updates/reshape.jl:266) does not."""
+    M, beta, m = 8, 1.0, 4
+    tau = beta / M
+    lz1, lz2 = 2 * AN._ln_ring(M, tau, 1.0, 0.5), 2 * AN._ln_ring(2 * M, tau, 1.0, 0.5)
+    p = math.exp(lz2) / (math.exp(2 * lz1) + math.exp(lz2))
+    out = {}
+    for rule in (0, 1):
+        s = _oracle_trap(2, M, beta)
+        f = []
+        for sw in range(150000):
+            s.sweep_perm(0, sw, 91, 4, 1.0, 2, m, rule)
+            if sw >= 2000:
+                f.append(float(s.next[0] != 0))
+        out[rule] = _blocked(f)
+    (f0, e0), (f1, e1) = out[0], out[1]
+    assert abs(f0 - p) < 3.5 * e0, (f0, e0, p)
+    assert (f1 - p) / e1 > 3.0, (f1, e1, p)
+
+
+def test_swap_boson_energy_closed_form():
+    """Three trapped bosons, M = 8, beta = 2: the thermodynamic energy with
+    swaps matches -d ln Z_3 / d beta of the primitive action (canonical
+    recursion over cycles), far from the distinguishable-particle value."""
+    N, M, beta = 3, 8, 2.0
+    s = _oracle_trap(N, M, beta, seed=1)
+    E = []
+    for sw in range(150000):
+        st = s.sweep_perm(0, sw, 77, 4, 1.0, 3, 4, 0)
+        if sw >= 2000:
+            E.append(s.energy()[0])
+    nxt = s.next
+    assert sorted(nxt.tolist()) == list(range(N))  # still a permutation
+    e, err = _blocked(E)
+    exact = AN.energy_boson_primitive(N, M, beta)
+    assert abs(e - exact) < 3.5 * err, (e, err, exact)
+    assert AN.energy_boltzmann_primitive(N, M, beta) - e > 20 * err
