@@ -60,13 +60,17 @@ typedef struct pimc_chain_desc {
                                 reference's behaviour, where the pair sums of
                                 interaction_action! are computed and dropped */
     int32_t ell;             /* staging segment length >= 2 (0: library default) */
-    int32_t reserved;
+    int32_t swaps;           /* swap tries per sweep (ReshapeSwapLinear, src/updates/reshape.jl:99-283);
+                                chains with swaps have no pair action and run whole per workgroup */
     double beta, lambda, L;  /* L is the half-width: box [-L, L)^dim (src/system.jl:101) */
     double omega;            /* harmonic trap */
     double V0, a;            /* square lattice */
     double g, sigma;         /* Gaussian pair repulsion */
     double r_a;              /* cell-list range: nbins = floor(2L / r_a); 0 -> L/4 (src/system.jl:22-24) */
     double com_step;         /* initial centre-of-mass step (SingleCenterOfMass(s, step)) */
+    int32_t swap_m;          /* links of a swap try, 2 .. M - 1 (0: ell) */
+    int32_t swap_rule;       /* 0: acceptance with the heat-bath normalisations of the partner choice
+                                (detailed balance); 1: the reference's exp(w_updated - w_initial) */
 } pimc_chain_desc;
 
 typedef struct pimc_options {
@@ -96,6 +100,7 @@ typedef struct pimc_counters {
     double wall_ms;            /* device time of the whole call (events on the library stream) */
     double kernel_ms;          /* summed device time of the sweep kernel launches */
     int64_t kernel_launches;
+    int64_t swap_tries, swap_acc;
 } pimc_counters;
 
 /* Estimates of one chain from the measured blocks (block means, standard
@@ -129,6 +134,11 @@ int pimc_get_paths(pimc_ctx *ctx, int32_t chain, double *r);
  * to it and levy! over the M - 1 links in between; Philox-keyed by (seed,
  * chain, particle).  chain = -1 initialises every chain.  Cell lists are rebuilt. */
 int pimc_init_paths(pimc_ctx *ctx, int32_t chain, uint64_t seed);
+/* permutation of one chain (Worldline.next, src/system.jl:1-6; next(), src/updates/helper.jl:287-300):
+ * next[p] is the particle whose slice 0 follows slice M - 1 of particle p (0-based).  Identity
+ * unless the chain has swap moves; set_next checks that it is a permutation */
+int pimc_set_next(pimc_ctx *ctx, int32_t chain, const int32_t *next);
+int pimc_get_next(pimc_ctx *ctx, int32_t chain, int32_t *next);
 int pimc_get_com_step(pimc_ctx *ctx, int32_t chain, double *step);
 int pimc_set_com_step(pimc_ctx *ctx, int32_t chain, double step);
 

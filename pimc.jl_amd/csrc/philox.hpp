@@ -20,6 +20,8 @@ enum Purpose : uint32_t {
     kOffset = 4,     // staging offset of one particle in one sweep, item = particle
     kInitStart = 5,  // first bead of a particle's initial path (words 0..dim-1), item = particle
     kInitLevy = 6,   // normals of the initial path, pairs (i, i+1) with i odd, item = particle * M + i
+    kSwap = 7,       // swap try a of a sweep: slice, first particle, partner uniform, Metropolis uniform; item = a
+    kSwapLevy = 8,   // normals of swap try a's path w (0, 1), pairs (i, i+1) with i odd, item = (2a + w) M + i
 };
 
 struct U4 {

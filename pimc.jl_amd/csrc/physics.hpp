@@ -32,7 +32,8 @@ struct ChainDev {
     int N, M, dim, pot;
     int pair, pair_in_accept, ell, nbins;
     int cap, ncell, part_base, nseg;
-    int sm, pad0, pad1, pad2; // sm: positions stored slice-major (chains with a pair action)
+    int sm;                   // positions stored slice-major (chains run whole per workgroup)
+    int swaps, swap_m, swap_rule; // swap tries per sweep, links per try, acceptance rule
 };
 
 // The chain constants converted to the arithmetic type (built once per chain

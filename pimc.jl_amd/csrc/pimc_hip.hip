@@ -77,7 +77,7 @@ struct pimc_ctx {
     int maxl = 8;
     int threads_tile = 256, threads_whole = 256;
     size_t lds_tile = 0, lds_whole = 0;
-    int64_t moves_per_sweep = 0, levy_tries_per_sweep = 0, com_tries_per_sweep = 0;
+    int64_t moves_per_sweep = 0, levy_tries_per_sweep = 0; // staging only: COM and swap tries are counted on the device
     uint32_t sweep = 0;
     int nblk = 0, nblk_cap = 0;
     // device
@@ -95,6 +95,7 @@ struct pimc_ctx {
     unsigned long long *d_radial = nullptr, *d_cart = nullptr;
     int *d_whole = nullptr, *d_whole_cells = nullptr, *d_whole_brute = nullptr, *d_is_whole = nullptr;
     int *d_whole_lds = nullptr;
+    int *d_next = nullptr; // per particle: the particle that follows it across slice M - 1
     int *d_cell_cnt = nullptr, *d_bins = nullptr;
     unsigned short *d_cell_mem = nullptr;
     int *d_status = nullptr;
@@ -180,7 +181,7 @@ void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &ids, const
     size_t lds = c->lds_whole;
     for (int id : ids) {
         maxM = std::max(maxM, c->ch[id].M);
-        if (LDSR) lds = std::max(lds, whole_lds_bytes(c->ch[id].M, DIM, sizeof(T)) + whole_path_bytes(c->ch[id], sizeof(T)));
+        if (LDSR) lds = std::max(lds, whole_lds_bytes(c->ch[id], sizeof(T)) + whole_path_bytes(c->ch[id], sizeof(T)));
     }
     // lanes per link: as many as the workgroup holds for one pass over the links (2..16)
     int G = 1;
@@ -239,6 +240,7 @@ SweepArgs sweep_args(pimc_ctx *c)
     a.tiles = c->d_tiles;
     a.part_chain = c->d_pchain;
     a.part_idx = c->d_pidx;
+    a.next = c->d_next;
     a.pos = c->d_pos;
     a.part_acc = c->d_part_acc;
     a.part_lacc = c->d_placc;
@@ -403,6 +405,14 @@ int validate(const pimc_chain_desc &d, std::string &why)
     if (d.potential == PIMC_POT_LATTICE && !(d.a > 0)) return why = "lattice constant a must be positive", -1;
     if (d.ell != 0 && (d.ell < 2 || d.ell > 32 || d.ell > d.M)) return why = "ell must be in [2, min(32, M)]", -1;
     if (!(d.com_step >= 0)) return why = "com_step must be >= 0", -1;
+    if (d.swaps < 0) return why = "swaps must be >= 0", -1;
+    if (d.swaps > 0) {
+        if (d.pair != PIMC_PAIR_NONE) return why = "swap moves are for chains without a pair action", -1;
+        if (d.N > 4096) return why = "swap moves need N <= 4096", -1;
+        const int sm = d.swap_m ? d.swap_m : (d.ell ? d.ell : std::min(8, d.M));
+        if (sm < 2 || sm > d.M - 1) return why = "swap_m must be in [2, M - 1]", -1;
+        if (d.swap_rule != 0 && d.swap_rule != 1) return why = "swap_rule must be 0 or 1", -1;
+    }
     return 0;
 }
 
@@ -513,8 +523,11 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             const double ra = d.r_a > 0 ? d.r_a : d.L / 4.0;                // src/system.jl:22-24
             h.nbins = std::max(1, (int)std::floor((2.0 * d.L) / ra));     // src/system.jl:81
             h.h = (2.0 * d.L) / h.nbins;                                  // src/nearest_neighbours.jl:127
+            h.swaps = d.swaps;
+            h.swap_m = d.swaps > 0 ? (d.swap_m ? d.swap_m : h.ell) : 0;
+            h.swap_rule = d.swap_rule;
             ellmax = std::max(ellmax, h.ell);
-            c->is_whole[i] = (d.pair != PIMC_PAIR_NONE || opt->derivatives) ? 1 : 0;
+            c->is_whole[i] = (d.pair != PIMC_PAIR_NONE || opt->derivatives || d.swaps > 0) ? 1 : 0;
         }
         c->maxl = next_pow2_maxl(ellmax);
         if (c->maxl > 32) throw HipError{hipErrorInvalidValue, "ell > 32"};
@@ -645,7 +658,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         for (int id : c->whole_ids) maxMw = std::max(maxMw, c->ch[id].M);
         c->threads_whole = 1024;
         if (maxMw > 0) {
-            c->lds_whole = whole_lds_bytes(maxMw, c->dim, ts);
+            c->lds_whole = 0;
+            for (int id : c->whole_ids) c->lds_whole = std::max(c->lds_whole, whole_lds_bytes(c->ch[id], ts));
             if (c->lds_whole > 160 * 1024) throw HipError{hipErrorInvalidValue, "M too large for the whole-chain kernel"};
         }
         // all-pairs chains whose positions fit in LDS next to the buffers keep them there
@@ -653,7 +667,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             std::vector<int> keep;
             for (int id : c->whole_brute_ids) {
                 const ChainDev &h = c->ch[id];
-                if (whole_lds_bytes(h.M, c->dim, ts) + whole_path_bytes(h, ts) <= 160 * 1024) c->whole_lds_ids.push_back(id);
+                if (whole_lds_bytes(h, ts) + whole_path_bytes(h, ts) <= 160 * 1024) c->whole_lds_ids.push_back(id);
                 else keep.push_back(id);
             }
             c->whole_brute_ids.swap(keep);
@@ -671,8 +685,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             }
             c->levy_tries_per_sweep += (int64_t)h.N * segs;
             c->levy_tries_chain.push_back((int64_t)h.N * segs);
-            c->com_tries_per_sweep += h.N;
-            c->moves_per_sweep += (int64_t)h.N * (per + h.M);
+            c->moves_per_sweep += (int64_t)h.N * per; // staging; COM and swap moves are counted on the device
         }
 
         // ---- device allocations
@@ -709,10 +722,17 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             for (int i = 0; i < n_chains; ++i) st[i] = chains[i].com_step;
             HIPCHK(hipMemcpy(c->d_step, st.data(), sizeof(double) * n_chains, hipMemcpyHostToDevice));
         }
-        c->d_cnt = dalloc<long long>((size_t)n_chains * 2);
-        HIPCHK(hipMemset(c->d_cnt, 0, sizeof(long long) * n_chains * 2));
-        c->d_chain_meas = dalloc<double>((size_t)n_chains * (kNQ + 2));
-        HIPCHK(hipMemset(c->d_chain_meas, 0, sizeof(double) * n_chains * (kNQ + 2)));
+        {
+            std::vector<int> ident(c->n_part);
+            for (int i = 0; i < n_chains; ++i)
+                for (int p = 0; p < c->ch[i].N; ++p) ident[(size_t)c->ch[i].part_base + p] = p;
+            c->d_next = dalloc<int>(c->n_part);
+            HIPCHK(hipMemcpy(c->d_next, ident.data(), sizeof(int) * c->n_part, hipMemcpyHostToDevice));
+        }
+        c->d_cnt = dalloc<long long>((size_t)n_chains * kCnt);
+        HIPCHK(hipMemset(c->d_cnt, 0, sizeof(long long) * n_chains * kCnt));
+        c->d_chain_meas = dalloc<double>((size_t)n_chains * (kNQ + kCnt));
+        HIPCHK(hipMemset(c->d_chain_meas, 0, sizeof(double) * n_chains * (kNQ + kCnt)));
         c->d_is_whole = dalloc<int>(n_chains);
         HIPCHK(hipMemcpy(c->d_is_whole, c->is_whole.data(), sizeof(int) * n_chains, hipMemcpyHostToDevice));
         c->d_whole = dalloc<int>(c->whole_ids.size());
@@ -773,7 +793,7 @@ int pimc_destroy(pimc_ctx *c)
     void *ptrs[] = {c->d_ch, c->d_prm, c->d_tiles, c->d_pchain, c->d_pidx, c->d_pos, c->d_part_acc, c->d_placc, c->d_pcacc,
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
-                    c->d_status, c->d_work, c->d_io, c->d_whole_lds};
+                    c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next};
     for (void *p : ptrs) dfree(p);
     for (auto &e : c->events) {
         (void)hipEventDestroy(e.a);
@@ -824,6 +844,43 @@ int pimc_set_paths(pimc_ctx *c, int32_t chain, const double *r)
             HIPCHK(hipStreamSynchronize(c->stream));
             return check_status(c);
         }
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_set_next(pimc_ctx *c, int32_t chain, const int32_t *next)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!next) return set_err(c, PIMC_ERR_ARG, "null next buffer");
+    const ChainDev &h = c->ch[chain];
+    std::vector<char> seen(h.N, 0);
+    bool ident = true;
+    for (int p = 0; p < h.N; ++p) {
+        if (next[p] < 0 || next[p] >= h.N || seen[next[p]]) return set_err(c, PIMC_ERR_ARG, "next is not a permutation");
+        seen[next[p]] = 1;
+        ident = ident && next[p] == p;
+    }
+    if (!ident && h.swaps == 0) return set_err(c, PIMC_ERR_ARG, "permutation cycles need a chain with swap moves");
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(c->d_next + h.part_base, next, sizeof(int) * h.N, hipMemcpyHostToDevice));
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_get_next(pimc_ctx *c, int32_t chain, int32_t *next)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!next) return set_err(c, PIMC_ERR_ARG, "null next buffer");
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(next, c->d_next + c->ch[chain].part_base, sizeof(int) * c->ch[chain].N, hipMemcpyDeviceToHost));
     } catch (const HipError &e) {
         return hip_err(c, e);
     }
@@ -937,7 +994,7 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
     if (measure && n_sweeps % c->opt.measure_every != 0)
         return set_err(c, PIMC_ERR_ARG, "a measuring run must be a multiple of measure_every sweeps");
     try {
-        std::vector<long long> before((size_t)c->n_chains * 2);
+        std::vector<long long> before((size_t)c->n_chains * kCnt);
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipMemcpy(before.data(), c->d_cnt, sizeof(long long) * before.size(), hipMemcpyDeviceToHost));
         const int K = c->opt.sweeps_per_launch;
@@ -983,14 +1040,18 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
             std::memset(out, 0, sizeof(*out));
             std::vector<long long> after(before.size());
             HIPCHK(hipMemcpy(after.data(), c->d_cnt, sizeof(long long) * after.size(), hipMemcpyDeviceToHost));
+            out->bead_moves = c->moves_per_sweep * n_sweeps;
             for (int i = 0; i < c->n_chains; ++i) {
-                out->levy_acc += after[i * 2] - before[i * 2];
-                out->com_acc += after[i * 2 + 1] - before[i * 2 + 1];
+                const long long *b = before.data() + (size_t)i * kCnt, *e = after.data() + (size_t)i * kCnt;
+                out->levy_acc += e[0] - b[0];
+                out->com_acc += e[1] - b[1];
+                out->com_tries += e[2] - b[2];
+                out->swap_acc += e[3] - b[3];
+                out->swap_tries += e[4] - b[4];
+                out->bead_moves += (int64_t)c->ch[i].M * (e[2] - b[2]) + 2 * (int64_t)(c->ch[i].swap_m - 1) * (e[4] - b[4]);
             }
             out->sweeps = n_sweeps;
-            out->bead_moves = c->moves_per_sweep * n_sweeps;
             out->levy_tries = c->levy_tries_per_sweep * n_sweeps;
-            out->com_tries = c->com_tries_per_sweep * n_sweeps;
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
             out->wall_ms = ms;
@@ -1020,15 +1081,15 @@ int pimc_get_counters(pimc_ctx *c, int32_t chain, int64_t *out)
     try {
         const int n = (int)c->ch.size();
         const int first = chain < 0 ? 0 : chain, count = chain < 0 ? n : 1;
-        std::vector<long long> acc((size_t)count * 2);
+        std::vector<long long> acc((size_t)count * kCnt);
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipMemcpy(acc.data(), c->d_cnt + (size_t)first * 2, sizeof(long long) * acc.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(acc.data(), c->d_cnt + (size_t)first * kCnt, sizeof(long long) * acc.size(), hipMemcpyDeviceToHost));
         for (int i = 0; i < count; ++i) {
             const int ch = first + i;
-            out[4 * i + 0] = acc[2 * i];
+            out[4 * i + 0] = acc[kCnt * i];
             out[4 * i + 1] = c->levy_tries_chain[ch] * (int64_t)c->sweep;
-            out[4 * i + 2] = acc[2 * i + 1];
-            out[4 * i + 3] = (int64_t)c->ch[ch].N * (int64_t)c->sweep;
+            out[4 * i + 2] = acc[kCnt * i + 1];
+            out[4 * i + 3] = acc[kCnt * i + 2];
         }
     } catch (const HipError &e) {
         return hip_err(c, e);

@@ -22,6 +22,8 @@ namespace pimc {
 
 constexpr int kItemsPerThread = 1; // register slots of the estimator sums per thread
 constexpr int kTileMaxThreads = 256; // workgroup size bound of the tile kernel
+// cumulative counters per chain: staging accepted, COM accepted, COM tries, swaps accepted, swap tries
+constexpr int kCnt = 5;
 constexpr size_t kTileLdsBudget = 40 * 1024; // LDS of one tile: 4 workgroups share a CU's 160 KiB
 constexpr int kNQ = 12;            // per-chain per-block estimator quantities
 
@@ -118,6 +120,7 @@ struct SweepArgs {
     const struct RunState *rs; // first sweep of this launch (device side, so a captured graph replays)
     int nsweeps, measure;
     int measure_every; // estimators after the sweeps s with (s + 1) % measure_every == 0
+    int *next;         // per particle: the particle whose slice 0 follows its slice M - 1
     int tile0;             // first tile of this launch (tiles are grouped by potential kind)
     int link_group;        // lanes per link in the whole-chain kernel (power of two <= 64)
     // whole-chain kernel
@@ -951,7 +954,7 @@ template <typename T, int DIM, bool CELLS, typename PathT>
 __device__ inline ChainMeas measure_chain(const PathT &path, const Cells &C, const ChainDev &ch, const Prm<T> &P,
                                           double *wbuf, unsigned long long *radial, int radial_bins,
                                           double radial_winv, unsigned long long *cart, int cart_bins, int c,
-                                          int G = 1)
+                                          int G = 1, const int *nxt = nullptr)
 {
     const int N = ch.N, M = ch.M;
     const double inv_s2 = (ch.pair != kPairNone) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
@@ -971,7 +974,7 @@ __device__ inline ChainMeas measure_chain(const PathT &path, const Cells &C, con
         const int jn = (j + 1 == M) ? 0 : j + 1;
         T r0[DIM], r1[DIM];
         path.get(i, j, r0);
-        path.get(i, jn, r1);
+        path.get((jn == 0 && nxt) ? nxt[i] : i, jn, r1); // the next bead along the permutation cycle
         T dd = T(0);
 #pragma unroll
         for (int e = 0; e < DIM; ++e) {
@@ -1042,7 +1045,7 @@ __device__ inline void fold_meas(const ChainMeas &m, const ChainDev &ch, double 
 // (M x DIM T each), Delta V per bead and Delta lnU per link (M T each), accept
 // flags (M ints), two sets of 16 wave partials (double), 16 doubles for block
 // sums, kNQ doubles of estimator sums.
-__host__ __device__ inline size_t whole_lds_bytes(int M, int dim, int tsize)
+__host__ __device__ inline size_t whole_lds_bytes(int M, int dim, int tsize, int nperm = 0)
 {
     size_t o = 3 * align16((size_t)M * dim * tsize);
     o += 2 * align16((size_t)M * tsize);
@@ -1050,7 +1053,27 @@ __host__ __device__ inline size_t whole_lds_bytes(int M, int dim, int tsize)
     o += align16(32 * sizeof(double));
     o += align16(16 * sizeof(double));
     o += align16(kNQ * sizeof(double));
+    // chains with swap moves: next[] and the starts and ends of a swap try (nperm = N)
+    o += align16((size_t)nperm * 4) + 2 * align16((size_t)nperm * dim * tsize) + align16(8 * 4);
     return o;
+}
+__host__ __device__ inline size_t whole_lds_bytes(const ChainDev &h, int tsize)
+{
+    return whole_lds_bytes(h.M, h.dim, tsize, h.swaps > 0 ? h.N : 0);
+}
+
+// Free-particle propagator over m links up to its normalisation,
+// exp(lnK) = exp(-|x - y|^2 / (4 lambda m tau)) with the minimum image
+// (src/propagator.jl:16-19): the weights of the swap partner choice.
+template <typename T, int DIM> __device__ __forceinline__ T rho0(const T *x, const T *y, const Prm<T> &P, int m)
+{
+    T dd = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const T r = mindist(x[d], y[d], P);
+        dd += r * r;
+    }
+    return ex(-dd / ((T(4) * P.lambda) * ((T)m * P.tau)));
 }
 
 // LDS that holds a whole chain's positions (chains whose path stays in LDS).
@@ -1082,7 +1105,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     const int tid = threadIdx.x, nt = blockDim.x;
     const int N = ch.N, M = ch.M, ell = ch.ell, nseg = ch.nseg;
     T *const gpath = reinterpret_cast<T *>(a.pos) + ch.pos_off;
-    T *const lpath = reinterpret_cast<T *>(smem + whole_lds_bytes(M, DIM, sizeof(T)));
+    T *const lpath = reinterpret_cast<T *>(smem + whole_lds_bytes(ch, sizeof(T)));
     if constexpr (LDSR) {
         for (int i = tid; i < N * M * DIM; i += nt) lpath[i] = gpath[i];
         __syncthreads();
@@ -1114,9 +1137,26 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     double *s_buf = reinterpret_cast<double *>(smem + o);
     o += align16(16 * sizeof(double));
     double *s_q = reinterpret_cast<double *>(smem + o); // estimator sums of the launch
+    o += align16(kNQ * sizeof(double));
+    // permutation cycles (chains with swap moves): next[] in LDS for the launch
+    const bool perm = ch.swaps > 0;
+    const int np_ = perm ? N : 0;
+    int *s_next = reinterpret_cast<int *>(smem + o);
+    o += align16((size_t)np_ * 4);
+    T *s_E = reinterpret_cast<T *>(smem + o); // a swap try's end beads E_i
+    o += align16((size_t)np_ * DIM * sizeof(T));
+    T *s_B = reinterpret_cast<T *>(smem + o); // ... and start beads r_i(j0)
+    o += align16((size_t)np_ * DIM * sizeof(T));
+    int *s_sw = reinterpret_cast<int *>(smem + o);
+    int *const gnext = a.next + ch.part_base;
+    for (int i = tid; i < np_; i += nt) s_next[i] = gnext[i];
+    // the particle holding bead u of the window that starts on particle pp:
+    // u < M on pp, u >= M on next[pp] (pp itself without permutations)
+    auto owner = [&](int pp, int u) { return (perm && u >= M) ? s_next[pp] : pp; };
 
     if (tid < kNQ) s_q[tid] = 0.0;
-    int lacc = 0, cacc = 0, overflow = 0;
+    int lacc = 0, cacc = 0, ctry = 0, sacc = 0, stry = 0, overflow = 0;
+    __syncthreads();
     const int hp = ell >> 1; // Philox blocks per segment: interior bead pairs (i, i + 1), i odd
 
     for (int s = 0; s < a.nsweeps; ++s) {
@@ -1151,13 +1191,13 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
             for (int k = tid; k < nseg; k += nt) {
                 const int m = min(ell, M - k * ell);
                 if (m >= 2) {
-                    int start = off + k * ell;
-                    if (start >= M) start -= M;
+                    const int us = off + k * ell; // window position of the start bead (< 2M)
+                    const int start = us >= M ? us - M : us;
                     int jm = start + m;
                     if (jm >= M) jm -= M;
                     T cur[DIM], em[DIM];
-                    path.get(p, start, cur);
-                    path.get(p, jm, em);
+                    path.get(owner(p, us), start, cur);
+                    path.get(owner(p, us + m), jm, em);
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
                     for (int i = 1; i < m; ++i) {
@@ -1185,9 +1225,10 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                     const int m = min(ell, M - k * ell);
                     if (m >= 2) {
                         const int jn = (j + 1 == M) ? 0 : j + 1;
+                        const int u = j >= off ? j : j + M; // window position of slice j
                         T o0[DIM], o1[DIM], n0[DIM], n1[DIM];
-                        path.get(p, j, o0);
-                        path.get(p, jn, o1);
+                        path.get(owner(p, u), j, o0);
+                        path.get(owner(p, u + 1), jn, o1);
 #pragma unroll
                         for (int e = 0; e < DIM; ++e) {
                             n0[e] = (t == 0) ? o0[e] : buf[j * DIM + e];
@@ -1243,14 +1284,17 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                     T rn[DIM];
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) rn[e] = buf[j * DIM + e];
-                    path.put(p, j, rn);
+                    path.put(owner(p, j >= off ? j : j + M), j, rn);
                     if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn);
                 }
             }
         }
         __syncthreads();
-        // ---- centre of mass: one particle after another
+        // ---- centre of mass: one particle after another, 1-cycles only
+        // (SingleCenterOfMass picks among next[i] == i, src/updates/com.jl:141-160)
         for (int p = 0; p < N; ++p) {
+            if (perm && s_next[p] != p) continue; // uniform: LDS value
+            ctry += (tid == 0) ? 1 : 0;
             const U4 y = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kCom);
             const T step2 = (T)a.com_step[c] * T(2);
             T d[DIM];
@@ -1297,24 +1341,153 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
             }
             __syncthreads();
         }
+        // ---- swap tries (chains without a pair action): the partner choice and the
+        // acceptance are serial in particle order on one thread, as in the oracle
+        for (int aa = 0; aa < ch.swaps; ++aa) {
+            const U4 xs = draw(a.seed, (uint32_t)aa, sw, (uint32_t)c, kSwap);
+            const int m = ch.swap_m;
+            const int j0 = (int)(((uint64_t)xs.x * (uint64_t)M) >> 32);
+            const int n1 = (int)(((uint64_t)xs.y * (uint64_t)N) >> 32);
+            const int jm = j0 + m, sm = jm >= M ? jm - M : jm;
+            stry += (tid == 0) ? 1 : 0;
+            if (N < 2) continue;
+            for (int i = tid; i < N; i += nt) {
+                T e[DIM], b[DIM];
+                path.get(jm < M ? i : s_next[i], sm, e);
+                path.get(i, j0, b);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    s_E[i * DIM + d] = e[d];
+                    s_B[i * DIM + d] = b[d];
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                const T *A = s_B + n1 * DIM, *C = s_E + n1 * DIM;
+                T W1 = T(0);
+                for (int i = 0; i < N; ++i) W1 += rho0<T, DIM>(A, s_E + i * DIM, P, m) * rho0<T, DIM>(s_B + i * DIM, C, P, m);
+                const T target = Uniform<T>::get(xs.z) * W1;
+                T cum = T(0);
+                int n2 = N - 1;
+                for (int i = 0; i < N; ++i) {
+                    cum += rho0<T, DIM>(A, s_E + i * DIM, P, m) * rho0<T, DIM>(s_B + i * DIM, C, P, m);
+                    if (cum > target) {
+                        n2 = i;
+                        break;
+                    }
+                }
+                s_sw[0] = n2;
+            }
+            __syncthreads();
+            const int n2 = s_sw[0];
+            if (n2 == n1) continue; // uniform: LDS value
+            // the two bridges: n1 from r_n1(j0) to E_n2, n2 from r_n2(j0) to E_n1
+            if (tid < 2) {
+                const int w = tid;
+                T *bw = w == 0 ? s_new0 : s_new1;
+                const T *st0 = s_B + (w == 0 ? n1 : n2) * DIM, *en = s_E + (w == 0 ? n2 : n1) * DIM;
+                T cur[DIM], em[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    cur[d] = st0[d];
+                    em[d] = shift_end(cur[d], en[d], P);
+                }
+                U4 xr{0u, 0u, 0u, 0u};
+                for (int i = 1; i < m; ++i) {
+                    if (i & 1) xr = draw(a.seed, (uint32_t)((2 * aa + w) * M + i), sw, (uint32_t)c, kSwapLevy);
+                    T nn[2];
+                    if (i & 1) box_muller(xr.x, xr.y, nn[0], nn[1]);
+                    else box_muller(xr.z, xr.w, nn[0], nn[1]);
+                    const T alpha = bridge_alpha<T>(m, i);
+                    const T sd = bridge_sd(alpha, P);
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        cur[d] = bridge_step(cur[d], em[d], alpha, nn[d], sd);
+                        bw[i * DIM + d] = teleport(cur[d], P);
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                // Delta lnW over the interior beads of both paths (old beads along the old cycles)
+                T dv = T(0);
+                for (int w = 0; w < 2; ++w) {
+                    const int nw = w == 0 ? n1 : n2;
+                    const T *bw = w == 0 ? s_new0 : s_new1;
+                    for (int i = 1; i < m; ++i) {
+                        const int u = j0 + i;
+                        T ob[DIM];
+                        path.get(owner(nw, u), u >= M ? u - M : u, ob);
+                        dv += potV<T, DIM>(bw + i * DIM, P) - potV<T, DIM>(ob, P);
+                    }
+                }
+                T ratio = ex(-P.tau * dv);
+                if (ch.swap_rule == 0) {
+                    // reverse over forward partner-choice probability, either particle drawn first
+                    const T *A = s_B + n1 * DIM, *C = s_E + n1 * DIM, *Bp = s_B + n2 * DIM, *D = s_E + n2 * DIM;
+                    T W1 = T(0), W2 = T(0), W1p = T(0), W2p = T(0);
+                    for (int i = 0; i < N; ++i) {
+                        const T *Ei = s_E + i * DIM, *Bi = s_B + i * DIM;
+                        const T *Epi = i == n1 ? D : (i == n2 ? C : Ei);
+                        W1 += rho0<T, DIM>(A, Ei, P, m) * rho0<T, DIM>(Bi, C, P, m);
+                        W2 += rho0<T, DIM>(Bp, Ei, P, m) * rho0<T, DIM>(Bi, D, P, m);
+                        W1p += rho0<T, DIM>(A, Epi, P, m) * rho0<T, DIM>(Bi, D, P, m);
+                        W2p += rho0<T, DIM>(Bp, Epi, P, m) * rho0<T, DIM>(Bi, C, P, m);
+                    }
+                    ratio *= (T(1) / W1p + T(1) / W2p) / (T(1) / W1 + T(1) / W2);
+                }
+                s_sw[1] = metropolis(ratio, Uniform<T>::get(xs.w)) ? 1 : 0;
+            }
+            __syncthreads();
+            if (s_sw[1]) { // uniform: LDS value
+                const int q1 = s_next[n1], q2 = s_next[n2];
+                for (int t = tid; t < 2 * (m - 1); t += nt) {
+                    const int w = t / (m - 1), i = t - w * (m - 1) + 1;
+                    const int u = j0 + i;
+                    const int q = u < M ? (w == 0 ? n1 : n2) : (w == 0 ? q2 : q1); // along the new cycles
+                    path.put(q, u >= M ? u - M : u, (w == 0 ? s_new0 : s_new1) + i * DIM);
+                }
+                if (jm < M) {
+                    for (int j = jm + tid; j < M; j += nt) {
+                        T r1[DIM], r2[DIM];
+                        path.get(n1, j, r1);
+                        path.get(n2, j, r2);
+                        path.put(n1, j, r2);
+                        path.put(n2, j, r1);
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    s_next[n1] = q2;
+                    s_next[n2] = q1;
+                    sacc += 1;
+                }
+            }
+            __syncthreads();
+        }
         if (a.measure && (sw + 1) % (uint32_t)a.measure_every == 0) {
             const ChainMeas m = measure_chain<T, DIM, CELLS>(path, cells, ch, P, s_buf, a.radial, a.radial_bins,
-                                                              a.radial_winv, a.cart, a.cart_bins, c, 8);
+                                                              a.radial_winv, a.cart, a.cart_bins, c, 8,
+                                                              perm ? s_next : nullptr);
             if (tid == 0) fold_meas(m, ch, s_q);
         }
     }
+    __syncthreads();
     if constexpr (LDSR) {
-        __syncthreads();
         for (int i = tid; i < N * M * DIM; i += nt) gpath[i] = lpath[i];
     }
+    for (int i = tid; i < np_; i += nt) gnext[i] = s_next[i];
     // per-launch outputs (accept counts reduced deterministically)
     const double la = block_sum((double)lacc, s_buf);
     const double ca = block_sum((double)cacc, s_buf);
     if (tid == 0) {
-        double *out = a.chain_meas + (size_t)c * (kNQ + 2);
+        double *out = a.chain_meas + (size_t)c * (kNQ + kCnt);
         for (int k = 0; k < kNQ; ++k) out[k] = s_q[k];
         out[kNQ] = la;
         out[kNQ + 1] = ca;
+        out[kNQ + 2] = ctry;
+        out[kNQ + 3] = sacc;
+        out[kNQ + 4] = stry;
     }
     if (overflow) atomicOr(a.status + c, 1);
 }
@@ -1346,11 +1519,15 @@ __global__ void k_post(PostArgs a)
     long long la = 0, ca = 0;
     double q[kNQ];
     for (int k = 0; k < kNQ; ++k) q[k] = 0.0;
+    long long ct = (long long)ch.N * a.nsweeps, sa = 0, st = 0;
     if (a.is_whole[c]) {
-        const double *src = a.chain_meas + (size_t)c * (kNQ + 2);
+        const double *src = a.chain_meas + (size_t)c * (kNQ + kCnt);
         for (int k = 0; k < kNQ; ++k) q[k] = src[k];
         la = (long long)src[kNQ];
         ca = (long long)src[kNQ + 1];
+        ct = (long long)src[kNQ + 2];
+        sa = (long long)src[kNQ + 3];
+        st = (long long)src[kNQ + 4];
     } else {
         double l = 0.0, v = 0.0, g = 0.0, nm = 0.0;
         for (int p = 0; p < ch.N; ++p) {
@@ -1376,13 +1553,17 @@ __global__ void k_post(PostArgs a)
             }
         }
     }
-    a.counters[(size_t)c * 2 + 0] += la;
-    a.counters[(size_t)c * 2 + 1] += ca;
+    long long *cnt = a.counters + (size_t)c * kCnt;
+    cnt[0] += la;
+    cnt[1] += ca;
+    cnt[2] += ct;
+    cnt[3] += sa;
+    cnt[4] += st;
     if (a.measure) {
         double *b = a.blocks + ((size_t)a.rs->blk * a.n_chains + c) * kNQ;
         for (int k = 0; k < kNQ; ++k) b[k] = q[k];
     } else {
-        const long long tries = (long long)ch.N * a.nsweeps;
+        const long long tries = ct;
         double step = a.com_step[c];
         if (tries > 0) {
             const double acc = (double)ca / (double)tries;

@@ -20,7 +20,8 @@ FP32, FP64 = 0, 1
 
 EXPORTS = [
     "pimc_create", "pimc_destroy", "pimc_last_error", "pimc_device_info",
-    "pimc_set_paths", "pimc_get_paths", "pimc_init_paths", "pimc_get_com_step", "pimc_set_com_step",
+    "pimc_set_paths", "pimc_get_paths", "pimc_init_paths", "pimc_set_next", "pimc_get_next",
+    "pimc_get_com_step", "pimc_set_com_step",
     "pimc_run", "pimc_get_estimates", "pimc_get_counters", "pimc_get_blocks", "pimc_get_radial", "pimc_get_cartesian",
     "pimc_reset_estimators", "pimc_levy_propose", "pimc_reshape_linear", "pimc_com_propose",
     "pimc_center_of_mass", "pimc_interaction", "pimc_energy", "pimc_density", "pimc_adjoint", "pimc_action",
@@ -31,10 +32,11 @@ class ChainDesc(C.Structure):
     _fields_ = [
         ("N", C.c_int32), ("M", C.c_int32), ("dim", C.c_int32),
         ("potential", C.c_int32), ("pair", C.c_int32), ("pair_in_accept", C.c_int32),
-        ("ell", C.c_int32), ("reserved", C.c_int32),
+        ("ell", C.c_int32), ("swaps", C.c_int32),
         ("beta", C.c_double), ("lam", C.c_double), ("L", C.c_double),
         ("omega", C.c_double), ("V0", C.c_double), ("a", C.c_double),
         ("g", C.c_double), ("sigma", C.c_double), ("r_a", C.c_double), ("com_step", C.c_double),
+        ("swap_m", C.c_int32), ("swap_rule", C.c_int32),
     ]
 
 
@@ -53,6 +55,7 @@ class Counters(C.Structure):
         ("sweeps", C.c_int64), ("bead_moves", C.c_int64),
         ("levy_tries", C.c_int64), ("levy_acc", C.c_int64), ("com_tries", C.c_int64), ("com_acc", C.c_int64),
         ("wall_ms", C.c_double), ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64),
+        ("swap_tries", C.c_int64), ("swap_acc", C.c_int64),
     ]
 
 
@@ -94,6 +97,8 @@ def load(path: str = LIB_PATH):
     L.pimc_set_paths.argtypes = [vp, C.c_int32, _D]
     L.pimc_get_paths.argtypes = [vp, C.c_int32, _D]
     L.pimc_init_paths.argtypes = [vp, C.c_int32, C.c_uint64]
+    L.pimc_set_next.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.pimc_get_next.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     L.pimc_get_com_step.argtypes = [vp, C.c_int32, _D]
     L.pimc_set_com_step.argtypes = [vp, C.c_int32, C.c_double]
     L.pimc_run.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(Counters)]
@@ -157,6 +162,16 @@ class Context:
             pass
 
     # -- paths
+    def set_next(self, chain: int, nxt):
+        """The permutation of one chain: nxt[p] follows particle p across slice M - 1."""
+        a = np.ascontiguousarray(np.asarray(nxt, dtype=np.int32))
+        self.check(self.lib.pimc_set_next(self.h, chain, a.ctypes.data_as(C.POINTER(C.c_int32))), "pimc_set_next")
+
+    def get_next(self, chain: int) -> np.ndarray:
+        a = np.zeros(self.chains[chain].N, dtype=np.int32)
+        self.check(self.lib.pimc_get_next(self.h, chain, a.ctypes.data_as(C.POINTER(C.c_int32))), "pimc_get_next")
+        return a
+
     def init_paths(self, chain: int = -1, seed: int = 0):
         """init_world on the device for one chain (or every chain: -1)."""
         self.check(self.lib.pimc_init_paths(self.h, chain, seed), "pimc_init_paths")

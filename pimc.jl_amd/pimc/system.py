@@ -51,7 +51,10 @@ def init_world(N: int, M: int, dim: int, L: float, tau: float, lam: float, rng: 
 
 
 def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_in_accept=True, ell=0,
-               com_step=1.0) -> ChainDesc:
+               com_step=1.0, swaps=0, swap_m=0, swap_rule=0) -> ChainDesc:
+    """pimc_chain_desc of one chain.  swaps > 0 adds that many swap tries per
+    sweep (ReshapeSwapLinear) of swap_m links (0: ell); swap_rule 1 keeps the
+    reference's acceptance exp(w_updated - w_initial)."""
     d = ChainDesc()
     d.N, d.M, d.dim = int(N), int(M), int(dim)
     d.potential = getattr(potential, "kind", 0)
@@ -70,6 +73,7 @@ def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_i
     d.pair_in_accept = 1 if pair_in_accept else 0
     d.ell = int(ell)
     d.com_step = float(com_step)
+    d.swaps, d.swap_m, d.swap_rule = int(swaps), int(swap_m), int(swap_rule)
     return d
 
 

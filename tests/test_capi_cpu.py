@@ -29,9 +29,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     # field counts and sizes of the ABI structs (8-byte aligned doubles after int32 groups)
-    assert ctypes.sizeof(A.ChainDesc) == 8 * 4 + 10 * 8
+    assert ctypes.sizeof(A.ChainDesc) == 8 * 4 + 10 * 8 + 2 * 4
     assert ctypes.sizeof(A.Options) == 4 * 4 + 8 + 4 * 4 + 8 + 2 * 4
-    assert ctypes.sizeof(A.Counters) == 6 * 8 + 2 * 8 + 8
+    assert ctypes.sizeof(A.Counters) == 6 * 8 + 2 * 8 + 8 + 2 * 8
     assert ctypes.sizeof(A.Estimates) == 2 * 8 + 12 * 8
 
 

@@ -512,3 +512,53 @@ def test_init_paths_matches_oracle(precision):
     assert np.max(np.abs(ctx.get_paths(1) - ref1)) <= tol * 100.0
     cnt = ctx.run(4, measure=False)
     assert cnt.levy_acc > 0
+
+
+def test_swap_sweeps_match_oracle_fp64():
+    """Whole-chain kernel with permutation cycles and swap tries == the
+    oracle's sweep_perm, FP64: paths, next[], all counters, energy blocks
+    (swap windows inside the ring and across slice M - 1, both rules)."""
+    rng = np.random.default_rng(43)
+    specs = [(4, 12, 4, 3, 4, 0, 2.0), (3, 16, 8, 2, 10, 0, 1.0), (5, 8, 4, 4, 3, 1, 3.0)]
+    descs, inits = [], []
+    for N, M, ell, nsw, m, rule, beta in specs:
+        descs.append(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=N, L=100.0, beta=beta, lam=0.5, ell=ell,
+                                com_step=0.8, swaps=nsw, swap_m=m, swap_rule=rule))
+        inits.append(rand_paths(rng, N, M, 100.0, scale=0.6))
+    seed, n_sweeps = 29, 12
+    ctx = make_ctx(descs, A.FP64, seed=seed, spl=n_sweeps)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    cnt = ctx.run(n_sweeps, measure=True)
+    tot = dict(levy_acc=0, com_acc=0, com_tries=0, swap_acc=0, swap_tries=0)
+    for c, (d, r) in enumerate(zip(descs, inits)):
+        s = oracle_sys(d, r)
+        Es = []
+        for sw in range(n_sweeps):
+            st = s.sweep_perm(c, sw, seed, d.ell, d.com_step, d.swaps, d.swap_m, d.swap_rule)
+            for k in tot:
+                tot[k] += getattr(st, k)
+            Es.append(s.energy()[0])
+        assert np.array_equal(ctx.get_next(c), s.next), c
+        assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
+        E, _, _ = ctx.blocks(c)
+        assert abs(E[0] - np.mean(Es)) < 1e-9 * max(1.0, abs(E[0])), (c, E[0], np.mean(Es))
+    assert tot["swap_acc"] > 3 and tot["swap_tries"] == n_sweeps * sum(d.swaps for d in descs)
+    for k in tot:
+        assert getattr(cnt, k) == tot[k], (k, getattr(cnt, k), tot[k])
+
+
+def test_set_next_validation():
+    d_swap = chain_desc(HarmonicTrap(1.0), dim=2, M=8, N=3, L=100.0, beta=1.0, lam=0.5, ell=4, swaps=1)
+    d_plain = harmonic_desc(N=3, M=8)
+    ctx = make_ctx([d_swap, d_plain], A.FP64)
+    ctx.set_next(0, [1, 2, 0])
+    assert list(ctx.get_next(0)) == [1, 2, 0]
+    with pytest.raises(A.PimcError):
+        ctx.set_next(0, [1, 1, 0])  # not a permutation
+    with pytest.raises(A.PimcError):
+        ctx.set_next(1, [1, 0, 2])  # cycles need swap moves
+    pair = GaussianPair(1.0, 0.5)
+    with pytest.raises(A.PimcError):
+        make_ctx([chain_desc(NoPotential(), dim=2, M=8, N=3, L=4.0, beta=1.0, lam=0.5, pair=pair, ell=4, swaps=1)],
+                 A.FP64)

@@ -183,3 +183,25 @@ def test_virial_matches_thermodynamic_with_pairs():
     print("E", E.mean(), "Ev", Ev.mean(), "Epair", Ep.mean(), "diff", m, err)
     assert Ep.mean() > 0.5
     assert abs(m) < 3.5 * err
+
+
+def test_boson_energy_with_swaps_fp32():
+    """N = 10 trapped bosons, M = 64, beta = 1 (the headline single-chain
+    configuration): with swap moves the thermodynamic energy is the
+    primitive-action boson value (canonical recursion, 13.50), not the
+    distinguishable-particle 21.64."""
+    nch = 64
+    descs = [chain_desc(HarmonicTrap(1.0), dim=2, M=64, N=10, L=100.0, beta=1.0, lam=0.5, ell=8, com_step=1.0,
+                        swaps=10, swap_m=8) for _ in range(nch)]
+    opt = A.Options()
+    opt.precision, opt.sweeps_per_launch, opt.seed, opt.use_graph = A.FP32, 100, 2024, 1
+    ctx = A.Context(descs, opt)
+    ctx.init_paths(-1, 7)
+    ctx.run(1000, measure=False)
+    cnt = ctx.run(3000, measure=True)
+    assert cnt.swap_acc > 0.05 * cnt.swap_tries
+    E = np.array([ctx.estimates(c).E for c in range(nch)])
+    m, err = E.mean(), E.std(ddof=1) / math.sqrt(nch)
+    exact = AN.energy_boson_primitive(10, 64, 1.0)
+    assert abs(m - exact) < 3.0 * err, (m, err, exact)
+    assert AN.energy_boltzmann_primitive(10, 64, 1.0) - m > 20 * err
