@@ -159,7 +159,9 @@ int pimc_get_radial(pimc_ctx *ctx, int32_t chain, double *hist, int64_t *ndata);
 int pimc_get_cartesian(pimc_ctx *ctx, int32_t chain, double *hist, int64_t *ndata);
 int pimc_reset_estimators(pimc_ctx *ctx);
 
-/* ---- one reference move or estimator on the device (parity hooks) ---- */
+/* ---- one reference move or estimator on the device (parity hooks) ----
+ * The four single moves below treat each particle as its own ring and return
+ * PIMC_ERR_ARG for chains with swap moves (which sample through pimc_run). */
 
 /* levy! + the ReshapeLinear weights for particle n, start slice j0 (0-based),
  * m links, with injected normals ((m-1) x dim, row-major per bead).  Writes

@@ -21,6 +21,7 @@ struct HookArgs {
     double *out;              // r' then the four weights, then the accept flag
     double *work;             // scratch, (M + 1) * dim doubles
     int use_cells;
+    const int *next;          // the chain's permutation (next[p] follows p across slice M - 1)
 };
 
 template <typename T, int DIM> __device__ inline Cells hook_cells(const HookArgs &a, const ChainDev &ch)
@@ -190,7 +191,7 @@ template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_energ
         const int jn = (j + 1 == M) ? 0 : j + 1;
         T r0[DIM], r1[DIM];
         path.get(i, j, r0);
-        path.get(i, jn, r1);
+        path.get(jn == 0 ? a.next[i] : i, jn, r1); // inext of src/measurement.jl:98
         T dd = T(0);
 #pragma unroll
         for (int e = 0; e < DIM; ++e) {

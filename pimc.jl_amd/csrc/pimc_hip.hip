@@ -329,6 +329,7 @@ HookArgs hook_args(pimc_ctx *c, int chain)
     h.chain = chain;
     h.work = c->d_work;
     h.use_cells = (c->ch[chain].ncell > 0) ? 1 : 0;
+    h.next = c->d_next + c->ch[chain].part_base;
     return h;
 }
 
@@ -1286,6 +1287,9 @@ static int hook_move(pimc_ctx *c, int chain, int n, int j0, int m, const double 
     const ChainDev &h = c->ch[chain];
     if (n < 0 || n >= h.N) return set_err(c, PIMC_ERR_ARG, "particle out of range");
     if (!com && (j0 < 0 || j0 >= h.M || m < 2 || m > h.M)) return set_err(c, PIMC_ERR_ARG, "bad j0 or m");
+    if (h.swaps > 0)
+        return set_err(c, PIMC_ERR_ARG, "single moves treat each particle as its own ring; chains with swap moves "
+                                        "sample through pimc_run");
     if (!in) return set_err(c, PIMC_ERR_ARG, "null input");
     try {
         const int rows = com ? h.M : m + 1;

@@ -80,6 +80,11 @@ def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_i
 class System:
     """System(potential; kwargs...) (src/system.jl:129-167) on one GPU chain.
 
+    swaps > 0 adds that many swap tries of swap_m links to every HIPSweep
+    sweep (ReshapeSwapLinear, src/updates/reshape.jl:99-283; the chain then
+    carries permutation cycles, see `next`); swap_rule = 1 keeps the
+    reference's acceptance.
+
     Keyword names follow the reference (lam for lambda, mu for mu, r_a for
     r_a).  `pair` replaces `interactions`/`propint` with a device-side pair
     potential; `reference_pair_semantics=True` keeps the reference's
@@ -91,7 +96,7 @@ class System:
                  L: float = 4.0, T: float = 1.0, lam: float = 1.0, pair: GaussianPair | None = None,
                  r_a: float = 0.0, length_measurement_cycle: int = 10, precision: str = "fp64",
                  seed: int | None = None, r0: np.ndarray | None = None, reference_pair_semantics: bool = False,
-                 ell: int = 8):
+                 ell: int = 8, swaps: int = 0, swap_m: int = 0, swap_rule: int = 0):
         self.potential = potential if potential is not None else NoPotential()
         self.dV = dV if dV is not None else getattr(self.potential, "dV", None)
         self.dim, self.M, self.N, self.Ninit = dim, M, N, N
@@ -105,7 +110,8 @@ class System:
         self.Nctr = {N: 0}
         self.rng = np.random.default_rng(seed)
         desc = chain_desc(self.potential, dim=dim, M=M, N=N, L=L, beta=self.beta, lam=lam, pair=pair, r_a=r_a,
-                          pair_in_accept=not reference_pair_semantics, ell=min(ell, M))
+                          pair_in_accept=not reference_pair_semantics, ell=min(ell, M), swaps=swaps, swap_m=swap_m,
+                          swap_rule=swap_rule)
         opt = Options()
         opt.precision = FP64 if precision == "fp64" else FP32
         opt.sweeps_per_launch = 1
@@ -128,12 +134,22 @@ class System:
 
     @property
     def world(self) -> list[Particle]:
-        """s.world with Particle.r as M x dim matrices (read back from the device)."""
+        """s.world with Particle.r as M x dim matrices and Particle.next (0-based)
+        read back from the device; V[j] is the link action to the next bead
+        along the permutation cycle."""
         r = self.paths
+        nxt = self.next
         out = []
         for n in range(self.N):
             rn = r[n].T.copy()
-            V = np.array([-0.5 * self.tau * (self.potential(rn[j]) + self.potential(rn[(j + 1) % self.M]))
+            after = r[nxt[n]].T[0]  # slice 0 of the particle that follows across slice M - 1
+            V = np.array([-0.5 * self.tau * (self.potential(rn[j]) +
+                                             self.potential(rn[j + 1] if j + 1 < self.M else after))
                           for j in range(self.M)])
-            out.append(Particle(rn, V, n))
+            out.append(Particle(rn, V, int(nxt[n])))
         return out
+
+    @property
+    def next(self) -> np.ndarray:
+        """The permutation: next[p] follows particle p across slice M - 1."""
+        return self.ctx.get_next(0)
