@@ -185,6 +185,28 @@ def test_virial_matches_thermodynamic_with_pairs():
     assert abs(m) < 3.5 * err
 
 
+def _boson_run(rule, nch, sweeps=3000):
+    descs = [chain_desc(HarmonicTrap(1.0), dim=2, M=64, N=10, L=100.0, beta=1.0, lam=0.5, ell=8, com_step=1.0,
+                        swaps=10, swap_m=8, swap_rule=rule) for _ in range(nch)]
+    opt = A.Options()
+    opt.precision, opt.sweeps_per_launch, opt.seed, opt.use_graph = A.FP32, 100, 2024 + rule, 1
+    ctx = A.Context(descs, opt)
+    ctx.init_paths(-1, 7)
+    ctx.run(1000, measure=False)
+    cnt = ctx.run(sweeps, measure=True)
+    E = np.array([ctx.estimates(c).E for c in range(nch)])
+    return E.mean(), E.std(ddof=1) / math.sqrt(nch), cnt
+
+
+def test_reference_swap_rule_is_biased_fp32():
+    """The reference's swap acceptance exp(w_updated - w_initial)
+    (src/updates/reshape.jl:266) leaves out the ratio of the partner-choice
+    normalisations; at N = 10, M = 64, beta = 1 its boson energy is low by
+    about 0.5 (kept behind swap_rule = 1, off by default)."""
+    m, err, _ = _boson_run(1, 128)
+    assert AN.energy_boson_primitive(10, 64, 1.0) - m > 8 * err, (m, err)
+
+
 def test_boson_energy_with_swaps_fp32():
     """N = 10 trapped bosons, M = 64, beta = 1 (the headline single-chain
     configuration): with swap moves the thermodynamic energy is the
