@@ -133,6 +133,8 @@ def cpu_baseline(wl, seconds: float):
         "unit": "bead-moves/s",
         "cores": cores,
         "kind": "port",
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model,
         "sample": (f"{cores} chains of the batched workload (one per thread), {iters} attempts each of the "
                    f"reference schedule (COM + ReshapeLinear cap 8, adaptive), FP64 C restatement, {dt:.1f} s; "
                    f"CPU: {cpu_model}"),
