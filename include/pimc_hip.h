@@ -90,7 +90,12 @@ typedef struct pimc_options {
                                   measurement_Z_sector (src/measurement.jl:1-17), counted in sweeps.
                                   sweeps_per_launch must be a multiple of it, and so must the sweep
                                   count of a measuring pimc_run */
-    int32_t reserved;
+    int32_t adapt_ell;         /* 1: chains run whole per workgroup (pair action, swaps, derivatives) adapt
+                                  their staging length after every thermalising launch by adjust!(::NumbOfSlices)
+                                  (src/updates/helper.jl:42-52) on that launch's staging acceptance: -1 below
+                                  0.6, +1 above 0.8, within [2, min(32, M - 2)] (ReshapeLinear's minslices and
+                                  maxslices, src/updates/reshape.jl:12-21); tile chains keep theirs (set by the
+                                  autotuner).  pimc_get_ell reads the current value */
 } pimc_options;
 
 typedef struct pimc_counters {
@@ -141,6 +146,9 @@ int pimc_set_next(pimc_ctx *ctx, int32_t chain, const int32_t *next);
 int pimc_get_next(pimc_ctx *ctx, int32_t chain, int32_t *next);
 int pimc_get_com_step(pimc_ctx *ctx, int32_t chain, double *step);
 int pimc_set_com_step(pimc_ctx *ctx, int32_t chain, double step);
+/* staging length of one chain (NumbOfSlices.m of ReshapeLinear, src/updates/reshape.jl:7-28);
+ * changes only under pimc_options.adapt_ell */
+int pimc_get_ell(pimc_ctx *ctx, int32_t chain, int32_t *ell);
 
 /* run!(s, n, [(1, SingleCenterOfMass), (1, ReshapeLinear)]; Zmeasurements)
  * as batched sweeps (src/simulation.jl:29-42).  One sweep: per particle,

@@ -122,6 +122,7 @@ def lib():
         L.o_sweep.argtypes = [C.POINTER(OSys), C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_double,
                               C.POINTER(OSweepStats)]
         L.o_adapt_step.argtypes = [C.c_double, C.c_int64, C.c_int64, C.c_double]; L.o_adapt_step.restype = C.c_double
+        L.o_adapt_ell.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int]; L.o_adapt_ell.restype = C.c_int
         L.o_sweep_bead_moves.argtypes = [C.c_int, C.c_int, C.c_int]; L.o_sweep_bead_moves.restype = C.c_int64
         _lib = L
     return _lib
@@ -360,3 +361,7 @@ def init_world(seed, chain, N, M, dim, L, tau, lam):
 
 def adapt_step(step, acc, tries, L):
     return lib().o_adapt_step(step, acc, tries, L)
+
+
+def adapt_ell(ell, acc, tries, M):
+    return lib().o_adapt_ell(ell, acc, tries, M)

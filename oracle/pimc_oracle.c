@@ -852,6 +852,21 @@ double o_adapt_step(double step, int64_t acc, int64_t tries, double L)
     return step;
 }
 
+/* adjust!(::NumbOfSlices) (src/updates/helper.jl:42-52) with ReshapeLinear's
+ * bounds minslices = 2, maxslices = M - 2 (src/updates/reshape.jl:12-21), a cap
+ * of 32 links, and 2 as the floor when M - 2 < 2 */
+int o_adapt_ell(int ell, int64_t acc, int64_t tries, int M)
+{
+    if (tries <= 0) return ell;
+    double a = (double)acc / (double)tries;
+    if (a < 0.6) ell -= 1;
+    else if (a > 0.8) ell += 1;
+    int hi = M - 2 < 32 ? M - 2 : 32;
+    if (ell > hi) ell = hi;
+    if (ell < 2) ell = 2;
+    return ell;
+}
+
 /* pair link sum for particle n over link j -> j+1 with its beads at slices j
  * and j+1 replaced by pj, pj1 (neighbour set from slice j) */
 static double link_pair(const o_sys *s, int n, int j, const double *pj, const double *pj1)

@@ -162,6 +162,8 @@ void o_sweep_perm(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int e
                   int swap_m, int rule, o_sweepstats *st);
 /* the COM step adaptation applied between launches while thermalising */
 double o_adapt_step(double step, int64_t acc, int64_t tries, double L);
+/* the staging-length adaptation (NumbOfSlices) applied between thermalising launches */
+int o_adapt_ell(int ell, int64_t acc, int64_t tries, int M);
 /* bead-moves of one sweep (Levy m-1 per segment, M per COM try) */
 int64_t o_sweep_bead_moves(int N, int M, int ell);
 

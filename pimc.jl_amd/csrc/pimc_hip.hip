@@ -77,7 +77,6 @@ struct pimc_ctx {
     int maxl = 8;
     int threads_tile = 256, threads_whole = 256;
     size_t lds_tile = 0, lds_whole = 0;
-    int64_t moves_per_sweep = 0, levy_tries_per_sweep = 0; // staging only: COM and swap tries are counted on the device
     uint32_t sweep = 0;
     int nblk = 0, nblk_cap = 0;
     // device
@@ -106,7 +105,6 @@ struct pimc_ctx {
     long long cart_per_chain = 0;
     RunState *d_rs = nullptr;                   // device sweep/block counters
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // captured launch step (thermalising, measuring)
-    std::vector<int64_t> levy_tries_chain;      // staging tries per sweep, per chain
     std::vector<double> hblocks; // host copy of the estimator blocks
     int hblk_n = -1;             // blocks held in hblocks (-1: stale)
 };
@@ -379,6 +377,7 @@ void launch_step(pimc_ctx *c, int k, int measure)
     pa.rs = c->d_rs;
     pa.measure = measure ? 1 : 0;
     pa.nsweeps = k;
+    pa.adapt_ell = c->opt.adapt_ell;
     hipLaunchKernelGGL(k_post, dim3((c->n_chains + 127) / 128), dim3(128), 0, c->stream, pa);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->d_rs, k, measure ? 1 : 0);
@@ -673,22 +672,6 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             }
             c->whole_brute_ids.swap(keep);
         }
-        // schedule counts per sweep
-        for (int i = 0; i < n_chains; ++i) {
-            const ChainDev &h = c->ch[i];
-            int64_t per = 0, segs = 0;
-            for (int k = 0; k < h.nseg; ++k) {
-                const int m = std::min(h.ell, h.M - k * h.ell);
-                if (m >= 2) {
-                    per += m - 1;
-                    ++segs;
-                }
-            }
-            c->levy_tries_per_sweep += (int64_t)h.N * segs;
-            c->levy_tries_chain.push_back((int64_t)h.N * segs);
-            c->moves_per_sweep += (int64_t)h.N * per; // staging; COM and swap moves are counted on the device
-        }
-
         // ---- device allocations
         c->d_ch = dalloc<ChainDev>(n_chains);
         HIPCHK(hipMemcpy(c->d_ch, c->ch.data(), sizeof(ChainDev) * n_chains, hipMemcpyHostToDevice));
@@ -988,6 +971,23 @@ int pimc_set_com_step(pimc_ctx *c, int32_t chain, double step)
     return PIMC_OK;
 }
 
+int pimc_get_ell(pimc_ctx *c, int32_t chain, int32_t *ell)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (!ell) return set_err(c, PIMC_ERR_ARG, "ell is null");
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int v = 0;
+        HIPCHK(hipMemcpy(&v, reinterpret_cast<const char *>(c->d_ch + chain) + offsetof(ChainDev, ell), sizeof(int),
+                         hipMemcpyDeviceToHost));
+        *ell = v;
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
 int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
 {
     if (!c) return PIMC_ERR_ARG;
@@ -1041,7 +1041,6 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
             std::memset(out, 0, sizeof(*out));
             std::vector<long long> after(before.size());
             HIPCHK(hipMemcpy(after.data(), c->d_cnt, sizeof(long long) * after.size(), hipMemcpyDeviceToHost));
-            out->bead_moves = c->moves_per_sweep * n_sweeps;
             for (int i = 0; i < c->n_chains; ++i) {
                 const long long *b = before.data() + (size_t)i * kCnt, *e = after.data() + (size_t)i * kCnt;
                 out->levy_acc += e[0] - b[0];
@@ -1049,10 +1048,11 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
                 out->com_tries += e[2] - b[2];
                 out->swap_acc += e[3] - b[3];
                 out->swap_tries += e[4] - b[4];
-                out->bead_moves += (int64_t)c->ch[i].M * (e[2] - b[2]) + 2 * (int64_t)(c->ch[i].swap_m - 1) * (e[4] - b[4]);
+                out->levy_tries += e[5] - b[5];
+                out->bead_moves += (e[6] - b[6]) + (int64_t)c->ch[i].M * (e[2] - b[2]) +
+                                   2 * (int64_t)(c->ch[i].swap_m - 1) * (e[4] - b[4]);
             }
             out->sweeps = n_sweeps;
-            out->levy_tries = c->levy_tries_per_sweep * n_sweeps;
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
             out->wall_ms = ms;
@@ -1088,7 +1088,7 @@ int pimc_get_counters(pimc_ctx *c, int32_t chain, int64_t *out)
         for (int i = 0; i < count; ++i) {
             const int ch = first + i;
             out[4 * i + 0] = acc[kCnt * i];
-            out[4 * i + 1] = c->levy_tries_chain[ch] * (int64_t)c->sweep;
+            out[4 * i + 1] = acc[kCnt * i + 5];
             out[4 * i + 2] = acc[kCnt * i + 1];
             out[4 * i + 3] = acc[kCnt * i + 2];
         }

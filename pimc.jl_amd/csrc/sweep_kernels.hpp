@@ -22,8 +22,9 @@ namespace pimc {
 
 constexpr int kItemsPerThread = 1; // register slots of the estimator sums per thread
 constexpr int kTileMaxThreads = 256; // workgroup size bound of the tile kernel
-// cumulative counters per chain: staging accepted, COM accepted, COM tries, swaps accepted, swap tries
-constexpr int kCnt = 5;
+// cumulative counters per chain: staging accepted, COM accepted, COM tries, swaps accepted,
+// swap tries, staging tries, staging bead moves
+constexpr int kCnt = 7;
 constexpr size_t kTileLdsBudget = 40 * 1024; // LDS of one tile: 4 workgroups share a CU's 160 KiB
 constexpr int kNQ = 12;            // per-chain per-block estimator quantities
 
@@ -1498,7 +1499,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
 // rule of adjust!(::Step) (src/updates/helper.jl:22-32) on this launch's
 // acceptance.
 struct PostArgs {
-    const ChainDev *chains;
+    ChainDev *chains;       // ell and nseg of whole chains adapt while thermalising (adapt_ell)
     int n_chains;
     const int *is_whole;
     const double *part_acc;
@@ -1509,13 +1510,35 @@ struct PostArgs {
     double *blocks;      // [blk][chain][kNQ]
     const RunState *rs; // block index of this launch
     int measure, nsweeps;
+    int adapt_ell;      // NumbOfSlices rule on whole chains while thermalising
 };
+
+// Staging tries and proposed beads of one sweep of a chain (segments of ell
+// tiling the ring; a segment of m links proposes m - 1 beads).
+__host__ __device__ inline void staging_counts(const ChainDev &h, long long &tries, long long &beads)
+{
+    tries = 0;
+    beads = 0;
+    for (int k = 0; k < h.nseg; ++k) {
+        const int m = h.ell < h.M - k * h.ell ? h.ell : h.M - k * h.ell;
+        if (m >= 2) {
+            tries += 1;
+            beads += m - 1;
+        }
+    }
+    tries *= h.N;
+    beads *= h.N;
+}
 
 __global__ void k_post(PostArgs a)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n_chains) return;
-    const ChainDev &ch = a.chains[c];
+    ChainDev &ch = a.chains[c];
+    long long lt, lb; // staging tries and proposed beads of this launch (the launch's ell)
+    staging_counts(ch, lt, lb);
+    lt *= a.nsweeps;
+    lb *= a.nsweeps;
     long long la = 0, ca = 0;
     double q[kNQ];
     for (int k = 0; k < kNQ; ++k) q[k] = 0.0;
@@ -1559,6 +1582,21 @@ __global__ void k_post(PostArgs a)
     cnt[2] += ct;
     cnt[3] += sa;
     cnt[4] += st;
+    cnt[5] += lt;
+    cnt[6] += lb;
+    // adjust!(::NumbOfSlices) (src/updates/helper.jl:42-52) on this launch's staging
+    // acceptance: one slice less below 0.6, one more above 0.8, within
+    // [minslices, maxslices] = [2, M - 2] (ReshapeLinear's defaults,
+    // src/updates/reshape.jl:12-21), capped at 32 and never below 2
+    if (!a.measure && a.adapt_ell && a.is_whole[c] && lt > 0) {
+        const double acc = (double)la / (double)lt;
+        int ell = ch.ell;
+        if (acc < 0.6) ell -= 1;
+        else if (acc > 0.8) ell += 1;
+        ell = max(2, min(min(32, ch.M - 2), ell));
+        ch.ell = ell;
+        ch.nseg = (ch.M + ell - 1) / ell;
+    }
     if (a.measure) {
         double *b = a.blocks + ((size_t)a.rs->blk * a.n_chains + c) * kNQ;
         for (int k = 0; k < kNQ; ++k) b[k] = q[k];

@@ -21,7 +21,7 @@ FP32, FP64 = 0, 1
 EXPORTS = [
     "pimc_create", "pimc_destroy", "pimc_last_error", "pimc_device_info",
     "pimc_set_paths", "pimc_get_paths", "pimc_init_paths", "pimc_set_next", "pimc_get_next",
-    "pimc_get_com_step", "pimc_set_com_step",
+    "pimc_get_com_step", "pimc_set_com_step", "pimc_get_ell",
     "pimc_run", "pimc_get_estimates", "pimc_get_counters", "pimc_get_blocks", "pimc_get_radial", "pimc_get_cartesian",
     "pimc_reset_estimators", "pimc_levy_propose", "pimc_reshape_linear", "pimc_com_propose",
     "pimc_center_of_mass", "pimc_interaction", "pimc_energy", "pimc_density", "pimc_adjoint", "pimc_action",
@@ -46,7 +46,7 @@ class Options(C.Structure):
         ("radial_bins", C.c_int32), ("cart_bins", C.c_int32), ("radial_rmax", C.c_double),
         ("tile_beads", C.c_int32), ("block_threads", C.c_int32),
         ("use_graph", C.c_int32), ("derivatives", C.c_int32), ("seed", C.c_uint64),
-        ("measure_every", C.c_int32), ("reserved", C.c_int32),
+        ("measure_every", C.c_int32), ("adapt_ell", C.c_int32),
     ]
 
 
@@ -101,6 +101,7 @@ def load(path: str = LIB_PATH):
     L.pimc_get_next.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     L.pimc_get_com_step.argtypes = [vp, C.c_int32, _D]
     L.pimc_set_com_step.argtypes = [vp, C.c_int32, C.c_double]
+    L.pimc_get_ell.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     L.pimc_run.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(Counters)]
     L.pimc_get_estimates.argtypes = [vp, C.c_int32, C.POINTER(Estimates)]
     L.pimc_get_counters.argtypes = [vp, C.c_int32, _I64]
@@ -194,6 +195,11 @@ class Context:
 
     def set_com_step(self, chain: int, step: float):
         self.check(self.lib.pimc_set_com_step(self.h, chain, step), "pimc_set_com_step")
+
+    def ell(self, chain: int) -> int:
+        x = C.c_int32()
+        self.check(self.lib.pimc_get_ell(self.h, chain, C.byref(x)), "pimc_get_ell")
+        return x.value
 
     # -- batched sweeps
     def run(self, n_sweeps: int, measure: bool) -> Counters:

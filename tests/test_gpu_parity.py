@@ -25,6 +25,7 @@ def make_ctx(descs, precision, seed=7, **kw):
     opt.radial_rmax = kw.get("radial_rmax", 0.0)
     opt.cart_bins = kw.get("cart_bins", 0)
     opt.derivatives = kw.get("derivatives", 0)
+    opt.adapt_ell = kw.get("adapt_ell", 0)
     return A.Context(descs, opt)
 
 
@@ -546,6 +547,68 @@ def test_swap_sweeps_match_oracle_fp64():
     assert tot["swap_acc"] > 3 and tot["swap_tries"] == n_sweeps * sum(d.swaps for d in descs)
     for k in tot:
         assert getattr(cnt, k) == tot[k], (k, getattr(cnt, k), tot[k])
+
+
+def test_adapt_ell_matches_oracle_fp64():
+    """adapt_ell: whole chains (swap cycles, pair action) change their staging
+    length between thermalising launches by adjust!(::NumbOfSlices), on each
+    launch's staging acceptance, in step with the oracle's sweep schedule with
+    o_adapt_ell between launches: paths, next[], counters, the lengths read
+    back by pimc_get_ell.  The tile chain keeps its length, and so does every
+    chain during a measuring run."""
+    rng = np.random.default_rng(47)
+    pair = GaussianPair(3.0, 0.5)
+    descs = [
+        chain_desc(HarmonicTrap(1.0), dim=2, M=24, N=4, L=100.0, beta=1.0, lam=0.5, ell=4, com_step=0.8,
+                   swaps=2, swap_m=4),
+        chain_desc(HarmonicTrap(0.3), dim=2, M=20, N=24, L=3.0, beta=8.0, lam=0.5, pair=pair, ell=16, r_a=1.5,
+                   com_step=0.5),
+        harmonic_desc(N=5, M=16, beta=1.0, ell=4),
+    ]
+    inits = [rand_paths(rng, 4, 24, 100.0, scale=0.6), rng.uniform(-3.0, 3.0, size=(24, 2, 20)),
+             rand_paths(rng, 5, 16, 100.0, scale=0.6)]
+    seed, spl, n_launch = 31, 2, 6
+    ctx = make_ctx(descs, A.FP64, seed=seed, spl=spl, adapt_ell=1)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    cnt_therm = ctx.run(spl * n_launch, measure=False)
+    ells_after = [ctx.ell(c) for c in range(3)]
+    cnt_meas = ctx.run(spl * 2, measure=True)
+    assert [ctx.ell(c) for c in range(3)] == ells_after
+    tot = dict(levy_acc=0, levy_tries=0, com_acc=0, com_tries=0, swap_acc=0, swap_tries=0)
+    ells_oracle, changed = [], False
+    for c, (d, r) in enumerate(zip(descs, inits)):
+        s = oracle_sys(d, r)
+        step, ell, sweep = d.com_step, d.ell, 0
+        for launch in range(n_launch + 2):
+            measure = launch >= n_launch
+            la = lt = ca = ct = 0
+            for _ in range(spl):
+                if d.swaps:
+                    st = s.sweep_perm(c, sweep, seed, ell, step, d.swaps, d.swap_m, d.swap_rule)
+                else:
+                    st = s.sweep(c, sweep, seed, ell, step)
+                la += st.levy_acc
+                lt += st.levy_tries
+                ca += st.com_acc
+                ct += st.com_tries
+                for k in tot:
+                    tot[k] += getattr(st, k)
+                sweep += 1
+            if not measure:
+                step = O.adapt_step(step, ca, ct, d.L)
+                if c < 2:  # whole chains only
+                    ell = O.adapt_ell(ell, la, lt, d.M)
+        ells_oracle.append(ell)
+        changed |= ell != d.ell
+        if d.swaps:
+            assert np.array_equal(ctx.get_next(c), s.next), c
+        assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
+        assert ctx.com_step(c) == pytest.approx(step, rel=1e-12), c
+    assert ells_after == ells_oracle, (ells_after, ells_oracle)
+    assert changed and ells_after[2] == descs[2].ell
+    for k in tot:
+        assert getattr(cnt_therm, k) + getattr(cnt_meas, k) == tot[k], (k, getattr(cnt_therm, k), tot[k])
 
 
 def test_set_next_validation():

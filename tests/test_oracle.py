@@ -294,6 +294,23 @@ def test_sweep_bead_move_count_and_step_rule():
     assert O.adapt_step(49.0, 10, 10, 100.0) == 50.0
 
 
+def test_staging_length_rule():
+    """adjust!(::NumbOfSlices) (src/updates/helper.jl:42-52): strict
+    comparisons at 0.6 and 0.8, one step per call, clamped to [2, M - 2]
+    (ReshapeLinear's defaults) and to 32 links."""
+    assert O.adapt_ell(8, 59, 100, 64) == 7
+    assert O.adapt_ell(8, 60, 100, 64) == 8   # 0.6 is not below 0.6
+    assert O.adapt_ell(8, 80, 100, 64) == 8   # 0.8 is not above 0.8
+    assert O.adapt_ell(8, 81, 100, 64) == 9
+    assert O.adapt_ell(2, 0, 100, 64) == 2
+    assert O.adapt_ell(32, 100, 100, 64) == 32
+    assert O.adapt_ell(10, 100, 100, 12) == 10  # maxslices = M - 2
+    assert O.adapt_ell(12, 50, 100, 12) == 10   # above maxslices: clamped, not just decremented
+    assert O.adapt_ell(3, 100, 100, 4) == 2    # M - 2 = 2
+    assert O.adapt_ell(2, 100, 100, 3) == 2    # M - 2 < 2: floor 2
+    assert O.adapt_ell(5, 0, 0, 64) == 5       # no tries: unchanged
+
+
 def test_init_world_structure_and_keys():
     """init_world (src/system.jl:36-78, a = 0): first bead == last bead, all in
     [-L, L), reproducible for (seed, chain), different for another chain."""
