@@ -398,10 +398,14 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
 #pragma unroll
     for (int r = 0; r < kItemsPerThread; ++r) mL[r] = mV[r] = mR[r] = 0.0;
     int nmeas = 0;
+    // sweeps until the next measured one: (sw + 1) % measure_every == 0 without a divide per sweep
+    int meas_in = (int)((uint32_t)a.measure_every - 1u - sweep0 % (uint32_t)a.measure_every);
     __syncthreads();
 
     for (int s = 0; s < a.nsweeps; ++s) {
         const uint32_t sw = sweep0 + (uint32_t)s;
+        const bool meas_now = a.measure && meas_in == 0;
+        meas_in = (meas_in == 0) ? a.measure_every - 1 : meas_in - 1;
         // per-particle draws of this sweep: staging offset, COM displacement and uniform
         for (int q = tid; q < np; q += nt) {
             const int gq = tl.q0 + q;
@@ -568,7 +572,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
 
         // estimators (src/measurement.jl:45-55, 92-122): T sums over a chunk
         // of `ell` beads, FP64 accumulation across chunks, sweeps and particles
-        if (a.measure && (sw + 1) % (uint32_t)a.measure_every == 0) {
+        if (meas_now) {
 #pragma unroll
             for (int r = 0; r < kItemsPerThread; ++r) {
                 const int it = tid + r * nt;
@@ -1159,9 +1163,13 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     int lacc = 0, cacc = 0, ctry = 0, sacc = 0, stry = 0, overflow = 0;
     __syncthreads();
     const int hp = ell >> 1; // Philox blocks per segment: interior bead pairs (i, i + 1), i odd
+    // sweeps until the next measured one: (sw + 1) % measure_every == 0 without a divide per sweep
+    int meas_in = (int)((uint32_t)a.measure_every - 1u - sweep0 % (uint32_t)a.measure_every);
 
     for (int s = 0; s < a.nsweeps; ++s) {
         const uint32_t sw = sweep0 + (uint32_t)s;
+        const bool meas_now = a.measure && meas_in == 0;
+        meas_in = (meas_in == 0) ? a.measure_every - 1 : meas_in - 1;
         // ---- staging: particles in order, segments (and links) of one particle in parallel
         for (int p = 0; p < N; ++p) {
             T *buf = (p & 1) ? s_new1 : s_new0;
@@ -1466,7 +1474,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
             }
             __syncthreads();
         }
-        if (a.measure && (sw + 1) % (uint32_t)a.measure_every == 0) {
+        if (meas_now) {
             const ChainMeas m = measure_chain<T, DIM, CELLS>(path, cells, ch, P, s_buf, a.radial, a.radial_bins,
                                                               a.radial_winv, a.cart, a.cart_bins, c, 8,
                                                               perm ? s_next : nullptr);
