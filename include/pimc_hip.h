@@ -20,6 +20,8 @@
  *   interaction_action!(w, s, fpcycle, j)              src/updates/helper.jl:306-324 -> pimc_interaction
  *   (u::Energy)(s)                                     src/measurement.jl:92-122 -> pimc_energy
  *   (d::Density)(s)                                    src/measurement.jl:45-55 -> pimc_density
+ *   build_prop_int(L, g0, tau)                         src/propagator.jl:35-89 -> pimc_prop_int_terms +
+ *                                                      pimc_set_pair_table (PIMC_PAIR_TABLE chains)
  */
 #ifndef PIMC_HIP_H
 #define PIMC_HIP_H
@@ -43,10 +45,13 @@ extern "C" {
 #define PIMC_POT_HARMONIC 1 /* V = 1/2 omega^2 |r|^2, dV = omega^2 r */
 #define PIMC_POT_LATTICE 2  /* V = V0 sum_d cos^2(pi x_d / a)         */
 
-/* pair potentials, entering through the reference's lnU hook
- * (src/system.jl:26-28) as a primitive link action */
+/* pair actions, entering through the reference's lnU hook (src/system.jl:26-28) */
 #define PIMC_PAIR_NONE 0
-#define PIMC_PAIR_GAUSSIAN 1 /* v(r) = g exp(-r^2 / (2 sigma^2)) */
+#define PIMC_PAIR_GAUSSIAN 1 /* primitive link action of v(r) = g exp(-r^2 / (2 sigma^2)) */
+#define PIMC_PAIR_TABLE 2    /* pair-product propagator of build_prop_int (src/propagator.jl:35-89):
+                                lnU = log(propint) (-mu where propint < 0) with propint = 1 + terms(|r1|, |r2|)
+                                / prop_rel0(r1, r2, tau), terms bilinear in a table set by pimc_set_pair_table.
+                                Needs r_a > 0; no pair energy, no adjoint, no derivatives */
 
 #define PIMC_FP32 0
 #define PIMC_FP64 1
@@ -149,6 +154,19 @@ int pimc_set_com_step(pimc_ctx *ctx, int32_t chain, double step);
 /* staging length of one chain (NumbOfSlices.m of ReshapeLinear, src/updates/reshape.jl:7-28);
  * changes only under pimc_options.adapt_ell */
 int pimc_get_ell(pimc_ctx *ctx, int32_t chain, int32_t *ell);
+
+/* prop_rel_interpolate_terms (src/propagator.jl:36-70) on the device: terms[i * n + k] =
+ * -term1 - term2 + term3 at (r_i, r_k), r_i = r0 + i (r1 - r0) / (n - 1), for contact strength g0
+ * and time step tau; the k-integrals by a fixed Gauss-Legendre rule in place of quadgk.  The
+ * reference's grid is n = 600, r0 = 1e-20, r1 = L.  Takes no context: the current device, errors
+ * in pimc_last_error(NULL) */
+int pimc_prop_int_terms(int32_t n, double r0, double r1, double g0, double tau, double *terms);
+/* the table of a PIMC_PAIR_TABLE chain (build_prop_int's interpolation, src/propagator.jl:67-89):
+ * n x n terms on the grid r0..r1 (as pimc_prop_int_terms fills it), arguments clamped to the grid;
+ * mu is lnU where propint < 0 (System's mu, src/system.jl:26-28).  Must precede pimc_run and the
+ * single-move hooks of that chain; may be replaced between runs */
+int pimc_set_pair_table(pimc_ctx *ctx, int32_t chain, int32_t n, double r0, double r1, double mu,
+                        const double *terms);
 
 /* run!(s, n, [(1, SingleCenterOfMass), (1, ReshapeLinear)]; Zmeasurements)
  * as batched sweeps (src/simulation.jl:29-42).  One sweep: per particle,

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 POT_NONE, POT_HARMONIC, POT_LATTICE = 0, 1, 2
-PAIR_NONE, PAIR_GAUSSIAN = 0, 1
+PAIR_NONE, PAIR_GAUSSIAN, PAIR_TABLE = 0, 1, 2
 
 
 def build(force: bool = False) -> str:
@@ -34,7 +34,9 @@ class OPot(C.Structure):
 
 
 class OPair(C.Structure):
-    _fields_ = [("kind", C.c_int), ("g", C.c_double), ("sigma", C.c_double)]
+    _fields_ = [("kind", C.c_int), ("g", C.c_double), ("sigma", C.c_double),
+                ("tab", C.POINTER(C.c_double)), ("tab_n", C.c_int), ("tab_r0", C.c_double),
+                ("tab_r1", C.c_double), ("mu", C.c_double)]
 
 
 class OSys(C.Structure):
@@ -91,6 +93,12 @@ def lib():
         L.o_lnV.argtypes = [_D, _D, C.c_int, C.c_double, C.POINTER(OPot)]; L.o_lnV.restype = C.c_double
         L.o_lnU.argtypes = [_D, _D, C.c_int, C.c_double, C.POINTER(OPair)]; L.o_lnU.restype = C.c_double
         L.o_metropolis.argtypes = [C.c_double, C.c_double]; L.o_metropolis.restype = C.c_int
+        L.o_prop_quad_nodes.argtypes = [C.c_double, C.c_double, _D, _D, C.c_int]; L.o_prop_quad_nodes.restype = C.c_int
+        L.o_prop_int_term.argtypes = [C.c_double] * 5; L.o_prop_int_term.restype = C.c_double
+        L.o_prop_int_terms.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _D]
+        L.o_prop_table_eval.argtypes = [C.POINTER(OPair), C.c_double, C.c_double]
+        L.o_prop_table_eval.restype = C.c_double
+        L.o_prop_int.argtypes = [C.POINTER(OPair), _D, _D, C.c_int, C.c_double]; L.o_prop_int.restype = C.c_double
         L.o_levy.argtypes = [_D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _D]
         L.o_sys_init.argtypes = [C.POINTER(OSys), C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                  OPot, OPair, C.c_double, C.c_int, _D]
@@ -173,8 +181,49 @@ def make_pot(kind=POT_NONE, omega=1.0, V0=0.0, a=1.0) -> OPot:
     return OPot(kind, omega, V0, a)
 
 
-def make_pair(kind=PAIR_NONE, g=0.0, sigma=1.0) -> OPair:
-    return OPair(kind, g, sigma)
+def make_pair(kind=PAIR_NONE, g=0.0, sigma=1.0, table=None, r0=0.0, r1=0.0, mu=0.0) -> OPair:
+    """table: the n x n integral terms on the grid r0..r1 (kind PAIR_TABLE); the
+    array is kept alive by the returned structure."""
+    p = OPair(kind, g, sigma)
+    if table is not None:
+        t = np.ascontiguousarray(np.asarray(table, dtype=np.float64))
+        p.tab, p.tab_n, p.tab_r0, p.tab_r1, p.mu = _dp(t), t.shape[0], r0, r1, mu
+        p._keep = t
+    return p
+
+
+def prop_quad_nodes(rmax, tau):
+    n = lib().o_prop_quad_nodes(rmax, tau, None, None, 0)
+    k, w = np.zeros(n), np.zeros(n)
+    lib().o_prop_quad_nodes(rmax, tau, _dp(k), _dp(w), n)
+    return k, w
+
+
+def prop_int_term(r1, r2, g0, tau, rmax):
+    """int_terms(r1, r2) of prop_rel_interpolate_terms by the oracle's quadrature rule."""
+    return lib().o_prop_int_term(r1, r2, g0, tau, rmax)
+
+
+def prop_int_terms(n, r0, r1, g0, tau):
+    out = np.zeros((n, n))
+    lib().o_prop_int_terms(n, r0, r1, g0, tau, _dp(out))
+    return out
+
+
+def prop_table_eval(pair: OPair, x, y):
+    return lib().o_prop_table_eval(C.byref(pair), x, y)
+
+
+def prop_int(pair: OPair, r1rel, r2rel, tau):
+    a = np.ascontiguousarray(r1rel, dtype=np.float64)
+    b = np.ascontiguousarray(r2rel, dtype=np.float64)
+    return lib().o_prop_int(C.byref(pair), _dp(a), _dp(b), len(a), tau)
+
+
+def lnU(pair: OPair, r1rel, r2rel, tau):
+    a = np.ascontiguousarray(r1rel, dtype=np.float64)
+    b = np.ascontiguousarray(r2rel, dtype=np.float64)
+    return lib().o_lnU(_dp(a), _dp(b), len(a), tau, C.byref(pair))
 
 
 class System:
@@ -189,6 +238,7 @@ class System:
                               pair if pair is not None else make_pair(), r_a, int(pair_in_accept), _dp(r))
         if rc != 0:
             raise ValueError(f"o_sys_init failed ({rc})")
+        self._pair = pair  # keeps a pair table alive while the system holds its pointer
         self.N, self.dim, self.M = N, dim, M
 
     def __del__(self):

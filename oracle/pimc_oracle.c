@@ -10,6 +10,8 @@
  * function cites the reference file:line it follows.  Pinning of each part
  * (closed forms, brute force, known-answer vectors) lives in tests/.
  */
+/* j0 and y0 (POSIX Bessel functions of libm) */
+#define _DEFAULT_SOURCE
 #include "pimc_oracle.h"
 
 #include <math.h>
@@ -137,6 +139,158 @@ double o_lnV(const double *r1, const double *r2, int dim, double tau, const o_po
     return (-0.5 * tau) * (o_V(p, r1, dim) + o_V(p, r2, dim));
 }
 
+/* ------------------------------------------------------------------------ */
+/* pair-product propagator (src/propagator.jl:35-89)                         */
+/* ------------------------------------------------------------------------ */
+#define GL_N 16
+#define GEO_PANELS 40
+
+/* nodes and weights of the n-point Gauss-Legendre rule on [-1, 1] by Newton's
+ * iteration on P_n (three-term recurrence), x_i descending */
+static void gauss_legendre(int n, double *x, double *w)
+{
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < n; ++i) {
+        double z = cos(pi * (i + 0.75) / (n + 0.5)), dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int l = 2; l <= n; ++l) {
+                double p2 = ((2.0 * l - 1.0) * z * p1 - (l - 1.0) * p0) / l;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            double dz = p1 / dp;
+            z -= dz;
+            if (fabs(dz) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = z;
+        for (int l = 2; l <= n; ++l) {
+            double p2 = ((2.0 * l - 1.0) * z * p1 - (l - 1.0) * p0) / l;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1.0);
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+static int add_panel(double a, double b, const double *gx, const double *gw, double *k, double *w, int cnt, int cap)
+{
+    for (int i = 0; i < GL_N; ++i, ++cnt) {
+        if (cnt < cap) {
+            k[cnt] = 0.5 * (b - a) * gx[i] + 0.5 * (b + a);
+            w[cnt] = 0.5 * (b - a) * gw[i];
+        }
+    }
+    return cnt;
+}
+
+int o_prop_quad_nodes(double rmax, double tau, double *k, double *w, int cap)
+{
+    double gx[GL_N], gw[GL_N];
+    gauss_legendre(GL_N, gx, gw);
+    double h = 4.0 / rmax;
+    if (h > 0.5) h = 0.5;
+    int cnt = 0;
+    for (int p = GEO_PANELS - 1; p >= 0; --p) cnt = add_panel(h * ldexp(1.0, -(p + 1)), h * ldexp(1.0, -p), gx, gw, k, w, cnt, cap);
+    double K = sqrt(40.0 / tau);
+    if (K < 2.0 * h) K = 2.0 * h;
+    int np = (int)ceil((K - h) / h);
+    double hw = (K - h) / np;
+    for (int p = 0; p < np; ++p) cnt = add_panel(h + p * hw, h + (p + 1) * hw, gx, gw, k, w, cnt, cap);
+    return cnt;
+}
+
+/* the k-dependent factors of the integrand at node k with weight w:
+ * a = w k e^{-tau k^2} / (2 pi) * tk^2 / (1 + tk^2), b = ... * tk / (1 + tk^2),
+ * tk = 1 / d, d = (2/pi)(eulergamma + log(k/2)) - 4/g0 (src/propagator.jl:45) */
+static void prop_coef(double k, double w, double g0, double tau, double *a, double *b)
+{
+    const double pi = 3.14159265358979323846, eg = 0.57721566490153286061;
+    double d = (2.0 / pi) * (eg + log(k / 2.0)) - 4.0 / g0;
+    double c = w * k * exp(-tau * k * k) / (2.0 * pi);
+    *a = c / (1.0 + d * d);
+    *b = c * d / (1.0 + d * d);
+}
+
+double o_prop_int_term(double r1, double r2, double g0, double tau, double rmax)
+{
+    int n = o_prop_quad_nodes(rmax, tau, NULL, NULL, 0);
+    double *k = (double *)malloc(sizeof(double) * n), *w = (double *)malloc(sizeof(double) * n);
+    o_prop_quad_nodes(rmax, tau, k, w, n);
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double a, b;
+        prop_coef(k[i], w[i], g0, tau, &a, &b);
+        double j1 = j0(k[i] * r1), j2 = j0(k[i] * r2), y1 = y0(k[i] * r1), y2 = y0(k[i] * r2);
+        t += -a * (j1 * j2) - b * (j1 * y2 + j2 * y1) + a * (y1 * y2);
+    }
+    free(k);
+    free(w);
+    return t;
+}
+
+void o_prop_int_terms(int n, double r0, double r1, double g0, double tau, double *out)
+{
+    int nk = o_prop_quad_nodes(r1, tau, NULL, NULL, 0);
+    double *k = (double *)malloc(sizeof(double) * nk), *w = (double *)malloc(sizeof(double) * nk);
+    double *a = (double *)malloc(sizeof(double) * nk), *b = (double *)malloc(sizeof(double) * nk);
+    double *J = (double *)malloc(sizeof(double) * nk * n), *Y = (double *)malloc(sizeof(double) * nk * n);
+    o_prop_quad_nodes(r1, tau, k, w, nk);
+    for (int q = 0; q < nk; ++q) prop_coef(k[q], w[q], g0, tau, &a[q], &b[q]);
+    for (int i = 0; i < n; ++i) {
+        double r = (n > 1) ? r0 + i * ((r1 - r0) / (n - 1)) : r0;
+        for (int q = 0; q < nk; ++q) {
+            J[(size_t)i * nk + q] = j0(k[q] * r);
+            Y[(size_t)i * nk + q] = y0(k[q] * r);
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int l = 0; l < n; ++l) {
+            const double *Ji = J + (size_t)i * nk, *Yi = Y + (size_t)i * nk;
+            const double *Jl = J + (size_t)l * nk, *Yl = Y + (size_t)l * nk;
+            double t = 0.0;
+            for (int q = 0; q < nk; ++q) t += -a[q] * (Ji[q] * Jl[q]) - b[q] * (Ji[q] * Yl[q] + Jl[q] * Yi[q]) + a[q] * (Yi[q] * Yl[q]);
+            out[(size_t)i * n + l] = t;
+        }
+    free(k); free(w); free(a); free(b); free(J); free(Y);
+}
+
+double o_prop_table_eval(const o_pairpot *pp, double x, double y)
+{
+    const int n = pp->tab_n;
+    const double h = (pp->tab_r1 - pp->tab_r0) / (n - 1);
+    double px = (x - pp->tab_r0) / h, py = (y - pp->tab_r0) / h;
+    if (px < 0.0) px = 0.0;
+    if (px > n - 1) px = n - 1;
+    if (py < 0.0) py = 0.0;
+    if (py > n - 1) py = n - 1;
+    int ix = (int)floor(px), iy = (int)floor(py);
+    if (ix > n - 2) ix = n - 2;
+    if (iy > n - 2) iy = n - 2;
+    const double fx = px - ix, fy = py - iy;
+    const double *t = pp->tab;
+    const double a0 = (1.0 - fy) * t[(size_t)ix * n + iy] + fy * t[(size_t)ix * n + iy + 1];
+    const double a1 = (1.0 - fy) * t[(size_t)(ix + 1) * n + iy] + fy * t[(size_t)(ix + 1) * n + iy + 1];
+    return (1.0 - fx) * a0 + fx * a1;
+}
+
+double o_prop_int(const o_pairpot *pp, const double *r1rel, const double *r2rel, int dim, double tau)
+{
+    const double pi = 3.14159265358979323846;
+    double n1 = 0.0, n2 = 0.0, dd = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        n1 += r1rel[d] * r1rel[d];
+        n2 += r2rel[d] * r2rel[d];
+        const double x = r1rel[d] - r2rel[d];
+        dd += x * x;
+    }
+    const double p0 = exp(-dd / (4.0 * tau)) / (4.0 * pi * tau);
+    return 1.0 + o_prop_table_eval(pp, sqrt(n1), sqrt(n2)) / p0;
+}
+
 double o_vpair(const o_pairpot *pp, double r)
 {
     if (pp->kind != O_PAIR_GAUSSIAN) return 0.0;
@@ -150,6 +304,11 @@ double o_vpair(const o_pairpot *pp, double r)
  * the pair at slices j and j+1. */
 double o_lnU(const double *r1rel, const double *r2rel, int dim, double tau, const o_pairpot *pp)
 {
+    if (pp->kind == O_PAIR_TABLE) {
+        /* src/system.jl:26-28: propint < 0 ? -mu : log(propint) */
+        const double p = o_prop_int(pp, r1rel, r2rel, dim, tau);
+        return p < 0.0 ? -pp->mu : log(p);
+    }
     double a = 0.0, b = 0.0;
     for (int d = 0; d < dim; ++d) {
         a += r1rel[d] * r1rel[d];
@@ -539,7 +698,7 @@ void o_energy(const o_sys *s, double *E, double *Ev)
 void o_pair_energy(const o_sys *s, double *Epair, double *Evir_pair)
 {
     *Epair = 0.0; *Evir_pair = 0.0;
-    if (s->pair.kind == O_PAIR_NONE) return;
+    if (s->pair.kind != O_PAIR_GAUSSIAN) return; /* no pair energy for the tabulated propagator */
     const int M = s->M, dim = s->dim;
     int list[NNCAP];
     double ri[3], rk[3], rin[3], rkn[3];

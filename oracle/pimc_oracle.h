@@ -24,7 +24,7 @@ extern "C" {
 
 /* ---- potentials (the Julia closures become an enum + parameters) ---- */
 enum { O_POT_NONE = 0, O_POT_HARMONIC = 1, O_POT_LATTICE = 2 };
-enum { O_PAIR_NONE = 0, O_PAIR_GAUSSIAN = 1 };
+enum { O_PAIR_NONE = 0, O_PAIR_GAUSSIAN = 1, O_PAIR_TABLE = 2 };
 
 typedef struct {
     int kind;        /* O_POT_* */
@@ -35,6 +35,13 @@ typedef struct {
 typedef struct {
     int kind;        /* O_PAIR_* */
     double g, sigma; /* v(r) = g exp(-r^2 / (2 sigma^2)) */
+    /* O_PAIR_TABLE: the integral terms of the pair-product propagator
+     * (prop_rel_interpolate_terms, src/propagator.jl:36-70) on the n x n grid
+     * r_i = r0 + i (r1 - r0) / (n - 1), row-major tab[i * n + k] = terms(r_i, r_k);
+     * mu: the value of lnU where the propagator is negative (src/system.jl:27) */
+    const double *tab;
+    int tab_n;
+    double tab_r0, tab_r1, mu;
 } o_pairpot;
 
 /* ---- system state (src/system.jl:93-126 restricted to the hot path) ---- */
@@ -68,6 +75,21 @@ double o_lnV(const double *r1, const double *r2, int dim, double tau, const o_po
 double o_vpair(const o_pairpot *pp, double r);
 double o_lnU(const double *r1rel, const double *r2rel, int dim, double tau, const o_pairpot *pp); /* src/system.jl:26-28 */
 int    o_metropolis(double delta, double u);           /* src/updates/helper.jl:3-5 */
+
+/* ---- pair-product propagator, src/propagator.jl:35-89 ----
+ * int_terms(r1, r2) = -term1 - term2 + term3 of prop_rel_interpolate_terms
+ * (:36-70) by a fixed Gauss-Legendre rule over k in place of quadgk: 16-point
+ * panels, 40 geometric panels [h 2^-(p+1), h 2^-p] below h = min(1/2, 4/rmax),
+ * uniform panels of width <= h from h to K = sqrt(40/tau) (exp(-tau K^2) =
+ * 4e-18), so that (r1 + r2) * width <= 8 on every panel. */
+int    o_prop_quad_nodes(double rmax, double tau, double *k, double *w, int cap); /* node count */
+double o_prop_int_term(double r1, double r2, double g0, double tau, double rmax);
+void   o_prop_int_terms(int n, double r0, double r1, double g0, double tau, double *out); /* n x n, grid r0..r1 */
+/* bilinear interpolation on the grid (scale(interpolate(A, BSpline(Linear())), ...)),
+ * arguments clamped to [r0, r1] */
+double o_prop_table_eval(const o_pairpot *pp, double x, double y);
+/* prop_int(r1_rel, r2_rel, tau) = 1 + terms(|r1|, |r2|) / prop_rel0(r1_rel, r2_rel, tau) (:72-89) */
+double o_prop_int(const o_pairpot *pp, const double *r1rel, const double *r2rel, int dim, double tau);
 
 /* levy! (src/updates/helper.jl:118-139) with injected normals.  rp holds
  * nrows x dim column-major (rp[d*nrows + j]); rows 0 and nrows-1 are the

@@ -184,7 +184,7 @@ template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_energ
     const int N = ch.N, M = ch.M;
     const Path<T, DIM, SM> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M, ch.N};
     const Cells C = hook_cells<T, DIM>(a, ch);
-    const double inv_s2 = (ch.pair != kPairNone) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
+    const double inv_s2 = (ch.pair == kPairGaussian) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
     double link = 0.0, pot = 0.0, vkin = 0.0, su = 0.0, rgp = 0.0;
     for (int b = threadIdx.x; b < N * M; b += blockDim.x) {
         const int i = b / M, j = b - i * M;
@@ -201,7 +201,7 @@ template <typename T, int DIM, bool CELLS, bool SM> __global__ void k_hook_energ
         link += (double)dd;
         pot += (double)(potV<T, DIM>(r0, P) + potV<T, DIM>(r1, P));
         vkin += (double)rgradV<T, DIM>(r0, P);
-        if (ch.pair != kPairNone) {
+        if (ch.pair == kPairGaussian) { // no pair energy for the tabulated propagator
             double x, y;
             pair_energy_bead<T, DIM, CELLS>(path, C, N, ch.nbins, P, inv_s2, i, j, x, y);
             su += x;

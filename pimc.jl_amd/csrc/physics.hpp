@@ -15,7 +15,7 @@
 namespace pimc {
 
 constexpr int kPotNone = 0, kPotHarmonic = 1, kPotLattice = 2;
-constexpr int kPairNone = 0, kPairGaussian = 1;
+constexpr int kPairNone = 0, kPairGaussian = 1, kPairTable = 2;
 
 // One chain as the kernels see it (device-resident table, 8-byte fields
 // first).  Positions of particle n, coordinate d, slice j sit at
@@ -29,11 +29,14 @@ struct ChainDev {
     long long cell_off; // offset of slice 0, cell 0 in the cell-count array (pair chains)
     long long mem_off;  // offset of slice 0, cell 0, slot 0 in the cell-member array
     long long bin_off;  // offset of the chain's per-bead cell index array (pair chains)
+    const double *tab;  // kPairTable: integral terms of the pair-product propagator, tab_n x tab_n
+    double tab_r0, tab_h, mu; // grid start and spacing; lnU where the propagator is negative
     int N, M, dim, pot;
     int pair, pair_in_accept, ell, nbins;
     int cap, ncell, part_base, nseg;
     int sm;                   // positions stored slice-major (chains run whole per workgroup)
     int swaps, swap_m, swap_rule; // swap tries per sweep, links per try, acceptance rule
+    int tab_n;
 };
 
 // The chain constants converted to the arithmetic type (built once per chain
@@ -44,6 +47,9 @@ template <typename T> struct Prm {
     T V0, inva;          // lattice depth, 1/a
     T g, inv2s2, h2;     // pair strength, 1/(2 sigma^2), cut-off radius^2
     T sdl;               // sqrt(2 lambda tau): free-particle displacement of one link
+    const double *tab;   // tabulated pair-product propagator (kPairTable), FP64 throughout
+    double tab_r0, tab_h, mu, tau64;
+    int tab_n;
     int pot, pair;
 
     __host__ __device__ __forceinline__ static Prm make(const ChainDev &c)
@@ -62,6 +68,12 @@ template <typename T> struct Prm {
         p.inv2s2 = (T)(1.0 / (2.0 * c.sigma * c.sigma));
         p.h2 = (T)(c.h * c.h);
         p.sdl = (T)sqrt(2.0 * c.lambda * c.tau);
+        p.tab = c.tab;
+        p.tab_r0 = c.tab_r0;
+        p.tab_h = c.tab_h;
+        p.mu = c.mu;
+        p.tau64 = c.tau;
+        p.tab_n = c.tab_n;
         p.pot = c.pot;
         p.pair = c.pair;
         return p;
@@ -174,6 +186,64 @@ template <typename T, int DIM> __device__ __forceinline__ T dVdomega(const T *r,
 template <typename T> __device__ __forceinline__ T vpair_r2(T r2, const Prm<T> &p)
 {
     return p.g * ex(-r2 * p.inv2s2);
+}
+
+// Bilinear interpolation of the tabulated integral terms at (x, y) (the
+// scale(interpolate(A, BSpline(Linear())), r, r) of src/propagator.jl:67-68),
+// arguments clamped to the grid.  tab[i * n + k] = terms(r_i, r_k).
+template <typename T> __device__ inline double table_eval(const Prm<T> &p, double x, double y)
+{
+    const int n = p.tab_n;
+    double px = (x - p.tab_r0) / p.tab_h, py = (y - p.tab_r0) / p.tab_h;
+    px = fmin(fmax(px, 0.0), (double)(n - 1));
+    py = fmin(fmax(py, 0.0), (double)(n - 1));
+    const int ix = min((int)floor(px), n - 2), iy = min((int)floor(py), n - 2);
+    const double fx = px - ix, fy = py - iy;
+    const double *t0 = p.tab + (size_t)ix * n + iy, *t1 = t0 + n;
+    const double a0 = (1.0 - fy) * t0[0] + fy * t0[1];
+    const double a1 = (1.0 - fy) * t1[0] + fy * t1[1];
+    return (1.0 - fx) * a0 + fx * a1;
+}
+
+// lnU of the tabulated pair-product propagator (src/system.jl:26-28 with
+// build_prop_int, src/propagator.jl:72-89): propint = 1 + terms(|r1|, |r2|) /
+// prop_rel0(r1, r2, tau), prop_rel0 = exp(-|r1 - r2|^2 / (4 tau)) / (4 pi tau),
+// lnU = propint < 0 ? -mu : log(propint).  c1, c2 are the per-component
+// |minimum-image| offsets of the pair at slices j and j+1 (the reference's
+// r1_rel, r2_rel, so r1 - r2 is taken between absolute values).
+template <typename T, int DIM> __device__ inline T lnu_table(const T *c1, const T *c2, const Prm<T> &p)
+{
+    double n1 = 0.0, n2 = 0.0, dd = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double a = (double)c1[d], b = (double)c2[d];
+        n1 += a * a;
+        n2 += b * b;
+        dd += (a - b) * (a - b);
+    }
+    const double p0 = exp(-dd / (4.0 * p.tau64)) / (4.0 * 3.14159265358979323846 * p.tau64);
+    const double pr = 1.0 + table_eval(p, sqrt(n1), sqrt(n2)) / p0;
+    return (T)(pr < 0.0 ? -p.mu : log(pr));
+}
+
+// lnU of one pair over one link, given the pair's offsets at both slices:
+// rk, rkn are the neighbour's beads at j and j+1, q, qn the particle's, r2 and
+// s2 the squared minimum-image distances at j and j+1.  Gaussian pair: the
+// primitive form -tau/2 (v(r_j) + v(r_j+1)).  TAB: 1 tabulated propagator,
+// 0 Gaussian, -1 chosen at run time from p.pair.
+template <typename T, int DIM, int TAB = -1>
+__device__ __forceinline__ T pair_lnU(const T *rk, const T *q, const T *rkn, const T *qn, T r2, T s2, const Prm<T> &p)
+{
+    if (TAB == 1 || (TAB < 0 && p.pair == kPairTable)) {
+        T c1[DIM], c2[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            c1[d] = mindist(rk[d], q[d], p);
+            c2[d] = mindist(rkn[d], qn[d], p);
+        }
+        return lnu_table<T, DIM>(c1, c2, p);
+    }
+    return (T(-0.5) * p.tau) * (vpair_r2(r2, p) + vpair_r2(s2, p));
 }
 
 // src/updates/helper.jl:3-5
@@ -315,7 +385,7 @@ struct Cells {
 // replaced by pj, pj1: the neighbour set is every particle k != n of the 9
 // stencil cells around pj at slice j whose periodic distance to pj is <= h
 // (find_nns, src/nearest_neighbours.jl:106-154), each contributing
-// lnU = -tau/2 (v(|r_jk|) + v(|r_j+1,k|)) (src/updates/helper.jl:306-366).
+// lnU (pair_lnU) of the pair over the link (src/updates/helper.jl:306-366).
 // Returns the sum of lnU (a log weight, <= 0 for a repulsion).
 template <typename T, int DIM, typename PathT>
 __device__ T pair_link(const PathT &P, const Cells &C, const Prm<T> &p, int n, int j, const T *pj, const T *pj1)
@@ -349,7 +419,7 @@ __device__ T pair_link(const PathT &P, const Cells &C, const Prm<T> &p, int n, i
                 T e1 = mindist(rkn[1], pj1[1], p);
                 s2 += e1 * e1;
             }
-            acc += (T(-0.5) * p.tau) * (vpair_r2(r2, p) + vpair_r2(s2, p));
+            acc += pair_lnU<T, DIM>(rk, pj, rkn, pj1, r2, s2, p);
         }
     }
     return acc;
@@ -394,7 +464,7 @@ __device__ T pair_link_all(const PathT &P, int N, int nbins, const Prm<T> &p, in
             T e1 = mindist(rkn[1], pj1[1], p);
             s2 += e1 * e1;
         }
-        acc += T(mult) * ((T(-0.5) * p.tau) * (vpair_r2(r2, p) + vpair_r2(s2, p)));
+        acc += T(mult) * pair_lnU<T, DIM>(rk, pj, rkn, pj1, r2, s2, p);
     }
     return acc;
 }

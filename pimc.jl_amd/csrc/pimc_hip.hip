@@ -13,6 +13,7 @@
 
 #include "hook_kernels.hpp"
 #include "pimc_hip.h"
+#include "propagator_kernels.hpp"
 #include "sweep_kernels.hpp"
 
 using namespace pimc;
@@ -69,6 +70,9 @@ struct pimc_ctx {
     std::vector<int> whole_cells_ids;  // whole chains with cell lists
     std::vector<int> whole_brute_ids;  // whole chains with brute-force neighbour sums
     std::vector<int> whole_lds_ids;    // ... whose path fits in LDS next to the kernel's buffers
+    // each of the three lists holds its Gaussian/pairless chains first, then from
+    // this index the chains with the tabulated propagator (a kernel of their own)
+    int cells_tab0 = 0, brute_tab0 = 0, lds_tab0 = 0;
     std::vector<TileDev> tiles;
     std::vector<TileGroup> tile_groups;
     std::vector<int> part_chain, part_idx;
@@ -107,6 +111,7 @@ struct pimc_ctx {
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // captured launch step (thermalising, measuring)
     std::vector<double> hblocks; // host copy of the estimator blocks
     int hblk_n = -1;             // blocks held in hblocks (-1: stale)
+    std::vector<double *> d_tab; // per chain: pair-product propagator table (PIMC_PAIR_TABLE chains)
 };
 
 namespace {
@@ -170,14 +175,15 @@ void launch_tiles(pimc_ctx *c, const SweepArgs &a)
     }
 }
 
-template <typename T, int DIM, bool CELLS, bool LDSR>
-void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &ids, const int *d_ids)
+template <typename T, int DIM, bool CELLS, bool LDSR, bool TAB>
+void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &all, int lo, int hi, const int *d_ids)
 {
-    if (ids.empty()) return;
-    a.whole = d_ids;
+    if (hi <= lo) return;
+    a.whole = d_ids + lo;
     int maxM = 0;
     size_t lds = c->lds_whole;
-    for (int id : ids) {
+    for (int x = lo; x < hi; ++x) {
+        const int id = all[x];
         maxM = std::max(maxM, c->ch[id].M);
         if (LDSR) lds = std::max(lds, whole_lds_bytes(c->ch[id], sizeof(T)) + whole_path_bytes(c->ch[id], sizeof(T)));
     }
@@ -185,17 +191,21 @@ void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &ids, const
     int G = 1;
     while (G < 16 && 2 * G * maxM <= c->threads_whole) G <<= 1;
     a.link_group = std::max(2, G);
-    auto k = k_sweep_whole<T, DIM, CELLS, LDSR>;
+    auto k = k_sweep_whole<T, DIM, CELLS, LDSR, TAB>;
     if (lds > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k, dim3((unsigned)ids.size()), dim3(c->threads_whole), lds, c->stream, a);
+    hipLaunchKernelGGL(k, dim3((unsigned)(hi - lo)), dim3(c->threads_whole), lds, c->stream, a);
     HIPCHK(hipGetLastError());
 }
 
 template <typename T, int DIM> void launch_whole_d(pimc_ctx *c, const SweepArgs &a)
 {
-    launch_whole_t<T, DIM, true, false>(c, a, c->whole_cells_ids, c->d_whole_cells);
-    launch_whole_t<T, DIM, false, false>(c, a, c->whole_brute_ids, c->d_whole_brute);
-    launch_whole_t<T, DIM, false, true>(c, a, c->whole_lds_ids, c->d_whole_lds);
+    const auto &wc = c->whole_cells_ids, &wb = c->whole_brute_ids, &wl = c->whole_lds_ids;
+    launch_whole_t<T, DIM, true, false, false>(c, a, wc, 0, c->cells_tab0, c->d_whole_cells);
+    launch_whole_t<T, DIM, true, false, true>(c, a, wc, c->cells_tab0, (int)wc.size(), c->d_whole_cells);
+    launch_whole_t<T, DIM, false, false, false>(c, a, wb, 0, c->brute_tab0, c->d_whole_brute);
+    launch_whole_t<T, DIM, false, false, true>(c, a, wb, c->brute_tab0, (int)wb.size(), c->d_whole_brute);
+    launch_whole_t<T, DIM, false, true, false>(c, a, wl, 0, c->lds_tab0, c->d_whole_lds);
+    launch_whole_t<T, DIM, false, true, true>(c, a, wl, c->lds_tab0, (int)wl.size(), c->d_whole_lds);
 }
 
 void launch_whole(pimc_ctx *c, const SweepArgs &a)
@@ -292,6 +302,17 @@ int check_chain(pimc_ctx *c, int chain)
 {
     if (!c) return PIMC_ERR_ARG;
     if (chain < 0 || chain >= c->n_chains) return set_err(c, PIMC_ERR_ARG, "chain index out of range");
+    return PIMC_OK;
+}
+
+// A chain with the tabulated pair propagator samples only once its table is set.
+int check_table(pimc_ctx *c, int chain)
+{
+    const int lo = chain < 0 ? 0 : chain, hi = chain < 0 ? c->n_chains : chain + 1;
+    for (int i = lo; i < hi; ++i)
+        if (c->ch[i].pair == kPairTable && !c->ch[i].tab)
+            return set_err(c, PIMC_ERR_STATE, "chain " + std::to_string(i) +
+                                                  ": the pair-product propagator table is not set (pimc_set_pair_table)");
     return PIMC_OK;
 }
 
@@ -400,8 +421,10 @@ int validate(const pimc_chain_desc &d, std::string &why)
     if (d.M < 4) return why = "M must be >= 4", -1;
     if (!(d.beta > 0) || !(d.lambda > 0) || !(d.L > 0)) return why = "beta, lambda and L must be positive", -1;
     if (d.potential < PIMC_POT_NONE || d.potential > PIMC_POT_LATTICE) return why = "unknown potential", -1;
-    if (d.pair < PIMC_PAIR_NONE || d.pair > PIMC_PAIR_GAUSSIAN) return why = "unknown pair potential", -1;
+    if (d.pair < PIMC_PAIR_NONE || d.pair > PIMC_PAIR_TABLE) return why = "unknown pair potential", -1;
     if (d.pair == PIMC_PAIR_GAUSSIAN && !(d.sigma > 0)) return why = "sigma must be positive", -1;
+    if (d.pair == PIMC_PAIR_TABLE && !(d.r_a > 0))
+        return why = "a tabulated pair propagator needs the neighbour range r_a > 0", -1;
     if (d.potential == PIMC_POT_LATTICE && !(d.a > 0)) return why = "lattice constant a must be positive", -1;
     if (d.ell != 0 && (d.ell < 2 || d.ell > 32 || d.ell > d.M)) return why = "ell must be in [2, min(32, M)]", -1;
     if (!(d.com_step >= 0)) return why = "com_step must be >= 0", -1;
@@ -416,12 +439,69 @@ int validate(const pimc_chain_desc &d, std::string &why)
     return 0;
 }
 
+// Nodes and weights of the k-integral of the pair-product propagator's terms:
+// 16-point Gauss-Legendre panels, 40 geometric panels [h 2^-(p+1), h 2^-p]
+// below h = min(1/2, 4/rmax) for the logarithmic behaviour near k = 0, then
+// uniform panels of width <= h up to K = sqrt(40/tau), where exp(-tau K^2) =
+// 4e-18; (r1 + r2) * width <= 8 keeps the Bessel products resolved.
+void prop_quad_nodes(double rmax, double tau, std::vector<double> &k, std::vector<double> &w)
+{
+    constexpr int ng = 16, npanel_geo = 40;
+    double gx[ng], gw[ng];
+    // Legendre roots by Newton's iteration on P_16 (three-term recurrence)
+    for (int i = 0; i < ng; ++i) {
+        double z = std::cos(3.14159265358979323846 * (i + 0.75) / (ng + 0.5)), dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int l = 2; l <= ng; ++l) {
+                const double p2 = ((2.0 * l - 1.0) * z * p1 - (l - 1.0) * p0) / l;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = ng * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = z;
+        for (int l = 2; l <= ng; ++l) {
+            const double p2 = ((2.0 * l - 1.0) * z * p1 - (l - 1.0) * p0) / l;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = ng * (z * p1 - p0) / (z * z - 1.0);
+        gx[i] = z;
+        gw[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    auto panel = [&](double a, double b) {
+        for (int i = 0; i < ng; ++i) {
+            k.push_back(0.5 * (b - a) * gx[i] + 0.5 * (b + a));
+            w.push_back(0.5 * (b - a) * gw[i]);
+        }
+    };
+    const double h = std::min(0.5, 4.0 / rmax);
+    k.clear();
+    w.clear();
+    for (int p = npanel_geo - 1; p >= 0; --p) panel(h * std::ldexp(1.0, -(p + 1)), h * std::ldexp(1.0, -p));
+    const double K = std::max(2.0 * h, std::sqrt(40.0 / tau));
+    const int np = (int)std::ceil((K - h) / h);
+    const double hw = (K - h) / np;
+    for (int p = 0; p < np; ++p) panel(h + p * hw, h + (p + 1) * hw);
+}
+
 } // namespace
 
 // ============================================================================
 extern "C" {
 
-const char *pimc_last_error(const pimc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+// message of the last failed call that takes no context (pimc_prop_int_terms), per thread
+static thread_local std::string g_err;
+
+const char *pimc_last_error(const pimc_ctx *ctx)
+{
+    if (ctx) return ctx->err.c_str();
+    return g_err.empty() ? "null context" : g_err.c_str();
+}
 
 int pimc_device_info(char *name, int32_t len, int32_t *cu_count)
 {
@@ -465,6 +545,14 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         *out = c;
         return PIMC_ERR_ARG;
     }
+    if (opt->derivatives) {
+        for (int i = 0; i < n_chains; ++i)
+            if (chains[i].pair == PIMC_PAIR_TABLE) {
+                c->err = "chain " + std::to_string(i) + ": derivatives need a Gaussian pair or none";
+                *out = c;
+                return PIMC_ERR_ARG;
+            }
+    }
     {
         const int K = opt->sweeps_per_launch > 0 ? opt->sweeps_per_launch : 50;
         const int me = opt->measure_every > 0 ? opt->measure_every : 1;
@@ -497,6 +585,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
 
         // ---- chain table; whole-chain kernel for pair chains (and all chains with derivatives)
         c->ch.resize(n_chains);
+        c->d_tab.assign(n_chains, nullptr);
         c->is_whole.assign(n_chains, 0);
         int ellmax = 2;
         for (int i = 0; i < n_chains; ++i) {
@@ -672,6 +761,16 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             }
             c->whole_brute_ids.swap(keep);
         }
+        // tabulated-propagator chains last in each list (their own kernel instantiation)
+        {
+            auto part = [&](std::vector<int> &v) {
+                return (int)(std::stable_partition(v.begin(), v.end(), [&](int id) { return c->ch[id].pair != kPairTable; }) -
+                             v.begin());
+            };
+            c->cells_tab0 = part(c->whole_cells_ids);
+            c->brute_tab0 = part(c->whole_brute_ids);
+            c->lds_tab0 = part(c->whole_lds_ids);
+        }
         // ---- device allocations
         c->d_ch = dalloc<ChainDev>(n_chains);
         HIPCHK(hipMemcpy(c->d_ch, c->ch.data(), sizeof(ChainDev) * n_chains, hipMemcpyHostToDevice));
@@ -779,6 +878,7 @@ int pimc_destroy(pimc_ctx *c)
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
                     c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next};
     for (void *p : ptrs) dfree(p);
+    for (double *p : c->d_tab) dfree(p);
     for (auto &e : c->events) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
@@ -988,12 +1088,51 @@ int pimc_get_ell(pimc_ctx *c, int32_t chain, int32_t *ell)
     return PIMC_OK;
 }
 
+int pimc_set_pair_table(pimc_ctx *c, int32_t chain, int32_t n, double r0, double r1, double mu,
+                        const double *terms)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    ChainDev &h = c->ch[chain];
+    if (h.pair != kPairTable) return set_err(c, PIMC_ERR_ARG, "the chain's pair kind is not PIMC_PAIR_TABLE");
+    if (n < 2 || !(r1 > r0) || !(r0 >= 0) || !terms) return set_err(c, PIMC_ERR_ARG, "need n >= 2, 0 <= r0 < r1, terms");
+    for (size_t i = 0; i < (size_t)n * n; ++i)
+        if (!std::isfinite(terms[i])) return set_err(c, PIMC_ERR_ARG, "non-finite table entry");
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double *t = dalloc<double>((size_t)n * n);
+        if (hipMemcpy(t, terms, sizeof(double) * n * n, hipMemcpyHostToDevice) != hipSuccess) {
+            dfree(t);
+            throw HipError{hipErrorInvalidValue, "hipMemcpy(pair table)"};
+        }
+        dfree(c->d_tab[chain]);
+        c->d_tab[chain] = t;
+        h.tab = t;
+        h.tab_n = n;
+        h.tab_r0 = r0;
+        h.tab_h = (r1 - r0) / (n - 1);
+        h.mu = mu;
+        HIPCHK(hipMemcpy(c->d_ch + chain, &h, sizeof(ChainDev), hipMemcpyHostToDevice));
+        if (c->prec == PIMC_FP64) {
+            const Prm<double> pr = Prm<double>::make(h);
+            HIPCHK(hipMemcpy(reinterpret_cast<Prm<double> *>(c->d_prm) + chain, &pr, sizeof(pr), hipMemcpyHostToDevice));
+        } else {
+            const Prm<float> pr = Prm<float>::make(h);
+            HIPCHK(hipMemcpy(reinterpret_cast<Prm<float> *>(c->d_prm) + chain, &pr, sizeof(pr), hipMemcpyHostToDevice));
+        }
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
 int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
 {
     if (!c) return PIMC_ERR_ARG;
     if (n_sweeps < 0) return set_err(c, PIMC_ERR_ARG, "n_sweeps < 0");
     if (measure && n_sweeps % c->opt.measure_every != 0)
         return set_err(c, PIMC_ERR_ARG, "a measuring run must be a multiple of measure_every sweeps");
+    if (int rc = check_table(c, -1)) return rc;
     try {
         std::vector<long long> before((size_t)c->n_chains * kCnt);
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1291,6 +1430,7 @@ static int hook_move(pimc_ctx *c, int chain, int n, int j0, int m, const double 
         return set_err(c, PIMC_ERR_ARG, "single moves treat each particle as its own ring; chains with swap moves "
                                         "sample through pimc_run");
     if (!in) return set_err(c, PIMC_ERR_ARG, "null input");
+    if ((rc = check_table(c, chain))) return rc;
     try {
         const int rows = com ? h.M : m + 1;
         const size_t nout = (size_t)rows * h.dim + 5;
@@ -1352,6 +1492,7 @@ int pimc_interaction(pimc_ctx *c, int32_t chain, int32_t n, int32_t j, double *w
     if (rc) return rc;
     const ChainDev &hc = c->ch[chain];
     if (n < 0 || n >= hc.N || j < 0 || j >= hc.M) return set_err(c, PIMC_ERR_ARG, "bad particle or slice");
+    if ((rc = check_table(c, chain))) return rc;
     try {
         ensure_io(c, 8);
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1430,6 +1571,8 @@ int pimc_adjoint(pimc_ctx *c, int32_t chain, double *dS_dr, double *dS_dparams, 
     int rc = check_chain(c, chain);
     if (rc) return rc;
     const ChainDev &hc = c->ch[chain];
+    if (hc.pair == kPairTable)
+        return set_err(c, PIMC_ERR_ARG, "the adjoint is defined for a Gaussian pair action or none");
     const size_t nb = (size_t)hc.N * hc.dim * hc.M;
     try {
         ensure_io(c, nb + 8);
@@ -1455,6 +1598,55 @@ int pimc_adjoint(pimc_ctx *c, int32_t chain, double *dS_dr, double *dS_dparams, 
 int pimc_action(pimc_ctx *c, int32_t chain, double *S_out)
 {
     return pimc_adjoint(c, chain, nullptr, nullptr, S_out);
+}
+
+int pimc_prop_int_terms(int32_t n, double r0, double r1, double g0, double tau, double *terms)
+{
+    g_err.clear();
+    if (n < 2 || !(r1 > r0) || !(r0 >= 0) || !(tau > 0) || !(g0 != 0) || !terms) {
+        g_err = "need n >= 2, 0 <= r0 < r1, tau > 0, g0 != 0 and an output buffer";
+        return PIMC_ERR_ARG;
+    }
+    std::vector<double> k, w;
+    prop_quad_nodes(r1, tau, k, w);
+    const int nk = (int)k.size();
+    std::vector<double> ca(nk), cb(nk);
+    for (int q = 0; q < nk; ++q) {
+        // tk = 1 / d (src/propagator.jl:45); tk^2 / (1 + tk^2) = 1 / (1 + d^2), tk / (1 + tk^2) = d / (1 + d^2)
+        const double pi = 3.14159265358979323846, eg = 0.57721566490153286061;
+        const double d = (2.0 / pi) * (eg + std::log(k[q] / 2.0)) - 4.0 / g0;
+        const double cq = w[q] * k[q] * std::exp(-tau * k[q] * k[q]) / (2.0 * pi);
+        ca[q] = cq / (1.0 + d * d);
+        cb[q] = cq * d / (1.0 + d * d);
+    }
+    double *dk = nullptr, *dca = nullptr, *dcb = nullptr, *dJ = nullptr, *dY = nullptr, *dout = nullptr;
+    int rc = PIMC_OK;
+    try {
+        dk = dalloc<double>(nk);
+        dca = dalloc<double>(nk);
+        dcb = dalloc<double>(nk);
+        dJ = dalloc<double>((size_t)n * nk);
+        dY = dalloc<double>((size_t)n * nk);
+        dout = dalloc<double>((size_t)n * n);
+        HIPCHK(hipMemcpy(dk, k.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dca, ca.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dcb, cb.data(), sizeof(double) * nk, hipMemcpyHostToDevice));
+        const double h = (r1 - r0) / (n - 1);
+        const long long tot = (long long)n * nk;
+        hipLaunchKernelGGL(k_prop_bessel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, nullptr, dk, nk, r0, h, n,
+                           dJ, dY);
+        HIPCHK(hipGetLastError());
+        const unsigned nt = (unsigned)((n + kPropTile - 1) / kPropTile);
+        hipLaunchKernelGGL(k_prop_terms, dim3(nt, nt), dim3(kPropThreads), 0, nullptr, dJ, dY, dca, dcb, nk, n, dout);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(terms, dout, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    } catch (const HipError &e) {
+        g_err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        rc = PIMC_ERR_HIP;
+    }
+    for (double *p : {dk, dca, dcb, dJ, dY, dout}) dfree(p);
+    return rc;
 }
 
 } // extern "C"

@@ -753,8 +753,9 @@ template <typename T, int DIM> __device__ __forceinline__ T dist2(const T *x, co
 // the ids, then the beads at slice j, of R candidates before it tests any,
 // then the beads at slice jn of the neighbours only.  A query whose cell is
 // query 0's shares query 0's list and loads.  Each query sees its candidates
-// in the order of a one-at-a-time scan.  term(u, rkn, r2, mult) is called for
-// every neighbour k != n of query u, with r2 = |r_kj - q_u|^2 and rkn = r_k,jn.
+// in the order of a one-at-a-time scan.  term(u, rk, rkn, r2, mult) is called
+// for every neighbour k != n of query u, with rk = r_kj, r2 = |r_kj - q_u|^2
+// and rkn = r_k,jn.
 template <typename T, int DIM, bool CELLS, int NQ, int R, typename PathT, typename F>
 __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P,
                                            int n, int j, int jn, const T (&q)[NQ][DIM], int g, int G, F &&term)
@@ -831,7 +832,8 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
             for (int u = 0; u < NQ; ++u) {
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    if (near[u][r]) term(u, lst[u] == 0 ? rkn[0][r] : rkn[u][r], r2[u][r], T(1));
+                    if (near[u][r])
+                        term(u, lst[u] == 0 ? rk[0][r] : rk[u][r], lst[u] == 0 ? rkn[0][r] : rkn[u][r], r2[u][r], T(1));
             }
         }
     } else {
@@ -868,16 +870,16 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
             for (int r = 0; r < R; ++r) {
 #pragma unroll
                 for (int u = 0; u < NQ; ++u)
-                    if (mult[u][r] > T(0)) term(u, rkn[r], r2[u][r], mult[u][r]);
+                    if (mult[u][r] > T(0)) term(u, rk[r], rkn[r], r2[u][r], mult[u][r]);
             }
         }
     }
 }
 
 // This lane's share of the pair link sum of particle n over link j -> j+1,
-// with its beads there replaced by pj, pj1 (see pair_link): lnU =
-// -tau/2 (v(r_j) + v(r_j+1)) for every neighbour found at slice j.
-template <typename T, int DIM, bool CELLS, typename PathT>
+// with its beads there replaced by pj, pj1 (see pair_link): pair_lnU for
+// every neighbour found at slice j.
+template <typename T, int DIM, bool CELLS, int TAB = -1, typename PathT>
 __device__ T link_part(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P, int n, int j,
                        const T *pj, const T *pj1, int g, int G)
 {
@@ -887,16 +889,16 @@ __device__ T link_part(const PathT &path, const Cells &C, int N, int nbins, cons
     for (int e = 0; e < DIM; ++e) q[0][e] = pj[e];
     T acc = T(0);
     scan_pairs<T, DIM, CELLS, 1, 4>(path, C, N, nbins, P, n, j, jn, q, g, G,
-                                    [&](int, const T *rkn, T r2, T mult) {
+                                    [&](int, const T *rk, const T *rkn, T r2, T mult) {
                                         const T s2 = dist2<T, DIM>(rkn, pj1, P);
-                                        acc += mult * ((T(-0.5) * P.tau) * (vpair_r2(r2, P) + vpair_r2(s2, P)));
+                                        acc += mult * pair_lnU<T, DIM, TAB>(rk, pj, rkn, pj1, r2, s2, P);
                                     });
     return acc;
 }
 
 // This lane's share of (link sum with beads n0, n1) - (link sum with o0, o1)
 // for particle n over link j: both scans in one batched pass.
-template <typename T, int DIM, bool CELLS, int R, typename PathT>
+template <typename T, int DIM, bool CELLS, int R, int TAB = -1, typename PathT>
 __device__ __forceinline__ T link_delta(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P, int n,
                                         int j, int jn, const T *n0, const T *n1, const T *o0, const T *o1, int g,
                                         int G)
@@ -909,9 +911,10 @@ __device__ __forceinline__ T link_delta(const PathT &path, const Cells &C, int N
     }
     T acc[2] = {T(0), T(0)};
     scan_pairs<T, DIM, CELLS, 2, R>(path, C, N, nbins, P, n, j, jn, q, g, G,
-                                    [&](int u, const T *rkn, T r2, T mult) {
-                                        const T s2 = dist2<T, DIM>(rkn, u == 0 ? n1 : o1, P);
-                                        acc[u] += mult * ((T(-0.5) * P.tau) * (vpair_r2(r2, P) + vpair_r2(s2, P)));
+                                    [&](int u, const T *rk, const T *rkn, T r2, T mult) {
+                                        const T *qn = u == 0 ? n1 : o1;
+                                        const T s2 = dist2<T, DIM>(rkn, qn, P);
+                                        acc[u] += mult * pair_lnU<T, DIM, TAB>(rk, u == 0 ? n0 : o0, rkn, qn, r2, s2, P);
                                     });
     return acc[0] - acc[1];
 }
@@ -941,7 +944,7 @@ __device__ inline void pair_energy_bead(const PathT &path, const Cells &C, int N
 #pragma unroll
     for (int e = 0; e < DIM; ++e) q[0][e] = ri[e];
     scan_pairs<T, DIM, CELLS, 1, 4>(path, C, N, nbins, P, i, j, jn, q, g, G,
-                                    [&](int, const T *rkn, T r2, T mult) {
+                                    [&](int, const T *, const T *rkn, T r2, T mult) {
                                         const T s2 = dist2<T, DIM>(rkn, rin, P);
                                         const double ea = (double)ex(-r2 * P.inv2s2), eb = (double)ex(-s2 * P.inv2s2);
                                         const double w = 0.5 * (0.5 * (double)P.tau) * (double)mult;
@@ -962,14 +965,14 @@ __device__ inline ChainMeas measure_chain(const PathT &path, const Cells &C, con
                                           int G = 1, const int *nxt = nullptr)
 {
     const int N = ch.N, M = ch.M;
-    const double inv_s2 = (ch.pair != kPairNone) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
+    const double inv_s2 = (ch.pair == kPairGaussian) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
     double l = 0.0, v = 0.0, g = 0.0, su = 0.0, rgp = 0.0, r2w = 0.0;
     // bead b = (slice j, particle i) per group of G lanes; the pair scan is split over
     // the lanes, the single-bead terms are taken by lane 0 of the group
     const int lg = threadIdx.x & (G - 1), ngroups = blockDim.x / G;
     for (int b = threadIdx.x / G; b < N * M; b += ngroups) {
         const int j = b / N, i = b - j * N;
-        if (ch.pair != kPairNone) {
+        if (ch.pair == kPairGaussian) { // no pair energy for the tabulated propagator
             double a, bb;
             pair_energy_bead<T, DIM, CELLS>(path, C, N, ch.nbins, P, inv_s2, i, j, a, bb, lg, G);
             su += a;
@@ -1099,7 +1102,7 @@ __host__ __device__ inline size_t whole_path_bytes(const ChainDev &h, int tsize)
 // Centre of mass, for each particle in order: the link groups form the moved
 // beads on the fly and add Delta V and the pair difference, one workgroup
 // reduction gives a uniform decision, and the commit: two barriers.
-template <typename T, int DIM, bool CELLS, bool LDSR>
+template <typename T, int DIM, bool CELLS, bool LDSR, bool TAB>
 __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
 {
     static_assert(!(CELLS && LDSR), "LDS-resident paths are for all-pairs chains");
@@ -1245,7 +1248,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                         }
                         if (g == 0) s_dvb[j] = (t >= 1) ? potV<T, DIM>(n0, P) - potV<T, DIM>(o0, P) : T(0);
                         if (use_pair)
-                            part = link_delta<T, DIM, CELLS, R>(path, cells, N, ch.nbins, P, p, j, jn, n0, n1, o0, o1, g,
+                            part = link_delta<T, DIM, CELLS, R, TAB ? 1 : 0>(path, cells, N, ch.nbins, P, p, j, jn, n0, n1, o0, o1, g,
                                                                 G);
                     }
                 }
@@ -1324,7 +1327,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                     }
                     if (g == 0) part += -P.tau * (potV<T, DIM>(n0, P) - potV<T, DIM>(o0, P));
                     if (use_pair)
-                        part += link_delta<T, DIM, CELLS, R>(path, cells, N, ch.nbins, P, p, j, jn, n0, n1, o0, o1, g, G);
+                        part += link_delta<T, DIM, CELLS, R, TAB ? 1 : 0>(path, cells, N, ch.nbins, P, p, j, jn, n0, n1, o0, o1, g, G);
                 }
             }
             // workgroup sum of the log-weight change: waves, then wave totals in order

@@ -15,13 +15,13 @@ LIB_PATH = os.path.join(_PKG, "libpimc_hip.so")
 PIMC_OK = 0
 ERRORS = {-1: "PIMC_ERR_ARG", -2: "PIMC_ERR_HIP", -3: "PIMC_ERR_STATE", -4: "PIMC_ERR_CAPACITY", -5: "PIMC_ERR_NOMEM"}
 POT_NONE, POT_HARMONIC, POT_LATTICE = 0, 1, 2
-PAIR_NONE, PAIR_GAUSSIAN = 0, 1
+PAIR_NONE, PAIR_GAUSSIAN, PAIR_TABLE = 0, 1, 2
 FP32, FP64 = 0, 1
 
 EXPORTS = [
     "pimc_create", "pimc_destroy", "pimc_last_error", "pimc_device_info",
     "pimc_set_paths", "pimc_get_paths", "pimc_init_paths", "pimc_set_next", "pimc_get_next",
-    "pimc_get_com_step", "pimc_set_com_step", "pimc_get_ell",
+    "pimc_get_com_step", "pimc_set_com_step", "pimc_get_ell", "pimc_prop_int_terms", "pimc_set_pair_table",
     "pimc_run", "pimc_get_estimates", "pimc_get_counters", "pimc_get_blocks", "pimc_get_radial", "pimc_get_cartesian",
     "pimc_reset_estimators", "pimc_levy_propose", "pimc_reshape_linear", "pimc_com_propose",
     "pimc_center_of_mass", "pimc_interaction", "pimc_energy", "pimc_density", "pimc_adjoint", "pimc_action",
@@ -102,6 +102,8 @@ def load(path: str = LIB_PATH):
     L.pimc_get_com_step.argtypes = [vp, C.c_int32, _D]
     L.pimc_set_com_step.argtypes = [vp, C.c_int32, C.c_double]
     L.pimc_get_ell.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.pimc_prop_int_terms.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _D]
+    L.pimc_set_pair_table.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, _D]
     L.pimc_run.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(Counters)]
     L.pimc_get_estimates.argtypes = [vp, C.c_int32, C.POINTER(Estimates)]
     L.pimc_get_counters.argtypes = [vp, C.c_int32, _I64]
@@ -195,6 +197,13 @@ class Context:
 
     def set_com_step(self, chain: int, step: float):
         self.check(self.lib.pimc_set_com_step(self.h, chain, step), "pimc_set_com_step")
+
+    def set_pair_table(self, chain: int, terms: np.ndarray, r0: float, r1: float, mu: float = 0.0):
+        """Table of a PIMC_PAIR_TABLE chain: terms[i, k] on the grid r0..r1."""
+        t = np.ascontiguousarray(np.asarray(terms, dtype=np.float64))
+        if t.ndim != 2 or t.shape[0] != t.shape[1]:
+            raise PimcError("the pair table must be square")
+        self.check(self.lib.pimc_set_pair_table(self.h, chain, t.shape[0], r0, r1, mu, dp(t)), "pimc_set_pair_table")
 
     def ell(self, chain: int) -> int:
         x = C.c_int32()
@@ -312,6 +321,17 @@ class Context:
         S = C.c_double()
         self.check(self.lib.pimc_action(self.h, chain, C.byref(S)), "pimc_action")
         return S.value
+
+
+def prop_int_terms(n: int, r0: float, r1: float, g0: float, tau: float) -> np.ndarray:
+    """pimc_prop_int_terms: the n x n integral terms of the pair-product
+    propagator on the grid r0..r1, computed on the GPU."""
+    L = load()
+    out = np.zeros((n, n))
+    rc = L.pimc_prop_int_terms(n, r0, r1, g0, tau, dp(out))
+    if rc != PIMC_OK:
+        raise PimcError(f"pimc_prop_int_terms: {ERRORS.get(rc, rc)} {L.pimc_last_error(None).decode()}")
+    return out
 
 
 def device_info():

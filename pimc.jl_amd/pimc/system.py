@@ -6,7 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._capi import FP32, FP64, PAIR_NONE, ChainDesc, Context, Options
+from ._capi import FP32, FP64, PAIR_NONE, PAIR_TABLE, ChainDesc, Context, Options
 from .potentials import GaussianPair, HarmonicTrap, NoPotential, SquareLattice
 
 
@@ -62,7 +62,11 @@ def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_i
     d.omega = getattr(potential, "omega", 1.0)
     d.V0 = getattr(potential, "V0", 0.0)
     d.a = getattr(potential, "a", 1.0)
-    if pair is not None:
+    if pair is not None and pair.kind == PAIR_TABLE:
+        d.pair = PAIR_TABLE
+        d.g, d.sigma = pair.g0, 1.0
+        d.r_a = r_a if r_a > 0 else pair.nn_range()  # determine_nnrange (src/system.jl:29-31)
+    elif pair is not None:
         d.pair = pair.kind
         d.g, d.sigma = pair.g, pair.sigma
         d.r_a = r_a if r_a > 0 else pair.cutoff
@@ -86,17 +90,25 @@ class System:
     reference's acceptance.
 
     Keyword names follow the reference (lam for lambda, mu for mu, r_a for
-    r_a).  `pair` replaces `interactions`/`propint` with a device-side pair
-    potential; `reference_pair_semantics=True` keeps the reference's
-    behaviour of computing the pair sums but leaving them out of the
-    Metropolis ratio.
+    r_a).  `pair` is a device-side pair action: GaussianPair, or the
+    tabulated pair-product propagator of build_prop_int (PropInt), which
+    `interactions=True, propint=...` also selects as in the reference;
+    `reference_pair_semantics=True` keeps the reference's behaviour of
+    computing the pair sums but leaving them out of the Metropolis ratio.
+    The reference's hard-sphere radius a = exp(-2 pi / g) of interacting
+    systems (hardspherelevy!, src/updates/helper.jl:141-181) is not applied.
     """
 
     def __init__(self, potential=None, *, dV=None, dim: int = 2, M: int = 100, N: int = 2, mu: float = 0.0,
                  L: float = 4.0, T: float = 1.0, lam: float = 1.0, pair: GaussianPair | None = None,
                  r_a: float = 0.0, length_measurement_cycle: int = 10, precision: str = "fp64",
                  seed: int | None = None, r0: np.ndarray | None = None, reference_pair_semantics: bool = False,
-                 ell: int = 8, swaps: int = 0, swap_m: int = 0, swap_rule: int = 0):
+                 ell: int = 8, swaps: int = 0, swap_m: int = 0, swap_rule: int = 0, interactions: bool = False,
+                 propint=None):
+        if interactions:
+            if propint is None:
+                raise ValueError("interactions=True needs propint (build_prop_int)")
+            pair = propint
         self.potential = potential if potential is not None else NoPotential()
         self.dV = dV if dV is not None else getattr(self.potential, "dV", None)
         self.dim, self.M, self.N, self.Ninit = dim, M, N, N
@@ -117,6 +129,9 @@ class System:
         opt.sweeps_per_launch = 1
         opt.seed = int(self.rng.integers(0, 2 ** 63))
         self.ctx = Context([desc], opt)
+        self.r_a = desc.r_a
+        if desc.pair == PAIR_TABLE:
+            self.ctx.set_pair_table(0, pair.terms, pair.r0, pair.r1, mu)
         self.nbins = max(1, int(math.floor(2 * L / (desc.r_a if desc.r_a > 0 else L / 4))))
         if r0 is None:
             self.ctx.init_paths(0, int(self.rng.integers(0, 2 ** 63)))  # init_world on the device

@@ -1,0 +1,191 @@
+"""The tabulated pair-product propagator on the GPU against the oracle.
+
+pimc_prop_int_terms (device quadrature of src/propagator.jl:36-70) against
+the oracle's table; PIMC_PAIR_TABLE chains (lnU = log prop_int, src/system.jl:
+26-28) through the single-move hook and the whole-chain sweep kernel, FP64,
+against the oracle's link sums and sweep schedule.
+
+Tolerances: tables to 1e-10 of max(1, |terms|) (the device's Bessel functions
+and summation order differ from the host's); link sums to 1e-12 of
+sum |lnU|; sweep paths to 1e-9.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pimc import _capi as A
+from pimc.propagator import PropInt, build_prop_int
+from pimc.potentials import GaussianPair, HarmonicTrap
+from pimc.system import System, chain_desc
+
+pytestmark = pytest.mark.gpu
+
+
+def table_for(L, g0, tau, n):
+    r1 = float(math.ceil(math.sqrt(2.0) * L))
+    return O.prop_int_terms(n, 1e-20, r1, g0, tau), 1e-20, r1
+
+
+def table_desc(N, L, M, ra, beta=2.0, pot=None, ell=4, com_step=0.5):
+    pi = PropInt(np.zeros((2, 2)), 0.0, 1.0, 4.0, beta / M)  # kind marker; the table is set separately
+    return chain_desc(pot if pot is not None else HarmonicTrap(0.3), dim=2, M=M, N=N, L=L, beta=beta, lam=0.5,
+                      pair=pi, ell=ell, r_a=ra, com_step=com_step)
+
+
+def oracle_table_sys(d, r, tab, r0, r1, mu):
+    pot = O.make_pot(d.potential, d.omega, d.V0, d.a)
+    pair = O.make_pair(O.PAIR_TABLE, table=tab, r0=r0, r1=r1, mu=mu)
+    return O.System(r, d.lam, d.L, d.beta, pot, pair, d.r_a, bool(d.pair_in_accept))
+
+
+def ctx_for(descs, precision, seed=7, spl=4):
+    opt = A.Options()
+    opt.precision = precision
+    opt.sweeps_per_launch = spl
+    opt.seed = seed
+    return A.Context(descs, opt)
+
+
+@pytest.mark.parametrize("n,g0,tau", [(40, 4.0, 0.05), (70, 20.0, 0.01), (130, 1.0, 0.25)])
+def test_prop_int_terms_match_oracle(n, g0, tau):
+    gpu = A.prop_int_terms(n, 1e-20, 4.0, g0, tau)
+    ref = O.prop_int_terms(n, 1e-20, 4.0, g0, tau)
+    assert np.all(np.isfinite(gpu))
+    assert np.max(np.abs(gpu - ref) / np.maximum(1.0, np.abs(ref))) < 1e-10
+    assert np.array_equal(gpu, gpu.T) or np.max(np.abs(gpu - gpu.T)) < 1e-12 * np.max(np.abs(gpu))
+
+
+def test_reference_size_table():
+    """The reference's grid (600 x 600 on [1e-20, L]) for tau = 0.01: entries
+    at sampled grid points equal the oracle's point values; prop_int(r, r) < 1
+    at short range (repulsive), -> 1 at long range; nn_range where it crosses
+    0.999 (determine_nnrange, src/system.jl:10-15)."""
+    L, g0, tau = 6.0, 4.0, 0.01
+    p = build_prop_int(L, g0, tau)
+    assert p.terms.shape == (600, 600)
+    grid = 1e-20 + np.arange(600) * ((L - 1e-20) / 599)
+    rng = np.random.default_rng(1)
+    for i, k in rng.integers(0, 600, size=(12, 2)):
+        ref = O.prop_int_term(grid[i], grid[k], g0, tau, L)
+        assert abs(p.terms[i, k] - ref) < 1e-10 * max(1.0, abs(ref)), (i, k)
+    c = 4 * math.pi * tau
+    assert 1 + c * p.terms[5, 5] < 0.999 and abs(c * p.terms[-1, -1]) < 1e-6
+    ra = p.nn_range()
+    assert 0.0 < ra < L
+    assert 1 + c * p._diag(ra) == pytest.approx(0.999, abs=1e-9)
+
+
+@pytest.mark.parametrize("N,L,ra", [(30, 3.0, 1.5), (110, 5.0, 2.0)])
+def test_table_interaction_hook_matches_oracle(N, L, ra):
+    rng = np.random.default_rng(21)
+    M, beta, mu = 8, 2.0, 0.4
+    d = table_desc(N, L, M, ra, beta=beta)
+    tab, r0, r1 = table_for(L, 4.0, beta / M, 48)
+    r = rng.uniform(-L, L, size=(N, 2, M))
+    ctx = ctx_for([d], A.FP64)
+    ctx.set_paths(0, r)
+    with pytest.raises(A.PimcError):
+        ctx.interaction(0, 0, 0)  # no table yet
+    ctx.set_pair_table(0, tab, r0, r1, mu)
+    s = oracle_table_sys(d, r, tab, r0, r1, mu)
+    for _ in range(25):
+        n, j = int(rng.integers(N)), int(rng.integers(M))
+        w = ctx.interaction(0, n, j)
+        wo = s.interaction_old(n, j)
+        assert abs(w - wo) <= 1e-12 * max(1.0, abs(wo) * 10), (n, j, w, wo)
+
+
+@pytest.mark.parametrize("N,L,ra", [(30, 3.0, 1.5), (110, 5.0, 2.0)])
+def test_table_sweeps_match_oracle_fp64(N, L, ra):
+    """Whole-chain kernel with the tabulated propagator in the acceptance
+    (all-pairs with LDS-resident paths for N = 30, cell lists for N = 110),
+    next to a Gaussian pair chain in the same context: paths and counters
+    equal the oracle's sweep schedule."""
+    rng = np.random.default_rng(23)
+    M, beta, mu, seed, n_sweeps = 16, 2.0, 0.4, 5, 3
+    dt = table_desc(N, L, M, ra, beta=beta, ell=8)
+    dg = chain_desc(HarmonicTrap(0.3), dim=2, M=M, N=N, L=L, beta=beta, lam=0.5, pair=GaussianPair(1.0, 0.5), ell=8,
+                    r_a=ra, com_step=0.5)
+    tab, r0, r1 = table_for(L, 4.0, beta / M, 64)
+    inits = [rng.uniform(-L, L, size=(N, 2, M)) for _ in range(2)]
+    ctx = ctx_for([dg, dt], A.FP64, seed=seed, spl=n_sweeps)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    with pytest.raises(A.PimcError):
+        ctx.run(n_sweeps, measure=False)  # the table chain has no table yet
+    ctx.set_pair_table(1, tab, r0, r1, mu)
+    cnt = ctx.run(n_sweeps, measure=True)
+    la = ca = 0
+    for c, d in enumerate([dg, dt]):
+        if c == 0:
+            pot = O.make_pot(d.potential, d.omega, d.V0, d.a)
+            s = O.System(inits[c], d.lam, d.L, d.beta, pot, O.make_pair(d.pair, d.g, d.sigma), d.r_a, True)
+        else:
+            s = oracle_table_sys(d, inits[c], tab, r0, r1, mu)
+        for sw in range(n_sweeps):
+            st = s.sweep(c, sw, seed, d.ell, d.com_step)
+            la += st.levy_acc
+            ca += st.com_acc
+        assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
+    assert cnt.levy_acc == la and cnt.com_acc == ca
+    assert 0 < cnt.levy_acc < cnt.levy_tries
+    E, Ev, Ep = ctx.blocks(1)
+    assert np.isfinite(E[0]) and Ep[0] == 0.0
+
+
+def test_table_chain_fp32_runs():
+    rng = np.random.default_rng(29)
+    N, L, M, ra = 40, 3.0, 16, 1.5
+    d = table_desc(N, L, M, ra, beta=2.0, ell=8)
+    tab, r0, r1 = table_for(L, 4.0, 2.0 / M, 64)
+    ctx = ctx_for([d], A.FP32, spl=10)
+    ctx.set_paths(0, rng.uniform(-L, L, size=(N, 2, M)))
+    ctx.set_pair_table(0, tab, r0, r1, 0.4)
+    ctx.run(40, measure=False)
+    cnt = ctx.run(40, measure=True)
+    assert 0 < cnt.levy_acc < cnt.levy_tries and 0 < cnt.com_acc < cnt.com_tries
+    assert np.all(np.isfinite(ctx.get_paths(0)))
+
+
+def test_table_refusals():
+    d = table_desc(10, 3.0, 8, 1.5)
+    ctx = ctx_for([d], A.FP64)
+    with pytest.raises(A.PimcError):
+        ctx.adjoint(0)
+    with pytest.raises(A.PimcError):
+        ctx.set_pair_table(0, np.zeros((3, 4)), 0.0, 1.0)
+    with pytest.raises(A.PimcError):
+        ctx.set_pair_table(0, np.zeros((1, 1)), 0.0, 1.0)
+    ctx.set_pair_table(0, np.zeros((4, 4)), 0.0, 1.0)
+    dg = chain_desc(HarmonicTrap(0.3), dim=2, M=8, N=10, L=3.0, beta=2.0, lam=0.5, pair=GaussianPair(1.0, 0.5),
+                    r_a=1.5)
+    ctx2 = ctx_for([dg], A.FP64)
+    with pytest.raises(A.PimcError):
+        ctx2.set_pair_table(0, np.zeros((4, 4)), 0.0, 1.0)
+    d0 = table_desc(10, 3.0, 8, 1.5)
+    d0.r_a = 0.0
+    with pytest.raises(A.PimcError):
+        ctx_for([d0], A.FP64)
+    opt = A.Options()
+    opt.precision = A.FP64
+    opt.derivatives = 1
+    with pytest.raises(A.PimcError):
+        A.Context([d], opt)
+    with pytest.raises(A.PimcError):
+        A.prop_int_terms(1, 0.0, 1.0, 4.0, 0.1)
+
+
+def test_system_with_interactions_and_propint():
+    """System(potential; interactions=true, propint=build_prop_int(...)) as in
+    examples/density_SRL_lattice.jl:16-20: r_a from determine_nnrange, the
+    sweep samples with the table on the device."""
+    from pimc import HIPSweep, run
+    T, M, L = 1.0, 16, 4.0
+    p = build_prop_int(round(math.sqrt(2 * L * L) + 0.5), 4.0, 1.0 / (T * M), n=200)
+    s = System(HarmonicTrap(1.0), lam=0.5, M=M, N=6, L=L, T=T, interactions=True, propint=p, seed=3)
+    assert s.r_a == pytest.approx(p.nn_range())
+    sweep = HIPSweep(s)
+    run(s, 20, [(1, sweep)])
+    assert np.all(np.isfinite(s.paths))
