@@ -76,6 +76,12 @@ typedef struct pimc_chain_desc {
     int32_t swap_m;          /* links of a swap try, 2 .. M - 1 (0: ell) */
     int32_t swap_rule;       /* 0: acceptance with the heat-bath normalisations of the partner choice
                                 (detailed balance); 1: the reference's exp(w_updated - w_initial) */
+    double hs_a;             /* hard-sphere radius a (System's exp(-2 pi / g) with interactions, src/system.jl:148):
+                                bridge beads and COM shifts are redrawn while a moved bead lies closer than a to
+                                another particle's bead at its slice (hardspherelevy!, move_polymer!,
+                                src/updates/helper.jl:141-181, 368-395), at most 10000 draws thermalising and
+                                1000 measuring; 0: off.  PIMC_PAIR_TABLE chains only, a <= 2L / nbins; their
+                                single-move hooks are refused */
 } pimc_chain_desc;
 
 typedef struct pimc_options {

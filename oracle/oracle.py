@@ -46,6 +46,7 @@ class OSys(C.Structure):
         ("pot", OPot), ("pair", OPair),
         ("nbins", C.c_int), ("h", C.c_double), ("pair_in_accept", C.c_int),
         ("r", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)), ("next", C.POINTER(C.c_int64)),
+        ("hs_a", C.c_double), ("hs_ctr", C.c_int),
     ]
 
 
@@ -108,6 +109,7 @@ def lib():
         L.o_bin.argtypes = [C.POINTER(OSys), _D]; L.o_bin.restype = C.c_int
         L.o_find_nns.argtypes = [C.POINTER(OSys), _D, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]
         L.o_find_nns.restype = C.c_int
+        L.o_find_nn.argtypes = [C.POINTER(OSys), _D, C.c_int, C.c_int]; L.o_find_nn.restype = C.c_int
         L.o_interaction_old.argtypes = [C.POINTER(OSys), C.c_int, C.c_int]; L.o_interaction_old.restype = C.c_double
         L.o_interaction_new.argtypes = [C.POINTER(OSys), _D, C.c_int, C.c_int, C.c_int, C.c_int]
         L.o_interaction_new.restype = C.c_double
@@ -230,7 +232,7 @@ class System:
     """Oracle system; r has shape (N, dim, M) (the reference's per-particle M x dim, column-major)."""
 
     def __init__(self, r, lam, L, beta, pot: OPot, pair: OPair | None = None, r_a: float = 0.0,
-                 pair_in_accept: bool = False):
+                 pair_in_accept: bool = False, hs_a: float = 0.0, hs_ctr: int = 10000):
         r = np.ascontiguousarray(np.asarray(r, dtype=np.float64))
         N, dim, M = r.shape
         self._s = OSys()
@@ -239,7 +241,20 @@ class System:
         if rc != 0:
             raise ValueError(f"o_sys_init failed ({rc})")
         self._pair = pair  # keeps a pair table alive while the system holds its pointer
+        self._s.hs_a, self._s.hs_ctr = hs_a, hs_ctr
         self.N, self.dim, self.M = N, dim, M
+
+    @property
+    def hs_ctr(self):
+        return self._s.hs_ctr
+
+    @hs_ctr.setter
+    def hs_ctr(self, v):
+        self._s.hs_ctr = int(v)
+
+    def find_nn(self, target, j, exclude):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        return lib().o_find_nn(C.byref(self._s), _dp(t), j, exclude)
 
     def __del__(self):
         try:

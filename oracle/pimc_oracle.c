@@ -60,7 +60,7 @@ void o_box_muller(uint32_t x0, uint32_t x1, double *n0, double *n1)
 
 /* draw purposes: the fourth counter word of every keyed draw */
 enum { PUR_LEVY = 1, PUR_LEVY_ACC = 2, PUR_COM = 3, PUR_OFFSET = 4, PUR_INIT_START = 5, PUR_INIT_LEVY = 6,
-       PUR_SWAP = 7, PUR_SWAP_LEVY = 8 };
+       PUR_SWAP = 7, PUR_SWAP_LEVY = 8, PUR_HS = 9, PUR_HS_COM = 10 };
 
 static void keyed(uint64_t seed, uint32_t c0, uint32_t sweep, uint32_t chain, uint32_t purpose,
                   uint32_t out[4])
@@ -385,6 +385,8 @@ int o_sys_init(o_sys *s, int dim, int M, int N, double lambda, double L, double 
     if (!s->r || !s->V || !s->next) return -2;
     memcpy(s->r, r, sizeof(double) * (size_t)N * dim * M);
     for (int n = 0; n < N; ++n) s->next[n] = n;
+    s->hs_a = 0.0;
+    s->hs_ctr = 10000;
     o_sys_refresh_V(s);
     return 0;
 }
@@ -423,6 +425,27 @@ static double periodic_norm(const o_sys *s, const double *a, const double *b)
  * + :106-154 (find_nns).  Candidate order: stencil order, then particle index
  * (the reference's per-cell order depends on its update history; only the
  * summation order of the pair terms depends on it). */
+int o_find_nn(const o_sys *s, const double *target, int j, int exclude)
+{
+    const int nb = s->nbins, dim = s->dim;
+    int b = o_bin(s, target);
+    int bx = b % nb, by = (dim >= 2) ? (b / nb) % nb : 0;
+    int best = -1;
+    double bd = 0.0, rk[3];
+    for (int k = 0; k < s->N; ++k) {
+        if (k == exclude) continue;
+        bead(s, k, j, rk);
+        int ck = o_bin(s, rk), kx = ck % nb, ky = (dim >= 2) ? (ck / nb) % nb : 0;
+        /* k is a stencil candidate when its cell is within one cell of target's, periodically */
+        int dx = mod1(kx - bx, nb), dy = mod1(ky - by, nb);
+        int inx = dx == 0 || dx == 1 || dx == nb - 1, iny = (dim < 2) || dy == 0 || dy == 1 || dy == nb - 1;
+        if (!(inx && iny)) continue;
+        double d = periodic_norm(s, rk, target);
+        if (best < 0 || d < bd) { best = k; bd = d; }
+    }
+    return best;
+}
+
 int o_find_nns(const o_sys *s, const double *target, int j, int exclude, int *out, int cap)
 {
     const int nb = s->nbins, dim = s->dim;
@@ -1055,6 +1078,63 @@ static inline int owner(const o_sys *s, int p, int u) { return u < s->M ? p : (i
 static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int n_swap, int m, int rule,
                     o_sweepstats *st);
 
+/* whether the bead r of particle n at slice j is closer than a to its nearest
+ * neighbour (the test of hardspherelevy! and move_polymer!, src/updates/helper.jl:166-170, 385-389) */
+static int hs_near(const o_sys *s, const double *r, int j, int n)
+{
+    int nn = o_find_nn(s, r, j, n);
+    if (nn < 0) return 0;
+    double rk[3];
+    bead(s, nn, j, rk);
+    return periodic_norm(s, rk, r) < s->hs_a;
+}
+
+/* hardspherelevy! (src/updates/helper.jl:141-181) for particle p, the rows of
+ * rp holding the segment from slice `start`: the end shift of levy!, then bead
+ * by bead the bridge step, redrawn while the teleported bead lies closer than a
+ * to its nearest neighbour.  Draw 1 of interior bead i is normals[i - 1]; draw
+ * t >= 2 is Box-Muller of words 0, 1 of keyed(seed, p M + slice, sweep, chain,
+ * PUR_HS | t << 8).  Returns 0 when some bead needs more than hs_ctr draws. */
+static int hs_levy(const o_sys *s, int p, int start, double *rp, int nrows, const double *normals, uint32_t chain,
+                   uint32_t sweep, uint64_t seed)
+{
+    const int dim = s->dim, last = nrows - 1, M = s->M;
+    const double L = s->L;
+    for (int d = 0; d < dim; ++d) {
+        if (fabs(rp[d * nrows] - rp[d * nrows + last]) > L) {
+            double sg = rp[d * nrows] > 0.0 ? 1.0 : (rp[d * nrows] < 0.0 ? -1.0 : 0.0);
+            rp[d * nrows + last] += sg * (2.0 * L);
+        }
+    }
+    const int m = nrows - 2;
+    uint32_t x[4];
+    for (int j = 1; j <= m; ++j) {
+        double alpha = (double)(m + 1 - j) / (double)(m + 2 - j);
+        double sd = sqrt(((2.0 * s->lambda) * alpha) * s->tau);
+        const int slice = (start + j) % M;
+        double nr[3] = {0, 0, 0}, t[3];
+        for (int d = 0; d < dim; ++d) nr[d] = normals[(j - 1) * dim + d];
+        for (int attempt = 1;; ++attempt) {
+            if (attempt > s->hs_ctr) return 0;
+            if (attempt > 1) {
+                keyed(seed, (uint32_t)(p * M + slice), sweep, chain, PUR_HS | ((uint32_t)attempt << 8), x);
+                o_box_muller(x[0], x[1], &nr[0], &nr[1]);
+            }
+            for (int d = 0; d < dim; ++d) {
+                double a = alpha * rp[d * nrows + (j - 1)];
+                double b = (1.0 - alpha) * rp[d * nrows + last];
+                double c = nr[d] * sd;
+                rp[d * nrows + j] = (a + b) + c;
+                t[d] = o_teleport(rp[d * nrows + j], L);
+            }
+            if (!hs_near(s, t, slice, p)) break;
+        }
+    }
+    for (int j = 0; j < nrows; ++j)
+        for (int d = 0; d < dim; ++d) rp[d * nrows + j] = o_teleport(rp[d * nrows + j], L);
+    return 1;
+}
+
 static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
                        int n_swap, int swap_m, int rule, o_sweepstats *st)
 {
@@ -1090,11 +1170,18 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
                 nrm[(i - 1) * dim + 0] = n0;
                 if (dim > 1) nrm[(i - 1) * dim + 1] = n1;
             }
-            /* the bridge of levy!, with the fixed ends kept as stored */
+            /* the bridge of levy! (hardspherelevy! with a > 0), with the fixed ends kept as stored */
             double e0[3], em[3];
             for (int d = 0; d < dim; ++d) { e0[d] = rp[d * nrows]; em[d] = rp[d * nrows + m]; }
-            o_levy(rp, nrows, dim, s->tau, s->L, s->lambda, nrm);
+            int bridged = 1;
+            if (s->hs_a > 0.0) bridged = hs_levy(s, p, start, rp, nrows, nrm, chain, sweep, seed);
+            else o_levy(rp, nrows, dim, s->tau, s->L, s->lambda, nrm);
             for (int d = 0; d < dim; ++d) { rp[d * nrows] = e0[d]; rp[d * nrows + m] = em[d]; }
+            if (!bridged) { /* no position outside the hard spheres: a rejected try */
+                st->levy_tries++;
+                st->bead_moves += m - 1;
+                continue;
+            }
             /* Delta ln W = -tau sum_interior (V' - V) + Delta lnU */
             double dv = 0.0, a[3], b[3];
             for (int i = 1; i < m; ++i) {
@@ -1135,8 +1222,32 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
         double dsp[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) dsp[d] = (com_step * 2.0) * (o_u01(x[d]) - 0.5);
         double dv = 0.0, a[3], b[3];
-        for (int j = 0; j < M; ++j)
-            for (int d = 0; d < dim; ++d) rp[d * M + j] = o_teleport(R(s, p, d, j) + dsp[d], s->L);
+        /* move_polymer! (src/updates/helper.jl:368-395): with a > 0 the shift is redrawn
+         * while a moved bead lies closer than a to its nearest neighbour; draw t >= 2 is
+         * words 0..dim-1 of keyed(seed, p, sweep, chain, PUR_HS_COM | t << 8) */
+        int placed = 1;
+        for (int attempt = 1;; ++attempt) {
+            if (attempt > 1) {
+                if (attempt > s->hs_ctr) { placed = 0; break; }
+                uint32_t y[4];
+                keyed(seed, (uint32_t)p, sweep, chain, PUR_HS_COM | ((uint32_t)attempt << 8), y);
+                for (int d = 0; d < dim; ++d) dsp[d] = (com_step * 2.0) * (o_u01(y[d]) - 0.5);
+            }
+            for (int j = 0; j < M; ++j)
+                for (int d = 0; d < dim; ++d) rp[d * M + j] = o_teleport(R(s, p, d, j) + dsp[d], s->L);
+            if (!(s->hs_a > 0.0)) break;
+            int near = 0;
+            for (int j = 0; j < M && !near; ++j) {
+                for (int d = 0; d < dim; ++d) a[d] = rp[d * M + j];
+                near = hs_near(s, a, j, p);
+            }
+            if (!near) break;
+        }
+        if (!placed) { /* a rejected try */
+            st->com_tries++;
+            st->bead_moves += M;
+            continue;
+        }
         for (int j = 0; j < M; ++j) {
             for (int d = 0; d < dim; ++d) { a[d] = rp[d * M + j]; b[d] = R(s, p, d, j); }
             dv += o_V(&s->pot, a, dim) - o_V(&s->pot, b, dim);

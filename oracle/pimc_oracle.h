@@ -58,6 +58,10 @@ typedef struct {
     double *r;          /* N*dim*M */
     double *V;          /* N*M cached potential link actions (src/system.jl:72-74) */
     int64_t *next;      /* N, 0-based; identity for the hot-path configs */
+    double hs_a;        /* hard-sphere radius a = exp(-2 pi / g) of interacting systems (src/system.jl:148);
+                           0 switches the rejection off */
+    int hs_ctr;         /* redraws allowed per bead or COM shift (s.ctr: 10000 thermalising, 1000 measuring,
+                           src/simulation.jl:31-32) */
 } o_sys;
 
 /* ---- Philox4x32-10 (Salmon et al., SC'11) and the draw helpers ---- */
@@ -108,6 +112,9 @@ int  o_bin(const o_sys *s, const double *r);           /* src/nearest_neighbours
  * 9-cell stencil (with the reference's repeats when nbins < 3) and are kept
  * when their periodic distance is <= h.  Returns the count (<= cap). */
 int o_find_nns(const o_sys *s, const double *target, int j, int exclude, int *out, int cap);
+/* find_nn (src/nearest_neighbours.jl:157-180): the nearest bead at slice j among the
+ * particles of the 9 stencil cells around target, other than exclude; -1 if none */
+int o_find_nn(const o_sys *s, const double *target, int j, int exclude);
 
 /* interaction_action! values (src/updates/helper.jl:306-366), i.e. what the
  * reference would add to w if w were passed by reference. */

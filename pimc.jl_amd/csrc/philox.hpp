@@ -22,6 +22,9 @@ enum Purpose : uint32_t {
     kInitLevy = 6,   // normals of the initial path, pairs (i, i+1) with i odd, item = particle * M + i
     kSwap = 7,       // swap try a of a sweep: slice, first particle, partner uniform, Metropolis uniform; item = a
     kSwapLevy = 8,   // normals of swap try a's path w (0, 1), pairs (i, i+1) with i odd, item = (2a + w) M + i
+    kHardSphere = 9, // redraw t >= 2 of a bridge bead (words x, y), item = particle * M + slice; the purpose
+                     // word is kHardSphere | t << 8
+    kHardSphereCom = 10, // redraw t >= 2 of a COM shift (words 0..dim-1), item = particle; kHardSphereCom | t << 8
 };
 
 struct U4 {

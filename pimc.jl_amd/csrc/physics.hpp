@@ -31,6 +31,7 @@ struct ChainDev {
     long long bin_off;  // offset of the chain's per-bead cell index array (pair chains)
     const double *tab;  // kPairTable: integral terms of the pair-product propagator, tab_n x tab_n
     double tab_r0, tab_h, mu; // grid start and spacing; lnU where the propagator is negative
+    double hs_a;        // hard-sphere radius of hardspherelevy!/move_polymer! (0: off)
     int N, M, dim, pot;
     int pair, pair_in_accept, ell, nbins;
     int cap, ncell, part_base, nseg;
@@ -49,6 +50,7 @@ template <typename T> struct Prm {
     T sdl;               // sqrt(2 lambda tau): free-particle displacement of one link
     const double *tab;   // tabulated pair-product propagator (kPairTable), FP64 throughout
     double tab_r0, tab_h, mu, tau64;
+    T hs2;               // hard-sphere radius^2
     int tab_n;
     int pot, pair;
 
@@ -73,6 +75,7 @@ template <typename T> struct Prm {
         p.tab_h = c.tab_h;
         p.mu = c.mu;
         p.tau64 = c.tau;
+        p.hs2 = (T)(c.hs_a * c.hs_a);
         p.tab_n = c.tab_n;
         p.pot = c.pot;
         p.pair = c.pair;

@@ -425,6 +425,13 @@ int validate(const pimc_chain_desc &d, std::string &why)
     if (d.pair == PIMC_PAIR_GAUSSIAN && !(d.sigma > 0)) return why = "sigma must be positive", -1;
     if (d.pair == PIMC_PAIR_TABLE && !(d.r_a > 0))
         return why = "a tabulated pair propagator needs the neighbour range r_a > 0", -1;
+    if (!(d.hs_a >= 0)) return why = "hs_a must be >= 0", -1;
+    if (d.hs_a > 0) {
+        if (d.pair != PIMC_PAIR_TABLE) return why = "hard spheres (hs_a > 0) come with the tabulated pair propagator", -1;
+        const double ra = d.r_a > 0 ? d.r_a : d.L / 4.0;
+        const int nb = std::max(1, (int)std::floor((2.0 * d.L) / ra));
+        if (d.hs_a > (2.0 * d.L) / nb) return why = "hs_a must not exceed the cell side 2L / nbins", -1;
+    }
     if (d.potential == PIMC_POT_LATTICE && !(d.a > 0)) return why = "lattice constant a must be positive", -1;
     if (d.ell != 0 && (d.ell < 2 || d.ell > 32 || d.ell > d.M)) return why = "ell must be in [2, min(32, M)]", -1;
     if (!(d.com_step >= 0)) return why = "com_step must be >= 0", -1;
@@ -615,6 +622,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             h.swaps = d.swaps;
             h.swap_m = d.swaps > 0 ? (d.swap_m ? d.swap_m : h.ell) : 0;
             h.swap_rule = d.swap_rule;
+            h.hs_a = d.hs_a;
             ellmax = std::max(ellmax, h.ell);
             c->is_whole[i] = (d.pair != PIMC_PAIR_NONE || opt->derivatives || d.swaps > 0) ? 1 : 0;
         }
@@ -1428,6 +1436,9 @@ static int hook_move(pimc_ctx *c, int chain, int n, int j0, int m, const double 
     if (!com && (j0 < 0 || j0 >= h.M || m < 2 || m > h.M)) return set_err(c, PIMC_ERR_ARG, "bad j0 or m");
     if (h.swaps > 0)
         return set_err(c, PIMC_ERR_ARG, "single moves treat each particle as its own ring; chains with swap moves "
+                                        "sample through pimc_run");
+    if (h.hs_a > 0)
+        return set_err(c, PIMC_ERR_ARG, "single moves take injected draws; chains with hard spheres (hs_a > 0) "
                                         "sample through pimc_run");
     if (!in) return set_err(c, PIMC_ERR_ARG, "null input");
     if ((rc = check_table(c, chain))) return rc;

@@ -926,6 +926,43 @@ __device__ __forceinline__ T link_sum(const PathT &path, const Cells &C, int N, 
     return link_part<T, DIM, CELLS>(path, C, N, nbins, P, n, j, pj, pj1, 0, 1);
 }
 
+// The hard-sphere test of hardspherelevy! and move_polymer! (src/updates/
+// helper.jl:166-170, 385-389): whether the nearest bead at slice j among the
+// stencil candidates of q (find_nn), other than particle n's, is closer than
+// a.  With a <= h every bead closer than a is a stencil candidate, so this is
+// whether any such bead exists.  The 64 lanes of the calling wave split the
+// candidates; every lane of the wave must call it, and all get the same answer.
+template <typename T, int DIM, bool CELLS, typename PathT>
+__device__ bool hs_near_wave(const PathT &path, const Cells &C, int N, const Prm<T> &P, int n, int j, const T *q,
+                             int lane)
+{
+    bool near = false;
+    if constexpr (CELLS) {
+        constexpr int NS = (DIM == 1) ? 3 : 9;
+        const int c0 = cell_of<T, DIM>(q, P, C.nbins);
+        for (int s = 0; s < NS; ++s) {
+            const int cell = stencil_cell(c0, s, C.nbins, DIM);
+            const int cnt = C.count(j, cell);
+            const unsigned short *mem = C.members(j, cell);
+            for (int i = lane; i < cnt; i += 64) {
+                const int k = mem[i];
+                if (k == n) continue;
+                T rk[DIM];
+                path.get(k, j, rk);
+                near = near || dist2<T, DIM>(rk, q, P) < P.hs2;
+            }
+        }
+    } else {
+        for (int k = lane; k < N; k += 64) {
+            if (k == n) continue;
+            T rk[DIM];
+            path.get(k, j, rk);
+            near = near || dist2<T, DIM>(rk, q, P) < P.hs2;
+        }
+    }
+    return __ballot(near) != 0ull;
+}
+
 // Pair energy of bead (i, j) per unit coupling g: its share of S_U / g
 // (each unordered pair counted from both ends with weight 1/2) and of
 // (r . grad S_U) / g.  Kept free of g so that dS/dg is defined at g = 0.
@@ -1122,6 +1159,9 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, M, ch.ncell, ch.cap,
                       ch.nbins};
     const bool use_pair = (ch.pair != kPairNone) && ch.pair_in_accept;
+    // hard spheres (tabulated-propagator chains only): s.ctr of run!, src/simulation.jl:31-32
+    const bool hs = TAB && ch.hs_a > 0.0;
+    const int hs_ctr = a.measure ? 1000 : 10000;
     const uint32_t sweep0 = a.rs->sweep;
     const int G = a.link_group, ngroups = nt / G, g = tid & (G - 1), grp = tid / G;
     const int lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
@@ -1200,6 +1240,70 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
             }
             __syncthreads();
             // (A2) bridges
+            if (TAB && hs) {
+                // hardspherelevy! (src/updates/helper.jl:141-181): one wave per segment walks the
+                // bridge bead by bead, redrawing a bead (draw t >= 2: kHardSphere | t << 8) while it
+                // lies closer than a to another particle's bead at its slice; a segment that runs out
+                // of draws is a rejected try (s_flag = -1) and keeps its old beads
+                for (int k = wave; k < nseg; k += nw) {
+                    const int m = min(ell, M - k * ell);
+                    if (m < 2) continue;
+                    const int start = off + k * ell >= M ? off + k * ell - M : off + k * ell;
+                    int jm = start + m;
+                    if (jm >= M) jm -= M;
+                    T cur[DIM], em[DIM];
+                    path.get(p, start, cur);
+                    path.get(p, jm, em);
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
+                    bool failed = false;
+                    for (int i = 1; i < m && !failed; ++i) {
+                        int b = start + i;
+                        if (b >= M) b -= M;
+                        const T alpha = bridge_alpha<T>(m, i);
+                        const T sd = bridge_sd(alpha, P);
+                        T nr[2] = {T(0), T(0)};
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) nr[e] = s_nrm[b * DIM + e];
+                        for (int t = 1;; ++t) {
+                            if (t > hs_ctr) {
+                                failed = true;
+                                break;
+                            }
+                            if (t > 1) {
+                                const U4 xr = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c,
+                                                   (uint32_t)kHardSphere | ((uint32_t)t << 8));
+                                box_muller(xr.x, xr.y, nr[0], nr[1]);
+                            }
+                            T nx[DIM], tb[DIM];
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) {
+                                nx[e] = bridge_step(cur[e], em[e], alpha, nr[e], sd);
+                                tb[e] = teleport(nx[e], P);
+                            }
+                            if (!hs_near_wave<T, DIM, CELLS>(path, cells, N, P, p, b, tb, lane)) {
+#pragma unroll
+                                for (int e = 0; e < DIM; ++e) {
+                                    cur[e] = nx[e];
+                                    if (lane == 0) buf[b * DIM + e] = tb[e];
+                                }
+                                break;
+                            }
+                        }
+                    }
+                    if (failed) { // the old beads: the link pass then sees no change
+                        for (int i = 1 + lane; i < m; i += 64) {
+                            int b = start + i;
+                            if (b >= M) b -= M;
+                            T ro[DIM];
+                            path.get(p, b, ro);
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) buf[b * DIM + e] = ro[e];
+                        }
+                    }
+                    if (lane == 0) s_flag[k] = failed ? -1 : 0;
+                }
+            } else
             for (int k = tid; k < nseg; k += nt) {
                 const int m = min(ell, M - k * ell);
                 if (m >= 2) {
@@ -1281,6 +1385,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                     }
                     const U4 xa = draw(a.seed, (uint32_t)(p * M + start), sw, (uint32_t)c, kLevyAccept);
                     acc = metropolis(ex(lw), Uniform<T>::get(xa.x)) ? 1 : 0;
+                    if (TAB && hs && s_flag[k] < 0) acc = 0; // no bridge outside the hard spheres
                 }
                 s_flag[k] = acc;
                 lacc += acc;
@@ -1312,6 +1417,35 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
             T d[DIM];
             d[0] = step2 * (Uniform<T>::get(y.x) - T(0.5));
             if (DIM > 1) d[1] = step2 * (Uniform<T>::get(y.y) - T(0.5));
+            if (TAB && hs) {
+                // move_polymer! (src/updates/helper.jl:368-395): redraw the shift (draw t >= 2:
+                // kHardSphereCom | t << 8) while a moved bead lies closer than a to another
+                // particle's bead at its slice; out of draws, the try is rejected
+                bool placed = false;
+                for (int t = 1; t <= hs_ctr; ++t) {
+                    if (t > 1) {
+                        const U4 yy = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, (uint32_t)kHardSphereCom | ((uint32_t)t << 8));
+                        d[0] = step2 * (Uniform<T>::get(yy.x) - T(0.5));
+                        if (DIM > 1) d[1] = step2 * (Uniform<T>::get(yy.y) - T(0.5));
+                    }
+                    if (tid == 0) s_sw[1] = 0;
+                    __syncthreads();
+                    bool nr = false;
+                    for (int j = wave; j < M; j += nw) {
+                        T q[DIM];
+                        path.get(p, j, q);
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) q[e] = teleport(q[e] + d[e], P);
+                        nr = nr || hs_near_wave<T, DIM, CELLS>(path, cells, N, P, p, j, q, lane);
+                    }
+                    if (nr && lane == 0) s_sw[1] = 1;
+                    __syncthreads();
+                    placed = s_sw[1] == 0;
+                    __syncthreads();
+                    if (placed) break;
+                }
+                if (!placed) continue; // uniform: LDS value; counted as a try
+            }
             T part = T(0);
             for (int j0 = 0; j0 < M; j0 += ngroups) {
                 const int j = j0 + grp;

@@ -36,7 +36,7 @@ class ChainDesc(C.Structure):
         ("beta", C.c_double), ("lam", C.c_double), ("L", C.c_double),
         ("omega", C.c_double), ("V0", C.c_double), ("a", C.c_double),
         ("g", C.c_double), ("sigma", C.c_double), ("r_a", C.c_double), ("com_step", C.c_double),
-        ("swap_m", C.c_int32), ("swap_rule", C.c_int32),
+        ("swap_m", C.c_int32), ("swap_rule", C.c_int32), ("hs_a", C.c_double),
     ]
 
 

@@ -51,7 +51,7 @@ def init_world(N: int, M: int, dim: int, L: float, tau: float, lam: float, rng: 
 
 
 def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_in_accept=True, ell=0,
-               com_step=1.0, swaps=0, swap_m=0, swap_rule=0) -> ChainDesc:
+               com_step=1.0, swaps=0, swap_m=0, swap_rule=0, hs_a=0.0) -> ChainDesc:
     """pimc_chain_desc of one chain.  swaps > 0 adds that many swap tries per
     sweep (ReshapeSwapLinear) of swap_m links (0: ell); swap_rule 1 keeps the
     reference's acceptance exp(w_updated - w_initial)."""
@@ -78,6 +78,7 @@ def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_i
     d.ell = int(ell)
     d.com_step = float(com_step)
     d.swaps, d.swap_m, d.swap_rule = int(swaps), int(swap_m), int(swap_rule)
+    d.hs_a = float(hs_a)
     return d
 
 
@@ -95,8 +96,11 @@ class System:
     `interactions=True, propint=...` also selects as in the reference;
     `reference_pair_semantics=True` keeps the reference's behaviour of
     computing the pair sums but leaving them out of the Metropolis ratio.
-    The reference's hard-sphere radius a = exp(-2 pi / g) of interacting
-    systems (hardspherelevy!, src/updates/helper.jl:141-181) is not applied.
+    With interactions, a coupling g != 0 sets the reference's hard-sphere
+    radius a = exp(-2 pi / g) (src/system.jl:148): bridge beads and COM shifts
+    are redrawn on the device while they come closer than a to another
+    particle (hardspherelevy!, move_polymer!, src/updates/helper.jl:141-181,
+    368-395).  g = 0, the reference's default, gives a = 0.
     """
 
     def __init__(self, potential=None, *, dV=None, dim: int = 2, M: int = 100, N: int = 2, mu: float = 0.0,
@@ -104,11 +108,14 @@ class System:
                  r_a: float = 0.0, length_measurement_cycle: int = 10, precision: str = "fp64",
                  seed: int | None = None, r0: np.ndarray | None = None, reference_pair_semantics: bool = False,
                  ell: int = 8, swaps: int = 0, swap_m: int = 0, swap_rule: int = 0, interactions: bool = False,
-                 propint=None):
+                 propint=None, g: float = 0.0):
+        hs_a = 0.0
         if interactions:
             if propint is None:
                 raise ValueError("interactions=True needs propint (build_prop_int)")
             pair = propint
+            hs_a = math.exp(-2 * math.pi / g) if g != 0 else 0.0  # exp(-2 pi / 0.0) = 0 in the reference
+        self.a = hs_a
         self.potential = potential if potential is not None else NoPotential()
         self.dV = dV if dV is not None else getattr(self.potential, "dV", None)
         self.dim, self.M, self.N, self.Ninit = dim, M, N, N
@@ -123,7 +130,7 @@ class System:
         self.rng = np.random.default_rng(seed)
         desc = chain_desc(self.potential, dim=dim, M=M, N=N, L=L, beta=self.beta, lam=lam, pair=pair, r_a=r_a,
                           pair_in_accept=not reference_pair_semantics, ell=min(ell, M), swaps=swaps, swap_m=swap_m,
-                          swap_rule=swap_rule)
+                          swap_rule=swap_rule, hs_a=hs_a)
         opt = Options()
         opt.precision = FP64 if precision == "fp64" else FP32
         opt.sweeps_per_launch = 1

@@ -27,12 +27,28 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == set(A.EXPORTS)
 
 
-def test_struct_layouts_match_header():
-    # field counts and sizes of the ABI structs (8-byte aligned doubles after int32 groups)
-    assert ctypes.sizeof(A.ChainDesc) == 8 * 4 + 10 * 8 + 2 * 4
-    assert ctypes.sizeof(A.Options) == 4 * 4 + 8 + 4 * 4 + 8 + 2 * 4
-    assert ctypes.sizeof(A.Counters) == 6 * 8 + 2 * 8 + 8 + 2 * 8
-    assert ctypes.sizeof(A.Estimates) == 2 * 8 + 12 * 8
+C_NAMES = {"lam": "lambda"}  # ctypes field -> header field where the names differ
+
+
+@pytest.mark.parametrize("cstruct,pystruct", [("pimc_chain_desc", A.ChainDesc), ("pimc_options", A.Options),
+                                              ("pimc_counters", A.Counters), ("pimc_estimates", A.Estimates)])
+def test_struct_layouts_match_header(tmp_path, cstruct, pystruct):
+    """sizeof and every field offset of the ctypes mirrors equal the C header's,
+    as gcc lays the header out."""
+    fields = [f[0] for f in pystruct._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pimc_hip.h"', "int main(void) {",
+             f'printf("%zu\\n", sizeof({cstruct}));']
+    lines += [f'printf("%zu\\n", offsetof({cstruct}, {C_NAMES.get(f, f)}));' for f in fields]
+    lines += ["return 0; }"]
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(pystruct)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(pystruct, f).offset == off, f
 
 
 @pytest.mark.parametrize("field,value,msg", [

@@ -28,16 +28,16 @@ def table_for(L, g0, tau, n):
     return O.prop_int_terms(n, 1e-20, r1, g0, tau), 1e-20, r1
 
 
-def table_desc(N, L, M, ra, beta=2.0, pot=None, ell=4, com_step=0.5):
+def table_desc(N, L, M, ra, beta=2.0, pot=None, ell=4, com_step=0.5, hs_a=0.0):
     pi = PropInt(np.zeros((2, 2)), 0.0, 1.0, 4.0, beta / M)  # kind marker; the table is set separately
     return chain_desc(pot if pot is not None else HarmonicTrap(0.3), dim=2, M=M, N=N, L=L, beta=beta, lam=0.5,
-                      pair=pi, ell=ell, r_a=ra, com_step=com_step)
+                      pair=pi, ell=ell, r_a=ra, com_step=com_step, hs_a=hs_a)
 
 
 def oracle_table_sys(d, r, tab, r0, r1, mu):
     pot = O.make_pot(d.potential, d.omega, d.V0, d.a)
     pair = O.make_pair(O.PAIR_TABLE, table=tab, r0=r0, r1=r1, mu=mu)
-    return O.System(r, d.lam, d.L, d.beta, pot, pair, d.r_a, bool(d.pair_in_accept))
+    return O.System(r, d.lam, d.L, d.beta, pot, pair, d.r_a, bool(d.pair_in_accept), hs_a=d.hs_a)
 
 
 def ctx_for(descs, precision, seed=7, spl=4):
@@ -135,6 +135,44 @@ def test_table_sweeps_match_oracle_fp64(N, L, ra):
     assert np.isfinite(E[0]) and Ep[0] == 0.0
 
 
+@pytest.mark.parametrize("N,L,ra,hs_a", [(30, 3.0, 1.5, 0.3), (110, 5.0, 2.0, 0.25)])
+def test_hard_sphere_sweeps_match_oracle_fp64(N, L, ra, hs_a):
+    """hardspherelevy! and move_polymer! in the whole-chain kernel (one wave per
+    segment, keyed redraws) against the oracle's sweep schedule: thermalising
+    launches (10000 draws allowed), then measuring ones (1000), FP64: paths and
+    counters."""
+    rng = np.random.default_rng(31)
+    M, beta, mu, seed, spl = 16, 2.0, 0.4, 11, 2
+    d = table_desc(N, L, M, ra, beta=beta, ell=8, hs_a=hs_a)
+    tab, r0, r1 = table_for(L, 4.0, beta / M, 64)
+    r = rng.uniform(-L, L, size=(N, 2, M))
+    ctx = ctx_for([d], A.FP64, seed=seed, spl=spl)
+    ctx.set_paths(0, r)
+    ctx.set_pair_table(0, tab, r0, r1, mu)
+    with pytest.raises(A.PimcError):
+        ctx.reshape_linear(0, 0, 0, 3, np.zeros((2, 2)), 0.5)  # single moves refuse hard-sphere chains
+    c1 = ctx.run(2 * spl, measure=False)
+    c2 = ctx.run(spl, measure=True)
+    s = oracle_table_sys(d, r, tab, r0, r1, mu)
+    tot = dict(levy_acc=0, levy_tries=0, com_acc=0, com_tries=0)
+    step = d.com_step
+    for launch in range(3):
+        s.hs_ctr = 10000 if launch < 2 else 1000
+        ca = ct = 0
+        for k in range(spl):
+            st = s.sweep(0, launch * spl + k, seed, d.ell, step)
+            for key in tot:
+                tot[key] += getattr(st, key)
+            ca += st.com_acc
+            ct += st.com_tries
+        if launch < 2:
+            step = O.adapt_step(step, ca, ct, d.L)
+    assert np.max(np.abs(ctx.get_paths(0) - s.r)) < 1e-9
+    for key in tot:
+        assert getattr(c1, key) + getattr(c2, key) == tot[key], key
+    assert tot["levy_acc"] > 0 and tot["com_acc"] > 0
+
+
 def test_table_chain_fp32_runs():
     rng = np.random.default_rng(29)
     N, L, M, ra = 40, 3.0, 16, 1.5
@@ -175,6 +213,20 @@ def test_table_refusals():
         A.Context([d], opt)
     with pytest.raises(A.PimcError):
         A.prop_int_terms(1, 0.0, 1.0, 4.0, 0.1)
+
+
+def test_system_hard_sphere_radius_from_g():
+    """System(...; interactions=true, g) sets a = exp(-2 pi / g) (src/system.jl:148);
+    without g the reference's default g = 0 gives a = 0."""
+    from pimc import HIPSweep, run
+    T, M, L = 1.0, 16, 4.0
+    p = build_prop_int(6.0, 4.0, 1.0 / (T * M), n=200)
+    s = System(HarmonicTrap(1.0), lam=0.5, M=M, N=6, L=L, T=T, interactions=True, propint=p, g=4.0, seed=5)
+    assert s.a == pytest.approx(math.exp(-2 * math.pi / 4.0))
+    run(s, 20, [(1, HIPSweep(s))])
+    assert np.all(np.isfinite(s.paths))
+    s0 = System(HarmonicTrap(1.0), lam=0.5, M=M, N=6, L=L, T=T, interactions=True, propint=p, seed=5)
+    assert s0.a == 0.0
 
 
 def test_system_with_interactions_and_propint():
