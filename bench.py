@@ -54,6 +54,8 @@ def parse():
                     help="staging length of every chain (capped at M); with --tile-beads and --threads this "
                          "fixes the configuration the autotuner would otherwise pick")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
+    ap.add_argument("--measure-kernel", type=int, default=0, choices=[0, 1, 2],
+                    help="estimators of whole chains: 0 automatic, 1 in the sweep kernel, 2 a wide kernel")
     ap.add_argument("--workload", default="trapped_batch",
                     choices=["trapped_batch", "trapped_plain", "gauss_box", "trapped_gauss", "lattice"],
                     help="workload of the timed sweeps (the default is the headline batched form)")
@@ -161,7 +163,8 @@ def other_workload(args, A):
     spl = args.spl
     bt = Batch = B.Batch
     t = time.perf_counter()
-    b = Batch(wl, sweeps_per_launch=spl, seed=99, tile_beads=args.tile_beads, block_threads=args.threads)
+    b = Batch(wl, sweeps_per_launch=spl, seed=99, tile_beads=args.tile_beads, block_threads=args.threads,
+              measure_kernel=args.measure_kernel)
     log(f"{args.workload}: {len(wl.descs)} chains, setup {time.perf_counter() - t:.1f} s")
     warm = max(spl, (args.warmup // spl) * spl)
     steps = max(spl, (args.steps // spl) * spl)

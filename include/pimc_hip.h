@@ -107,6 +107,10 @@ typedef struct pimc_options {
                                   0.6, +1 above 0.8, within [2, min(32, M - 2)] (ReshapeLinear's minslices and
                                   maxslices, src/updates/reshape.jl:12-21); tile chains keep theirs (set by the
                                   autotuner).  pimc_get_ell reads the current value */
+    int32_t measure_kernel;    /* estimators of chains run whole per workgroup, in measuring runs: 1 inside the
+                                  sweep kernel; 2 in a wide kernel of their own after each sweep (the sweep
+                                  kernel then runs one sweep per launch); 0: 2 when such a chain has a pair
+                                  action and at least 8192 beads, else 1.  Same samples either way */
 } pimc_options;
 
 typedef struct pimc_counters {

@@ -73,6 +73,12 @@ struct pimc_ctx {
     // each of the three lists holds its Gaussian/pairless chains first, then from
     // this index the chains with the tabulated propagator (a kernel of their own)
     int cells_tab0 = 0, brute_tab0 = 0, lds_tab0 = 0;
+    // measuring steps with the estimators of whole chains in their own wide kernel
+    bool split_meas = false;
+    int meas_nblk = 1;
+    std::vector<int> meas_ids;      // whole chains: cell-list chains first, then all-pairs chains
+    int *d_meas_ids = nullptr;
+    double *d_meas_part = nullptr;  // [id][blk][6] partial sums
     std::vector<TileDev> tiles;
     std::vector<TileGroup> tile_groups;
     std::vector<int> part_chain, part_idx;
@@ -376,6 +382,63 @@ HookArgs hook_args(pimc_ctx *c, int chain)
 
 // One launch step: the sweep kernels for k sweeps, the fold into counters and
 // blocks, and the advance of the device-side sweep/block counters.
+template <typename T, int DIM> void launch_meas_d(pimc_ctx *c, int sweep_off)
+{
+    MeasArgs m{};
+    m.chains = c->d_ch;
+    m.prm = c->d_prm;
+    m.nblk = c->meas_nblk;
+    m.pos = c->d_pos;
+    m.cell_cnt = c->d_cell_cnt;
+    m.cell_mem = c->d_cell_mem;
+    m.bins = c->d_bins;
+    m.next = c->d_next;
+    m.radial = c->d_radial;
+    m.cart = c->d_cart;
+    m.radial_bins = c->opt.radial_bins;
+    m.radial_winv = c->opt.radial_bins > 0 ? c->opt.radial_bins / c->opt.radial_rmax : 0.0;
+    m.cart_bins = c->opt.cart_bins;
+    m.chain_meas = c->d_chain_meas;
+    m.rs = c->d_rs;
+    m.sweep_off = sweep_off;
+    m.measure_every = c->opt.measure_every;
+    const int ncell = (int)c->whole_cells_ids.size(), nall = (int)c->meas_ids.size();
+    if (ncell > 0) {
+        MeasArgs mc = m;
+        mc.ids = c->d_meas_ids;
+        mc.n_ids = ncell;
+        mc.part = c->d_meas_part;
+        hipLaunchKernelGGL((k_meas_beads<T, DIM, true>), dim3((unsigned)c->meas_nblk, (unsigned)ncell), dim3(256), 0,
+                           c->stream, mc);
+        HIPCHK(hipGetLastError());
+    }
+    if (nall > ncell) {
+        MeasArgs mb = m;
+        mb.ids = c->d_meas_ids + ncell;
+        mb.n_ids = nall - ncell;
+        mb.part = c->d_meas_part + (size_t)ncell * c->meas_nblk * 6;
+        hipLaunchKernelGGL((k_meas_beads<T, DIM, false>), dim3((unsigned)c->meas_nblk, (unsigned)(nall - ncell)),
+                           dim3(256), 0, c->stream, mb);
+        HIPCHK(hipGetLastError());
+    }
+    m.ids = c->d_meas_ids;
+    m.n_ids = nall;
+    m.part = c->d_meas_part;
+    hipLaunchKernelGGL(k_meas_fold, dim3((unsigned)((nall + 127) / 128)), dim3(128), 0, c->stream, m);
+    HIPCHK(hipGetLastError());
+}
+
+void launch_meas(pimc_ctx *c, int sweep_off)
+{
+    if (c->prec == PIMC_FP64) {
+        if (c->dim == 1) launch_meas_d<double, 1>(c, sweep_off);
+        else launch_meas_d<double, 2>(c, sweep_off);
+    } else {
+        if (c->dim == 1) launch_meas_d<float, 1>(c, sweep_off);
+        else launch_meas_d<float, 2>(c, sweep_off);
+    }
+}
+
 void launch_step(pimc_ctx *c, int k, int measure)
 {
     SweepArgs a = sweep_args(c);
@@ -383,7 +446,21 @@ void launch_step(pimc_ctx *c, int k, int measure)
     a.measure = measure ? 1 : 0;
     a.measure_every = c->opt.measure_every;
     if (!c->tiles.empty()) launch_tiles(c, a);
-    if (!c->whole_ids.empty()) launch_whole(c, a);
+    if (!c->whole_ids.empty()) {
+        if (measure && c->split_meas) {
+            // one sweep per launch of the whole-chain kernel, estimators in their own wide kernel
+            for (int s = 0; s < k; ++s) {
+                SweepArgs b = a;
+                b.nsweeps = 1;
+                b.sweep_off = s;
+                b.split = 1;
+                launch_whole(c, b);
+                launch_meas(c, s);
+            }
+        } else {
+            launch_whole(c, a);
+        }
+    }
     PostArgs pa{};
     pa.chains = c->d_ch;
     pa.n_chains = c->n_chains;
@@ -559,6 +636,11 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 *out = c;
                 return PIMC_ERR_ARG;
             }
+    }
+    if (opt->measure_kernel < 0 || opt->measure_kernel > 2) {
+        c->err = "measure_kernel must be 0, 1 or 2";
+        *out = c;
+        return PIMC_ERR_ARG;
     }
     {
         const int K = opt->sweeps_per_launch > 0 ? opt->sweeps_per_launch : 50;
@@ -779,6 +861,23 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             c->brute_tab0 = part(c->whole_brute_ids);
             c->lds_tab0 = part(c->whole_lds_ids);
         }
+        // estimators of whole chains: in the sweep kernel, or (split) a wide kernel after each sweep;
+        // automatic: split when a whole chain with a pair action has >= 8192 beads
+        {
+            long long most = 0;
+            bool big_pair = false;
+            for (int id : c->whole_ids) {
+                const long long nb = (long long)c->ch[id].N * c->ch[id].M;
+                most = std::max(most, nb);
+                big_pair = big_pair || (c->ch[id].pair != kPairNone && nb >= 8192);
+            }
+            const int mk = c->opt.measure_kernel;
+            c->split_meas = !c->whole_ids.empty() && (mk == 2 || (mk == 0 && big_pair));
+            c->meas_nblk = (int)std::max(1LL, (most + 1023) / 1024);
+            c->meas_ids = c->whole_cells_ids;
+            c->meas_ids.insert(c->meas_ids.end(), c->whole_brute_ids.begin(), c->whole_brute_ids.end());
+            c->meas_ids.insert(c->meas_ids.end(), c->whole_lds_ids.begin(), c->whole_lds_ids.end());
+        }
         // ---- device allocations
         c->d_ch = dalloc<ChainDev>(n_chains);
         HIPCHK(hipMemcpy(c->d_ch, c->ch.data(), sizeof(ChainDev) * n_chains, hipMemcpyHostToDevice));
@@ -830,6 +929,10 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         c->d_whole_cells = dalloc<int>(c->whole_cells_ids.size());
         c->d_whole_brute = dalloc<int>(c->whole_brute_ids.size());
         c->d_whole_lds = dalloc<int>(c->whole_lds_ids.size());
+        c->d_meas_ids = dalloc<int>(c->meas_ids.size());
+        if (!c->meas_ids.empty())
+            HIPCHK(hipMemcpy(c->d_meas_ids, c->meas_ids.data(), sizeof(int) * c->meas_ids.size(), hipMemcpyHostToDevice));
+        c->d_meas_part = dalloc<double>(c->split_meas ? c->meas_ids.size() * (size_t)c->meas_nblk * 6 : 1);
         if (!c->whole_lds_ids.empty())
             HIPCHK(hipMemcpy(c->d_whole_lds, c->whole_lds_ids.data(), sizeof(int) * c->whole_lds_ids.size(), hipMemcpyHostToDevice));
         if (!c->whole_ids.empty())
@@ -884,7 +987,7 @@ int pimc_destroy(pimc_ctx *c)
     void *ptrs[] = {c->d_ch, c->d_prm, c->d_tiles, c->d_pchain, c->d_pidx, c->d_pos, c->d_part_acc, c->d_placc, c->d_pcacc,
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
-                    c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next};
+                    c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next, c->d_meas_ids, c->d_meas_part};
     for (void *p : ptrs) dfree(p);
     for (double *p : c->d_tab) dfree(p);
     for (auto &e : c->events) {

@@ -121,6 +121,8 @@ struct SweepArgs {
     const struct RunState *rs; // first sweep of this launch (device side, so a captured graph replays)
     int nsweeps, measure;
     int measure_every; // estimators after the sweeps s with (s + 1) % measure_every == 0
+    int sweep_off;     // whole-chain kernel: this launch starts sweep_off sweeps after the run state's sweep
+    int split;         // whole-chain kernel: estimators left to k_meas_beads / k_meas_fold (one sweep per launch)
     int *next;         // per particle: the particle whose slice 0 follows its slice M - 1
     int tile0;             // first tile of this launch (tiles are grouped by potential kind)
     int link_group;        // lanes per link in the whole-chain kernel (power of two <= 64)
@@ -999,15 +1001,16 @@ template <typename T, int DIM, bool CELLS, typename PathT>
 __device__ inline ChainMeas measure_chain(const PathT &path, const Cells &C, const ChainDev &ch, const Prm<T> &P,
                                           double *wbuf, unsigned long long *radial, int radial_bins,
                                           double radial_winv, unsigned long long *cart, int cart_bins, int c,
-                                          int G = 1, const int *nxt = nullptr)
+                                          int G = 1, const int *nxt = nullptr, int b0 = 0, int b1 = -1)
 {
     const int N = ch.N, M = ch.M;
+    if (b1 < 0) b1 = N * M;
     const double inv_s2 = (ch.pair == kPairGaussian) ? 1.0 / (ch.sigma * ch.sigma) : 0.0;
     double l = 0.0, v = 0.0, g = 0.0, su = 0.0, rgp = 0.0, r2w = 0.0;
     // bead b = (slice j, particle i) per group of G lanes; the pair scan is split over
     // the lanes, the single-bead terms are taken by lane 0 of the group
     const int lg = threadIdx.x & (G - 1), ngroups = blockDim.x / G;
-    for (int b = threadIdx.x / G; b < N * M; b += ngroups) {
+    for (int b = b0 + (int)threadIdx.x / G; b < b1; b += ngroups) {
         const int j = b / N, i = b - j * N;
         if (ch.pair == kPairGaussian) { // no pair energy for the tabulated propagator
             double a, bb;
@@ -1162,7 +1165,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     // hard spheres (tabulated-propagator chains only): s.ctr of run!, src/simulation.jl:31-32
     const bool hs = TAB && ch.hs_a > 0.0;
     const int hs_ctr = a.measure ? 1000 : 10000;
-    const uint32_t sweep0 = a.rs->sweep;
+    const uint32_t sweep0 = a.rs->sweep + (uint32_t)a.sweep_off;
     const int G = a.link_group, ngroups = nt / G, g = tid & (G - 1), grp = tid / G;
     const int lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
     constexpr int R = 2; // candidates per lane and query loaded together
@@ -1211,7 +1214,7 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
 
     for (int s = 0; s < a.nsweeps; ++s) {
         const uint32_t sw = sweep0 + (uint32_t)s;
-        const bool meas_now = a.measure && meas_in == 0;
+        const bool meas_now = a.measure && !a.split && meas_in == 0;
         meas_in = (meas_in == 0) ? a.measure_every - 1 : meas_in - 1;
         // ---- staging: particles in order, segments (and links) of one particle in parallel
         for (int p = 0; p < N; ++p) {
@@ -1628,14 +1631,98 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     const double ca = block_sum((double)cacc, s_buf);
     if (tid == 0) {
         double *out = a.chain_meas + (size_t)c * (kNQ + kCnt);
-        for (int k = 0; k < kNQ; ++k) out[k] = s_q[k];
-        out[kNQ] = la;
-        out[kNQ + 1] = ca;
-        out[kNQ + 2] = ctry;
-        out[kNQ + 3] = sacc;
-        out[kNQ + 4] = stry;
+        if (a.split && a.sweep_off > 0) { // later one-sweep launches of a split step add their counts
+            out[kNQ] += la;
+            out[kNQ + 1] += ca;
+            out[kNQ + 2] += ctry;
+            out[kNQ + 3] += sacc;
+            out[kNQ + 4] += stry;
+        } else {
+            for (int k = 0; k < kNQ; ++k) out[k] = s_q[k]; // zero when split: k_meas_fold adds
+            out[kNQ] = la;
+            out[kNQ + 1] = ca;
+            out[kNQ + 2] = ctry;
+            out[kNQ + 3] = sacc;
+            out[kNQ + 4] = stry;
+        }
     }
     if (overflow) atomicOr(a.status + c, 1);
+}
+
+// Estimators of whole chains as a wide kernel (split measuring steps): the
+// sweep kernel runs one sweep per launch and leaves the estimators out; after
+// each such launch k_meas_beads spreads the chains' beads over nblk workgroups
+// per chain (the per-bead sums of measure_chain with 2 lanes per bead, FP64
+// partial sums per workgroup) and k_meas_fold adds the partials in workgroup order and folds the
+// measurement into the chain's kNQ sums.  Both skip sweeps that are not
+// measured ((sweep + 1) % measure_every != 0).
+struct MeasArgs {
+    const ChainDev *chains;
+    const void *prm;
+    const int *ids; // whole chains of this launch
+    int n_ids, nblk;
+    const void *pos;
+    int *cell_cnt;
+    unsigned short *cell_mem;
+    int *bins;
+    const int *next;
+    unsigned long long *radial, *cart;
+    double radial_winv;
+    int radial_bins, cart_bins;
+    double *part; // [id][blk][6]
+    double *chain_meas;
+    const RunState *rs;
+    int sweep_off, measure_every;
+};
+
+__device__ __forceinline__ bool split_measured(const MeasArgs &a)
+{
+    const uint32_t sw = a.rs->sweep + (uint32_t)a.sweep_off;
+    return (sw + 1u) % (uint32_t)a.measure_every == 0u;
+}
+
+template <typename T, int DIM, bool CELLS> __global__ void __launch_bounds__(256) k_meas_beads(MeasArgs a)
+{
+    __shared__ double wbuf[32];
+    if (!split_measured(a)) return;
+    const int y = blockIdx.y, c = a.ids[y];
+    const ChainDev ch = a.chains[c];
+    const Prm<T> P = reinterpret_cast<const Prm<T> *>(a.prm)[c];
+    const int nb = ch.N * ch.M, per = (nb + a.nblk - 1) / a.nblk;
+    const int b0 = min(nb, (int)blockIdx.x * per), b1 = min(nb, b0 + per);
+    const Path<T, DIM, true> path{reinterpret_cast<T *>(const_cast<void *>(a.pos)) + ch.pos_off, ch.M, ch.N};
+    const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, ch.M, ch.ncell, ch.cap,
+                      ch.nbins};
+    const int *nxt = ch.swaps > 0 ? a.next + ch.part_base : nullptr;
+    const ChainMeas m = measure_chain<T, DIM, CELLS>(path, cells, ch, P, wbuf, a.radial, a.radial_bins, a.radial_winv,
+                                                      a.cart, a.cart_bins, c, 2, nxt, b0, b1);
+    if (threadIdx.x == 0) {
+        double *o = a.part + ((size_t)y * a.nblk + blockIdx.x) * 6;
+        o[0] = m.link;
+        o[1] = m.V;
+        o[2] = m.rg;
+        o[3] = m.su;
+        o[4] = m.rgp;
+        o[5] = m.r2w;
+    }
+}
+
+__global__ void k_meas_fold(MeasArgs a)
+{
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= a.n_ids || !split_measured(a)) return;
+    const int c = a.ids[y];
+    ChainMeas m{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < a.nblk; ++b) {
+        const double *o = a.part + ((size_t)y * a.nblk + b) * 6;
+        m.link += o[0];
+        m.V += o[1];
+        m.rg += o[2];
+        m.su += o[3];
+        m.rgp += o[4];
+        m.r2w += o[5];
+    }
+    fold_meas(m, a.chains[c], a.chain_meas + (size_t)c * (kNQ + kCnt));
 }
 
 // After each launch: fold per-particle (tile kernel) or per-chain (whole

@@ -46,7 +46,7 @@ class Options(C.Structure):
         ("radial_bins", C.c_int32), ("cart_bins", C.c_int32), ("radial_rmax", C.c_double),
         ("tile_beads", C.c_int32), ("block_threads", C.c_int32),
         ("use_graph", C.c_int32), ("derivatives", C.c_int32), ("seed", C.c_uint64),
-        ("measure_every", C.c_int32), ("adapt_ell", C.c_int32),
+        ("measure_every", C.c_int32), ("adapt_ell", C.c_int32), ("measure_kernel", C.c_int32),
     ]
 
 

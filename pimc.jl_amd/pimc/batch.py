@@ -115,11 +115,13 @@ class Batch:
     workgroup sizes first (pimc.autotune); use_graph=True replays the launch
     sequence from a captured hipGraph; measure_every=k takes the estimators
     after every k-th sweep (the reference's measurement cycle, counted in
-    sweeps)."""
+    sweeps); measure_kernel chooses where whole chains take their estimators
+    (pimc_options.measure_kernel: 0 automatic, 1 in the sweep kernel, 2 a
+    wide kernel of their own)."""
 
     def __init__(self, wl: Workload, sweeps_per_launch: int = 50, seed: int = 12345, tile_beads: int = 0,
                  block_threads: int = 0, autotune: bool = False, use_graph: bool = False, tune_sweeps: int = 100,
-                 measure_every: int = 1):
+                 measure_every: int = 1, measure_kernel: int = 0):
         import copy
         descs = [copy.copy(d) for d in wl.descs]
         self.tune = None
@@ -141,6 +143,7 @@ class Batch:
         opt.use_graph = 1 if use_graph else 0
         opt.seed = seed
         opt.measure_every = measure_every
+        opt.measure_kernel = measure_kernel
         self.wl = wl
         self.descs = descs
         self.ctx = Context(descs, opt)

@@ -625,3 +625,50 @@ def test_set_next_validation():
     with pytest.raises(A.PimcError):
         make_ctx([chain_desc(NoPotential(), dim=2, M=8, N=3, L=4.0, beta=1.0, lam=0.5, pair=pair, ell=4, swaps=1)],
                  A.FP64)
+
+
+def test_split_estimators_match_in_kernel_fp64():
+    """measure_kernel 2 (one sweep per launch, estimators in a wide kernel of
+    their own) against 1 (estimators inside the sweep kernel), FP64, for cell-list,
+    all-pairs and permutation-cycle chains, with measure_every = 2: identical
+    paths, next[] and counters, estimator blocks to 1e-12, identical histograms."""
+    rng = np.random.default_rng(53)
+    pair = GaussianPair(1.0, 0.5)
+    descs = [
+        chain_desc(HarmonicTrap(0.3), dim=2, M=16, N=110, L=5.0, beta=2.0, lam=0.5, pair=pair, ell=8, r_a=2.0,
+                   com_step=0.5),
+        chain_desc(HarmonicTrap(0.3), dim=2, M=16, N=30, L=3.0, beta=2.0, lam=0.5, pair=pair, ell=8, r_a=1.5,
+                   com_step=0.5),
+        chain_desc(HarmonicTrap(1.0), dim=2, M=12, N=4, L=100.0, beta=2.0, lam=0.5, ell=4, com_step=0.8, swaps=3,
+                   swap_m=4),
+    ]
+    inits = [rng.uniform(-5.0, 5.0, size=(110, 2, 16)), rng.uniform(-3.0, 3.0, size=(30, 2, 16)),
+             rand_paths(rng, 4, 12, 100.0, scale=0.6)]
+    out = []
+    for mk in (1, 2):
+        opt = A.Options()
+        opt.precision = A.FP64
+        opt.sweeps_per_launch = 4
+        opt.seed = 19
+        opt.measure_every = 2
+        opt.radial_bins = 16
+        opt.radial_rmax = 4.0
+        opt.measure_kernel = mk
+        ctx = A.Context(descs, opt)
+        for c, r in enumerate(inits):
+            ctx.set_paths(c, r)
+        ctx.run(4, measure=False)
+        cnt = ctx.run(8, measure=True)
+        out.append(dict(cnt=(cnt.levy_acc, cnt.com_acc, cnt.swap_acc, cnt.com_tries),
+                        paths=[ctx.get_paths(c) for c in range(3)], next=ctx.get_next(2),
+                        blocks=[np.array(ctx.blocks(c)) for c in range(3)],
+                        radial=[ctx.radial(c) for c in range(3)]))
+        ctx.close()
+    a, b = out
+    assert a["cnt"] == b["cnt"]
+    for c in range(3):
+        assert np.array_equal(a["paths"][c], b["paths"][c]), c
+        assert np.allclose(a["blocks"][c], b["blocks"][c], rtol=1e-12, atol=1e-12), c
+        assert np.array_equal(a["radial"][c][0], b["radial"][c][0]), c
+    assert np.array_equal(a["next"], b["next"])
+    assert a["blocks"][0].shape[1] == 2  # two blocks of 4 sweeps, each with 2 measured sweeps
