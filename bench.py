@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--measure-kernel", type=int, default=0, choices=[0, 1, 2],
                     help="estimators of whole chains: 0 automatic, 1 in the sweep kernel, 2 a wide kernel")
     ap.add_argument("--workload", default="trapped_batch",
-                    choices=["trapped_batch", "trapped_plain", "gauss_box", "trapped_gauss", "lattice"],
+                    choices=["trapped_batch", "trapped_plain", "gauss_box", "trapped_gauss", "lattice", "contact_box"],
                     help="workload of the timed sweeps (the default is the headline batched form)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for "
@@ -151,6 +151,9 @@ WORKLOAD_TEXT = {
                      "FP32, reweighting sums for d<E>/d omega and d<E>/d g",
     "lattice": "64 chains of N=400 bosons on a 20x20 periodic square lattice V0=4 a=1, Gaussian pair g=1 "
                "sigma=0.3, M=512, beta=4, FP32, FP64 256x256 density histogram and energies",
+    "contact_box": "256 chains of N=256 bosons in a periodic 16x16 box with the tabulated pair-product propagator "
+                   "of a contact coupling g0=2 (600x600 table built on the GPU) and hard spheres a=exp(-2pi/g0), "
+                   "M=128, beta=2, FP32 paths, FP64 lnU and energy sums",
 }
 
 
@@ -158,7 +161,7 @@ def other_workload(args, A):
     """Throughput of one of the other workloads (no CPU leg)."""
     from pimc import batch as B
     build = {"trapped_plain": B.trapped_plain, "gauss_box": B.gauss_box, "trapped_gauss": B.trapped_gauss,
-             "lattice": B.lattice}[args.workload]
+             "lattice": B.lattice, "contact_box": B.contact_box}[args.workload]
     wl = build()
     spl = args.spl
     bt = Batch = B.Batch

@@ -940,19 +940,32 @@ __device__ bool hs_near_wave(const PathT &path, const Cells &C, int N, const Prm
 {
     bool near = false;
     if constexpr (CELLS) {
+        // all stencil counts first (independent loads), then the flattened candidate list
         constexpr int NS = (DIM == 1) ? 3 : 9;
         const int c0 = cell_of<T, DIM>(q, P, C.nbins);
+        int cs[NS], pre[NS + 1];
+        pre[0] = 0;
+#pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const int cell = stencil_cell(c0, s, C.nbins, DIM);
-            const int cnt = C.count(j, cell);
-            const unsigned short *mem = C.members(j, cell);
-            for (int i = lane; i < cnt; i += 64) {
-                const int k = mem[i];
-                if (k == n) continue;
-                T rk[DIM];
-                path.get(k, j, rk);
-                near = near || dist2<T, DIM>(rk, q, P) < P.hs2;
+            cs[s] = stencil_cell(c0, s, C.nbins, DIM);
+            pre[s + 1] = C.count(j, cs[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) pre[s + 1] += pre[s];
+        for (int idx = lane; idx < pre[NS]; idx += 64) {
+            int cell = cs[0], base = 0;
+#pragma unroll
+            for (int s = 1; s < NS; ++s) {
+                if (idx >= pre[s]) {
+                    cell = cs[s];
+                    base = pre[s];
+                }
             }
+            const int k = C.members(j, cell)[idx - base];
+            if (k == n) continue;
+            T rk[DIM];
+            path.get(k, j, rk);
+            near = near || dist2<T, DIM>(rk, q, P) < P.hs2;
         }
     } else {
         for (int k = lane; k < N; k += 64) {

@@ -15,6 +15,11 @@ reference's half-width, box [-L, L)^2):
 * lattice         N=400, M=512, beta=4, V0=4, a=1, box 20x20 (L=10), Gaussian
                   g=1, sigma=0.3, 64 chains, beads at lattice minima plus
                   Normal(0, 0.1), 256x256 density histogram.
+* contact_box     gauss_box's geometry with the reference's own interaction:
+                  the pair-product propagator of a contact coupling g0 = 2
+                  (build_prop_int on a 600 x 600 grid up to ceil(sqrt(2) L)),
+                  hard-sphere radius exp(-2 pi / g0), r_a from the table's
+                  0.999 crossing; 256 chains.
 """
 from __future__ import annotations
 
@@ -39,6 +44,7 @@ class Workload:
     cart_bins: int = 0
     derivatives: int = 0
     meta: dict = field(default_factory=dict)
+    pair_table: tuple | None = None  # (terms, r0, r1, mu) for every PIMC_PAIR_TABLE chain
 
 
 def _normal_start(rng, N, M, omega, L):
@@ -108,6 +114,23 @@ def lattice(n_chains: int = 64, seed: int = 5, precision: int = FP32, M: int = 5
     return Workload("lattice", descs, init, precision, cart_bins=256)
 
 
+def contact_box(n_chains: int = 256, seed: int = 6, precision: int = FP32, N: int = 256, M: int = 128,
+                ell: int = 8, g0: float = 2.0) -> Workload:
+    from .propagator import build_prop_int
+    rng = np.random.default_rng(seed)
+    L, beta = 8.0, 2.0
+    p = build_prop_int(float(math.ceil(math.sqrt(2.0) * L)), g0, beta / M)
+    ra, a = p.nn_range(), math.exp(-2.0 * math.pi / g0)
+    descs, init = [], []
+    for _ in range(n_chains):
+        descs.append(chain_desc(NoPotential(), dim=2, M=M, N=N, L=L, beta=beta, lam=0.5, pair=p, r_a=ra, ell=ell,
+                                com_step=1.0, hs_a=a))
+        x = rng.uniform(-L, L, size=(N, 2))
+        init.append(np.repeat(x[:, :, None], M, axis=2))
+    return Workload("contact_box", descs, init, precision, pair_table=(p.terms, p.r0, p.r1, 0.0),
+                    meta={"r_a": ra, "hs_a": a})
+
+
 class Batch:
     """Many independent chains in one context (the batched HIPSweep form).
 
@@ -149,6 +172,11 @@ class Batch:
         self.ctx = Context(descs, opt)
         for c, r in enumerate(wl.init):
             self.ctx.set_paths(c, r)
+        if wl.pair_table is not None:
+            from ._capi import PAIR_TABLE
+            for c, d in enumerate(descs):
+                if d.pair == PAIR_TABLE:
+                    self.ctx.set_pair_table(c, *wl.pair_table)
 
     def run(self, n_sweeps: int, measure: bool = False):
         return self.ctx.run(n_sweeps, measure)
