@@ -210,8 +210,9 @@ def other_workload(args, A):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of chains is design-only "
-                                   "(see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU: chains are independent and would shard across "
+                                   "GPUs with one final FP64 reduction of the estimator sums, which is not built",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
     from pimc import _capi as A
     from pimc.batch import Batch, trapped_batch, trapped_plain
