@@ -3,6 +3,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from pimc import _capi as A
@@ -104,3 +105,43 @@ def test_counter_and_adjust_follow_the_reference():
     for _ in range(5):
         adjust(n, 0.1)
     assert n.m == 2
+
+
+@pytest.mark.parametrize("case,msg", [
+    ("table_without_ra", "r_a"), ("hs_without_table", "hard spheres"), ("hs_above_cell_side", "cell side"),
+    ("negative_hs", "hs_a"), ("table_with_derivatives", "derivatives"), ("measure_kernel", "measure_kernel"),
+    ("pair_kind", "pair"),
+])
+def test_table_and_hard_sphere_descriptors_are_checked_before_any_device_call(case, msg):
+    from pimc.propagator import PropInt
+    p = PropInt(np.zeros((4, 4)), 0.0, 3.0, 4.0, 0.05)
+    d = chain_desc(HarmonicTrap(1.0), dim=2, M=32, N=4, L=4.0, beta=1.0, lam=0.5, pair=p, r_a=1.0)
+    opt = A.Options()
+    opt.precision = A.FP32
+    if case == "table_without_ra":
+        d.r_a = 0.0
+    elif case == "hs_without_table":
+        d = chain_desc(HarmonicTrap(1.0), dim=2, M=32, N=4, L=4.0, beta=1.0, lam=0.5, pair=GaussianPair(1.0, 0.5),
+                       hs_a=0.1)
+    elif case == "hs_above_cell_side":
+        d.hs_a = 1.5  # nbins = floor(8 / 1.0) = 8, cell side 1.0
+    elif case == "negative_hs":
+        d.hs_a = -0.1
+    elif case == "table_with_derivatives":
+        opt.derivatives = 1
+    elif case == "measure_kernel":
+        opt.measure_kernel = 3
+    elif case == "pair_kind":
+        d.pair = 3
+    with pytest.raises(A.PimcError) as e:
+        A.Context([d], opt)
+    assert "PIMC_ERR_ARG" in str(e.value)
+    assert msg in str(e.value)
+
+
+def test_prop_int_terms_checks_its_arguments_without_a_device():
+    for args in [(1, 0.0, 1.0, 4.0, 0.1), (8, 1.0, 0.5, 4.0, 0.1), (8, 0.0, 1.0, 0.0, 0.1), (8, 0.0, 1.0, 4.0, 0.0),
+                 (8, -1.0, 1.0, 4.0, 0.1)]:
+        with pytest.raises(A.PimcError) as e:
+            A.prop_int_terms(*args)
+        assert "PIMC_ERR_ARG" in str(e.value)
