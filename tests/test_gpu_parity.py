@@ -472,6 +472,43 @@ def test_graph_replay_matches_eager_launches():
     assert ctxs[0].counters(0)[1] == 60 * d.N * segs and ctxs[0].counters(0)[3] == 60 * d.N
 
 
+def test_graph_replay_with_split_estimators_and_tables():
+    """Graph replay of measuring steps split into one-sweep launches plus the
+    estimator kernels (a cell-list chain large enough for the automatic split),
+    next to a tabulated-propagator chain with hard spheres: bit-identical to
+    eager launches over several replays."""
+    from pimc.propagator import PropInt
+    rng = np.random.default_rng(61)
+    tau = 2.0 / 16
+    terms = O.prop_int_terms(64, 1e-20, 8.0, 2.0, tau)
+    p = PropInt(terms, 1e-20, 8.0, 2.0, tau)
+    descs = [chain_desc(NoPotential(), dim=2, M=96, N=110, L=5.0, beta=2.0, lam=0.5, pair=GaussianPair(1.0, 0.5),
+                        ell=8, r_a=2.0, com_step=0.5),
+             chain_desc(HarmonicTrap(0.5), dim=2, M=16, N=20, L=5.0, beta=2.0, lam=0.5, pair=p, r_a=1.0, ell=8,
+                        com_step=0.5, hs_a=0.05)]
+    inits = [rng.uniform(-5.0, 5.0, size=(110, 2, 96)), rng.uniform(-4.0, 4.0, size=(20, 2, 16))]
+    ctxs = []
+    for g in (0, 1):
+        opt = A.Options()
+        opt.precision = A.FP32
+        opt.sweeps_per_launch = 3
+        opt.seed = 5
+        opt.use_graph = g
+        ctx = A.Context(descs, opt)
+        for c, r in enumerate(inits):
+            ctx.set_paths(c, r)
+        ctx.set_pair_table(1, terms, 1e-20, 8.0, 0.0)
+        ctxs.append(ctx)
+    for rep in range(4):
+        measure = rep % 2 == 1
+        cnts = [ctx.run(6, measure=measure) for ctx in ctxs]
+        assert (cnts[0].levy_acc, cnts[0].com_acc) == (cnts[1].levy_acc, cnts[1].com_acc), rep
+        for c in range(2):
+            assert np.array_equal(ctxs[0].get_paths(c), ctxs[1].get_paths(c)), (rep, c)
+    for c in range(2):
+        assert all(np.array_equal(x, y) for x, y in zip(ctxs[0].blocks(c), ctxs[1].blocks(c)))
+
+
 def test_cell_list_overflow_is_reported():
     """All particles of a pair chain in one cell at every slice: set_paths
     reports PIMC_ERR_CAPACITY for that chain instead of sampling with a
