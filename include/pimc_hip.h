@@ -44,6 +44,10 @@ extern "C" {
 #define PIMC_POT_NONE 0
 #define PIMC_POT_HARMONIC 1 /* V = 1/2 omega^2 |r|^2, dV = omega^2 r */
 #define PIMC_POT_LATTICE 2  /* V = V0 sum_d cos^2(pi x_d / a)         */
+#define PIMC_POT_OPTICAL 3  /* V = V0 normalized_intensity(r, angles, scale; helical) of
+                               examples/tools/potentialtools.jl:1-38 (2D): the squared mean of the
+                               beams' plane waves; V0 < 0 is the attractive lattice of generate_V */
+#define PIMC_MAX_BEAMS 16
 
 /* pair actions, entering through the reference's lnU hook (src/system.jl:26-28) */
 #define PIMC_PAIR_NONE 0
@@ -82,6 +86,10 @@ typedef struct pimc_chain_desc {
                                 src/updates/helper.jl:141-181, 368-395), at most 10000 draws thermalising and
                                 1000 measuring; 0: off.  PIMC_PAIR_TABLE chains only, a <= 2L / nbins; their
                                 single-move hooks are refused */
+    double beam_scale;       /* PIMC_POT_OPTICAL: wave number of the beams (generate_V's scale) */
+    int32_t n_beams;         /* 1 .. PIMC_MAX_BEAMS */
+    int32_t helical;         /* 1: beam b carries the phase a_b (normalized_intensity's helical) */
+    double beam_angle[PIMC_MAX_BEAMS]; /* beam directions a_b: the wave vector is scale (sin a_b, cos a_b) */
 } pimc_chain_desc;
 
 typedef struct pimc_options {

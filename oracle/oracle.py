@@ -18,7 +18,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-POT_NONE, POT_HARMONIC, POT_LATTICE = 0, 1, 2
+POT_NONE, POT_HARMONIC, POT_LATTICE, POT_OPTICAL = 0, 1, 2, 3
 PAIR_NONE, PAIR_GAUSSIAN, PAIR_TABLE = 0, 1, 2
 
 
@@ -30,7 +30,8 @@ def build(force: bool = False) -> str:
 
 
 class OPot(C.Structure):
-    _fields_ = [("kind", C.c_int), ("omega", C.c_double), ("V0", C.c_double), ("a", C.c_double)]
+    _fields_ = [("kind", C.c_int), ("omega", C.c_double), ("V0", C.c_double), ("a", C.c_double),
+                ("nbeam", C.c_int), ("helical", C.c_int), ("scale", C.c_double), ("angle", C.c_double * 16)]
 
 
 class OPair(C.Structure):
@@ -179,8 +180,12 @@ def levy(rp: np.ndarray, tau: float, L: float, lam: float, normals: np.ndarray) 
     return buf.T.copy()
 
 
-def make_pot(kind=POT_NONE, omega=1.0, V0=0.0, a=1.0) -> OPot:
-    return OPot(kind, omega, V0, a)
+def make_pot(kind=POT_NONE, omega=1.0, V0=0.0, a=1.0, angles=(), scale=1.0, helical=False) -> OPot:
+    p = OPot(kind, omega, V0, a)
+    p.nbeam, p.helical, p.scale = len(angles), int(bool(helical)), scale
+    for i, x in enumerate(angles):
+        p.angle[i] = x
+    return p
 
 
 def make_pair(kind=PAIR_NONE, g=0.0, sigma=1.0, table=None, r0=0.0, r1=0.0, mu=0.0) -> OPair:
@@ -192,6 +197,25 @@ def make_pair(kind=PAIR_NONE, g=0.0, sigma=1.0, table=None, r0=0.0, r1=0.0, mu=0
         p.tab, p.tab_n, p.tab_r0, p.tab_r1, p.mu = _dp(t), t.shape[0], r0, r1, mu
         p._keep = t
     return p
+
+
+def pot_from_desc(d) -> OPot:
+    """The oracle potential of a chain descriptor (any object with the
+    pimc_chain_desc field names)."""
+    angles = [d.beam_angle[i] for i in range(d.n_beams)] if d.potential == POT_OPTICAL else ()
+    return make_pot(d.potential, d.omega, d.V0, d.a, angles=angles, scale=d.beam_scale, helical=bool(d.helical))
+
+
+def pot_V(pot: OPot, r):
+    a = np.ascontiguousarray(r, dtype=np.float64)
+    return lib().o_V(C.byref(pot), _dp(a), len(a))
+
+
+def pot_dV(pot: OPot, r):
+    a = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros(len(a))
+    lib().o_dV(C.byref(pot), _dp(a), len(a), _dp(out))
+    return out
 
 
 def prop_quad_nodes(rmax, tau):

@@ -116,6 +116,21 @@ double o_V(const o_pot *p, const double *r, int dim)
         }
         return p->V0 * v;
     }
+    case O_POT_OPTICAL: {
+        /* normalized_intensity (examples/tools/potentialtools.jl:1-16): plane waves of the
+         * beams summed, each sum divided by the beam count, squared and added */
+        const double pi = 3.14159265358979323846264338327950288;
+        double sn = 0.0, cs = 0.0;
+        for (int b = 0; b < p->nbeam; ++b) {
+            double rb = r[0] * sin(p->angle[b]) + r[1] * cos(p->angle[b]);
+            double arg = 2.0 * pi * rb * p->scale + (p->helical ? p->angle[b] : 0.0);
+            sn += sin(arg);
+            cs += cos(arg);
+        }
+        sn /= p->nbeam;
+        cs /= p->nbeam;
+        return p->V0 * (sn * sn + cs * cs);
+    }
     default:
         return 0.0;
     }
@@ -130,6 +145,23 @@ void o_dV(const o_pot *p, const double *r, int dim, double *out)
         const double pi = 3.14159265358979323846264338327950288;
         /* d/dx cos^2(pi x/a) = -(pi/a) sin(2 pi x / a) */
         for (int d = 0; d < dim; ++d) out[d] = -p->V0 * (pi / p->a) * sin(2.0 * pi * r[d] / p->a);
+    } else if (p->kind == O_POT_OPTICAL) {
+        /* V = V0 (S^2 + C^2) / n^2: dV/dx_d = 2 V0 / n^2 (S dS/dx_d + C dC/dx_d) */
+        const double pi = 3.14159265358979323846264338327950288;
+        double sn = 0.0, cs = 0.0, dsn[2] = {0, 0}, dcs[2] = {0, 0};
+        for (int b = 0; b < p->nbeam; ++b) {
+            double k[2] = {2.0 * pi * p->scale * sin(p->angle[b]), 2.0 * pi * p->scale * cos(p->angle[b])};
+            double arg = k[0] * r[0] + k[1] * r[1] + (p->helical ? p->angle[b] : 0.0);
+            double sa = sin(arg), ca = cos(arg);
+            sn += sa;
+            cs += ca;
+            for (int d = 0; d < 2; ++d) {
+                dsn[d] += ca * k[d];
+                dcs[d] -= sa * k[d];
+            }
+        }
+        const double f = 2.0 * p->V0 / ((double)p->nbeam * p->nbeam);
+        for (int d = 0; d < 2 && d < dim; ++d) out[d] = f * (sn * dsn[d] + cs * dcs[d]);
     }
 }
 

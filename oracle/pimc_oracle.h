@@ -23,13 +23,19 @@ extern "C" {
 #endif
 
 /* ---- potentials (the Julia closures become an enum + parameters) ---- */
-enum { O_POT_NONE = 0, O_POT_HARMONIC = 1, O_POT_LATTICE = 2 };
+enum { O_POT_NONE = 0, O_POT_HARMONIC = 1, O_POT_LATTICE = 2, O_POT_OPTICAL = 3 };
+#define O_MAX_BEAMS 16
 enum { O_PAIR_NONE = 0, O_PAIR_GAUSSIAN = 1, O_PAIR_TABLE = 2 };
 
 typedef struct {
     int kind;        /* O_POT_* */
     double omega;    /* harmonic: V = 1/2 omega^2 |r|^2 (m = 1) */
     double V0, a;    /* lattice: V = V0 * sum_d cos^2(pi x_d / a) */
+    /* optical: V = V0 * normalized_intensity(r, angles, scale; helical)
+     * (examples/tools/potentialtools.jl:1-16), 2D */
+    int nbeam, helical;
+    double scale;
+    double angle[O_MAX_BEAMS];
 } o_pot;
 
 typedef struct {

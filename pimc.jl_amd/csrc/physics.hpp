@@ -14,7 +14,8 @@
 
 namespace pimc {
 
-constexpr int kPotNone = 0, kPotHarmonic = 1, kPotLattice = 2;
+constexpr int kPotNone = 0, kPotHarmonic = 1, kPotLattice = 2, kPotOptical = 3;
+constexpr int kMaxBeams = 16;
 constexpr int kPairNone = 0, kPairGaussian = 1, kPairTable = 2;
 
 // One chain as the kernels see it (device-resident table, 8-byte fields
@@ -32,12 +33,14 @@ struct ChainDev {
     const double *tab;  // kPairTable: integral terms of the pair-product propagator, tab_n x tab_n
     double tab_r0, tab_h, mu; // grid start and spacing; lnU where the propagator is negative
     double hs_a;        // hard-sphere radius of hardspherelevy!/move_polymer! (0: off)
+    const void *beam;   // kPotOptical: nbeam triples (kx, ky, phase) in revolutions, in the arithmetic type
     int N, M, dim, pot;
     int pair, pair_in_accept, ell, nbins;
     int cap, ncell, part_base, nseg;
     int sm;                   // positions stored slice-major (chains run whole per workgroup)
     int swaps, swap_m, swap_rule; // swap tries per sweep, links per try, acceptance rule
     int tab_n;
+    int nbeam;
 };
 
 // The chain constants converted to the arithmetic type (built once per chain
@@ -51,6 +54,9 @@ template <typename T> struct Prm {
     const double *tab;   // tabulated pair-product propagator (kPairTable), FP64 throughout
     double tab_r0, tab_h, mu, tau64;
     T hs2;               // hard-sphere radius^2
+    const T *beam;       // optical lattice: (kx, ky, phase) per beam, revolutions
+    T nbeamT;            // beam count
+    int nbeam;
     int tab_n;
     int pot, pair;
 
@@ -76,6 +82,9 @@ template <typename T> struct Prm {
         p.mu = c.mu;
         p.tau64 = c.tau;
         p.hs2 = (T)(c.hs_a * c.hs_a);
+        p.beam = static_cast<const T *>(c.beam);
+        p.nbeam = c.nbeam;
+        p.nbeamT = (T)c.nbeam;
         p.tab_n = c.tab_n;
         p.pot = c.pot;
         p.pair = c.pair;
@@ -93,6 +102,45 @@ __device__ __forceinline__ double ex(double x) { return exp(x); }
 __device__ __forceinline__ float ex(float x) { return __expf(x); }
 __device__ __forceinline__ double sq(double x) { return sqrt(x); }
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); } // v_sqrt_f32, 1 ulp
+// sin and cos of 2 pi t: FP64 through sinpi/cospi, FP32 through v_sin/v_cos (revolutions) of
+// the fractional part (the hardware range is |t| <= 256)
+__device__ __forceinline__ void sincos2pi(double t, double &s, double &c)
+{
+    s = sinpi(2.0 * t);
+    c = cospi(2.0 * t);
+}
+__device__ __forceinline__ void sincos2pi(float t, float &s, float &c)
+{
+    const float f = __builtin_amdgcn_fractf(t);
+    s = __builtin_amdgcn_sinf(f);
+    c = __builtin_amdgcn_cosf(f);
+}
+
+// The optical-lattice intensity normalized_intensity (examples/tools/
+// potentialtools.jl:1-16): S = sum_b sin(2 pi t_b), C = sum_b cos(2 pi t_b),
+// t_b = scale (x sin a_b + y cos a_b) (+ a_b / 2 pi for helical beams);
+// V = V0 ((S/n)^2 + (C/n)^2).  With grad, also dS/dx_d and dC/dx_d.
+template <typename T, bool GRAD>
+__device__ __forceinline__ void optical_sums(const T *r, const Prm<T> &p, T &S, T &Cc, T *dS, T *dC)
+{
+    S = T(0);
+    Cc = T(0);
+    if (GRAD) dS[0] = dS[1] = dC[0] = dC[1] = T(0);
+    for (int b = 0; b < p.nbeam; ++b) {
+        const T kx = p.beam[3 * b], ky = p.beam[3 * b + 1];
+        const T t = kx * r[0] + ky * r[1] + p.beam[3 * b + 2];
+        T sn, cs;
+        sincos2pi(t, sn, cs);
+        S += sn;
+        Cc += cs;
+        if (GRAD) {
+            dS[0] += cs * kx;
+            dS[1] += cs * ky;
+            dC[0] -= sn * kx;
+            dC[1] -= sn * ky;
+        }
+    }
+}
 
 // src/propagator.jl:30-32, teleport(x) = (x + L) - floor(x / 2L + 1/2) 2L - L
 template <typename T> __device__ __forceinline__ T teleport(T x, const Prm<T> &p)
@@ -124,6 +172,12 @@ template <typename T, int DIM, int POT> __device__ __forceinline__ T potVk(const
             v += c1 * c1;
         }
         return p.V0 * v;
+    } else if constexpr (POT == kPotOptical && DIM == 2) {
+        T S, Cc;
+        optical_sums<T, false>(r, p, S, Cc, nullptr, nullptr);
+        S /= p.nbeamT;
+        Cc /= p.nbeamT;
+        return p.V0 * (S * S + Cc * Cc);
     } else {
         return T(0);
     }
@@ -142,6 +196,12 @@ template <typename T, int DIM, int POT> __device__ __forceinline__ T rgradVk(con
         T s = r[0] * spi(T(2) * r[0] * p.inva);
         if (DIM > 1) s += r[1] * spi(T(2) * r[1] * p.inva);
         return -p.V0 * pia * s;
+    } else if constexpr (POT == kPotOptical && DIM == 2) {
+        // dV/dx_d = 2 V0 / n^2 (S dS/dx_d + C dC/dx_d), the k_b carrying the factor 2 pi
+        T S, Cc, dS[2], dC[2];
+        optical_sums<T, true>(r, p, S, Cc, dS, dC);
+        const T f = T(2) * T(6.283185307179586476925286766559) * p.V0 / (p.nbeamT * p.nbeamT);
+        return f * (r[0] * (S * dS[0] + Cc * dC[0]) + r[1] * (S * dS[1] + Cc * dC[1]));
     } else {
         return T(0);
     }
@@ -153,6 +213,7 @@ template <typename T, int DIM> __device__ __forceinline__ T potV(const T *r, con
 {
     if (p.pot == kPotHarmonic) return potVk<T, DIM, kPotHarmonic>(r, p);
     if (p.pot == kPotLattice) return potVk<T, DIM, kPotLattice>(r, p);
+    if (p.pot == kPotOptical) return potVk<T, DIM, kPotOptical>(r, p);
     return T(0);
 }
 
@@ -160,6 +221,7 @@ template <typename T, int DIM> __device__ __forceinline__ T rgradV(const T *r, c
 {
     if (p.pot == kPotHarmonic) return rgradVk<T, DIM, kPotHarmonic>(r, p);
     if (p.pot == kPotLattice) return rgradVk<T, DIM, kPotLattice>(r, p);
+    if (p.pot == kPotOptical) return rgradVk<T, DIM, kPotOptical>(r, p);
     return T(0);
 }
 
@@ -174,6 +236,12 @@ template <typename T, int DIM> __device__ __forceinline__ void gradV(const T *r,
         const T pia = T(3.14159265358979323846) * p.inva;
         g[0] = -p.V0 * pia * spi(T(2) * r[0] * p.inva);
         if (DIM > 1) g[1] = -p.V0 * pia * spi(T(2) * r[1] * p.inva);
+    } else if (p.pot == kPotOptical && DIM == 2) {
+        T S, Cc, dS[2], dC[2];
+        optical_sums<T, true>(r, p, S, Cc, dS, dC);
+        const T f = T(2) * T(6.283185307179586476925286766559) * p.V0 / (p.nbeamT * p.nbeamT);
+        g[0] = f * (S * dS[0] + Cc * dC[0]);
+        if (DIM > 1) g[DIM - 1] = f * (S * dS[1] + Cc * dC[1]);
     }
 }
 

@@ -118,6 +118,7 @@ struct pimc_ctx {
     std::vector<double> hblocks; // host copy of the estimator blocks
     int hblk_n = -1;             // blocks held in hblocks (-1: stale)
     std::vector<double *> d_tab; // per chain: pair-product propagator table (PIMC_PAIR_TABLE chains)
+    void *d_beams = nullptr;     // per chain: kMaxBeams (kx, ky, phase) triples of PIMC_POT_OPTICAL, arithmetic type
 };
 
 namespace {
@@ -156,6 +157,7 @@ template <typename T, int DIM, int MAXL> void launch_tiles_t(pimc_ctx *c, const 
 {
     if (g.pot == PIMC_POT_HARMONIC) launch_tiles_p<T, DIM, MAXL, kPotHarmonic>(c, a, g);
     else if (g.pot == PIMC_POT_LATTICE) launch_tiles_p<T, DIM, MAXL, kPotLattice>(c, a, g);
+    else if (g.pot == PIMC_POT_OPTICAL) launch_tiles_p<T, DIM, MAXL, kPotOptical>(c, a, g);
     else launch_tiles_p<T, DIM, MAXL, kPotNone>(c, a, g);
 }
 
@@ -497,7 +499,12 @@ int validate(const pimc_chain_desc &d, std::string &why)
     if (d.N < 1 || d.N > 65535) return why = "N must be in [1, 65535]", -1;
     if (d.M < 4) return why = "M must be >= 4", -1;
     if (!(d.beta > 0) || !(d.lambda > 0) || !(d.L > 0)) return why = "beta, lambda and L must be positive", -1;
-    if (d.potential < PIMC_POT_NONE || d.potential > PIMC_POT_LATTICE) return why = "unknown potential", -1;
+    if (d.potential < PIMC_POT_NONE || d.potential > PIMC_POT_OPTICAL) return why = "unknown potential", -1;
+    if (d.potential == PIMC_POT_OPTICAL) {
+        if (d.dim != 2) return why = "the optical lattice potential is two-dimensional", -1;
+        if (d.n_beams < 1 || d.n_beams > PIMC_MAX_BEAMS) return why = "n_beams must be in [1, 16]", -1;
+        if (!(d.beam_scale > 0)) return why = "beam_scale must be positive", -1;
+    }
     if (d.pair < PIMC_PAIR_NONE || d.pair > PIMC_PAIR_TABLE) return why = "unknown pair potential", -1;
     if (d.pair == PIMC_PAIR_GAUSSIAN && !(d.sigma > 0)) return why = "sigma must be positive", -1;
     if (d.pair == PIMC_PAIR_TABLE && !(d.r_a > 0))
@@ -705,11 +712,37 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             h.swap_m = d.swaps > 0 ? (d.swap_m ? d.swap_m : h.ell) : 0;
             h.swap_rule = d.swap_rule;
             h.hs_a = d.hs_a;
+            h.nbeam = d.potential == PIMC_POT_OPTICAL ? d.n_beams : 0;
             ellmax = std::max(ellmax, h.ell);
             c->is_whole[i] = (d.pair != PIMC_PAIR_NONE || opt->derivatives || d.swaps > 0) ? 1 : 0;
         }
         c->maxl = next_pow2_maxl(ellmax);
         if (c->maxl > 32) throw HipError{hipErrorInvalidValue, "ell > 32"};
+        // optical-lattice beams in revolutions: t_b = scale (x sin a_b + y cos a_b) + [helical] a_b / 2 pi
+        {
+            const size_t per = 3 * kMaxBeams, n = (size_t)n_chains * per;
+            std::vector<double> bd(n, 0.0);
+            for (int i = 0; i < n_chains; ++i) {
+                const pimc_chain_desc &d = chains[i];
+                for (int b = 0; b < c->ch[i].nbeam; ++b) {
+                    bd[i * per + 3 * b] = d.beam_scale * std::sin(d.beam_angle[b]);
+                    bd[i * per + 3 * b + 1] = d.beam_scale * std::cos(d.beam_angle[b]);
+                    bd[i * per + 3 * b + 2] = d.helical ? d.beam_angle[b] / (2.0 * 3.14159265358979323846) : 0.0;
+                }
+            }
+            if (c->prec == PIMC_FP64) {
+                double *p = dalloc<double>(n);
+                c->d_beams = p;
+                HIPCHK(hipMemcpy(p, bd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> bf(bd.begin(), bd.end());
+                float *p = dalloc<float>(n);
+                c->d_beams = p;
+                HIPCHK(hipMemcpy(p, bf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+            }
+            for (int i = 0; i < n_chains; ++i)
+                c->ch[i].beam = static_cast<const char *>(c->d_beams) + (size_t)i * per * tsize(c);
+        }
 
         // ---- memory order: tile chains sorted by (M, ell), then whole chains
         std::vector<int> order(n_chains);
@@ -987,7 +1020,8 @@ int pimc_destroy(pimc_ctx *c)
     void *ptrs[] = {c->d_ch, c->d_prm, c->d_tiles, c->d_pchain, c->d_pidx, c->d_pos, c->d_part_acc, c->d_placc, c->d_pcacc,
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
-                    c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next, c->d_meas_ids, c->d_meas_part};
+                    c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next, c->d_meas_ids, c->d_meas_part,
+                    c->d_beams};
     for (void *p : ptrs) dfree(p);
     for (double *p : c->d_tab) dfree(p);
     for (auto &e : c->events) {

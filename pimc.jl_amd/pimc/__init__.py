@@ -18,7 +18,7 @@ from __future__ import annotations
 
 from ._capi import (FP32, FP64, PAIR_GAUSSIAN, PAIR_NONE, PAIR_TABLE, POT_HARMONIC, POT_LATTICE, POT_NONE, ChainDesc,
                     Context, Counters, Estimates, Options, PimcError, device_info, load, prop_int_terms)
-from .potentials import GaussianPair, HarmonicTrap, NoPotential, SquareLattice
+from .potentials import GaussianPair, HarmonicTrap, NoPotential, OpticalLattice, SquareLattice, generate_V
 from .propagator import PropInt, build_prop_int
 from .system import Particle, System, init_world
 from .updates import (Counter, NumbOfSlices, ReshapeLinear, SingleCenterOfMass, HIPSweep, Step, Update, UpdateC,
@@ -34,7 +34,7 @@ __all__ = [
     "ReshapeLinear", "SingleCenterOfMass", "HIPSweep", "Update", "UpdateC",
     "Counter", "Step", "NumbOfSlices", "adjust", "queue", "acceptance",
     "Energy", "Density", "Measurement", "ZMeasurement", "measurement_Z_sector",
-    "HarmonicTrap", "SquareLattice", "NoPotential", "GaussianPair", "PropInt", "build_prop_int", "prop_int_terms",
+    "HarmonicTrap", "SquareLattice", "OpticalLattice", "generate_V", "NoPotential", "GaussianPair", "PropInt", "build_prop_int", "prop_int_terms",
     "Batch", "Context", "ChainDesc", "Options", "Counters", "Estimates", "PimcError", "device_info", "load",
     "FP32", "FP64", "POT_NONE", "POT_HARMONIC", "POT_LATTICE", "PAIR_NONE", "PAIR_GAUSSIAN", "PAIR_TABLE",
 ]

@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(_PKG, "libpimc_hip.so")
 
 PIMC_OK = 0
 ERRORS = {-1: "PIMC_ERR_ARG", -2: "PIMC_ERR_HIP", -3: "PIMC_ERR_STATE", -4: "PIMC_ERR_CAPACITY", -5: "PIMC_ERR_NOMEM"}
-POT_NONE, POT_HARMONIC, POT_LATTICE = 0, 1, 2
+POT_NONE, POT_HARMONIC, POT_LATTICE, POT_OPTICAL = 0, 1, 2, 3
+MAX_BEAMS = 16
 PAIR_NONE, PAIR_GAUSSIAN, PAIR_TABLE = 0, 1, 2
 FP32, FP64 = 0, 1
 
@@ -37,6 +38,8 @@ class ChainDesc(C.Structure):
         ("omega", C.c_double), ("V0", C.c_double), ("a", C.c_double),
         ("g", C.c_double), ("sigma", C.c_double), ("r_a", C.c_double), ("com_step", C.c_double),
         ("swap_m", C.c_int32), ("swap_rule", C.c_int32), ("hs_a", C.c_double),
+        ("beam_scale", C.c_double), ("n_beams", C.c_int32), ("helical", C.c_int32),
+        ("beam_angle", C.c_double * 16),
     ]
 
 

@@ -9,7 +9,9 @@ from __future__ import annotations
 
 import numpy as np
 
-from ._capi import PAIR_GAUSSIAN, POT_HARMONIC, POT_LATTICE, POT_NONE
+import math
+
+from ._capi import MAX_BEAMS, PAIR_GAUSSIAN, POT_HARMONIC, POT_LATTICE, POT_NONE, POT_OPTICAL
 
 
 class NoPotential:
@@ -60,6 +62,68 @@ class SquareLattice:
 
     def params(self):
         return {"V0": self.V0, "a": self.a}
+
+
+class OpticalLattice:
+    """V(r) = depth * normalized_intensity(r, angles, scale; helical)
+    (examples/tools/potentialtools.jl:1-16, generate_V :18-23): the squared
+    mean of the beams' plane waves exp(i 2 pi scale (x sin a + y cos a)),
+    2D, at most 16 beams.  A negative depth is generate_V's attractive form."""
+    kind = POT_OPTICAL
+
+    def __init__(self, angles, scale: float = 1.0, depth: float = 1.0, helical: bool = False):
+        self.angles = [float(a) for a in angles]
+        if not 1 <= len(self.angles) <= MAX_BEAMS:
+            raise ValueError(f"1 to {MAX_BEAMS} beams")
+        self.scale, self.depth, self.helical = float(scale), float(depth), bool(helical)
+        self.V0 = self.depth
+
+    def _sums(self, r):
+        x, y = float(r[0]), float(r[1])
+        s = c = 0.0
+        ds, dc = np.zeros(2), np.zeros(2)
+        for a in self.angles:
+            k = 2 * math.pi * self.scale * np.array([math.sin(a), math.cos(a)])
+            t = k[0] * x + k[1] * y + (a if self.helical else 0.0)
+            s += math.sin(t)
+            c += math.cos(t)
+            ds += math.cos(t) * k
+            dc -= math.sin(t) * k
+        return s, c, ds, dc
+
+    def __call__(self, r):
+        s, c, _, _ = self._sums(r)
+        n = len(self.angles)
+        return self.depth * ((s / n) ** 2 + (c / n) ** 2)
+
+    def dV(self, r):
+        s, c, ds, dc = self._sums(r)
+        n = len(self.angles)
+        return 2 * self.depth / n ** 2 * (s * ds + c * dc)
+
+    def params(self):
+        return {"angles": self.angles, "scale": self.scale, "depth": self.depth, "helical": self.helical}
+
+
+# beam directions of generate_V(scale, depth, sym) (examples/tools/potentialtools.jl:25-39)
+BEAMS = {
+    "cubic": [2 * math.pi * k / 4 for k in range(4)],
+    "l25": [2.214297435588181, 0.9272952180016122, -0.6435011087932844, 0.6435011087932844, -2.498091544796509,
+            3.141592653589793, 2.498091544796509, 0.0, 1.5707963267948966, -2.2142974355881813,
+            -1.5707963267948968, -0.9272952180016123],
+    "l65": [0.5191461142465229, 1.695151321341658, -0.12435499454676144, -1.4464413322481353, 2.0899424410414196,
+            2.62244653934327, 0.12435499454676144, -1.0516502125483738, -1.695151321341658, 1.446441332248135,
+            -2.0899424410414196, -3.017237659043032, -0.5191461142465229, 3.017237659043032, -2.62244653934327,
+            1.0516502125483738],
+}
+
+
+def generate_V(scale: float, depth: float, sym: str, attractive: bool = True):
+    """generate_V(scale, depth, sym; attractive) (examples/tools/potentialtools.jl:25-39):
+    the square (cubic) and the 12- and 16-beam superlattices (l25, l65), or the harmonic trap."""
+    if sym == "harmonic":
+        return HarmonicTrap(1.0)
+    return OpticalLattice(BEAMS[sym], scale, -depth if attractive else depth)
 
 
 class GaussianPair:

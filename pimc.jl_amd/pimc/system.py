@@ -6,7 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._capi import FP32, FP64, PAIR_NONE, PAIR_TABLE, ChainDesc, Context, Options
+from ._capi import FP32, FP64, PAIR_NONE, PAIR_TABLE, POT_OPTICAL, ChainDesc, Context, Options
 from .potentials import GaussianPair, HarmonicTrap, NoPotential, SquareLattice
 
 
@@ -62,6 +62,10 @@ def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_i
     d.omega = getattr(potential, "omega", 1.0)
     d.V0 = getattr(potential, "V0", 0.0)
     d.a = getattr(potential, "a", 1.0)
+    if getattr(potential, "kind", 0) == POT_OPTICAL:
+        d.beam_scale, d.n_beams, d.helical = potential.scale, len(potential.angles), int(potential.helical)
+        for i, a in enumerate(potential.angles):
+            d.beam_angle[i] = a
     if pair is not None and pair.kind == PAIR_TABLE:
         d.pair = PAIR_TABLE
         d.g, d.sigma = pair.g0, 1.0

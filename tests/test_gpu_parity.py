@@ -30,7 +30,7 @@ def make_ctx(descs, precision, seed=7, **kw):
 
 
 def oracle_sys(desc, r, pair_in_accept=None):
-    pot = O.make_pot(desc.potential, desc.omega, desc.V0, desc.a)
+    pot = O.pot_from_desc(desc)
     pair = O.make_pair(desc.pair, desc.g, desc.sigma)
     pia = desc.pair_in_accept if pair_in_accept is None else pair_in_accept
     return O.System(r, desc.lam, desc.L, desc.beta, pot, pair, desc.r_a, bool(pia))
@@ -709,3 +709,62 @@ def test_split_estimators_match_in_kernel_fp64():
         assert np.array_equal(a["radial"][c][0], b["radial"][c][0]), c
     assert np.array_equal(a["next"], b["next"])
     assert a["blocks"][0].shape[1] == 2  # two blocks of 4 sweeps, each with 2 measured sweeps
+
+
+@pytest.mark.parametrize("precision", [A.FP64, A.FP32])
+@pytest.mark.parametrize("helical", [False, True])
+def test_optical_lattice_moves_match_oracle(precision, helical):
+    """normalized_intensity potentials (the 12-beam l25 superlattice of generate_V
+    and helical cubic beams) through the staging and COM hooks: bridge beads and
+    Delta V equal the oracle's."""
+    from pimc.potentials import BEAMS, OpticalLattice
+    rng = np.random.default_rng(81)
+    pot = OpticalLattice(BEAMS["cubic"] if helical else BEAMS["l25"], 1.0, -4.0, helical=helical)
+    d = chain_desc(pot, dim=2, M=24, N=4, L=8.0, beta=2.0, lam=0.5, ell=8)
+    r = rng.uniform(-3.0, 3.0, size=(4, 2, 24))
+    ctx = make_ctx([d], precision)
+    ctx.set_paths(0, r)
+    rr = ctx.get_paths(0)
+    osys = oracle_sys(d, rr)
+    tol = 1e-11 if precision == A.FP64 else 3e-5
+    for trial in range(20):
+        n, j0, m = int(rng.integers(4)), int(rng.integers(24)), int(rng.integers(2, 12))
+        nrm = rng.standard_normal((m - 1, 2))
+        r_gpu, w_gpu = ctx.levy_propose(0, n, j0, m, nrm)
+        r_or, w_or = osys.reshape_propose(n, j0, m, nrm)
+        assert np.max(np.abs(r_gpu - r_or)) <= tol * 8.0, trial
+        scale = abs(w_or.wV_updated) + abs(w_or.wV_initial) + 1e-300
+        assert abs((w_gpu[1] - w_gpu[0]) - (w_or.wV_updated - w_or.wV_initial)) <= 10 * tol * scale, trial
+    for trial in range(10):
+        n = int(rng.integers(4))
+        disp = rng.uniform(-0.5, 0.5, size=2)
+        r_gpu, w_gpu = ctx.com_propose(0, n, disp)
+        r_or, w_or = osys.com_propose(n, disp)
+        scale = abs(w_or.wV_updated) + abs(w_or.wV_initial) + 1e-300
+        assert abs((w_gpu[1] - w_gpu[0]) - (w_or.wV_updated - w_or.wV_initial)) <= 10 * tol * scale, trial
+
+
+def test_optical_lattice_sweeps_match_oracle_fp64():
+    """Tile kernel (no pair action) and whole-chain kernel (Gaussian pair) with
+    the optical potential: paths, counters and energy blocks equal the oracle's
+    sweep schedule."""
+    from pimc.potentials import BEAMS, OpticalLattice
+    rng = np.random.default_rng(83)
+    pot = OpticalLattice(BEAMS["l25"], 1.0, -4.0)
+    descs = [chain_desc(pot, dim=2, M=32, N=6, L=8.0, beta=2.0, lam=0.5, ell=8, com_step=0.5),
+             chain_desc(pot, dim=2, M=16, N=20, L=4.0, beta=2.0, lam=0.5, ell=8, com_step=0.5,
+                        pair=GaussianPair(1.0, 0.3), r_a=1.2)]
+    inits = [rng.uniform(-3.0, 3.0, size=(6, 2, 32)), rng.uniform(-4.0, 4.0, size=(20, 2, 16))]
+    seed, n_sweeps = 23, 4
+    ctx = make_ctx(descs, A.FP64, seed=seed, spl=n_sweeps)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    cnt = ctx.run(n_sweeps, measure=True)
+    ref = sweep_oracle(descs, inits, seed, n_sweeps, n_sweeps, measure=True)
+    assert cnt.levy_acc == sum(x[1] for x in ref) and cnt.com_acc == sum(x[2] for x in ref)
+    for c in range(2):
+        assert np.max(np.abs(ctx.get_paths(c) - ref[c][0])) < 1e-9, c
+        E, Ev, Ep = ctx.blocks(c)
+        Eo = np.mean([e[0] for e in ref[c][4]])  # the reference's Energy: no pair term
+        assert abs((E[0] - Ep[0]) - Eo) < 1e-9 * max(1.0, abs(Eo)), (c, E[0], Ep[0], Eo)
+    assert ctx.blocks(1)[2][0] > 0.0

@@ -18,7 +18,7 @@ import analytic as AN
 import oracle as O
 from pimc import _capi as A
 from pimc.batch import Batch, Workload
-from pimc.potentials import GaussianPair, HarmonicTrap, NoPotential, SquareLattice
+from pimc.potentials import GaussianPair, HarmonicTrap, NoPotential, OpticalLattice, SquareLattice
 from pimc.system import chain_desc
 
 pytestmark = pytest.mark.gpu
@@ -34,7 +34,7 @@ def ctx_for(descs, precision=A.FP64, seed=3, spl=10, derivatives=0):
 
 
 def oracle_action(d, r):
-    s = O.System(r, d.lam, d.L, d.beta, O.make_pot(d.potential, d.omega, d.V0, d.a),
+    s = O.System(r, d.lam, d.L, d.beta, O.pot_from_desc(d),
                  O.make_pair(d.pair, d.g, d.sigma), d.r_a, True)
     return -(s.total_lnV() + s.total_lnU()), s
 
@@ -46,6 +46,8 @@ CASES = {
                                          pair=GaussianPair(1.0, 0.5)),
     "lattice_pair_cells": lambda: chain_desc(SquareLattice(4.0, 1.0), dim=2, M=8, N=110, L=5.0, beta=4.0,
                                              lam=0.5, pair=GaussianPair(1.0, 0.3)),
+    "optical_helical": lambda: chain_desc(OpticalLattice([0.0, 1.2, 2.5, 4.0], 0.8, -3.0, helical=True), dim=2,
+                                          M=8, N=12, L=5.0, beta=2.0, lam=0.5),
 }
 
 

@@ -35,7 +35,7 @@ def table_desc(N, L, M, ra, beta=2.0, pot=None, ell=4, com_step=0.5, hs_a=0.0):
 
 
 def oracle_table_sys(d, r, tab, r0, r1, mu):
-    pot = O.make_pot(d.potential, d.omega, d.V0, d.a)
+    pot = O.pot_from_desc(d)
     pair = O.make_pair(O.PAIR_TABLE, table=tab, r0=r0, r1=r1, mu=mu)
     return O.System(r, d.lam, d.L, d.beta, pot, pair, d.r_a, bool(d.pair_in_accept), hs_a=d.hs_a)
 
@@ -120,7 +120,7 @@ def test_table_sweeps_match_oracle_fp64(N, L, ra):
     la = ca = 0
     for c, d in enumerate([dg, dt]):
         if c == 0:
-            pot = O.make_pot(d.potential, d.omega, d.V0, d.a)
+            pot = O.pot_from_desc(d)
             s = O.System(inits[c], d.lam, d.L, d.beta, pot, O.make_pair(d.pair, d.g, d.sigma), d.r_a, True)
         else:
             s = oracle_table_sys(d, inits[c], tab, r0, r1, mu)

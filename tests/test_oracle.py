@@ -399,3 +399,26 @@ def test_swap_boson_energy_closed_form():
     exact = AN.energy_boson_primitive(N, M, beta)
     assert abs(e - exact) < 3.5 * err, (e, err, exact)
     assert AN.energy_boltzmann_primitive(N, M, beta) - e > 20 * err
+
+
+def test_optical_lattice_known_answers_and_gradient():
+    """normalized_intensity of the reference's own test (test/testpotential.jl:
+    19-30): three beams at 0, 2pi/3, 4pi/3 with scale 1 give intensity 1 at the
+    origin, at (2/sqrt(3), 0) and at (0, 2/3); the gradient matches central
+    differences, for plain and helical beams."""
+    cyclic_3 = [2 * math.pi * k / 3 for k in range(3)]
+    pot = O.make_pot(O.POT_OPTICAL, V0=1.0, angles=cyclic_3, scale=1.0)
+    for r in ([0.0, 0.0], [2 / math.sqrt(3), 0.0], [0.0, 2 / 3]):
+        assert O.pot_V(pot, r) == pytest.approx(1.0, rel=1e-12)
+    # intensity is at most 1 and 0 where the beams cancel
+    rng = np.random.default_rng(2)
+    vals = [O.pot_V(pot, x) for x in rng.uniform(-2, 2, size=(200, 2))]
+    assert max(vals) <= 1.0 + 1e-12 and min(vals) >= 0.0
+    cubic = [2 * math.pi * k / 4 for k in range(4)]
+    for helical in (False, True):
+        p = O.make_pot(O.POT_OPTICAL, V0=-4.0, angles=cubic + [0.3], scale=0.7, helical=helical)
+        for x in rng.uniform(-3, 3, size=(10, 2)):
+            g = O.pot_dV(p, x)
+            h = 1e-6
+            fd = [(O.pot_V(p, x + h * e) - O.pot_V(p, x - h * e)) / (2 * h) for e in np.eye(2)]
+            assert np.allclose(g, fd, rtol=1e-6, atol=1e-7)
