@@ -770,17 +770,24 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
             c0[u] = cell_of<T, DIM>(q[u], P, C.nbins);
             lst[u] = (u > 0 && c0[u] == c0[0]) ? 0 : u;
         }
-        int cs[NQ][NS], pre[NQ][NS + 1], total = 0;
+        // per stencil slot s: prefix pre[s] of the candidate counts and the offset moff[s]
+        // with C.mem[moff[s] + idx] = member idx - pre[s] of cell s, so that a candidate
+        // index picks its member with one select per slot
+        int moff[NQ][NS], pre[NQ][NS + 1], total = 0;
 #pragma unroll
         for (int u = 0; u < NQ; ++u) {
             pre[u][0] = 0;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                cs[u][s] = stencil_cell(c0[u], s, C.nbins, DIM);
-                pre[u][s + 1] = (lst[u] == u) ? C.count(j, cs[u][s]) : 0;
+                const int cell = stencil_cell(c0[u], s, C.nbins, DIM);
+                moff[u][s] = (j * C.ncell + cell) * C.cap;
+                pre[u][s + 1] = (lst[u] == u) ? C.count(j, cell) : 0;
             }
 #pragma unroll
-            for (int s = 0; s < NS; ++s) pre[u][s + 1] += pre[u][s];
+            for (int s = 0; s < NS; ++s) {
+                moff[u][s] -= pre[u][s];
+                pre[u][s + 1] += pre[u][s];
+            }
             total = max(total, pre[u][NS]);
         }
         for (int i0 = g; i0 < total; i0 += G * R) {
@@ -789,17 +796,15 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
 #pragma unroll
             for (int u = 0; u < NQ; ++u) {
 #pragma unroll
+                for (int r = 0; r < R; ++r) k[u][r] = -1;
+                if (lst[u] != u) continue; // a query sharing query 0's list reads query 0's candidates
+#pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int idx = i0 + r * G;
-                    int cell = cs[u][0], base = 0;
+                    int off = moff[u][0];
 #pragma unroll
-                    for (int s = 1; s < NS; ++s) {
-                        if (idx >= pre[u][s]) {
-                            cell = cs[u][s];
-                            base = pre[u][s];
-                        }
-                    }
-                    k[u][r] = (idx < pre[u][NS]) ? (int)C.members(j, cell)[idx - base] : -1;
+                    for (int s = 1; s < NS; ++s) off = (idx >= pre[u][s]) ? moff[u][s] : off;
+                    k[u][r] = (idx < pre[u][NS]) ? (int)C.mem[off + idx] : -1;
                 }
             }
 #pragma unroll
