@@ -23,6 +23,28 @@ def test_system_initialisation():
         assert np.array_equal(p.r[0], p.r[-1])  # init_world closes the path on its first bead
 
 
+def test_system_with_the_references_optical_potential():
+    """testsystem.jl Initialization2d with its own potential: generate_V(cyclic_4,
+    scale 0.5, depth 8) of normalized_intensity, System(V; dim=2, M=100, N=5,
+    L=4.0, T=1.0); testpotential.jl's intensity 1 at the lattice maxima; and the
+    periodic-bounds check of testsystem.jl after run! (no bead outside [-L, L))."""
+    import math
+    from pimc import OpticalLattice
+    cyclic_4 = [2 * math.pi * k / 4 for k in range(4)]
+    V = OpticalLattice(cyclic_4, 0.5, 8.0)
+    assert V([0.0, 0.0]) == pytest.approx(8.0)
+    cyclic_3 = [2 * math.pi * k / 3 for k in range(3)]
+    I3 = OpticalLattice(cyclic_3, 1.0, 1.0)
+    for r in ([0.0, 0.0], [2 / math.sqrt(3), 0.0], [0.0, 2 / 3]):
+        assert I3(r) == pytest.approx(1.0)
+    s = System(V, dim=2, M=100, N=5, L=4.0, T=1.0, seed=7)
+    assert s.N == 5
+    updates = [(2, SingleCenterOfMass(s, 3.0)), (1, ReshapeLinear(s, 20))]
+    run(s, 2000, updates)
+    r = s.paths
+    assert np.all(r >= -4.0) and np.all(r < 4.0)
+
+
 def test_density_integrates_to_particle_number():
     """testmeasurements.jl: after run! with SingleCenterOfMass and
     ReshapeLinear, the Density histogram integrates to N (up to the beads in
