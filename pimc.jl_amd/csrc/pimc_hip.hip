@@ -1370,7 +1370,6 @@ int pimc_get_counters(pimc_ctx *c, int32_t chain, int64_t *out)
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipMemcpy(acc.data(), c->d_cnt + (size_t)first * kCnt, sizeof(long long) * acc.size(), hipMemcpyDeviceToHost));
         for (int i = 0; i < count; ++i) {
-            const int ch = first + i;
             out[4 * i + 0] = acc[kCnt * i];
             out[4 * i + 1] = acc[kCnt * i + 5];
             out[4 * i + 2] = acc[kCnt * i + 1];
