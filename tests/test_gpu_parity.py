@@ -353,7 +353,7 @@ def test_sweep_trajectory_pair_1d_fp64(N, L, ra):
     assert np.isfinite(E[0]) and np.isfinite(Ep[0])
 
 
-@pytest.mark.parametrize("case", ["trap", "lattice_box"])
+@pytest.mark.parametrize("case", ["trap", "lattice_box", "optical"])
 def test_sweep_trajectory_fp32(case):
     """FP32 tile kernel == oracle sweep schedule up to FP32 rounding.
 
@@ -369,6 +369,12 @@ def test_sweep_trajectory_fp32(case):
             descs.append(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=N, L=100.0, beta=beta, lam=0.5, ell=ell,
                                     com_step=1.0))
             inits.append(rand_paths(rng, N, M, 100.0))
+    elif case == "optical":
+        from pimc.potentials import BEAMS, OpticalLattice
+        pot = OpticalLattice(BEAMS["l25"], 1.0, -4.0)
+        for N, M, ell in [(4, 32, 8), (3, 64, 16), (3, 40, 6)]:
+            descs.append(chain_desc(pot, dim=2, M=M, N=N, L=6.0, beta=2.0, lam=0.5, ell=ell, com_step=0.5))
+            inits.append(rng.uniform(-3.0, 3.0, size=(N, 2, M)))
     else:
         for N, M, ell in [(4, 32, 8), (3, 64, 16), (3, 40, 6)]:
             descs.append(chain_desc(SquareLattice(1.0, 1.0), dim=2, M=M, N=N, L=1.5, beta=2.0, lam=0.5, ell=ell,
