@@ -983,6 +983,37 @@ __device__ bool hs_near_wave(const PathT &path, const Cells &C, int N, const Prm
     return __ballot(near) != 0ull;
 }
 
+// The same test by one thread alone (candidates of the 9 stencil cells, or all
+// particles, one after another).
+template <typename T, int DIM, bool CELLS, typename PathT>
+__device__ bool hs_near_thread(const PathT &path, const Cells &C, int N, const Prm<T> &P, int n, int j, const T *q)
+{
+    if constexpr (CELLS) {
+        constexpr int NS = (DIM == 1) ? 3 : 9;
+        const int c0 = cell_of<T, DIM>(q, P, C.nbins);
+        for (int s = 0; s < NS; ++s) {
+            const int cell = stencil_cell(c0, s, C.nbins, DIM);
+            const int cnt = C.count(j, cell);
+            const unsigned short *mem = C.members(j, cell);
+            for (int i = 0; i < cnt; ++i) {
+                const int k = mem[i];
+                if (k == n) continue;
+                T rk[DIM];
+                path.get(k, j, rk);
+                if (dist2<T, DIM>(rk, q, P) < P.hs2) return true;
+            }
+        }
+    } else {
+        for (int k = 0; k < N; ++k) {
+            if (k == n) continue;
+            T rk[DIM];
+            path.get(k, j, rk);
+            if (dist2<T, DIM>(rk, q, P) < P.hs2) return true;
+        }
+    }
+    return false;
+}
+
 // Pair energy of bead (i, j) per unit coupling g: its share of S_U / g
 // (each unordered pair counted from both ends with weight 1/2) and of
 // (r . grad S_U) / g.  Kept free of g so that dS/dg is defined at g = 0.
@@ -1260,15 +1291,64 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                 }
             }
             __syncthreads();
-            // (A2) bridges
+            // (A2) bridges with the first draws; with hard spheres s_flag[k] = m: no bead of
+            // segment k found too close yet
+            for (int k = tid; k < nseg; k += nt) {
+                const int m = min(ell, M - k * ell);
+                if (TAB && hs) s_flag[k] = m;
+                if (m >= 2) {
+                    const int us = off + k * ell; // window position of the start bead (< 2M)
+                    const int start = us >= M ? us - M : us;
+                    int jm = start + m;
+                    if (jm >= M) jm -= M;
+                    T cur[DIM], em[DIM];
+                    path.get(owner(p, us), start, cur);
+                    path.get(owner(p, us + m), jm, em);
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
+                    for (int i = 1; i < m; ++i) {
+                        int b = start + i;
+                        if (b >= M) b -= M;
+                        const T alpha = bridge_alpha<T>(m, i);
+                        const T sd = bridge_sd(alpha, P);
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) {
+                            cur[e] = bridge_step(cur[e], em[e], alpha, s_nrm[b * DIM + e], sd);
+                            buf[b * DIM + e] = teleport(cur[e], P);
+                        }
+                    }
+                }
+            }
             if (TAB && hs) {
-                // hardspherelevy! (src/updates/helper.jl:141-181): one wave per segment walks the
-                // bridge bead by bead, redrawing a bead (draw t >= 2: kHardSphere | t << 8) while it
-                // lies closer than a to another particle's bead at its slice; a segment that runs out
-                // of draws is a rejected try (s_flag = -1) and keeps its old beads
+                // hardspherelevy! (src/updates/helper.jl:141-181).  The first draws of every
+                // interior bead are tested in parallel, one thread per bead, keeping in s_flag the
+                // first bead of each segment that lies closer than a to another particle's bead
+                __syncthreads();
+                for (int j = tid; j < M; j += nt) {
+                    int rel = j - off;
+                    if (rel < 0) rel += M;
+                    const int k = rel / ell, t = rel - k * ell;
+                    const int m = min(ell, M - k * ell);
+                    if (m >= 2 && t >= 1) {
+                        T q[DIM];
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) q[e] = buf[j * DIM + e];
+                        if (hs_near_thread<T, DIM, CELLS>(path, cells, N, P, p, j, q)) atomicMin(&s_flag[k], t);
+                    }
+                }
+                __syncthreads();
+                // segments with such a bead: one wave walks the bridge on from it, bead by bead,
+                // redrawing a bead (draw t >= 2: kHardSphere | t << 8) while it lies closer than a;
+                // the beads before it keep their first draws.  A segment that runs out of draws is a
+                // rejected try (s_flag = -1) and keeps its old beads; the others get s_flag = 0
                 for (int k = wave; k < nseg; k += nw) {
                     const int m = min(ell, M - k * ell);
                     if (m < 2) continue;
+                    const int i0 = s_flag[k];
+                    if (i0 >= m) {
+                        if (lane == 0) s_flag[k] = 0;
+                        continue;
+                    }
                     const int start = off + k * ell >= M ? off + k * ell - M : off + k * ell;
                     int jm = start + m;
                     if (jm >= M) jm -= M;
@@ -1277,8 +1357,16 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                     path.get(p, jm, em);
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
+                    for (int i = 1; i < i0; ++i) { // the bridge up to the failing bead, as built above
+                        int b = start + i;
+                        if (b >= M) b -= M;
+                        const T alpha = bridge_alpha<T>(m, i);
+                        const T sd = bridge_sd(alpha, P);
+#pragma unroll
+                        for (int e = 0; e < DIM; ++e) cur[e] = bridge_step(cur[e], em[e], alpha, s_nrm[b * DIM + e], sd);
+                    }
                     bool failed = false;
-                    for (int i = 1; i < m && !failed; ++i) {
+                    for (int i = i0; i < m && !failed; ++i) {
                         int b = start + i;
                         if (b >= M) b -= M;
                         const T alpha = bridge_alpha<T>(m, i);
@@ -1323,31 +1411,6 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                         }
                     }
                     if (lane == 0) s_flag[k] = failed ? -1 : 0;
-                }
-            } else
-            for (int k = tid; k < nseg; k += nt) {
-                const int m = min(ell, M - k * ell);
-                if (m >= 2) {
-                    const int us = off + k * ell; // window position of the start bead (< 2M)
-                    const int start = us >= M ? us - M : us;
-                    int jm = start + m;
-                    if (jm >= M) jm -= M;
-                    T cur[DIM], em[DIM];
-                    path.get(owner(p, us), start, cur);
-                    path.get(owner(p, us + m), jm, em);
-#pragma unroll
-                    for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
-                    for (int i = 1; i < m; ++i) {
-                        int b = start + i;
-                        if (b >= M) b -= M;
-                        const T alpha = bridge_alpha<T>(m, i);
-                        const T sd = bridge_sd(alpha, P);
-#pragma unroll
-                        for (int e = 0; e < DIM; ++e) {
-                            cur[e] = bridge_step(cur[e], em[e], alpha, s_nrm[b * DIM + e], sd);
-                            buf[b * DIM + e] = teleport(cur[e], P);
-                        }
-                    }
                 }
             }
             __syncthreads();
@@ -1452,14 +1515,14 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
                     if (tid == 0) s_sw[1] = 0;
                     __syncthreads();
                     bool nr = false;
-                    for (int j = wave; j < M; j += nw) {
+                    for (int j = tid; j < M; j += nt) { // one thread per moved bead
                         T q[DIM];
                         path.get(p, j, q);
 #pragma unroll
                         for (int e = 0; e < DIM; ++e) q[e] = teleport(q[e] + d[e], P);
-                        nr = nr || hs_near_wave<T, DIM, CELLS>(path, cells, N, P, p, j, q, lane);
+                        nr = nr || hs_near_thread<T, DIM, CELLS>(path, cells, N, P, p, j, q);
                     }
-                    if (nr && lane == 0) s_sw[1] = 1;
+                    if (nr) s_sw[1] = 1;
                     __syncthreads();
                     placed = s_sw[1] == 0;
                     __syncthreads();
