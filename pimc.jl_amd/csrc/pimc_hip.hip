@@ -85,7 +85,7 @@ struct pimc_ctx {
     long long n_pos = 0, n_cell_cnt = 0, n_bins = 0;
     int n_part = 0;
     int maxl = 8;
-    int threads_tile = 256, threads_whole = 256;
+    int threads_tile = 256;
     size_t lds_tile = 0, lds_whole = 0;
     uint32_t sweep = 0;
     int nblk = 0, nblk_cap = 0;
@@ -196,12 +196,13 @@ void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &all, int l
         if (LDSR) lds = std::max(lds, whole_lds_bytes(c->ch[id], sizeof(T)) + whole_path_bytes(c->ch[id], sizeof(T)));
     }
     // lanes per link: as many as the workgroup holds for one pass over the links (2..16)
+    const int nt = whole_threads<TAB>();
     int G = 1;
-    while (G < 16 && 2 * G * maxM <= c->threads_whole) G <<= 1;
+    while (G < 16 && 2 * G * maxM <= nt) G <<= 1;
     a.link_group = std::max(2, G);
     auto k = k_sweep_whole<T, DIM, CELLS, LDSR, TAB>;
     if (lds > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k, dim3((unsigned)(hi - lo)), dim3(c->threads_whole), lds, c->stream, a);
+    hipLaunchKernelGGL(k, dim3((unsigned)(hi - lo)), dim3(nt), lds, c->stream, a);
     HIPCHK(hipGetLastError());
 }
 
@@ -868,7 +869,6 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         // whole-chain kernel
         int maxMw = 0;
         for (int id : c->whole_ids) maxMw = std::max(maxMw, c->ch[id].M);
-        c->threads_whole = 1024;
         if (maxMw > 0) {
             c->lds_whole = 0;
             for (int id : c->whole_ids) c->lds_whole = std::max(c->lds_whole, whole_lds_bytes(c->ch[id], ts));

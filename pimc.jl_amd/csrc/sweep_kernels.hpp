@@ -1191,8 +1191,11 @@ __host__ __device__ inline size_t whole_path_bytes(const ChainDev &h, int tsize)
 // Centre of mass, for each particle in order: the link groups form the moved
 // beads on the fly and add Delta V and the pair difference, one workgroup
 // reduction gives a uniform decision, and the commit: two barriers.
+// Workgroup size: 1024 threads, or 512 with the tabulated propagator, whose
+// FP64 table lookups need more than the 128 VGPRs of a 1024-thread launch.
+template <bool TAB> constexpr int whole_threads() { return TAB ? 512 : 1024; }
 template <typename T, int DIM, bool CELLS, bool LDSR, bool TAB>
-__global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
+__global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs a)
 {
     static_assert(!(CELLS && LDSR), "LDS-resident paths are for all-pairs chains");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1217,7 +1220,9 @@ __global__ void __launch_bounds__(1024) k_sweep_whole(SweepArgs a)
     const uint32_t sweep0 = a.rs->sweep + (uint32_t)a.sweep_off;
     const int G = a.link_group, ngroups = nt / G, g = tid & (G - 1), grp = tid / G;
     const int lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
-    constexpr int R = 2; // candidates per lane and query loaded together
+    // candidates per lane and query loaded together (3 suits the cell-list scans of the
+    // analytic pairs; more spills registers)
+    constexpr int R = (CELLS && !TAB) ? 3 : 2;
 
     size_t o = 0;
     T *s_new0 = reinterpret_cast<T *>(smem + o);
