@@ -54,6 +54,7 @@ template <typename T> struct Prm {
     const double *tab;   // tabulated pair-product propagator (kPairTable), FP64 throughout
     double tab_r0, tab_h, mu, tau64;
     T hs2;               // hard-sphere radius^2
+    T cellw;             // cell side 2L / nbins (pair chains)
     const T *beam;       // optical lattice: (kx, ky, phase) per beam, revolutions
     T nbeamT;            // beam count
     int nbeam;
@@ -82,6 +83,7 @@ template <typename T> struct Prm {
         p.mu = c.mu;
         p.tau64 = c.tau;
         p.hs2 = (T)(c.hs_a * c.hs_a);
+        p.cellw = p.twoL / (T)c.nbins;
         p.beam = static_cast<const T *>(c.beam);
         p.nbeam = c.nbeam;
         p.nbeamT = (T)c.nbeam;
@@ -376,32 +378,48 @@ template <typename T> __device__ __forceinline__ T bridge_step(T prev, T em, T a
     return (alpha * prev + (T(1) - alpha) * em) + xi * sd;
 }
 
-// Cell index of a point (src/nearest_neighbours.jl:26-33), 0-based, x fastest.
+// Bin coordinates of a point (src/nearest_neighbours.jl:26-33), 0-based;
+// nbins is the chain's, whose cell side p.cellw = 2L / nbins is rounded in T
+// as the quotient would be here.
+template <typename T, int DIM>
+__device__ __forceinline__ void bin_xy(const T *r, const Prm<T> &p, int nbins, int &bx, int &by)
+{
+    bx = (int)fl((r[0] + p.L) / p.cellw);
+    bx = bx < 0 ? 0 : (bx >= nbins ? nbins - 1 : bx);
+    by = 0;
+    if (DIM > 1) {
+        by = (int)fl((r[1] + p.L) / p.cellw);
+        by = by < 0 ? 0 : (by >= nbins ? nbins - 1 : by);
+    }
+}
+
+// Cell index of a point, x fastest.
 template <typename T, int DIM> __device__ __forceinline__ int cell_of(const T *r, const Prm<T> &p, int nbins)
 {
-    const T w = p.twoL / T(nbins);
-    int bx = (int)fl((r[0] + p.L) / w);
-    bx = bx < 0 ? 0 : (bx >= nbins ? nbins - 1 : bx);
-    if (DIM == 1) return bx;
-    int by = (int)fl((r[1] + p.L) / w);
-    by = by < 0 ? 0 : (by >= nbins ? nbins - 1 : by);
+    int bx, by;
+    bin_xy<T, DIM>(r, p, nbins, bx, by);
     return bx + nbins * by;
 }
 
-// Stencil of bin_neighbors (src/nearest_neighbours.jl:55-65), in its order.
-__device__ __forceinline__ int stencil_cell(int c, int s, int nbins, int dim)
+// x + d wrapped into [0, n) for |d| <= 1 and 0 <= x < n: (x + d + n) % n
+// without an integer divide.
+__device__ __forceinline__ int wrap_bin(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
+
+// Stencil of bin_neighbors (src/nearest_neighbours.jl:55-65), in its order,
+// around the cell with bin coordinates (bx, by).
+__device__ __forceinline__ int stencil_xy(int bx, int by, int s, int nbins, int dim)
 {
-    const int bx = c % nbins;
     if (dim == 1) {
         const int dx = (s == 0) ? 0 : (s == 1 ? -1 : 1);
-        return (bx + dx + nbins) % nbins;
+        return wrap_bin(bx + dx, nbins);
     }
-    const int by = c / nbins;
     constexpr int sx[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1};
     constexpr int sy[9] = {0, 1, 1, 1, 0, 0, -1, -1, -1};
-    const int cx = (bx + sx[s] + nbins) % nbins;
-    const int cy = (by + sy[s] + nbins) % nbins;
-    return cx + nbins * cy;
+    return wrap_bin(bx + sx[s], nbins) + nbins * wrap_bin(by + sy[s], nbins);
+}
+__device__ __forceinline__ int stencil_cell(int c, int s, int nbins, int dim)
+{
+    return stencil_xy(c % nbins, c / nbins, s, nbins, dim);
 }
 
 // Accessor for one chain's positions.  Particle-major (SM = false): the

@@ -764,10 +764,11 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
 {
     constexpr int NS = (DIM == 1) ? 3 : 9;
     if constexpr (CELLS) {
-        int c0[NQ], lst[NQ]; // lst[u]: the list query u reads (its own, or query 0's)
+        int c0[NQ], lst[NQ], bx[NQ], by[NQ]; // lst[u]: the list query u reads (its own, or query 0's)
 #pragma unroll
         for (int u = 0; u < NQ; ++u) {
-            c0[u] = cell_of<T, DIM>(q[u], P, C.nbins);
+            bin_xy<T, DIM>(q[u], P, C.nbins, bx[u], by[u]);
+            c0[u] = bx[u] + C.nbins * by[u];
             lst[u] = (u > 0 && c0[u] == c0[0]) ? 0 : u;
         }
         // per stencil slot s: prefix pre[s] of the candidate counts and the offset moff[s]
@@ -779,7 +780,7 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
             pre[u][0] = 0;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                const int cell = stencil_cell(c0[u], s, C.nbins, DIM);
+                const int cell = stencil_xy(bx[u], by[u], s, C.nbins, DIM);
                 moff[u][s] = (j * C.ncell + cell) * C.cap;
                 pre[u][s + 1] = (lst[u] == u) ? C.count(j, cell) : 0;
             }
@@ -947,12 +948,13 @@ __device__ bool hs_near_wave(const PathT &path, const Cells &C, int N, const Prm
     if constexpr (CELLS) {
         // all stencil counts first (independent loads), then the flattened candidate list
         constexpr int NS = (DIM == 1) ? 3 : 9;
-        const int c0 = cell_of<T, DIM>(q, P, C.nbins);
+        int bx, by;
+        bin_xy<T, DIM>(q, P, C.nbins, bx, by);
         int cs[NS], pre[NS + 1];
         pre[0] = 0;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            cs[s] = stencil_cell(c0, s, C.nbins, DIM);
+            cs[s] = stencil_xy(bx, by, s, C.nbins, DIM);
             pre[s + 1] = C.count(j, cs[s]);
         }
 #pragma unroll
@@ -983,24 +985,50 @@ __device__ bool hs_near_wave(const PathT &path, const Cells &C, int N, const Prm
     return __ballot(near) != 0ull;
 }
 
-// The same test by one thread alone (candidates of the 9 stencil cells, or all
-// particles, one after another).
+// The same test by one thread alone.  With cell lists, the counts and the
+// first 8 member ids (one 16-byte load; lists are 16-byte aligned) of all
+// stencil cells are loaded first, then the beads of one cell's first 8
+// members together; members past the 8th, rare at the densities where cell
+// lists are used, one at a time.  Without, all particles one after another.
 template <typename T, int DIM, bool CELLS, typename PathT>
 __device__ bool hs_near_thread(const PathT &path, const Cells &C, int N, const Prm<T> &P, int n, int j, const T *q)
 {
     if constexpr (CELLS) {
         constexpr int NS = (DIM == 1) ? 3 : 9;
-        const int c0 = cell_of<T, DIM>(q, P, C.nbins);
+        int bx, by;
+        bin_xy<T, DIM>(q, P, C.nbins, bx, by);
+        int cnt[NS];
+        uint4 ids[NS];
+#pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const int cell = stencil_cell(c0, s, C.nbins, DIM);
-            const int cnt = C.count(j, cell);
-            const unsigned short *mem = C.members(j, cell);
-            for (int i = 0; i < cnt; ++i) {
-                const int k = mem[i];
-                if (k == n) continue;
-                T rk[DIM];
-                path.get(k, j, rk);
-                if (dist2<T, DIM>(rk, q, P) < P.hs2) return true;
+            const int cell = stencil_xy(bx, by, s, C.nbins, DIM);
+            cnt[s] = C.count(j, cell);
+            ids[s] = *reinterpret_cast<const uint4 *>(C.members(j, cell));
+        }
+        bool near = false;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const unsigned w[4] = {ids[s].x, ids[s].y, ids[s].z, ids[s].w};
+            int k[8];
+            T rk[8][DIM];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                k[t] = (t < cnt[s]) ? (int)((w[t >> 1] >> (16 * (t & 1))) & 0xffffu) : n;
+                path.get(k[t], j, rk[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) near = near || (k[t] != n && dist2<T, DIM>(rk[t], q, P) < P.hs2);
+            if (near) return true;
+            if (cnt[s] > 8) {
+                const int cell = stencil_xy(bx, by, s, C.nbins, DIM);
+                const unsigned short *mem = C.members(j, cell);
+                for (int i = 8; i < cnt[s]; ++i) {
+                    const int kk = mem[i];
+                    if (kk == n) continue;
+                    T r[DIM];
+                    path.get(kk, j, r);
+                    if (dist2<T, DIM>(r, q, P) < P.hs2) return true;
+                }
             }
         }
     } else {
