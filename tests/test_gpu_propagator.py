@@ -135,12 +135,14 @@ def test_table_sweeps_match_oracle_fp64(N, L, ra):
     assert np.isfinite(E[0]) and Ep[0] == 0.0
 
 
-@pytest.mark.parametrize("N,L,ra,hs_a", [(30, 3.0, 1.5, 0.3), (110, 5.0, 2.0, 0.25)])
+@pytest.mark.parametrize("N,L,ra,hs_a", [(30, 3.0, 1.5, 0.3), (110, 5.0, 2.0, 0.25), (240, 5.0, 2.0, 0.12)])
 def test_hard_sphere_sweeps_match_oracle_fp64(N, L, ra, hs_a):
-    """hardspherelevy! and move_polymer! in the whole-chain kernel (one wave per
-    segment, keyed redraws) against the oracle's sweep schedule: thermalising
+    """hardspherelevy! and move_polymer! in the whole-chain kernel (first draws
+    tested in parallel, a segment walked again by one wave from its first bead
+    too close, keyed redraws) against the oracle's sweep schedule: thermalising
     launches (10000 draws allowed), then measuring ones (1000), FP64: paths and
-    counters."""
+    counters.  The last case has about 9.6 beads per cell, so most stencils
+    hold cells with more than 8 members."""
     rng = np.random.default_rng(31)
     M, beta, mu, seed, spl = 16, 2.0, 0.4, 11, 2
     d = table_desc(N, L, M, ra, beta=beta, ell=8, hs_a=hs_a)
