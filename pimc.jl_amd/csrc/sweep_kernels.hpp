@@ -987,14 +987,15 @@ __device__ bool hs_near_wave(const PathT &path, const Cells &C, int N, const Prm
 
 // The same test by one thread alone.  With cell lists, the counts and the
 // first 8 member ids (one 16-byte load; lists are 16-byte aligned) of all
-// stencil cells are loaded first, then the beads of one cell's first 8
-// members together; members past the 8th, rare at the densities where cell
-// lists are used, one at a time.  Without, all particles one after another.
+// stencil cells are loaded first; then the beads of the first 4 members of 3
+// cells at a time, then those of members 4..7 of the cells that have them;
+// members past the 8th, rare at the densities where cell lists are used, one
+// at a time.  Without, all particles one after another.
 template <typename T, int DIM, bool CELLS, typename PathT>
 __device__ bool hs_near_thread(const PathT &path, const Cells &C, int N, const Prm<T> &P, int n, int j, const T *q)
 {
     if constexpr (CELLS) {
-        constexpr int NS = (DIM == 1) ? 3 : 9;
+        constexpr int NS = (DIM == 1) ? 3 : 9, SB = 3;
         int bx, by;
         bin_xy<T, DIM>(q, P, C.nbins, bx, by);
         int cnt[NS];
@@ -1005,20 +1006,45 @@ __device__ bool hs_near_thread(const PathT &path, const Cells &C, int N, const P
             cnt[s] = C.count(j, cell);
             ids[s] = *reinterpret_cast<const uint4 *>(C.members(j, cell));
         }
-        bool near = false;
+        // members 4h .. 4h + 3 of cells s0 .. s0 + SB - 1
+        auto batch = [&](int h, int s0) {
+            int k[SB][4];
+            T rk[SB][4][DIM];
+#pragma unroll
+            for (int s = 0; s < SB; ++s) {
+                const uint4 v = ids[s0 + s];
+                const unsigned w0 = h == 0 ? v.x : v.z, w1 = h == 0 ? v.y : v.w;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const unsigned w = t < 2 ? w0 : w1;
+                    k[s][t] = (4 * h + t < cnt[s0 + s]) ? (int)((w >> (16 * (t & 1))) & 0xffffu) : n;
+                    path.get(k[s][t], j, rk[s][t]);
+                }
+            }
+            bool near = false;
+#pragma unroll
+            for (int s = 0; s < SB; ++s)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) near = near || (k[s][t] != n && dist2<T, DIM>(rk[s][t], q, P) < P.hs2);
+            return near;
+        };
+        int more = 0; // cells with members past the 4th
+#pragma unroll
+        for (int s = 0; s < NS; ++s) more += cnt[s] > 4;
+#pragma unroll
+        for (int s0 = 0; s0 < NS; s0 += SB)
+            if (batch(0, s0)) return true;
+        if (more > 0) {
+#pragma unroll
+            for (int s0 = 0; s0 < NS; s0 += SB) {
+                bool any = false;
+#pragma unroll
+                for (int s = 0; s < SB; ++s) any = any || cnt[s0 + s] > 4;
+                if (any && batch(1, s0)) return true;
+            }
+        }
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const unsigned w[4] = {ids[s].x, ids[s].y, ids[s].z, ids[s].w};
-            int k[8];
-            T rk[8][DIM];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                k[t] = (t < cnt[s]) ? (int)((w[t >> 1] >> (16 * (t & 1))) & 0xffffu) : n;
-                path.get(k[t], j, rk[t]);
-            }
-#pragma unroll
-            for (int t = 0; t < 8; ++t) near = near || (k[t] != n && dist2<T, DIM>(rk[t], q, P) < P.hs2);
-            if (near) return true;
             if (cnt[s] > 8) {
                 const int cell = stencil_xy(bx, by, s, C.nbins, DIM);
                 const unsigned short *mem = C.members(j, cell);
