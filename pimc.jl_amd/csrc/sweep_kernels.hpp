@@ -463,17 +463,49 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
                     // the chunk is one skew block: slice k MAXL + jj sits at k (MAXL + 1) + jj
                     const T *x0 = xp + k * (MAXL + 1);
                     T vn = T(0), vo = T(0), mx = T(0);
+                    if constexpr (POT == kPotHarmonic) {
+                        // (w^2/2) sum_j (|r_j + d|^2 - |r_j|^2) = (w^2/2) (2 d.S + MAXL |d|^2),
+                        // S = sum_j r_j: two adds per bead instead of both potentials
+                        T sr[DIM];
 #pragma unroll
-                    for (int jj = 0; jj < MAXL; ++jj) {
-                        T ro[DIM], rn[DIM];
+                        for (int e = 0; e < DIM; ++e) sr[e] = T(0);
 #pragma unroll
-                        for (int e = 0; e < DIM; ++e) {
-                            ro[e] = x0[e * R.Mp + jj];
-                            rn[e] = ro[e] + d[e];
-                            mx = fmaxf(mx, fabsf(rn[e]));
+                        for (int jj = 0; jj < MAXL; ++jj)
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) {
+                                const T ro = x0[e * R.Mp + jj];
+                                sr[e] += ro;
+                                mx = fmaxf(mx, fabsf(ro + d[e]));
+                            }
+                        T ds = d[0] * sr[0], dd = d[0] * d[0];
+                        if (DIM > 1) {
+                            ds += d[DIM - 1] * sr[DIM - 1];
+                            dd += d[DIM - 1] * d[DIM - 1];
                         }
-                        vn += potVk<T, DIM, POT>(rn, P);
-                        vo += potVk<T, DIM, POT>(ro, P);
+                        vn = P.halfw2 * (T(2) * ds + T(MAXL) * dd);
+                        if (mx >= P.L) {
+                            vn = T(0);
+#pragma unroll
+                            for (int jj = 0; jj < MAXL; ++jj) {
+                                T ro[DIM];
+#pragma unroll
+                                for (int e = 0; e < DIM; ++e) ro[e] = x0[e * R.Mp + jj];
+                                vo += potVk<T, DIM, POT>(ro, P);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int jj = 0; jj < MAXL; ++jj) {
+                            T ro[DIM], rn[DIM];
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) {
+                                ro[e] = x0[e * R.Mp + jj];
+                                rn[e] = ro[e] + d[e];
+                                mx = fmaxf(mx, fabsf(rn[e]));
+                            }
+                            vn += potVk<T, DIM, POT>(rn, P);
+                            vo += potVk<T, DIM, POT>(ro, P);
+                        }
                     }
                     if (mx >= P.L) {
                         wrap |= 1u << r;
