@@ -295,10 +295,32 @@ __device__ __forceinline__ bool levy_full(float *xp, const LdsRow &R, int M, int
             mx = fmaxf(mx, fabsf(cur[d]));
             old[d] = xp[R.at(d, b)];
         }
-        vn += potVk<float, DIM, POT>(nb[i - 1], P);
-        vo += potVk<float, DIM, POT>(old, P);
+        if constexpr (POT == kPotHarmonic) {
+            // sum of |new|^2 - |old|^2, one FMA per coordinate; times w^2/2 below
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                vn = fmaf(nb[i - 1][d], nb[i - 1][d], vn);
+                vn = fmaf(-old[d], old[d], vn);
+            }
+        } else {
+            vn += potVk<float, DIM, POT>(nb[i - 1], P);
+            vo += potVk<float, DIM, POT>(old, P);
+        }
     }
+    if constexpr (POT == kPotHarmonic) vn *= P.halfw2;
     if (mx >= P.L) {
+        if constexpr (POT == kPotHarmonic) {
+            vo = 0.0f;
+#pragma unroll
+            for (int i = 1; i < MAXL; ++i) {
+                int b = start + i;
+                if (b >= M) b -= M;
+                float old[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) old[d] = xp[R.at(d, b)];
+                vo += potVk<float, DIM, POT>(old, P);
+            }
+        }
         vn = 0.0f;
 #pragma unroll
         for (int i = 1; i < MAXL; ++i) {
