@@ -642,16 +642,32 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
                     T l = T(0), v = T(0), g = T(0);
                     // link (r0 -> r1) of the kinetic sum, and the potential, virial and
                     // histogram terms of bead r0
+                    // (FP32 in the harmonic trap: v sums |r|^2, scaled to V and r.grad V after
+                    // the chunk)
+                    constexpr bool kSumR2 = kF32 && POT == kPotHarmonic;
                     auto bead = [&](const T *r0, const T *r1) {
-                        T dd = T(0);
+                        if constexpr (kF32) {
 #pragma unroll
-                        for (int e = 0; e < DIM; ++e) {
-                            const T dx = mindist(r0[e], r1[e], P);
-                            dd += dx * dx;
+                            for (int e = 0; e < DIM; ++e) {
+                                const T dx = mindist(r0[e], r1[e], P);
+                                l = fma(dx, dx, l);
+                            }
+                        } else {
+                            T dd = T(0);
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) {
+                                const T dx = mindist(r0[e], r1[e], P);
+                                dd += dx * dx;
+                            }
+                            l += dd;
                         }
-                        l += dd;
-                        v += potVk<T, DIM, POT>(r0, P);
-                        g += rgradVk<T, DIM, POT>(r0, P);
+                        if constexpr (kSumR2) {
+#pragma unroll
+                            for (int e = 0; e < DIM; ++e) v = fma(r0[e], r0[e], v);
+                        } else {
+                            v += potVk<T, DIM, POT>(r0, P);
+                            g += rgradVk<T, DIM, POT>(r0, P);
+                        }
                         if (rbins > 0) {
                             T rr = r0[0] * r0[0];
                             if (DIM > 1) rr += r0[1] * r0[1];
@@ -708,6 +724,10 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
 #pragma unroll
                             for (int e = 0; e < DIM; ++e) r0[e] = r1[e];
                         }
+                    }
+                    if constexpr (kSumR2) {
+                        g = P.w2 * v;
+                        v = P.halfw2 * v;
                     }
                     mL[r] += (double)l;
                     mV[r] += (double)v;
