@@ -1059,71 +1059,78 @@ __device__ bool hs_near_wave(const PathT &path, const Cells &C, int N, const Prm
     return __ballot(near) != 0ull;
 }
 
-// The same test by one thread alone.  With cell lists, the counts and the
-// first 8 member ids (one 16-byte load; lists are 16-byte aligned) of all
-// stencil cells are loaded first; then the beads of the first 4 members of 3
-// cells at a time, then those of members 4..7 of the cells that have them;
-// members past the 8th, rare at the densities where cell lists are used, one
-// at a time.  Without, all particles one after another.
+// Lane g's share of the same test when kHsLanes lanes test one bead together: the
+// stencil cells s = g, g + G, ... (with cell lists) or the particles
+// k = g, g + G, ... (without); the bead is too close when any lane's share
+// is.  With cell lists a lane loads the counts and first 8 member ids (one
+// 16-byte load; lists are 16-byte aligned) of its cells first, then the
+// beads of their first 4 members together, then those of members 4..7 of
+// the cells that have them; members past the 8th, rare at the densities
+// where cell lists are used, one at a time.
+constexpr int kHsLanes = 4;
 template <typename T, int DIM, bool CELLS, typename PathT>
-__device__ bool hs_near_thread(const PathT &path, const Cells &C, int N, const Prm<T> &P, int n, int j, const T *q)
+__device__ bool hs_near_part(const PathT &path, const Cells &C, int N, const Prm<T> &P, int n, int j, const T *q,
+                             int g)
 {
+    constexpr int G = kHsLanes;
     if constexpr (CELLS) {
-        constexpr int NS = (DIM == 1) ? 3 : 9, SB = 3;
+        constexpr int NS = (DIM == 1) ? 3 : 9, NL = (NS + G - 1) / G; // cells per lane
+        // stencil offsets of bin_neighbors' order, dx + 1 and dy + 1 in 2 bits per cell
+        constexpr unsigned kDx = (DIM == 1) ? (1u | (0u << 2) | (2u << 4))
+                                            : (1u | (0u << 2) | (1u << 4) | (2u << 6) | (0u << 8) | (2u << 10) |
+                                               (0u << 12) | (1u << 14) | (2u << 16));
+        constexpr unsigned kDy = (1u | (2u << 2) | (2u << 4) | (2u << 6) | (1u << 8) | (1u << 10) | (0u << 12) |
+                                  (0u << 14) | (0u << 16));
         int bx, by;
         bin_xy<T, DIM>(q, P, C.nbins, bx, by);
-        int cnt[NS];
-        uint4 ids[NS];
+        int cnt[NL], cell[NL];
+        uint4 ids[NL];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int cell = stencil_xy(bx, by, s, C.nbins, DIM);
-            cnt[s] = C.count(j, cell);
-            ids[s] = *reinterpret_cast<const uint4 *>(C.members(j, cell));
+        for (int i = 0; i < NL; ++i) {
+            const int s = g + i * G;
+            cnt[i] = 0;
+            cell[i] = 0;
+            ids[i] = make_uint4(0u, 0u, 0u, 0u);
+            if (s < NS) {
+                const int cx = wrap_bin(bx + (int)((kDx >> (2 * s)) & 3u) - 1, C.nbins);
+                const int cy = DIM == 1 ? 0 : wrap_bin(by + (int)((kDy >> (2 * s)) & 3u) - 1, C.nbins);
+                cell[i] = cx + C.nbins * cy;
+                cnt[i] = C.count(j, cell[i]);
+                ids[i] = *reinterpret_cast<const uint4 *>(C.members(j, cell[i]));
+            }
         }
-        // members 4h .. 4h + 3 of cells s0 .. s0 + SB - 1
-        auto batch = [&](int h, int s0) {
-            int k[SB][4];
-            T rk[SB][4][DIM];
+        // members 4h .. 4h + 3 of this lane's cells
+        auto batch = [&](int h) {
+            int k[NL][4];
+            T rk[NL][4][DIM];
 #pragma unroll
-            for (int s = 0; s < SB; ++s) {
-                const uint4 v = ids[s0 + s];
-                const unsigned w0 = h == 0 ? v.x : v.z, w1 = h == 0 ? v.y : v.w;
+            for (int i = 0; i < NL; ++i) {
+                const unsigned w0 = h == 0 ? ids[i].x : ids[i].z, w1 = h == 0 ? ids[i].y : ids[i].w;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const unsigned w = t < 2 ? w0 : w1;
-                    k[s][t] = (4 * h + t < cnt[s0 + s]) ? (int)((w >> (16 * (t & 1))) & 0xffffu) : n;
-                    path.get(k[s][t], j, rk[s][t]);
+                    k[i][t] = (4 * h + t < cnt[i]) ? (int)((w >> (16 * (t & 1))) & 0xffffu) : n;
+                    if (k[i][t] != n) path.get(k[i][t], j, rk[i][t]);
                 }
             }
             bool near = false;
 #pragma unroll
-            for (int s = 0; s < SB; ++s)
+            for (int i = 0; i < NL; ++i)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) near = near || (k[s][t] != n && dist2<T, DIM>(rk[s][t], q, P) < P.hs2);
+                for (int t = 0; t < 4; ++t) near = near || (k[i][t] != n && dist2<T, DIM>(rk[i][t], q, P) < P.hs2);
             return near;
         };
-        int more = 0; // cells with members past the 4th
+        if (batch(0)) return true;
+        bool more = false;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) more += cnt[s] > 4;
+        for (int i = 0; i < NL; ++i) more = more || cnt[i] > 4;
+        if (more && batch(1)) return true;
 #pragma unroll
-        for (int s0 = 0; s0 < NS; s0 += SB)
-            if (batch(0, s0)) return true;
-        if (more > 0) {
-#pragma unroll
-            for (int s0 = 0; s0 < NS; s0 += SB) {
-                bool any = false;
-#pragma unroll
-                for (int s = 0; s < SB; ++s) any = any || cnt[s0 + s] > 4;
-                if (any && batch(1, s0)) return true;
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            if (cnt[s] > 8) {
-                const int cell = stencil_xy(bx, by, s, C.nbins, DIM);
-                const unsigned short *mem = C.members(j, cell);
-                for (int i = 8; i < cnt[s]; ++i) {
-                    const int kk = mem[i];
+        for (int i = 0; i < NL; ++i) {
+            if (cnt[i] > 8) {
+                const unsigned short *mem = C.members(j, cell[i]);
+                for (int t = 8; t < cnt[i]; ++t) {
+                    const int kk = mem[t];
                     if (kk == n) continue;
                     T r[DIM];
                     path.get(kk, j, r);
@@ -1132,7 +1139,7 @@ __device__ bool hs_near_thread(const PathT &path, const Cells &C, int N, const P
             }
         }
     } else {
-        for (int k = 0; k < N; ++k) {
+        for (int k = g; k < N; k += G) {
             if (k == n) continue;
             T rk[DIM];
             path.get(k, j, rk);
@@ -1454,10 +1461,12 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
             }
             if (TAB && hs) {
                 // hardspherelevy! (src/updates/helper.jl:141-181).  The first draws of every
-                // interior bead are tested in parallel, one thread per bead, keeping in s_flag the
-                // first bead of each segment that lies closer than a to another particle's bead
+                // interior bead are tested in parallel, kHsLanes threads per bead, keeping in
+                // s_flag the first bead of each segment that lies closer than a to another
+                // particle's bead
                 __syncthreads();
-                for (int j = tid; j < M; j += nt) {
+                for (int w = tid; w < M * kHsLanes; w += nt) {
+                    const int j = w / kHsLanes, hg = w - j * kHsLanes;
                     int rel = j - off;
                     if (rel < 0) rel += M;
                     const int k = rel / ell, t = rel - k * ell;
@@ -1466,7 +1475,7 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                         T q[DIM];
 #pragma unroll
                         for (int e = 0; e < DIM; ++e) q[e] = buf[j * DIM + e];
-                        if (hs_near_thread<T, DIM, CELLS>(path, cells, N, P, p, j, q)) atomicMin(&s_flag[k], t);
+                        if (hs_near_part<T, DIM, CELLS>(path, cells, N, P, p, j, q, hg)) atomicMin(&s_flag[k], t);
                     }
                 }
                 __syncthreads();
@@ -1648,12 +1657,13 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                     if (tid == 0) s_sw[1] = 0;
                     __syncthreads();
                     bool nr = false;
-                    for (int j = tid; j < M; j += nt) { // one thread per moved bead
+                    for (int w = tid; w < M * kHsLanes; w += nt) { // kHsLanes threads per moved bead
+                        const int j = w / kHsLanes;
                         T q[DIM];
                         path.get(p, j, q);
 #pragma unroll
                         for (int e = 0; e < DIM; ++e) q[e] = teleport(q[e] + d[e], P);
-                        nr = nr || hs_near_thread<T, DIM, CELLS>(path, cells, N, P, p, j, q);
+                        nr = nr || hs_near_part<T, DIM, CELLS>(path, cells, N, P, p, j, q, w - j * kHsLanes);
                     }
                     if (nr) s_sw[1] = 1;
                     __syncthreads();
