@@ -985,6 +985,23 @@ __device__ __forceinline__ T link_delta(const PathT &path, const Cells &C, int N
                                         int j, int jn, const T *n0, const T *n1, const T *o0, const T *o1, int g,
                                         int G)
 {
+    if (CELLS && TAB == 1 && G >= 4) {
+        // (tabulated propagator, whose 512-thread kernel has the registers for it) half of
+        // the group scans for the new beads, half for the old ones, each with twice the
+        // candidates per lane; the group's sum of the returned shares is the difference
+        const int h = G >> 1, u = g >= h ? 1 : 0;
+        const T *a0 = u ? o0 : n0, *a1 = u ? o1 : n1;
+        T q[1][DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) q[0][e] = a0[e];
+        T acc = T(0);
+        scan_pairs<T, DIM, CELLS, 1, 2 * R>(path, C, N, nbins, P, n, j, jn, q, g - u * h, h,
+                                            [&](int, const T *rk, const T *rkn, T r2, T mult) {
+                                                const T s2 = dist2<T, DIM>(rkn, a1, P);
+                                                acc += mult * pair_lnU<T, DIM, TAB>(rk, a0, rkn, a1, r2, s2, P);
+                                            });
+        return u ? -acc : acc;
+    }
     T q[2][DIM];
 #pragma unroll
     for (int e = 0; e < DIM; ++e) {
