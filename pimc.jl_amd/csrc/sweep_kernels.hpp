@@ -1496,14 +1496,17 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                     }
                 }
                 __syncthreads();
-                // segments with such a bead: one wave walks the bridge on from it, bead by bead,
-                // redrawing a bead (draw t >= 2: kHardSphere | t << 8) while it lies closer than a;
-                // the beads before it keep their first draws.  A segment that runs out of draws is a
-                // rejected try (s_flag = -1) and keeps its old beads; the others get s_flag = 0
+                // segments with such a bead: one wave walks the bridge on from it.  The failing bead
+                // is redrawn (draw t >= 2: kHardSphere | t << 8; its first draw is known to fail)
+                // until it lies no closer than a; then the rest of the segment, built from its first
+                // draws, is tested together (kHsLanes lanes per bead), and the walk goes on from the
+                // first bead that fails, as a bead-by-bead walk would.  A segment that runs out of
+                // draws is a rejected try (s_flag = -1) and keeps its old beads; the others get
+                // s_flag = 0
                 for (int k = wave; k < nseg; k += nw) {
                     const int m = min(ell, M - k * ell);
                     if (m < 2) continue;
-                    const int i0 = s_flag[k];
+                    int i0 = s_flag[k];
                     if (i0 >= m) {
                         if (lane == 0) s_flag[k] = 0;
                         continue;
@@ -1516,48 +1519,79 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                     path.get(p, jm, em);
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
-                    for (int i = 1; i < i0; ++i) { // the bridge up to the failing bead, as built above
-                        int b = start + i;
-                        if (b >= M) b -= M;
+                    // cur: bead i - 1 of the segment; advance(i): bead i from its first draw
+                    auto advance = [&](T *x, int i) {
+                        int bb = start + i;
+                        if (bb >= M) bb -= M;
                         const T alpha = bridge_alpha<T>(m, i);
                         const T sd = bridge_sd(alpha, P);
 #pragma unroll
-                        for (int e = 0; e < DIM; ++e) cur[e] = bridge_step(cur[e], em[e], alpha, s_nrm[b * DIM + e], sd);
-                    }
+                        for (int e = 0; e < DIM; ++e) x[e] = bridge_step(x[e], em[e], alpha, s_nrm[bb * DIM + e], sd);
+                    };
+                    for (int i = 1; i < i0; ++i) advance(cur, i); // the bridge up to the failing bead, as built above
                     bool failed = false;
-                    for (int i = i0; i < m && !failed; ++i) {
-                        int b = start + i;
+                    while (i0 < m) {
+                        // redraws of bead i0
+                        int b = start + i0;
                         if (b >= M) b -= M;
-                        const T alpha = bridge_alpha<T>(m, i);
-                        const T sd = bridge_sd(alpha, P);
-                        T nr[2] = {T(0), T(0)};
+                        {
+                            const T alpha = bridge_alpha<T>(m, i0);
+                            const T sd = bridge_sd(alpha, P);
+                            bool placed = false;
+                            for (int t = 2; t <= hs_ctr; ++t) {
+                                const U4 xr = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c,
+                                                   (uint32_t)kHardSphere | ((uint32_t)t << 8));
+                                T nr[2];
+                                box_muller(xr.x, xr.y, nr[0], nr[1]);
+                                T nx[DIM], tb[DIM];
 #pragma unroll
-                        for (int e = 0; e < DIM; ++e) nr[e] = s_nrm[b * DIM + e];
-                        for (int t = 1;; ++t) {
-                            if (t > hs_ctr) {
+                                for (int e = 0; e < DIM; ++e) {
+                                    nx[e] = bridge_step(cur[e], em[e], alpha, nr[e], sd);
+                                    tb[e] = teleport(nx[e], P);
+                                }
+                                if (!hs_near_wave<T, DIM, CELLS>(path, cells, N, P, p, b, tb, lane)) {
+#pragma unroll
+                                    for (int e = 0; e < DIM; ++e) {
+                                        cur[e] = nx[e];
+                                        if (lane == 0) buf[b * DIM + e] = tb[e];
+                                    }
+                                    placed = true;
+                                    break;
+                                }
+                            }
+                            if (!placed) {
                                 failed = true;
                                 break;
                             }
-                            if (t > 1) {
-                                const U4 xr = draw(a.seed, (uint32_t)(p * M + b), sw, (uint32_t)c,
-                                                   (uint32_t)kHardSphere | ((uint32_t)t << 8));
-                                box_muller(xr.x, xr.y, nr[0], nr[1]);
-                            }
-                            T nx[DIM], tb[DIM];
+                        }
+                        // beads i0 + 1 .. from their first draws, 64 / kHsLanes at a time
+                        int next = m; // the first of them that fails
+                        for (int c0 = i0 + 1; c0 < m && next == m; c0 += 64 / kHsLanes) {
+                            const int mine = c0 + lane / kHsLanes;
+                            T x[DIM], xm[DIM];
 #pragma unroll
-                            for (int e = 0; e < DIM; ++e) {
-                                nx[e] = bridge_step(cur[e], em[e], alpha, nr[e], sd);
-                                tb[e] = teleport(nx[e], P);
-                            }
-                            if (!hs_near_wave<T, DIM, CELLS>(path, cells, N, P, p, b, tb, lane)) {
+                            for (int e = 0; e < DIM; ++e) x[e] = xm[e] = cur[e];
+                            for (int i = i0 + 1; i < min(m, c0 + 64 / kHsLanes); ++i) {
+                                advance(x, i);
+                                if (i == mine) {
 #pragma unroll
-                                for (int e = 0; e < DIM; ++e) {
-                                    cur[e] = nx[e];
-                                    if (lane == 0) buf[b * DIM + e] = tb[e];
+                                    for (int e = 0; e < DIM; ++e) xm[e] = teleport(x[e], P);
                                 }
-                                break;
+                            }
+                            int bm = start + mine;
+                            if (bm >= M) bm -= M;
+                            const bool near = mine < m && hs_near_part<T, DIM, CELLS>(path, cells, N, P, p, bm, xm, lane % kHsLanes);
+                            const unsigned long long bal = __ballot(near);
+                            if (bal != 0ull) next = c0 + (__ffsll((long long)bal) - 1) / kHsLanes;
+                            // the beads before the first failure keep their first draws
+                            if (mine < next && mine < m && lane % kHsLanes == 0) {
+#pragma unroll
+                                for (int e = 0; e < DIM; ++e) buf[bm * DIM + e] = xm[e];
                             }
                         }
+                        if (next >= m) break;
+                        for (int i = i0 + 1; i < next; ++i) advance(cur, i);
+                        i0 = next;
                     }
                     if (failed) { // the old beads: the link pass then sees no change
                         for (int i = 1 + lane; i < m; i += 64) {
