@@ -28,9 +28,9 @@ def table_for(L, g0, tau, n):
     return O.prop_int_terms(n, 1e-20, r1, g0, tau), 1e-20, r1
 
 
-def table_desc(N, L, M, ra, beta=2.0, pot=None, ell=4, com_step=0.5, hs_a=0.0):
+def table_desc(N, L, M, ra, beta=2.0, pot=None, ell=4, com_step=0.5, hs_a=0.0, dim=2):
     pi = PropInt(np.zeros((2, 2)), 0.0, 1.0, 4.0, beta / M)  # kind marker; the table is set separately
-    return chain_desc(pot if pot is not None else HarmonicTrap(0.3), dim=2, M=M, N=N, L=L, beta=beta, lam=0.5,
+    return chain_desc(pot if pot is not None else HarmonicTrap(0.3), dim=dim, M=M, N=N, L=L, beta=beta, lam=0.5,
                       pair=pi, ell=ell, r_a=ra, com_step=com_step, hs_a=hs_a)
 
 
@@ -143,16 +143,27 @@ def test_hard_sphere_sweeps_match_oracle_fp64(N, L, ra, hs_a):
     launches (10000 draws allowed), then measuring ones (1000), FP64: paths and
     counters.  The last case has about 9.6 beads per cell, so most stencils
     hold cells with more than 8 members."""
+    hard_sphere_case(N, L, ra, hs_a, 2)
+
+
+@pytest.mark.parametrize("N,L,ra,hs_a", [(40, 10.0, 2.0, 0.15), (120, 20.0, 2.0, 0.1)])
+def test_hard_sphere_sweeps_1d_match_oracle_fp64(N, L, ra, hs_a):
+    """The same in 1D (3-cell stencils): all-pairs for N = 40, cell lists with
+    about 6 beads per cell for N = 120."""
+    hard_sphere_case(N, L, ra, hs_a, 1)
+
+
+def hard_sphere_case(N, L, ra, hs_a, dim):
     rng = np.random.default_rng(31)
     M, beta, mu, seed, spl = 16, 2.0, 0.4, 11, 2
-    d = table_desc(N, L, M, ra, beta=beta, ell=8, hs_a=hs_a)
+    d = table_desc(N, L, M, ra, beta=beta, ell=8, hs_a=hs_a, dim=dim)
     tab, r0, r1 = table_for(L, 4.0, beta / M, 64)
-    r = rng.uniform(-L, L, size=(N, 2, M))
+    r = rng.uniform(-L, L, size=(N, dim, M))
     ctx = ctx_for([d], A.FP64, seed=seed, spl=spl)
     ctx.set_paths(0, r)
     ctx.set_pair_table(0, tab, r0, r1, mu)
     with pytest.raises(A.PimcError):
-        ctx.reshape_linear(0, 0, 0, 3, np.zeros((2, 2)), 0.5)  # single moves refuse hard-sphere chains
+        ctx.reshape_linear(0, 0, 0, 3, np.zeros((2, dim)), 0.5)  # single moves refuse hard-sphere chains
     c1 = ctx.run(2 * spl, measure=False)
     c2 = ctx.run(spl, measure=True)
     s = oracle_table_sys(d, r, tab, r0, r1, mu)
