@@ -558,14 +558,14 @@ __device__ T pair_link_all(const PathT &P, int N, int nbins, const Prm<T> &p, in
     return acc;
 }
 
-// Move particle n's bead at slice j from its stored cell to the cell of r.
-template <typename T, int DIM>
-__device__ __forceinline__ bool cells_move(const Cells &C, const Prm<T> &p, int n, int j, const T *r)
+// File particle n's bead at slice j, listed in cell oc (-1: in no list, after
+// an overflow), under cell nc != oc instead.  One thread per slice may move
+// beads at once: the lists of a slice are touched by its thread only.
+__device__ __forceinline__ bool cells_refile(const Cells &C, int n, int j, int oc, int nc)
 {
-    const int nc = cell_of<T, DIM>(r, p, C.nbins);
     int &b = C.bin[(long long)n * C.M + j];
-    const int oc = b;
-    if (nc == oc) return true;
+    int &ncnt = C.count(j, nc);
+    const int nc0 = ncnt; // loaded with the old cell's list: nc != oc
     if (oc >= 0) {
         // find n in the old cell's list, read in 16-byte chunks (cap is a multiple
         // of 8 and every list starts 16-byte aligned) issued before any is tested
@@ -594,15 +594,35 @@ __device__ __forceinline__ bool cells_move(const Cells &C, const Prm<T> &p, int 
             ocnt = cnt - 1;
         }
     }
-    int &ncnt = C.count(j, nc);
-    if (ncnt >= C.cap) {
+    if (nc0 >= C.cap) {
         b = -1; // overflow: reported through the status word
         return false;
     }
-    C.members(j, nc)[ncnt] = (unsigned short)n;
-    ncnt += 1;
+    C.members(j, nc)[nc0] = (unsigned short)n;
+    ncnt = nc0 + 1;
     b = nc;
     return true;
+}
+
+// Move particle n's bead at slice j from its stored cell to the cell of r.
+template <typename T, int DIM>
+__device__ __forceinline__ bool cells_move(const Cells &C, const Prm<T> &p, int n, int j, const T *r)
+{
+    const int nc = cell_of<T, DIM>(r, p, C.nbins);
+    const int oc = C.bin[(long long)n * C.M + j];
+    if (nc == oc) return true;
+    return cells_refile(C, n, j, oc, nc);
+}
+
+// The same move when the bead's old position rold is at hand: its cell is the
+// one the bead was filed under, so the stored cell need not be read first.
+template <typename T, int DIM>
+__device__ __forceinline__ bool cells_move(const Cells &C, const Prm<T> &p, int n, int j, const T *r, const T *rold)
+{
+    const int nc = cell_of<T, DIM>(r, p, C.nbins);
+    const int oc = cell_of<T, DIM>(rold, p, C.nbins);
+    if (nc == oc) return true;
+    return cells_refile(C, n, j, oc, nc);
 }
 
 } // namespace pimc

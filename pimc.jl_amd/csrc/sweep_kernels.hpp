@@ -1290,13 +1290,13 @@ __device__ inline void fold_meas(const ChainMeas &m, const ChainDev &ch, double 
     q[qWhole] = 1.0;
 }
 
-// LDS of k_sweep_whole: two buffers of new beads and one of bridge normals
-// (M x DIM T each), Delta V per bead and Delta lnU per link (M T each), accept
+// LDS of k_sweep_whole: two buffers of new beads, one of bridge normals and
+// one of a staging move's old beads (M x DIM T each), Delta V per bead and Delta lnU per link (M T each), accept
 // flags (M ints), two sets of 16 wave partials (double), 16 doubles for block
 // sums, kNQ doubles of estimator sums.
 __host__ __device__ inline size_t whole_lds_bytes(int M, int dim, int tsize, int nperm = 0)
 {
-    size_t o = 3 * align16((size_t)M * dim * tsize);
+    size_t o = 4 * align16((size_t)M * dim * tsize);
     o += 2 * align16((size_t)M * tsize);
     o += align16((size_t)M * 4);
     o += align16(32 * sizeof(double));
@@ -1382,6 +1382,8 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
     T *s_new1 = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)M * DIM * sizeof(T));
     T *s_nrm = reinterpret_cast<T *>(smem + o);
+    o += align16((size_t)M * DIM * sizeof(T));
+    T *s_old = reinterpret_cast<T *>(smem + o); // old beads of the staging move, for the cell lists
     o += align16((size_t)M * DIM * sizeof(T));
     T *s_dvb = reinterpret_cast<T *>(smem + o);
     o += align16((size_t)M * sizeof(T));
@@ -1627,7 +1629,13 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                             n0[e] = (t == 0) ? o0[e] : buf[j * DIM + e];
                             n1[e] = (t + 1 == m) ? o1[e] : buf[jn * DIM + e];
                         }
-                        if (g == 0) s_dvb[j] = (t >= 1) ? potV<T, DIM>(n0, P) - potV<T, DIM>(o0, P) : T(0);
+                        if (g == 0) {
+                            s_dvb[j] = (t >= 1) ? potV<T, DIM>(n0, P) - potV<T, DIM>(o0, P) : T(0);
+                            if (CELLS) {
+#pragma unroll
+                                for (int e = 0; e < DIM; ++e) s_old[j * DIM + e] = o0[e];
+                            }
+                        }
                         if (use_pair)
                             part = link_delta<T, DIM, CELLS, R, TAB ? 1 : 0>(path, cells, N, ch.nbins, P, p, j, jn, n0, n1, o0, o1, g,
                                                                 G);
@@ -1679,7 +1687,7 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) rn[e] = buf[j * DIM + e];
                     path.put(owner(p, j >= off ? j : j + M), j, rn);
-                    if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn);
+                    if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn, s_old + j * DIM);
                 }
             }
         }
@@ -1755,12 +1763,12 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
             cacc += (acc && tid == 0) ? 1 : 0;
             if (acc) {
                 for (int j = tid; j < M; j += nt) {
-                    T rn[DIM];
-                    path.get(p, j, rn);
+                    T ro[DIM], rn[DIM];
+                    path.get(p, j, ro);
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) rn[e] = teleport(rn[e] + d[e], P);
+                    for (int e = 0; e < DIM; ++e) rn[e] = teleport(ro[e] + d[e], P);
                     path.put(p, j, rn);
-                    if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn);
+                    if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn, ro);
                 }
             }
             __syncthreads();
