@@ -353,15 +353,17 @@ def test_sweep_trajectory_pair_1d_fp64(N, L, ra):
     assert np.isfinite(E[0]) and np.isfinite(Ep[0])
 
 
-@pytest.mark.parametrize("case", ["trap", "lattice_box", "optical"])
+@pytest.mark.parametrize("case", ["trap", "trap_box", "lattice_box", "optical"])
 def test_sweep_trajectory_fp32(case):
     """FP32 tile kernel == oracle sweep schedule up to FP32 rounding.
 
     Covers both FP32 code paths of the tile kernel: segments that are one
     skew block of the LDS rows (ell a power of two dividing M: compile-time
     bridge constants, wrap only for beads outside the box) and the general
-    segments (ell = 6, M = 40), in a wide trap and in a periodic lattice box
-    of half-width 1.5 where beads leave the box and are wrapped."""
+    segments (ell = 6, M = 40), in a wide trap, in a trap whose box of
+    half-width 1.5 the beads leave (the harmonic sums of staging and COM then
+    take the per-bead form with wrapped beads) and in a periodic lattice box
+    of half-width 1.5."""
     rng = np.random.default_rng(23)
     descs, inits = [], []
     if case == "trap":
@@ -369,6 +371,11 @@ def test_sweep_trajectory_fp32(case):
             descs.append(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=N, L=100.0, beta=beta, lam=0.5, ell=ell,
                                     com_step=1.0))
             inits.append(rand_paths(rng, N, M, 100.0))
+    elif case == "trap_box":
+        for N, M, ell in [(4, 32, 8), (3, 64, 16), (3, 40, 6)]:
+            descs.append(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=N, L=1.5, beta=2.0, lam=0.5, ell=ell,
+                                    com_step=1.0))
+            inits.append(rng.uniform(-1.5, 1.5, size=(N, 2, M)))
     elif case == "optical":
         from pimc.potentials import BEAMS, OpticalLattice
         pot = OpticalLattice(BEAMS["l25"], 1.0, -4.0)
@@ -398,7 +405,7 @@ def test_sweep_trajectory_fp32(case):
         Eo = np.mean([e[0] for e in ref[c][4]])
         assert abs(E[0] - Eo) <= 1e-4 * max(1.0, abs(Eo)), (c, E[0], Eo)
     assert wrapped <= 2
-    if case == "lattice_box":
+    if case in ("lattice_box", "trap_box"):
         moved_out = sum(int(np.sum(np.abs(r0 - x[0]) > 1.5)) for r0, x in zip(inits, ref))
         assert moved_out > 0  # the wrap branch ran
 
