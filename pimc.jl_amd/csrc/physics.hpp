@@ -470,94 +470,6 @@ struct Cells {
     }
 };
 
-// Pair link sum of particle n over link j -> j+1 with its beads there
-// replaced by pj, pj1: the neighbour set is every particle k != n of the 9
-// stencil cells around pj at slice j whose periodic distance to pj is <= h
-// (find_nns, src/nearest_neighbours.jl:106-154), each contributing
-// lnU (pair_lnU) of the pair over the link (src/updates/helper.jl:306-366).
-// Returns the sum of lnU (a log weight, <= 0 for a repulsion).
-template <typename T, int DIM, typename PathT>
-__device__ T pair_link(const PathT &P, const Cells &C, const Prm<T> &p, int n, int j, const T *pj, const T *pj1)
-{
-    const int M = P.M;
-    const int jn = (j + 1 == M) ? 0 : j + 1;
-    const int c0 = cell_of<T, DIM>(pj, p, C.nbins);
-    const int ns = (DIM == 1) ? 3 : 9;
-    T acc = T(0);
-    for (int s = 0; s < ns; ++s) {
-        const int c = stencil_cell(c0, s, C.nbins, DIM);
-        const int cnt = C.count(j, c);
-        const unsigned short *mem = C.members(j, c);
-        for (int q = 0; q < cnt; ++q) {
-            const int k = mem[q];
-            if (k == n) continue;
-            T rk[DIM];
-            P.get(k, j, rk);
-            T d0 = mindist(rk[0], pj[0], p);
-            T r2 = d0 * d0;
-            if (DIM > 1) {
-                T d1 = mindist(rk[1], pj[1], p);
-                r2 += d1 * d1;
-            }
-            if (r2 > p.h2) continue;
-            T rkn[DIM];
-            P.get(k, jn, rkn);
-            T e0 = mindist(rkn[0], pj1[0], p);
-            T s2 = e0 * e0;
-            if (DIM > 1) {
-                T e1 = mindist(rkn[1], pj1[1], p);
-                s2 += e1 * e1;
-            }
-            acc += pair_lnU<T, DIM>(rk, pj, rkn, pj1, r2, s2, p);
-        }
-    }
-    return acc;
-}
-
-// The same link sum without cell lists: every k != n is tested against the
-// radius.  With nbins >= 3 the 9 stencil cells around pj hold every point
-// within h of pj exactly once, so this enumerates the same neighbour set; for
-// nbins < 3 the stencil repeats cells and the reference counts a candidate
-// once per repeat, which `mult` reproduces.
-template <typename T, int DIM, typename PathT>
-__device__ T pair_link_all(const PathT &P, int N, int nbins, const Prm<T> &p, int n, int j, const T *pj,
-                           const T *pj1)
-{
-    const int M = P.M;
-    const int jn = (j + 1 == M) ? 0 : j + 1;
-    const int ns = (DIM == 1) ? 3 : 9;
-    const int c0 = (nbins < 3) ? cell_of<T, DIM>(pj, p, nbins) : 0;
-    T acc = T(0);
-    for (int k = 0; k < N; ++k) {
-        if (k == n) continue;
-        T rk[DIM];
-        P.get(k, j, rk);
-        T d0 = mindist(rk[0], pj[0], p);
-        T r2 = d0 * d0;
-        if (DIM > 1) {
-            T d1 = mindist(rk[1], pj[1], p);
-            r2 += d1 * d1;
-        }
-        if (r2 > p.h2) continue;
-        int mult = 1;
-        if (nbins < 3) {
-            const int ck = cell_of<T, DIM>(rk, p, nbins);
-            mult = 0;
-            for (int s = 0; s < ns; ++s) mult += (stencil_cell(c0, s, nbins, DIM) == ck);
-        }
-        T rkn[DIM];
-        P.get(k, jn, rkn);
-        T e0 = mindist(rkn[0], pj1[0], p);
-        T s2 = e0 * e0;
-        if (DIM > 1) {
-            T e1 = mindist(rkn[1], pj1[1], p);
-            s2 += e1 * e1;
-        }
-        acc += T(mult) * pair_lnU<T, DIM>(rk, pj, rkn, pj1, r2, s2, p);
-    }
-    return acc;
-}
-
 // File particle n's bead at slice j, listed in cell oc (-1: in no list, after
 // an overflow), under cell nc != oc instead.  One thread per slice may move
 // beads at once: the lists of a slice are touched by its thread only.

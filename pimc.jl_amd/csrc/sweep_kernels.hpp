@@ -959,7 +959,7 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
 }
 
 // This lane's share of the pair link sum of particle n over link j -> j+1,
-// with its beads there replaced by pj, pj1 (see pair_link): pair_lnU for
+// with its beads there replaced by pj, pj1: pair_lnU for
 // every neighbour found at slice j.
 template <typename T, int DIM, bool CELLS, int TAB = -1, typename PathT>
 __device__ T link_part(const PathT &path, const Cells &C, int N, int nbins, const Prm<T> &P, int n, int j,
