@@ -47,7 +47,7 @@ class OSys(C.Structure):
         ("pot", OPot), ("pair", OPair),
         ("nbins", C.c_int), ("h", C.c_double), ("pair_in_accept", C.c_int),
         ("r", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)), ("next", C.POINTER(C.c_int64)),
-        ("hs_a", C.c_double), ("hs_ctr", C.c_int),
+        ("hs_a", C.c_double), ("hs_ctr", C.c_int), ("cells", C.c_void_p),
     ]
 
 
@@ -94,6 +94,7 @@ def lib():
         L.o_dV.argtypes = [C.POINTER(OPot), _D, C.c_int, _D]
         L.o_lnV.argtypes = [_D, _D, C.c_int, C.c_double, C.POINTER(OPot)]; L.o_lnV.restype = C.c_double
         L.o_lnU.argtypes = [_D, _D, C.c_int, C.c_double, C.POINTER(OPair)]; L.o_lnU.restype = C.c_double
+        L.o_vpair.argtypes = [C.POINTER(OPair), C.c_double]; L.o_vpair.restype = C.c_double
         L.o_metropolis.argtypes = [C.c_double, C.c_double]; L.o_metropolis.restype = C.c_int
         L.o_prop_quad_nodes.argtypes = [C.c_double, C.c_double, _D, _D, C.c_int]; L.o_prop_quad_nodes.restype = C.c_int
         L.o_prop_int_term.argtypes = [C.c_double] * 5; L.o_prop_int_term.restype = C.c_double
@@ -244,6 +245,11 @@ def prop_int(pair: OPair, r1rel, r2rel, tau):
     a = np.ascontiguousarray(r1rel, dtype=np.float64)
     b = np.ascontiguousarray(r2rel, dtype=np.float64)
     return lib().o_prop_int(C.byref(pair), _dp(a), _dp(b), len(a), tau)
+
+
+def vpair(pair: OPair, r):
+    """The pair potential v(r) of a Gaussian pair (0 for other kinds)."""
+    return lib().o_vpair(C.byref(pair), float(r))
 
 
 def lnU(pair: OPair, r1rel, r2rel, tau):

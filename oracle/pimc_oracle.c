@@ -387,8 +387,104 @@ static inline void bead(const o_sys *s, int n, int j, double *out)
 }
 static inline int mod1(int j, int M) { return ((j % M) + M) % M; } /* 0-based wrap */
 
+/* ---- per-slice index of the beads by cell (speeds up find_nns and find_nn) ----
+ * For each slice j: the particle indices sorted by (o_bin of their bead,
+ * index) and the start of each cell's run, so a query visits its stencil
+ * cells' members in index order, the order of a scan over all particles.
+ * Bucket ncell holds beads whose o_bin lies outside 0..ncell-1.  Every
+ * position write goes through set_r, which moves the bead to the bucket of
+ * its new position; o_sys_refresh_V (called after positions are replaced
+ * from outside) drops the index, which is then rebuilt on the next query. */
+typedef struct {
+    int ncell;   /* nbins^dim */
+    int *start;  /* M x (ncell + 2) */
+    int *ord;    /* M x N */
+    int *bucket; /* N x M: bucket of bead (n, j) */
+    int valid;
+} o_cells;
+
+static int bucket_of(const o_sys *s, int ncell, int n, int j)
+{
+    double a[3];
+    bead(s, n, j, a);
+    int b = o_bin(s, a);
+    return (b >= 0 && b < ncell) ? b : ncell;
+}
+
+static o_cells *cells_get(o_sys *s)
+{
+    o_cells *c = (o_cells *)s->cells;
+    const int N = s->N, M = s->M;
+    if (!c) {
+        c = (o_cells *)calloc(1, sizeof(o_cells));
+        long nc = 1;
+        for (int d = 0; d < s->dim; ++d) nc *= s->nbins;
+        c->ncell = (int)nc;
+        c->start = (int *)malloc(sizeof(int) * (size_t)M * (size_t)(c->ncell + 2));
+        c->ord = (int *)malloc(sizeof(int) * (size_t)M * N);
+        c->bucket = (int *)malloc(sizeof(int) * (size_t)N * M);
+        s->cells = c;
+    }
+    if (!c->valid) {
+        const int nc = c->ncell;
+        int *fill = (int *)malloc(sizeof(int) * (size_t)(nc + 1));
+        for (int j = 0; j < M; ++j) {
+            int *st = c->start + (size_t)j * (nc + 2), *ord = c->ord + (size_t)j * N;
+            memset(st, 0, sizeof(int) * (size_t)(nc + 2));
+            for (int n = 0; n < N; ++n) {
+                const int b = bucket_of(s, nc, n, j);
+                c->bucket[(size_t)n * M + j] = b;
+                st[b + 1]++;
+            }
+            for (int b = 0; b <= nc; ++b) st[b + 1] += st[b];
+            memcpy(fill, st, sizeof(int) * (size_t)(nc + 1));
+            for (int n = 0; n < N; ++n) ord[fill[c->bucket[(size_t)n * M + j]]++] = n;
+        }
+        free(fill);
+        c->valid = 1;
+    }
+    return c;
+}
+
+static void cells_free(o_sys *s)
+{
+    o_cells *c = (o_cells *)s->cells;
+    if (!c) return;
+    free(c->start); free(c->ord); free(c->bucket); free(c);
+    s->cells = NULL;
+}
+
+/* bead (n, j) was written: move it to the bucket of its new position */
+static void cells_update(o_sys *s, int n, int j)
+{
+    o_cells *c = (o_cells *)s->cells;
+    if (!c || !c->valid) return;
+    const int nc = c->ncell, N = s->N, M = s->M;
+    const int b = bucket_of(s, nc, n, j), a = c->bucket[(size_t)n * M + j];
+    if (a == b) return;
+    int *st = c->start + (size_t)j * (nc + 2), *ord = c->ord + (size_t)j * N;
+    int p = st[a];
+    while (ord[p] != n) ++p;
+    memmove(ord + p, ord + p + 1, sizeof(int) * (size_t)(N - 1 - p));
+    for (int k = a + 1; k <= nc + 1; ++k) st[k]--;
+    int q = st[b];
+    while (q < st[b + 1] && ord[q] < n) ++q;
+    memmove(ord + q + 1, ord + q, sizeof(int) * (size_t)(N - 1 - q));
+    ord[q] = n;
+    for (int k = b + 1; k <= nc + 1; ++k) st[k]++;
+    c->bucket[(size_t)n * M + j] = b;
+}
+
+/* the one way positions change inside the oracle */
+static inline void set_r(o_sys *s, int n, int d, int j, double v)
+{
+    s->r[((size_t)n * s->dim + d) * s->M + j] = v;
+    cells_update(s, n, j);
+}
+
 void o_sys_refresh_V(o_sys *s)
 {
+    if (s->cells) ((o_cells *)s->cells)->valid = 0;
     double a[3], b[3];
     for (int n = 0; n < s->N; ++n)
         for (int j = 0; j < s->M; ++j) {
@@ -425,6 +521,7 @@ int o_sys_init(o_sys *s, int dim, int M, int N, double lambda, double L, double 
 
 void o_sys_free(o_sys *s)
 {
+    cells_free(s);
     free(s->r); free(s->V); free(s->next);
     s->r = NULL; s->V = NULL; s->next = NULL;
 }
@@ -464,16 +561,33 @@ int o_find_nn(const o_sys *s, const double *target, int j, int exclude)
     int bx = b % nb, by = (dim >= 2) ? (b / nb) % nb : 0;
     int best = -1;
     double bd = 0.0, rk[3];
-    for (int k = 0; k < s->N; ++k) {
+    const o_cells *c = cells_get((o_sys *)s); /* an index of the positions, not state */
+    const int *st = c->start + (size_t)j * (c->ncell + 2), *ord = c->ord + (size_t)j * s->N;
+    /* the nearest of the stencil candidates, the lowest index among equal distances (the
+     * result of a scan over all particles in index order keeping the first minimum) */
+    for (int cc = 0; cc < ((dim == 1) ? 3 : 9); ++cc) {
+        const int cx = mod1(bx + (dim == 1 ? cc - 1 : cc % 3 - 1), nb);
+        const int cy = (dim == 1) ? 0 : mod1(by + cc / 3 - 1, nb);
+        const int cell = cx + nb * cy;
+        for (int p = st[cell]; p < st[cell + 1]; ++p) {
+            const int k = ord[p];
+            if (k == exclude) continue;
+            bead(s, k, j, rk);
+            double d = periodic_norm(s, rk, target);
+            if (best < 0 || d < bd || (d == bd && k < best)) { best = k; bd = d; }
+        }
+    }
+    /* beads whose o_bin lies outside the grid take the cell their index names */
+    for (int p = st[c->ncell]; p < st[c->ncell + 1]; ++p) {
+        const int k = ord[p];
         if (k == exclude) continue;
         bead(s, k, j, rk);
         int ck = o_bin(s, rk), kx = ck % nb, ky = (dim >= 2) ? (ck / nb) % nb : 0;
-        /* k is a stencil candidate when its cell is within one cell of target's, periodically */
         int dx = mod1(kx - bx, nb), dy = mod1(ky - by, nb);
         int inx = dx == 0 || dx == 1 || dx == nb - 1, iny = (dim < 2) || dy == 0 || dy == 1 || dy == nb - 1;
         if (!(inx && iny)) continue;
         double d = periodic_norm(s, rk, target);
-        if (best < 0 || d < bd) { best = k; bd = d; }
+        if (best < 0 || d < bd || (d == bd && k < best)) { best = k; bd = d; }
     }
     return best;
 }
@@ -488,19 +602,22 @@ int o_find_nns(const o_sys *s, const double *target, int j, int exclude, int *ou
     int ncells = (dim == 1) ? 3 : 9;
     int cnt = 0;
     double rk[3];
-    for (int c = 0; c < ncells; ++c) {
+    const o_cells *c = cells_get((o_sys *)s); /* an index of the positions, not state */
+    const int *st = c->start + (size_t)j * (c->ncell + 2), *ord = c->ord + (size_t)j * s->N;
+    for (int cc = 0; cc < ncells; ++cc) {
         int cell;
         if (dim == 1) {
-            cell = mod1(bx + st1[c], nb);
+            cell = mod1(bx + st1[cc], nb);
         } else {
-            int cx = mod1(bx + st2[c][0], nb), cy = mod1(by + st2[c][1], nb);
+            int cx = mod1(bx + st2[cc][0], nb), cy = mod1(by + st2[cc][1], nb);
             cell = cx + nb * cy;
             /* 3D is not part of the reference's stencil; dim 3 is rejected at init */
         }
-        for (int k = 0; k < s->N; ++k) {
+        /* the members of the cell in index order */
+        for (int p = st[cell]; p < st[cell + 1]; ++p) {
+            const int k = ord[p];
             if (k == exclude) continue;
             bead(s, k, j, rk);
-            if (o_bin(s, rk) != cell) continue;
             if (periodic_norm(s, rk, target) <= s->h) {
                 if (cnt < cap) out[cnt] = k;
                 ++cnt;
@@ -655,7 +772,7 @@ int o_reshape_apply(o_sys *s, int n, int j0, int m, const double *normals, doubl
         for (int jp = 0; jp < m; ++jp) {
             int j = mod1(j0 + jp, M);
             for (int d = 0; d < dim; ++d) {
-                s->r[((size_t)n * dim + d) * M + j] = rp[d * nrows + jp];
+                set_r(s, n, d, j, rp[d * nrows + jp]);
                 a[d] = rp[d * nrows + jp];
                 b[d] = rp[d * nrows + jp + 1];
             }
@@ -705,7 +822,7 @@ int o_com_apply(o_sys *s, int n, const double *dsp, double u, int *accepted, o_w
     *accepted = o_metropolis(exp(lw), u);
     if (*accepted) {
         for (int j = 0; j < M; ++j)
-            for (int d = 0; d < dim; ++d) s->r[((size_t)n * dim + d) * M + j] = rp[d * M + j];
+            for (int d = 0; d < dim; ++d) set_r(s, n, d, j, rp[d * M + j]);
         o_sys_refresh_V(s); /* :211-219 store V' per link: same values */
     }
     free(rp);
@@ -971,7 +1088,7 @@ int o_run_reference(o_sys *s, o_refstate *st, int64_t n_iter, uint64_t seed, int
             int acc = (delta >= 1.0) || (delta > stream_u(&rs));
             if (acc) {
                 for (int j = 0; j < M; ++j)
-                    for (int d = 0; d < dim; ++d) s->r[((size_t)n * dim + d) * M + j] = rp[d * M + j];
+                    for (int d = 0; d < dim; ++d) set_r(s, n, d, j, rp[d * M + j]);
                 double a[3], b[3];
                 for (int j = 0; j < M; ++j) {
                     for (int d = 0; d < dim; ++d) { a[d] = rp[d * M + j]; b[d] = rp[d * M + (j + 1) % M]; }
@@ -1007,7 +1124,7 @@ int o_run_reference(o_sys *s, o_refstate *st, int64_t n_iter, uint64_t seed, int
                 for (int jp = 0; jp < m; ++jp) {
                     int j = mod1(j0 + jp, M);
                     for (int d = 0; d < dim; ++d) {
-                        s->r[((size_t)n * dim + d) * M + j] = rp[d * (m + 1) + jp];
+                        set_r(s, n, d, j, rp[d * (m + 1) + jp]);
                         a[d] = rp[d * (m + 1) + jp];
                         b[d] = rp[d * (m + 1) + jp + 1];
                     }
@@ -1242,7 +1359,7 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
                 st->levy_acc++;
                 for (int i = 1; i < m; ++i) {
                     int j = (start + i) % M, q = owner(s, p, us + i);
-                    for (int d = 0; d < dim; ++d) s->r[((size_t)q * dim + d) * M + j] = rp[d * nrows + i];
+                    for (int d = 0; d < dim; ++d) set_r(s, q, d, j, rp[d * nrows + i]);
                 }
             }
         }
@@ -1303,7 +1420,7 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
         if (acc) {
             st->com_acc++;
             for (int j = 0; j < M; ++j)
-                for (int d = 0; d < dim; ++d) s->r[((size_t)p * dim + d) * M + j] = rp[d * M + j];
+                for (int d = 0; d < dim; ++d) set_r(s, p, d, j, rp[d * M + j]);
         }
     }
     if (n_swap > 0) o_swaps(s, chain, sweep, seed, n_swap, swap_m, rule, st);
@@ -1439,15 +1556,15 @@ static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int
             const double *rp = w == 0 ? rp1 : rp2;
             for (int i = 1; i < m; ++i) {
                 const int u = j0 + i, j = u % M, q = u < M ? nw : qn;
-                for (int d = 0; d < dim; ++d) s->r[((size_t)q * dim + d) * M + j] = rp[d * (m + 1) + i];
+                for (int d = 0; d < dim; ++d) set_r(s, q, d, j, rp[d * (m + 1) + i]);
             }
         }
         if (jm < M) {
             for (int j = jm; j < M; ++j)
                 for (int d = 0; d < dim; ++d) {
                     double t = s->r[((size_t)n1 * dim + d) * M + j];
-                    s->r[((size_t)n1 * dim + d) * M + j] = s->r[((size_t)n2 * dim + d) * M + j];
-                    s->r[((size_t)n2 * dim + d) * M + j] = t;
+                    set_r(s, n1, d, j, s->r[((size_t)n2 * dim + d) * M + j]);
+                    set_r(s, n2, d, j, t);
                 }
         }
         s->next[n1] = q2;

@@ -68,6 +68,8 @@ typedef struct {
                            0 switches the rejection off */
     int hs_ctr;         /* redraws allowed per bead or COM shift (s.ctr: 10000 thermalising, 1000 measuring,
                            src/simulation.jl:31-32) */
+    void *cells;        /* per-slice index of the beads by cell (built on first use, kept in step with every
+                           position write); speeds up find_nns / find_nn without changing their results */
 } o_sys;
 
 /* ---- Philox4x32-10 (Salmon et al., SC'11) and the draw helpers ---- */

@@ -422,3 +422,146 @@ def test_optical_lattice_known_answers_and_gradient():
             h = 1e-6
             fd = [(O.pot_V(p, x + h * e) - O.pot_V(p, x - h * e)) / (2 * h) for e in np.eye(2)]
             assert np.allclose(g, fd, rtol=1e-6, atol=1e-7)
+
+
+# ---------------------------------------------------------------- square lattice and Gaussian pair
+@pytest.mark.parametrize("V0,a", [(4.0, 1.0), (-2.5, 0.7)])
+def test_square_lattice_special_values_period_and_gradient(V0, a):
+    """V = V0 sum_d cos^2(pi x_d / a) (the lattice of the interacting-lattice
+    workload): maxima dim V0 at the lattice points k a, zeros at (k + 1/2) a,
+    V0 (1 + 1/2) at (a/4, 0) and V0 (1 + 1/4) at (a/3, 0), period a in each
+    coordinate, a stationary point at every (k + 1/2) a and k a, and the
+    gradient equal to central differences.  A cos(2 pi x / a), a missing
+    square or a gradient off by a factor 2 fails one of these."""
+    pot = O.make_pot(O.POT_LATTICE, V0=V0, a=a)
+    assert O.pot_V(pot, [0.0, 0.0]) == pytest.approx(2 * V0, rel=1e-15)
+    assert O.pot_V(pot, [a / 2, a / 2]) == pytest.approx(0.0, abs=1e-15 * abs(V0))
+    assert O.pot_V(pot, [-a / 2, 3 * a / 2]) == pytest.approx(0.0, abs=1e-14 * abs(V0))
+    assert O.pot_V(pot, [a / 4, 0.0]) == pytest.approx(1.5 * V0, rel=1e-14)
+    assert O.pot_V(pot, [a / 3, 0.0]) == pytest.approx(1.25 * V0, rel=1e-14)
+    assert O.pot_V(pot, [0.0, a / 4]) == pytest.approx(1.5 * V0, rel=1e-14)
+    assert O.pot_V(pot, [a / 4]) == pytest.approx(0.5 * V0, rel=1e-14)  # 1D
+    rng = np.random.default_rng(91)
+    for x in rng.uniform(-5, 5, size=(20, 2)):
+        v = O.pot_V(pot, x)
+        for shift in ([a, 0.0], [0.0, -a], [3 * a, 2 * a]):
+            assert O.pot_V(pot, x + np.array(shift)) == pytest.approx(v, rel=1e-12, abs=1e-12)
+        g = O.pot_dV(pot, x)
+        h = 1e-6
+        fd = [(O.pot_V(pot, x + h * e) - O.pot_V(pot, x - h * e)) / (2 * h) for e in np.eye(2)]
+        assert np.allclose(g, fd, rtol=1e-7, atol=1e-7 * abs(V0))
+    for k in (-2, 0, 3):
+        for pt in ([(k + 0.5) * a, (k + 0.5) * a], [k * a, (k + 0.5) * a]):
+            assert np.allclose(O.pot_dV(pot, pt), 0.0, atol=1e-12 * abs(V0) / a)
+    # the extremes over a fine grid of one cell are 0 and 2 V0
+    xs = np.linspace(0, a, 41)
+    vals = [O.pot_V(pot, [x, y]) for x in xs for y in xs]
+    assert min(vals) == pytest.approx(min(0.0, 2 * V0), abs=1e-12)
+    assert max(vals) == pytest.approx(max(0.0, 2 * V0), abs=1e-12)
+
+
+def test_reference_sine_lattice_is_a_shifted_square_lattice():
+    """The reference's tests sample v1d(x) = depth sin^2(2 pi x scale)
+    (test/testsystem.jl, test/testmeasurements.jl).  That potential is
+    depth + V of the square lattice with V0 = -depth and a = 1 / (2 scale),
+    so a constant shift of the action: the same sampled paths."""
+    depth, scale = 3.0, 0.8
+    pot = O.make_pot(O.POT_LATTICE, V0=-depth, a=1.0 / (2 * scale))
+    for x in np.linspace(-3, 3, 37):
+        v1d = depth * math.sin(2 * math.pi * x * scale) ** 2
+        assert O.pot_V(pot, [x]) + depth == pytest.approx(v1d, abs=1e-13)
+
+
+def test_gaussian_pair_special_values_and_link_action():
+    """v(r) = g exp(-r^2 / (2 sigma^2)): v(0) = g, v(sigma) = g e^-1/2,
+    v(sqrt(2) sigma) = g / e, v(4 sigma) = g e^-8 (the cut-off of the
+    interacting workloads).  The primitive link action lnU of one pair is
+    -tau/2 (v(|r_j|) + v(|r_j+1|)) from the two per-component offsets."""
+    g, sigma, tau = 1.5, 0.5, 0.1
+    pair = O.make_pair(O.PAIR_GAUSSIAN, g, sigma)
+    assert O.vpair(pair, 0.0) == g
+    assert O.vpair(pair, sigma) == pytest.approx(g * math.exp(-0.5), rel=1e-15)
+    assert O.vpair(pair, math.sqrt(2) * sigma) == pytest.approx(g / math.e, rel=1e-15)
+    assert O.vpair(pair, 4 * sigma) == pytest.approx(g * math.exp(-8.0), rel=1e-14)
+    # offsets (0.3, 0.4): |r| = 0.5 = sigma; (0, 0): v = g; (2, 0): 4 sigma; (0.5, 0.5): sqrt(2) sigma
+    assert O.lnU(pair, [0.3, 0.4], [0.0, 0.0], tau) == pytest.approx(-0.05 * (g * math.exp(-0.5) + g), rel=1e-15)
+    assert O.lnU(pair, [2.0, 0.0], [0.5, 0.5], tau) == pytest.approx(-0.05 * (g * math.exp(-8) + g / math.e),
+                                                                     rel=1e-14)
+    assert O.lnU(pair, [0.5], [0.0], tau) == pytest.approx(-0.05 * (g * math.exp(-0.5) + g), rel=1e-15)  # 1D
+    assert O.vpair(O.make_pair(), 0.3) == 0.0
+
+
+def test_pair_energy_of_a_hand_placed_pair():
+    """Two particles, M = 4, beta = 4 (tau = 1), in a wide box: separation
+    (0.3, 0.4) at slices 0 and 2 (|r| = sigma) and (0.5, 0.5) at slices 1
+    and 3 (|r| = sqrt(2) sigma).  Every link finds the pair, so S_U =
+    tau/2 * 4 (v0 + v1) and sum r . grad S_U = tau/2 * 4 (-(r0^2/sigma^2) v0
+    - (r1^2/sigma^2) v1) = 2 (-v0 - 2 v1); E_pair = S_U / beta, the pair
+    virial E_pair + that / (2 beta)."""
+    g, sigma = 1.3, 0.5
+    r = np.zeros((2, 2, 4))
+    r[0, :, 0] = (1.0, -2.0)
+    r[1, :, 0] = (1.3, -1.6)
+    r[0, :, 1] = (0.2, 0.7)
+    r[1, :, 1] = (0.7, 1.2)
+    r[:, :, 2:] = r[:, :, :2] + np.array([0.1, -0.2])[None, :, None]
+    s = O.System(r, 0.5, 10.0, 4.0, O.make_pot(), O.make_pair(O.PAIR_GAUSSIAN, g, sigma), 2.0, True)
+    v0, v1 = g * math.exp(-0.5), g / math.e
+    Ep, Evp = s.pair_energy()
+    assert Ep == pytest.approx((v0 + v1) / 2.0, rel=1e-14)
+    assert Evp == pytest.approx((v0 + v1) / 2.0 + (-v0 - 2.0 * v1) / 4.0, rel=1e-14)
+    assert -s.total_lnU() == pytest.approx(2 * (v0 + v1), rel=1e-14)
+
+
+def test_pair_virial_against_scaling_derivative():
+    """sum r . grad S_U = d/ds S_U(s r) at s = 1 (the positions scaled about
+    the origin, inside a wide box so that no offset wraps and the neighbour
+    sets do not change): the pair virial of o_pair_energy against a central
+    difference of the brute-force total action."""
+    N, M, L, beta = 12, 6, 20.0, 2.0
+    rng = np.random.default_rng(92)
+    r = rng.uniform(-2.0, 2.0, size=(N, 2, M))
+    pair = O.make_pair(O.PAIR_GAUSSIAN, 0.7, 0.5)
+    # r_a = 8: nbins = 5, the stencil covers every particle and the cut-off 8 exceeds every distance
+    s = O.System(r, 0.5, L, beta, O.make_pot(), pair, 8.0, True)
+    Ep, Evp = s.pair_energy()
+    SU = -s.total_lnU()
+    assert Ep == pytest.approx(SU / beta, rel=1e-12)
+    h = 1e-5
+    s.r = r * (1 + h)
+    up = -s.total_lnU()
+    s.r = r * (1 - h)
+    dn = -s.total_lnU()
+    rgrad = (up - dn) / (2 * h)
+    assert (Evp - Ep) * 2 * beta == pytest.approx(rgrad, rel=1e-7)
+
+
+def test_thermodynamic_equals_virial_energy_with_pair():
+    """For a confined system the primitive-action thermodynamic and virial
+    estimators have the same mean at every M (scaling identity of the
+    discretised path integral), the pair terms included: E_pair thermodynamic
+    S_U / beta against the virial S_U / beta + (r . grad S_U) / (2 beta).
+    Trapped 2D particles with the Gaussian repulsion (the differentiation
+    workload's interaction, g = 0.5, sigma = 0.5), in the oracle's sweep
+    schedule.  A wrong sign or factor in grad V or in the pair virial breaks
+    the equality."""
+    N, M, beta = 4, 12, 2.0
+    rng = np.random.default_rng(93)
+    r = rng.normal(0, 0.7, size=(N, 2, M))
+    pair = O.make_pair(O.PAIR_GAUSSIAN, 0.5, 0.5)
+    s = O.System(r, 0.5, 100.0, beta, O.make_pot(O.POT_HARMONIC, 1.0), pair, 2.0, True)
+    E, Ev, Ep = [], [], []
+    for sw in range(16000):
+        s.sweep(0, sw, 5, 4, 0.8)
+        if sw >= 400:
+            e0, ev0 = s.energy()
+            p0, pv0 = s.pair_energy()
+            E.append(e0 + p0)
+            Ev.append(ev0 + pv0)
+            Ep.append(p0)
+    d = np.array(E) - np.array(Ev)
+    m, e = blockerr(d)
+    assert abs(m) < 3.5 * e, (m, e)
+    # the pair terms matter at this size: the pair energy is far from zero
+    mp, ep = blockerr(Ep)
+    assert mp > 20 * ep and mp > 0.05 * np.mean(Ev)
