@@ -411,4 +411,104 @@ __global__ void k_init_paths(const ChainDev *chains, const int *part_chain, cons
     }
 }
 
+// The batched kernel's FP32 fast paths on one particle of a tile chain whose
+// staging length ell = MAXL divides M (every segment and centre-of-mass chunk
+// is one skew block of the LDS rows): the particle's ring is staged into LDS
+// in the tile kernel's row layout, and one wave runs the same device code as
+// k_sweep_tiles with the caller's draws.  Staging (m = MAXL links from j0):
+// levy_full_propose with the injected normals; centre of mass: com_chunk_full
+// per chunk, summed by the lane-group butterfly when the chunks form an
+// aligned lane group (nseg a power of two <= 64), else in chunk order.  The
+// outputs follow k_hook_reshape / k_hook_com, with the log-weight change in
+// w[1] (w[0] = 0: the fast paths form the difference directly).
+template <int DIM, int MAXL, int POT> __global__ void __launch_bounds__(64) k_hook_tile_fast(HookArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const ChainDev ch = a.chains[a.chain];
+    const Prm<float> P = Prm<float>::make(ch);
+    const int M = ch.M, n = a.n, lane = threadIdx.x;
+    int sh = 0;
+    while ((2 << sh) <= MAXL) ++sh;
+    const LdsRow R{tile_row_len(M, sh), sh};
+    float *sp = reinterpret_cast<float *>(smem);
+    float *s_red = sp + DIM * R.Mp;
+    float *gp = reinterpret_cast<float *>(a.pos) + ch.pos_off + (size_t)n * DIM * M;
+    for (int i = lane; i < DIM * M; i += 64) {
+        const int d = i / M, j = i - d * M;
+        sp[R.at(d, j)] = gp[i];
+    }
+    __syncthreads();
+    double *w = nullptr;
+    if (a.m > 0) { // staging segment
+        const int j0 = a.j0, nrows = MAXL + 1;
+        w = a.out + nrows * DIM;
+        if (lane == 0) {
+            float nb[MAXL - 1][DIM];
+            GivenNormals<DIM> ns{a.in};
+            const float lw = levy_full_propose<DIM, MAXL, POT>(sp, R, M, j0, P, ns, nb);
+            const int jm = (j0 + MAXL) % M;
+            for (int e = 0; e < DIM; ++e) {
+                a.out[e * nrows] = (double)sp[R.at(e, j0)];
+                a.out[e * nrows + MAXL] = (double)sp[R.at(e, jm)];
+                for (int i = 1; i < MAXL; ++i) a.out[e * nrows + i] = (double)nb[i - 1][e];
+            }
+            w[0] = 0.0;
+            w[1] = (double)lw;
+            w[2] = w[3] = w[4] = 0.0;
+            if (a.op == 1 && metropolis(ex(lw), (float)a.u)) {
+                w[4] = 1.0;
+                for (int i = 1; i < MAXL; ++i) {
+                    const int j = (j0 + i) % M;
+                    for (int e = 0; e < DIM; ++e) gp[e * M + j] = nb[i - 1][e];
+                }
+            }
+        }
+        return;
+    }
+    // centre of mass: chunk k = lane (every lane takes part in the butterfly)
+    const int nseg = M / MAXL;
+    float d[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) d[e] = (float)a.in[e];
+    w = a.out + M * DIM;
+    const bool grouped = nseg <= 64 && (nseg & (nseg - 1)) == 0;
+    for (int k0 = 0; k0 < nseg; k0 += 64) {
+        const int k = k0 + lane;
+        bool wr = false;
+        float v = 0.0f;
+        if (k < nseg) {
+            v = com_chunk_full<DIM, MAXL, POT>(sp + k * (MAXL + 1), R, d, P, wr);
+            for (int jj = 0; jj < MAXL; ++jj) {
+                const int j = k * MAXL + jj;
+                for (int e = 0; e < DIM; ++e) {
+                    const float x = sp[R.at(e, j)] + d[e];
+                    a.out[e * M + j] = (double)(wr ? teleport(sp[R.at(e, j)] + d[e], P) : x);
+                }
+            }
+            s_red[k] = v;
+        }
+        if (grouped) {
+            const float tot = group_sum_dpp(v, nseg);
+            if (lane == 0) s_red[0] = tot;
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        float tot = s_red[0];
+        if (!grouped) {
+            tot = 0.0f;
+            for (int k = 0; k < nseg; ++k) tot += s_red[k];
+        }
+        const float lw = -P.tau * tot;
+        w[0] = 0.0;
+        w[1] = (double)lw;
+        w[2] = w[3] = w[4] = 0.0;
+        if (a.op == 1 && metropolis(ex(lw), (float)a.u)) {
+            w[4] = 1.0;
+            for (int j = 0; j < M; ++j)
+                for (int e = 0; e < DIM; ++e) gp[e * M + j] = (float)a.out[e * M + j];
+        }
+    }
+}
+
 } // namespace pimc

@@ -581,6 +581,31 @@ void prop_quad_nodes(double rmax, double tau, std::vector<double> &k, std::vecto
     for (int p = 0; p < np; ++p) panel(h + p * hw, h + (p + 1) * hw);
 }
 
+template <int DIM, int MAXL>
+void launch_tile_fast_t(pimc_ctx *c, const HookArgs &h, int pot, size_t lds)
+{
+    switch (pot) {
+    case PIMC_POT_HARMONIC: hipLaunchKernelGGL((k_hook_tile_fast<DIM, MAXL, kPotHarmonic>), dim3(1), dim3(64), lds, c->stream, h); break;
+    case PIMC_POT_LATTICE: hipLaunchKernelGGL((k_hook_tile_fast<DIM, MAXL, kPotLattice>), dim3(1), dim3(64), lds, c->stream, h); break;
+    case PIMC_POT_OPTICAL: hipLaunchKernelGGL((k_hook_tile_fast<DIM, MAXL, kPotOptical>), dim3(1), dim3(64), lds, c->stream, h); break;
+    default: hipLaunchKernelGGL((k_hook_tile_fast<DIM, MAXL, kPotNone>), dim3(1), dim3(64), lds, c->stream, h); break;
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void launch_tile_fast(pimc_ctx *c, const HookArgs &h, int maxl, int pot, size_t lds)
+{
+    if (c->dim == 1) {
+        if (maxl == 8) launch_tile_fast_t<1, 8>(c, h, pot, lds);
+        else if (maxl == 16) launch_tile_fast_t<1, 16>(c, h, pot, lds);
+        else launch_tile_fast_t<1, 32>(c, h, pot, lds);
+    } else {
+        if (maxl == 8) launch_tile_fast_t<2, 8>(c, h, pot, lds);
+        else if (maxl == 16) launch_tile_fast_t<2, 16>(c, h, pot, lds);
+        else launch_tile_fast_t<2, 32>(c, h, pot, lds);
+    }
+}
+
 } // namespace
 
 // ============================================================================
@@ -1578,6 +1603,15 @@ static int hook_move(pimc_ctx *c, int chain, int n, int j0, int m, const double 
                                         "sample through pimc_run");
     if (!in) return set_err(c, PIMC_ERR_ARG, "null input");
     if ((rc = check_table(c, chain))) return rc;
+    // FP32 tile chains whose segments and COM chunks are one skew block of the LDS rows: the
+    // batched kernel's fast-path code (levy_full_propose, com_chunk_full) with the given draws
+    const int maxl = next_pow2_maxl(h.ell);
+    const bool fast = c->prec == PIMC_FP32 && !c->is_whole[chain] && h.ell == maxl && h.M % maxl == 0 &&
+                      (com || m == h.ell);
+    const int pot = h.pot, nseg = h.M / maxl;
+    int sh = 0;
+    while ((2 << sh) <= maxl) ++sh;
+    const size_t lds_fast = ((size_t)h.dim * tile_row_len(h.M, sh) + (size_t)nseg) * sizeof(float);
     try {
         const int rows = com ? h.M : m + 1;
         const size_t nout = (size_t)rows * h.dim + 5;
@@ -1593,8 +1627,14 @@ static int hook_move(pimc_ctx *c, int chain, int n, int j0, int m, const double 
         h.in = d_in;
         h.u = u;
         h.out = c->d_io;
-        if (com) HOOK_DISPATCH(k_hook_com, dim3(1), dim3(64), 0, h);
-        else HOOK_DISPATCH(k_hook_reshape, dim3(1), dim3(64), 0, h);
+        if (fast) {
+            h.m = com ? 0 : m;
+            launch_tile_fast(c, h, maxl, pot, lds_fast);
+        } else if (com) {
+            HOOK_DISPATCH(k_hook_com, dim3(1), dim3(64), 0, h);
+        } else {
+            HOOK_DISPATCH(k_hook_reshape, dim3(1), dim3(64), 0, h);
+        }
         HIPCHK(hipStreamSynchronize(c->stream));
         std::vector<double> o(nout);
         HIPCHK(hipMemcpy(o.data(), c->d_io, sizeof(double) * nout, hipMemcpyDeviceToHost));

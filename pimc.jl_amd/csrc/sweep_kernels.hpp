@@ -257,15 +257,45 @@ template <typename T> __device__ __forceinline__ T group_sum_dpp(T v, int G)
     return v;
 }
 
-// FP32 staging segment of exactly MAXL links: the draws, the bridge and the
-// Metropolis test of levy_segment, with two changes that only touch
+// Normals of the interior beads of a staging segment, as the batched sweep
+// draws them: one Philox block per bead pair (i, i+1), i odd, keyed by the
+// slice b of bead i, words x, y for bead i and z, w for bead i + 1.
+struct PhiloxNormals {
+    uint64_t seed;
+    uint32_t item0, sweep, chain;
+    U4 xr;
+    __device__ __forceinline__ void operator()(int i, int b, float (&n)[2])
+    {
+        if (i & 1) {
+            xr = draw(seed, item0 + (uint32_t)b, sweep, chain, kLevy);
+            box_muller(xr.x, xr.y, n[0], n[1]);
+        } else {
+            box_muller(xr.z, xr.w, n[0], n[1]);
+        }
+    }
+};
+
+// Normals given by the caller ((MAXL - 1) x DIM, row-major per bead): the
+// single-move hooks run the batched kernel's staging code with them.
+template <int DIM> struct GivenNormals {
+    const double *nrm;
+    __device__ __forceinline__ void operator()(int i, int, float (&n)[2])
+    {
+        n[0] = (float)nrm[(i - 1) * DIM];
+        n[1] = DIM > 1 ? (float)nrm[(i - 1) * DIM + 1] : 0.0f;
+    }
+};
+
+// FP32 staging segment of exactly MAXL links: the bridge of levy_segment and
+// the log-weight change of the potential, with two changes that only touch
 // rounding.  The bridge coefficients are compile-time constants (kBridge),
 // and the periodic wrap is applied only when a new bead of the segment left
 // [-L, L): inside the box teleport is the identity, and its FP32 evaluation
-// ((x + L) - ...) - L would only round x to the spacing of L.
-template <int DIM, int MAXL, int POT>
-__device__ __forceinline__ bool levy_full(float *xp, const LdsRow &R, int M, int start, const Prm<float> &P,
-                                          uint64_t seed, uint32_t item0, uint32_t sweep, uint32_t chain)
+// ((x + L) - ...) - L would only round x to the spacing of L.  Fills the
+// interior beads nb and returns -tau (sum V' - sum V).
+template <int DIM, int MAXL, int POT, typename NS>
+__device__ __forceinline__ float levy_full_propose(const float *xp, const LdsRow &R, int M, int start,
+                                                   const Prm<float> &P, NS &ns, float (&nb)[MAXL - 1][DIM])
 {
     int jm = start + MAXL;
     if (jm >= M) jm -= M;
@@ -275,17 +305,13 @@ __device__ __forceinline__ bool levy_full(float *xp, const LdsRow &R, int M, int
         cur[d] = xp[R.at(d, start)];
         em[d] = shift_end(cur[d], xp[R.at(d, jm)], P);
     }
-    float nb[MAXL - 1][DIM];
     float vn = 0.0f, vo = 0.0f, mx = 0.0f;
-    U4 xr{0u, 0u, 0u, 0u};
 #pragma unroll
     for (int i = 1; i < MAXL; ++i) {
         int b = start + i;
         if (b >= M) b -= M;
-        if (i & 1) xr = draw(seed, item0 + (uint32_t)b, sweep, chain, kLevy);
         float n[2];
-        if (i & 1) box_muller(xr.x, xr.y, n[0], n[1]);
-        else box_muller(xr.z, xr.w, n[0], n[1]);
+        ns(i, b, n);
         const float al = kBridge<MAXL>.alpha[i], sd = P.sdl * kBridge<MAXL>.root[i];
         float old[DIM];
 #pragma unroll
@@ -329,7 +355,18 @@ __device__ __forceinline__ bool levy_full(float *xp, const LdsRow &R, int M, int
             vn += potVk<float, DIM, POT>(nb[i - 1], P);
         }
     }
-    const float lw = -P.tau * (vn - vo);
+    return -P.tau * (vn - vo);
+}
+
+// The batched sweep's FP32 staging segment of MAXL links: levy_full_propose
+// with the Philox draws, the Metropolis test, and the write-back on accept.
+template <int DIM, int MAXL, int POT>
+__device__ __forceinline__ bool levy_full(float *xp, const LdsRow &R, int M, int start, const Prm<float> &P,
+                                          uint64_t seed, uint32_t item0, uint32_t sweep, uint32_t chain)
+{
+    float nb[MAXL - 1][DIM];
+    PhiloxNormals ns{seed, item0, sweep, chain, U4{0u, 0u, 0u, 0u}};
+    const float lw = levy_full_propose<DIM, MAXL, POT>(xp, R, M, start, P, ns, nb);
     const U4 xa = draw(seed, item0 + (uint32_t)start, sweep, chain, kLevyAccept);
     const bool acc = metropolis(ex(lw), Uniform<float>::get(xa.x));
     if (acc) {
@@ -342,6 +379,73 @@ __device__ __forceinline__ bool levy_full(float *xp, const LdsRow &R, int M, int
         }
     }
     return acc;
+}
+
+// FP32 centre-of-mass chunk that is one skew block of the LDS rows (slice
+// k MAXL + jj of the chunk at x0[jj]): the potential change sum_j V(r_j + d)
+// - V(r_j) of its MAXL beads.  In the harmonic trap (w^2/2) (2 d.S + MAXL
+// |d|^2) with S the sum of the beads, two adds per bead instead of both
+// potentials.  wrap: a moved bead left [-L, L), so the moved beads are
+// teleported (and the sum taken over them).
+template <int DIM, int MAXL, int POT>
+__device__ __forceinline__ float com_chunk_full(const float *x0, const LdsRow &R, const float (&d)[DIM],
+                                                const Prm<float> &P, bool &wrap)
+{
+    float vn = 0.0f, vo = 0.0f, mx = 0.0f;
+    if constexpr (POT == kPotHarmonic) {
+        float sr[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) sr[e] = 0.0f;
+#pragma unroll
+        for (int jj = 0; jj < MAXL; ++jj)
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) {
+                const float ro = x0[e * R.Mp + jj];
+                sr[e] += ro;
+                mx = fmaxf(mx, fabsf(ro + d[e]));
+            }
+        float ds = d[0] * sr[0], dd = d[0] * d[0];
+        if (DIM > 1) {
+            ds += d[DIM - 1] * sr[DIM - 1];
+            dd += d[DIM - 1] * d[DIM - 1];
+        }
+        vn = P.halfw2 * (2.0f * ds + float(MAXL) * dd);
+        if (mx >= P.L) {
+            vn = 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < MAXL; ++jj) {
+                float ro[DIM];
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) ro[e] = x0[e * R.Mp + jj];
+                vo += potVk<float, DIM, POT>(ro, P);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int jj = 0; jj < MAXL; ++jj) {
+            float ro[DIM], rn[DIM];
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) {
+                ro[e] = x0[e * R.Mp + jj];
+                rn[e] = ro[e] + d[e];
+                mx = fmaxf(mx, fabsf(rn[e]));
+            }
+            vn += potVk<float, DIM, POT>(rn, P);
+            vo += potVk<float, DIM, POT>(ro, P);
+        }
+    }
+    wrap = mx >= P.L;
+    if (wrap) {
+        vn = 0.0f;
+#pragma unroll
+        for (int jj = 0; jj < MAXL; ++jj) {
+            float rn[DIM];
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) rn[e] = teleport(x0[e * R.Mp + jj] + d[e], P);
+            vn += potVk<float, DIM, POT>(rn, P);
+        }
+    }
+    return vn - vo;
 }
 
 // At most 256 threads; FP32 at 4 waves per SIMD (128 VGPRs: with tiles of
@@ -484,71 +588,18 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
                 if (aligned) {
                     // the chunk is one skew block: slice k MAXL + jj sits at k (MAXL + 1) + jj
                     const T *x0 = xp + k * (MAXL + 1);
-                    T vn = T(0), vo = T(0), mx = T(0);
-                    if constexpr (POT == kPotHarmonic) {
-                        // (w^2/2) sum_j (|r_j + d|^2 - |r_j|^2) = (w^2/2) (2 d.S + MAXL |d|^2),
-                        // S = sum_j r_j: two adds per bead instead of both potentials
-                        T sr[DIM];
-#pragma unroll
-                        for (int e = 0; e < DIM; ++e) sr[e] = T(0);
-#pragma unroll
-                        for (int jj = 0; jj < MAXL; ++jj)
-#pragma unroll
-                            for (int e = 0; e < DIM; ++e) {
-                                const T ro = x0[e * R.Mp + jj];
-                                sr[e] += ro;
-                                mx = fmaxf(mx, fabsf(ro + d[e]));
-                            }
-                        T ds = d[0] * sr[0], dd = d[0] * d[0];
-                        if (DIM > 1) {
-                            ds += d[DIM - 1] * sr[DIM - 1];
-                            dd += d[DIM - 1] * d[DIM - 1];
-                        }
-                        vn = P.halfw2 * (T(2) * ds + T(MAXL) * dd);
-                        if (mx >= P.L) {
-                            vn = T(0);
-#pragma unroll
-                            for (int jj = 0; jj < MAXL; ++jj) {
-                                T ro[DIM];
-#pragma unroll
-                                for (int e = 0; e < DIM; ++e) ro[e] = x0[e * R.Mp + jj];
-                                vo += potVk<T, DIM, POT>(ro, P);
-                            }
-                        }
-                    } else {
-#pragma unroll
-                        for (int jj = 0; jj < MAXL; ++jj) {
-                            T ro[DIM], rn[DIM];
-#pragma unroll
-                            for (int e = 0; e < DIM; ++e) {
-                                ro[e] = x0[e * R.Mp + jj];
-                                rn[e] = ro[e] + d[e];
-                                mx = fmaxf(mx, fabsf(rn[e]));
-                            }
-                            vn += potVk<T, DIM, POT>(rn, P);
-                            vo += potVk<T, DIM, POT>(ro, P);
-                        }
-                    }
-                    if (mx >= P.L) {
-                        wrap |= 1u << r;
-                        vn = T(0);
-#pragma unroll
-                        for (int jj = 0; jj < MAXL; ++jj) {
-                            T rn[DIM];
-#pragma unroll
-                            for (int e = 0; e < DIM; ++e) rn[e] = teleport(x0[e * R.Mp + jj] + d[e], P);
-                            vn += potVk<T, DIM, POT>(rn, P);
-                        }
-                    }
+                    bool wr = false;
+                    const T dvc = com_chunk_full<DIM, MAXL, POT>(x0, R, d, P, wr);
+                    if (wr) wrap |= 1u << r;
                     if (!grouped) {
-                        s_red[it] = vn - vo;
+                        s_red[it] = dvc;
                         continue;
                     }
                     // the nseg chunks of particle q are an aligned lane group of this
                     // wave: every lane gets the same total (each butterfly step adds
                     // the same two numbers in both partner lanes), decides, and moves
                     // its own chunk
-                    const T tot = group_sum_dpp(vn - vo, nseg);
+                    const T tot = group_sum_dpp(dvc, nseg);
                     const bool acc = metropolis(ex(-P.tau * tot), s_u[q]);
                     if (acc && k == 0) s_cacc[q] += 1;
                     if (acc) {

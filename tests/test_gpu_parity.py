@@ -76,7 +76,7 @@ def test_levy_propose_harmonic(precision):
         dS_gpu = w_gpu[1] - w_gpu[0]
         dS_or = w_or.wV_updated - w_or.wV_initial
         scale = abs(w_or.wV_updated) + abs(w_or.wV_initial) + 1e-300
-        assert abs(dS_gpu - dS_or) <= RTOL[precision] * scale * 10, (trial, dS_gpu, dS_or)
+        assert abs(dS_gpu - dS_or) <= RTOL[precision] * scale, (trial, dS_gpu, dS_or)
 
 
 @pytest.mark.parametrize("precision", [A.FP64, A.FP32])
@@ -117,7 +117,7 @@ def test_com_propose(precision):
         r_or, w_or = osys.com_propose(n, disp)
         assert close(r_gpu, r_or, RTOL[precision])
         scale = abs(w_or.wV_updated) + abs(w_or.wV_initial)
-        assert abs((w_gpu[1] - w_gpu[0]) - (w_or.wV_updated - w_or.wV_initial)) <= 10 * RTOL[precision] * scale
+        assert abs((w_gpu[1] - w_gpu[0]) - (w_or.wV_updated - w_or.wV_initial)) <= RTOL[precision] * scale
 
 
 def test_reshape_and_com_apply_fp64():
@@ -145,6 +145,64 @@ def test_reshape_and_com_apply_fp64():
         n_acc += a
     assert 20 < n_acc < 200
     assert np.max(np.abs(ctx.get_paths(0) - osys.r)) < 1e-11
+
+
+@pytest.mark.parametrize("pot", ["harmonic", "lattice"])
+@pytest.mark.parametrize("ell,M", [(16, 64), (16, 256), (32, 128), (8, 40)])
+def test_fp32_fast_path_moves_with_injected_draws(pot, ell, M):
+    """FP32 tile chains whose staging length ell (a power of two) divides M run
+    the batched kernel's fast paths: levy_full_propose (compile-time bridge
+    coefficients, no wrap inside the box, harmonic Delta V from sums of
+    |new|^2 - |old|^2) and com_chunk_full (harmonic Delta V from the chunk sums,
+    lane-group butterfly over the chunks when M / ell is a power of two).  The
+    single-move hooks run exactly that code with injected normals and shifts:
+    bridge beads to 1e-5 relative (of max(1, |r|)) and Delta S to 1e-5 of
+    sum |terms| against the oracle, in the C2 box (L = 100) and a periodic
+    lattice box; the committed move writes the proposed beads."""
+    rng = np.random.default_rng(211)
+    N = 3
+    if pot == "harmonic":
+        d = chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=N, L=100.0, beta=2.0, lam=0.5, ell=ell, com_step=1.0)
+        r = rand_paths(rng, N, M, 100.0)
+    else:
+        d = chain_desc(SquareLattice(4.0, 1.0), dim=2, M=M, N=N, L=3.0, beta=4.0, lam=0.5, ell=ell, com_step=0.5)
+        r = rng.uniform(-3.0, 3.0, size=(N, 2, M))
+    ctx = make_ctx([d], A.FP32)
+    ctx.set_paths(0, r)
+    rr = ctx.get_paths(0)
+    osys = oracle_sys(d, rr)
+    L = d.L
+    worst_r = worst_s = 0.0
+    for trial in range(30):
+        n, j0 = int(rng.integers(N)), int(rng.integers(M))
+        nrm = rng.standard_normal((ell - 1, 2))
+        r_gpu, w_gpu = ctx.levy_propose(0, n, j0, ell, nrm)
+        r_or, w_or = osys.reshape_propose(n, j0, ell, nrm)
+        diff = np.abs(r_gpu - r_or)
+        diff = np.minimum(diff, 2 * L - diff)  # a bead at the wall may sit at -L in one, +L in the other
+        worst_r = max(worst_r, float(np.max(diff / np.maximum(1.0, np.abs(r_or)))))
+        scale = abs(w_or.wV_updated) + abs(w_or.wV_initial) + 1e-30
+        worst_s = max(worst_s, abs((w_gpu[1] - w_gpu[0]) - (w_or.wV_updated - w_or.wV_initial)) / scale)
+    for trial in range(20):
+        n = int(rng.integers(N))
+        disp = rng.uniform(-1, 1, size=2)
+        r_gpu, w_gpu = ctx.com_propose(0, n, disp)
+        r_or, w_or = osys.com_propose(n, disp)
+        diff = np.abs(r_gpu - r_or)
+        diff = np.minimum(diff, 2 * L - diff)
+        worst_r = max(worst_r, float(np.max(diff / np.maximum(1.0, np.abs(r_or)))))
+        scale = abs(w_or.wV_updated) + abs(w_or.wV_initial) + 1e-30
+        worst_s = max(worst_s, abs((w_gpu[1] - w_gpu[0]) - (w_or.wV_updated - w_or.wV_initial)) / scale)
+    print(pot, ell, M, "bridge/COM beads", worst_r, "Delta S", worst_s)
+    assert worst_r <= 1e-5 and worst_s <= 1e-5, (worst_r, worst_s)
+    # committing: an accepted staging move (u = 0) writes the proposed interior beads
+    n, j0 = 1, 5
+    nrm = rng.standard_normal((ell - 1, 2)) * 0.1
+    r_prop, _ = ctx.levy_propose(0, n, j0, ell, nrm)
+    assert ctx.reshape_linear(0, n, j0, ell, nrm, 0.0)
+    after = ctx.get_paths(0)
+    for i in range(1, ell):
+        assert np.array_equal(after[n, :, (j0 + i) % M], r_prop[i].astype(np.float32).astype(np.float64))
 
 
 @pytest.mark.parametrize("precision", [A.FP64, A.FP32])
@@ -214,11 +272,26 @@ def test_pair_moves_and_energy(precision):
     assert abs(Evp - Evpo) <= tol * max(1.0, abs(Evpo)) * 10
 
 
+def oracle_meas(s):
+    """The kernels' energy record of one measurement on the oracle state:
+    (E, Ev, Epair) with the pair energy and pair virial added to the
+    reference's Energy (zero without a Gaussian pair)."""
+    e, ev = s.energy()
+    p, pv = s.pair_energy()
+    return e + p, ev + pv, p
+
+
+def block_means(vals, spl_meas):
+    """Means of consecutive groups of spl_meas measurements (one block each)."""
+    vals = np.asarray(vals, float)
+    return np.array([vals[i:i + spl_meas].mean() for i in range(0, len(vals), spl_meas)])
+
+
 def sweep_oracle(descs, inits, seed, n_sweeps, spl, measure=False, measure_every=1, sweep0=0):
     """Run the oracle's sweep schedule for every chain, with the COM-step
     adaptation between launches of `spl` sweeps when not measuring; when
-    measuring, the energies after the global sweeps s with (s + 1) %
-    measure_every == 0."""
+    measuring, the (E, Ev, Epair) of oracle_meas after the global sweeps s
+    with (s + 1) % measure_every == 0."""
     out = []
     for c, (d, r) in enumerate(zip(descs, inits)):
         s = oracle_sys(d, r)
@@ -237,7 +310,7 @@ def sweep_oracle(descs, inits, seed, n_sweeps, spl, measure=False, measure_every
                 ca += st.com_acc
                 sweep += 1
                 if measure and sweep % measure_every == 0:
-                    Es.append(s.energy())
+                    Es.append(oracle_meas(s))
             if not measure:
                 step = O.adapt_step(step, ca, d.N * k, d.L)
             left -= k
@@ -349,8 +422,18 @@ def test_sweep_trajectory_pair_1d_fp64(N, L, ra):
     ref = sweep_oracle([d], [r], seed, n_sweeps, spl, measure=True)
     assert cnt.levy_acc == ref[0][1] and cnt.com_acc == ref[0][2]
     assert np.max(np.abs(ctx.get_paths(0) - ref[0][0])) < 1e-9
-    E, Ev, Ep = ctx.blocks(0)
-    assert np.isfinite(E[0]) and np.isfinite(Ep[0])
+    assert_blocks(ctx, 0, ref[0][4], n_sweeps)
+
+
+def assert_blocks(ctx, c, meas, per_block, rtol=1e-9):
+    """The chain's estimator blocks (E, Ev, Epair) equal the block means of the
+    oracle's per-sweep energies (oracle_meas) to rtol of max(1, |value|)."""
+    got = ctx.blocks(c)
+    meas = np.asarray(meas, float)
+    assert len(got[0]) == len(meas) // per_block, (len(got[0]), len(meas))
+    for k, name in enumerate(("E", "Ev", "Epair")):
+        want = block_means(meas[:, k], per_block)
+        assert np.allclose(got[k], want, rtol=rtol, atol=rtol), (c, name, got[k], want)
 
 
 @pytest.mark.parametrize("case", ["trap", "trap_box", "lattice_box", "optical"])
@@ -400,7 +483,14 @@ def test_sweep_trajectory_fp32(case):
         diff = ctx.get_paths(c) - ref[c][0]
         wrapped += int(np.sum(np.abs(diff) > d.L))
         diff -= 2 * d.L * np.round(diff / (2 * d.L))  # a bead on the box edge may sit at -L in one, +L in the other
-        assert np.max(np.abs(diff)) < 2e-4 * max(1.0, d.L / 10), (c, np.max(np.abs(diff)))
+        rel = float(np.max(np.abs(diff) / np.maximum(1.0, np.abs(ref[c][0]))))
+        print(case, c, "max |r32 - r64| =", float(np.max(np.abs(diff))), "relative", rel, flush=True)
+        # the fast path (ell a power of two dividing M) leaves in-box beads unwrapped: FP32
+        # rounding of the bridge only.  The general path applies the reference's teleport,
+        # whose FP32 form ((x + L) - ...) - L rounds a bead to the spacing of L.
+        fast = d.ell in (8, 16, 32) and d.M % d.ell == 0
+        bound = 1e-5 if fast else 1e-5 + 2 * float(np.spacing(np.float32(d.L)))
+        assert np.all(np.abs(diff) <= bound * np.maximum(1.0, np.abs(ref[c][0]))), (c, rel, bound)
         E, Ev, _ = ctx.blocks(c)
         Eo = np.mean([e[0] for e in ref[c][4]])
         assert abs(E[0] - Eo) <= 1e-4 * max(1.0, abs(Eo)), (c, E[0], Eo)
@@ -425,11 +515,8 @@ def test_sweep_trajectory_pair_fp64(N, L, ra):
     ref = sweep_oracle([d], [r], seed, n_sweeps, spl, measure=True)
     assert cnt.levy_acc == ref[0][1] and cnt.com_acc == ref[0][2]
     assert np.max(np.abs(ctx.get_paths(0) - ref[0][0])) < 1e-9
-    E, Ev, Ep = ctx.blocks(0)
-    s = oracle_sys(d, ref[0][0])
-    # the last sweep's pair energy is part of the block: recompute the block from the oracle trajectory
-    Eo = np.mean([e[0] for e in ref[0][4]])
-    assert np.isfinite(E[0]) and np.isfinite(Ep[0])
+    assert_blocks(ctx, 0, ref[0][4], n_sweeps)
+    assert ctx.blocks(0)[2][0] > 0.0
 
 
 def test_batch_fp32_runs_and_counts():
@@ -777,7 +864,5 @@ def test_optical_lattice_sweeps_match_oracle_fp64():
     assert cnt.levy_acc == sum(x[1] for x in ref) and cnt.com_acc == sum(x[2] for x in ref)
     for c in range(2):
         assert np.max(np.abs(ctx.get_paths(c) - ref[c][0])) < 1e-9, c
-        E, Ev, Ep = ctx.blocks(c)
-        Eo = np.mean([e[0] for e in ref[c][4]])  # the reference's Energy: no pair term
-        assert abs((E[0] - Ep[0]) - Eo) < 1e-9 * max(1.0, abs(Eo)), (c, E[0], Ep[0], Eo)
+        assert_blocks(ctx, c, ref[c][4], n_sweeps)
     assert ctx.blocks(1)[2][0] > 0.0

@@ -19,6 +19,7 @@ from pimc import _capi as A
 from pimc.propagator import PropInt, build_prop_int
 from pimc.potentials import GaussianPair, HarmonicTrap
 from pimc.system import System, chain_desc
+from test_gpu_parity import assert_blocks, oracle_meas
 
 pytestmark = pytest.mark.gpu
 
@@ -124,15 +125,20 @@ def test_table_sweeps_match_oracle_fp64(N, L, ra):
             s = O.System(inits[c], d.lam, d.L, d.beta, pot, O.make_pair(d.pair, d.g, d.sigma), d.r_a, True)
         else:
             s = oracle_table_sys(d, inits[c], tab, r0, r1, mu)
+        meas = []
         for sw in range(n_sweeps):
             st = s.sweep(c, sw, seed, d.ell, d.com_step)
             la += st.levy_acc
             ca += st.com_acc
+            meas.append(oracle_meas(s))
         assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
+        # one block of n_sweeps measurements; the table chain has no pair energy, so its E is
+        # the reference's Energy
+        assert_blocks(ctx, c, meas, n_sweeps)
     assert cnt.levy_acc == la and cnt.com_acc == ca
     assert 0 < cnt.levy_acc < cnt.levy_tries
     E, Ev, Ep = ctx.blocks(1)
-    assert np.isfinite(E[0]) and Ep[0] == 0.0
+    assert Ep[0] == 0.0 and ctx.blocks(0)[2][0] > 0.0
 
 
 @pytest.mark.parametrize("N,L,ra,hs_a", [(30, 3.0, 1.5, 0.3), (110, 5.0, 2.0, 0.25), (240, 5.0, 2.0, 0.12)])
