@@ -227,6 +227,14 @@ int pimc_energy(pimc_ctx *ctx, int32_t chain, double *out);
  * column-major); *ndata += M */
 int pimc_density(pimc_ctx *ctx, int32_t chain, int32_t nbins, double *dens, int64_t *ndata);
 
+/* Density(s) as pimc_density, without allocating device memory.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight serving loops).  The
+ * counts go to a fixed device array of the library (one per GPU, 2 MiB, kept until the process
+ * ends) instead of a per-call allocation, so nbins^dim is at most 262144 (512 x 512 in 2D), else
+ * PIMC_ERR_ARG.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU (two
+ * host threads, two contexts) race. */
+int pimc_density_noalloc(pimc_ctx *ctx, int32_t chain, int32_t nbins, double *dens, int64_t *ndata);
+
 /* Adjoint of the potential action S_pot = sum_links tau/2 (V + V') + S_pair:
  * dS/d(beads) (N x dim x M), dS/domega and dS/dg on the current paths.
  * The adjoint kernel recomputes bead distances instead of storing them. */

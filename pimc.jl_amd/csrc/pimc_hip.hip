@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -18,7 +19,42 @@
 
 using namespace pimc;
 
+// Histogram counts of pimc_density_noalloc: a fixed device array instead of a
+// per-call allocation.  Each GPU holds its own copy until the process ends;
+// host code uses the address hipGetSymbolAddress returns for the current
+// device (kept per device in g_scratch), never the array's name, which on the
+// host is only a stand-in for the symbol.
+constexpr long long kScratchBins = 1ll << 18; // 2 MiB: nbins^dim up to 512 x 512
+static __device__ __align__(256) unsigned long long scratch[kScratchBins];
+
 namespace {
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 256;
+unsigned long long *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on device dev, which must be current;
+// resolved once per device.
+hipError_t scratch_address(int dev, unsigned long long **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = static_cast<unsigned long long *>(p);
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
 
 struct HipError {
     hipError_t e;
@@ -113,6 +149,7 @@ struct pimc_ctx {
     std::vector<Launch> events;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     long long cart_per_chain = 0;
+    unsigned long long *d_scratch = nullptr;   // this device's `scratch` (pimc_density_noalloc)
     RunState *d_rs = nullptr;                   // device sweep/block counters
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // captured launch step (thermalising, measuring)
     std::vector<double> hblocks; // host copy of the estimator blocks
@@ -694,6 +731,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         c->n_chains = n_chains;
         c->desc.assign(chains, chains + n_chains);
         HIPCHK(hipGetDevice(&c->dev));
+        HIPCHK(scratch_address(c->dev, &c->d_scratch));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&c->ev0));
         HIPCHK(hipEventCreate(&c->ev1));
@@ -1748,6 +1786,51 @@ int pimc_density(pimc_ctx *c, int32_t chain, int32_t nbins, double *dens, int64_
         if (ndata) *ndata += hc.M;
     } catch (const HipError &e) {
         dfree(d_h);
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_density_noalloc(pimc_ctx *c, int32_t chain, int32_t nbins, double *dens, int64_t *ndata)
+{
+    int rc = check_chain(c, chain);
+    if (rc) return rc;
+    if (nbins < 1 || !dens) return set_err(c, PIMC_ERR_ARG, "bad histogram");
+    const ChainDev &hc = c->ch[chain];
+    const long long nh = hc.dim == 1 ? (long long)nbins : (long long)nbins * nbins;
+    if (nh > kScratchBins)
+        return set_err(c, PIMC_ERR_ARG, "pimc_density_noalloc holds at most 262144 bins (nbins^dim; 512 x 512 in 2D): "
+                                        "use pimc_density for larger histograms");
+    if (!c->d_scratch) return set_err(c, PIMC_ERR_STATE, "no scratch address for this device");
+    try {
+        unsigned long long *d_h = c->d_scratch;
+        // the counts start at zero: cleared by a kernel (the array keeps the last call's counts)
+        const unsigned fb = (unsigned)std::min<long long>(1024, (nh + 255) / 256);
+        hipLaunchKernelGGL(k_zero_u64, dim3(fb), dim3(256), 0, c->stream, d_h, nh);
+        HIPCHK(hipGetLastError());
+        HookArgs h = hook_args(c, chain);
+        const int nb = std::max(1, std::min(1024, (hc.N * hc.M + 255) / 256));
+#define DENS(T, D, S) hipLaunchKernelGGL((k_hook_density<T, D, S>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h)
+        const bool sm = hc.sm != 0;
+        if (c->prec == PIMC_FP64) {
+            if (c->dim == 1) { if (sm) DENS(double, 1, true); else DENS(double, 1, false); }
+            else { if (sm) DENS(double, 2, true); else DENS(double, 2, false); }
+        } else {
+            if (c->dim == 1) { if (sm) DENS(float, 1, true); else DENS(float, 1, false); }
+            else { if (sm) DENS(float, 2, true); else DENS(float, 2, false); }
+        }
+#undef DENS
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        // counts to the caller's doubles through a fixed host buffer
+        unsigned long long buf[2048];
+        for (long long i0 = 0; i0 < nh; i0 += 2048) {
+            const long long cnt = std::min<long long>(2048, nh - i0);
+            HIPCHK(hipMemcpy(buf, d_h + i0, sizeof(unsigned long long) * (size_t)cnt, hipMemcpyDeviceToHost));
+            for (long long i = 0; i < cnt; ++i) dens[i0 + i] += (double)buf[i];
+        }
+        if (ndata) *ndata += hc.M;
+    } catch (const HipError &e) {
         return hip_err(c, e);
     }
     return PIMC_OK;

@@ -25,7 +25,8 @@ EXPORTS = [
     "pimc_get_com_step", "pimc_set_com_step", "pimc_get_ell", "pimc_prop_int_terms", "pimc_set_pair_table",
     "pimc_run", "pimc_get_estimates", "pimc_get_counters", "pimc_get_blocks", "pimc_get_radial", "pimc_get_cartesian",
     "pimc_reset_estimators", "pimc_levy_propose", "pimc_reshape_linear", "pimc_com_propose",
-    "pimc_center_of_mass", "pimc_interaction", "pimc_energy", "pimc_density", "pimc_adjoint", "pimc_action",
+    "pimc_center_of_mass", "pimc_interaction", "pimc_energy", "pimc_density", "pimc_density_noalloc",
+    "pimc_adjoint", "pimc_action",
 ]
 
 
@@ -122,6 +123,7 @@ def load(path: str = LIB_PATH):
     L.pimc_interaction.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, _D]
     L.pimc_energy.argtypes = [vp, C.c_int32, _D]
     L.pimc_density.argtypes = [vp, C.c_int32, C.c_int32, _D, _I64]
+    L.pimc_density_noalloc.argtypes = [vp, C.c_int32, C.c_int32, _D, _I64]
     L.pimc_adjoint.argtypes = [vp, C.c_int32, _D, _D, _D]
     L.pimc_action.argtypes = [vp, C.c_int32, _D]
     for name in EXPORTS:
@@ -310,6 +312,17 @@ class Context:
         buf = np.asfortranarray(dens, dtype=np.float64)
         nd = C.c_int64(ndata)
         self.check(self.lib.pimc_density(self.h, chain, nbins, buf.ctypes.data_as(_D), C.byref(nd)), "pimc_density")
+        return buf, nd.value
+
+    def density_noalloc(self, chain, nbins, dens=None, ndata=0):
+        """pimc_density_noalloc: Density(s) with the counts in the library's fixed device array."""
+        d = self.chains[chain]
+        if dens is None:
+            dens = np.zeros((nbins,) * d.dim, order="F")
+        buf = np.asfortranarray(dens, dtype=np.float64)
+        nd = C.c_int64(ndata)
+        self.check(self.lib.pimc_density_noalloc(self.h, chain, nbins, buf.ctypes.data_as(_D), C.byref(nd)),
+                   "pimc_density_noalloc")
         return buf, nd.value
 
     def adjoint(self, chain):
