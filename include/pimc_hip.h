@@ -173,6 +173,15 @@ int pimc_set_com_step(pimc_ctx *ctx, int32_t chain, double step);
  * changes only under pimc_options.adapt_ell */
 int pimc_get_ell(pimc_ctx *ctx, int32_t chain, int32_t *ell);
 
+/* Philox sweep counter of a context (the sweep word of every draw; pimc_run advances it).  A run
+ * resumed in a new context from saved paths (pimc_get_paths / pimc_set_paths), permutations
+ * (pimc_get_next / pimc_set_next), COM steps (pimc_get_com_step / pimc_set_com_step) and this
+ * counter continues the saved run's random streams exactly.  There is no reference counterpart (the
+ * reference has no checkpoint, only CSV output, examples/tools/savetools.jl:4-69); the counter is
+ * below 2^32. */
+int pimc_get_sweep(pimc_ctx *ctx, int64_t *sweep);
+int pimc_set_sweep(pimc_ctx *ctx, int64_t sweep);
+
 /* prop_rel_interpolate_terms (src/propagator.jl:36-70) on the device: terms[i * n + k] =
  * -term1 - term2 + term3 at (r_i, r_k), r_i = r0 + i (r1 - r0) / (n - 1), for contact strength g0
  * and time step tau; the k-integrals by a fixed Gauss-Legendre rule in place of quadgk.  The

@@ -55,7 +55,6 @@ __global__ void k_zero_u64(unsigned long long *p, long long n)
         p[i] = 0ull;
 }
 
-
 struct HipError {
     hipError_t e;
     const char *what;
@@ -1320,7 +1319,13 @@ int pimc_set_pair_table(pimc_ctx *c, int32_t chain, int32_t n, double r0, double
         h.tab_r0 = r0;
         h.tab_h = (r1 - r0) / (n - 1);
         h.mu = mu;
-        HIPCHK(hipMemcpy(c->d_ch + chain, &h, sizeof(ChainDev), hipMemcpyHostToDevice));
+        // only the table fields: the device copy's staging length may have been adapted (adapt_ell)
+        char *dc = reinterpret_cast<char *>(c->d_ch + chain);
+        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, tab), &h.tab, sizeof(h.tab), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, tab_n), &h.tab_n, sizeof(h.tab_n), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, tab_r0), &h.tab_r0, sizeof(h.tab_r0), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, tab_h), &h.tab_h, sizeof(h.tab_h), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, mu), &h.mu, sizeof(h.mu), hipMemcpyHostToDevice));
         if (c->prec == PIMC_FP64) {
             const Prm<double> pr = Prm<double>::make(h);
             HIPCHK(hipMemcpy(reinterpret_cast<Prm<double> *>(c->d_prm) + chain, &pr, sizeof(pr), hipMemcpyHostToDevice));
@@ -1328,6 +1333,31 @@ int pimc_set_pair_table(pimc_ctx *c, int32_t chain, int32_t n, double r0, double
             const Prm<float> pr = Prm<float>::make(h);
             HIPCHK(hipMemcpy(reinterpret_cast<Prm<float> *>(c->d_prm) + chain, &pr, sizeof(pr), hipMemcpyHostToDevice));
         }
+    } catch (const HipError &e) {
+        return hip_err(c, e);
+    }
+    return PIMC_OK;
+}
+
+int pimc_get_sweep(pimc_ctx *c, int64_t *sweep)
+{
+    if (!c) return PIMC_ERR_ARG;
+    if (!sweep) return set_err(c, PIMC_ERR_ARG, "sweep is null");
+    *sweep = (int64_t)c->sweep;
+    return PIMC_OK;
+}
+
+int pimc_set_sweep(pimc_ctx *c, int64_t sweep)
+{
+    if (!c) return PIMC_ERR_ARG;
+    if (sweep < 0 || sweep > 0xffffffffll) return set_err(c, PIMC_ERR_ARG, "sweep must be in [0, 2^32)");
+    try {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const unsigned v = (unsigned)sweep;
+        // the kernels read the counter from the device run state, so captured graphs stay valid
+        HIPCHK(hipMemcpy(reinterpret_cast<char *>(c->d_rs) + offsetof(RunState, sweep), &v, sizeof(v),
+                         hipMemcpyHostToDevice));
+        c->sweep = v;
     } catch (const HipError &e) {
         return hip_err(c, e);
     }

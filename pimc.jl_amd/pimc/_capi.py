@@ -22,7 +22,8 @@ FP32, FP64 = 0, 1
 EXPORTS = [
     "pimc_create", "pimc_destroy", "pimc_last_error", "pimc_device_info",
     "pimc_set_paths", "pimc_get_paths", "pimc_init_paths", "pimc_set_next", "pimc_get_next",
-    "pimc_get_com_step", "pimc_set_com_step", "pimc_get_ell", "pimc_prop_int_terms", "pimc_set_pair_table",
+    "pimc_get_com_step", "pimc_set_com_step", "pimc_get_ell", "pimc_get_sweep", "pimc_set_sweep",
+    "pimc_prop_int_terms", "pimc_set_pair_table",
     "pimc_run", "pimc_get_estimates", "pimc_get_counters", "pimc_get_blocks", "pimc_get_radial", "pimc_get_cartesian",
     "pimc_reset_estimators", "pimc_levy_propose", "pimc_reshape_linear", "pimc_com_propose",
     "pimc_center_of_mass", "pimc_interaction", "pimc_energy", "pimc_density", "pimc_density_noalloc",
@@ -106,6 +107,8 @@ def load(path: str = LIB_PATH):
     L.pimc_get_com_step.argtypes = [vp, C.c_int32, _D]
     L.pimc_set_com_step.argtypes = [vp, C.c_int32, C.c_double]
     L.pimc_get_ell.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.pimc_get_sweep.argtypes = [vp, _I64]
+    L.pimc_set_sweep.argtypes = [vp, C.c_int64]
     L.pimc_prop_int_terms.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _D]
     L.pimc_set_pair_table.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, _D]
     L.pimc_run.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(Counters)]
@@ -209,6 +212,15 @@ class Context:
         if t.ndim != 2 or t.shape[0] != t.shape[1]:
             raise PimcError("the pair table must be square")
         self.check(self.lib.pimc_set_pair_table(self.h, chain, t.shape[0], r0, r1, mu, dp(t)), "pimc_set_pair_table")
+
+    def sweep(self) -> int:
+        """The Philox sweep counter (pimc_get_sweep)."""
+        v = C.c_int64()
+        self.check(self.lib.pimc_get_sweep(self.h, C.byref(v)), "pimc_get_sweep")
+        return v.value
+
+    def set_sweep(self, sweep: int):
+        self.check(self.lib.pimc_set_sweep(self.h, int(sweep)), "pimc_set_sweep")
 
     def ell(self, chain: int) -> int:
         x = C.c_int32()

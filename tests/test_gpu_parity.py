@@ -866,3 +866,68 @@ def test_optical_lattice_sweeps_match_oracle_fp64():
         assert np.max(np.abs(ctx.get_paths(c) - ref[c][0])) < 1e-9, c
         assert_blocks(ctx, c, ref[c][4], n_sweeps)
     assert ctx.blocks(1)[2][0] > 0.0
+
+
+def test_resume_from_checkpoint_is_exact_fp64():
+    """Checkpoint and resume: a run of 6 + 4 thermalising sweeps in one context
+    equals 6 sweeps, then a new context given the saved paths, permutation,
+    COM steps and Philox sweep counter and run for 4 sweeps, bit for bit
+    (tile chain, cell-list pair chain, permutation-cycle chain); without the
+    counter the resumed run replays the first sweeps' random streams."""
+    rng = np.random.default_rng(401)
+    pair = GaussianPair(1.0, 0.5)
+    descs = [harmonic_desc(N=5, M=32, beta=2.0, ell=8),
+             chain_desc(HarmonicTrap(0.3), dim=2, M=16, N=110, L=5.0, beta=2.0, lam=0.5, pair=pair, ell=8, r_a=2.0,
+                        com_step=0.5),
+             chain_desc(HarmonicTrap(1.0), dim=2, M=12, N=4, L=100.0, beta=2.0, lam=0.5, ell=4, com_step=0.8, swaps=3,
+                        swap_m=4)]
+    inits = [rand_paths(rng, 5, 32, 100.0), rng.uniform(-5.0, 5.0, size=(110, 2, 16)),
+             rand_paths(rng, 4, 12, 100.0, scale=0.6)]
+    full = make_ctx(descs, A.FP64, seed=33, spl=2)
+    for c, r in enumerate(inits):
+        full.set_paths(c, r)
+    full.run(6, measure=False)
+    assert full.sweep() == 6
+    saved = dict(paths=[full.get_paths(c) for c in range(3)], steps=[full.com_step(c) for c in range(3)],
+                 nxt=full.get_next(2), sweep=full.sweep())
+    full.run(4, measure=False)
+    for use_counter in (True, False):
+        res = make_ctx(descs, A.FP64, seed=33, spl=2)
+        for c in range(3):
+            res.set_paths(c, saved["paths"][c])
+            res.set_com_step(c, saved["steps"][c])
+        res.set_next(2, saved["nxt"])
+        if use_counter:
+            res.set_sweep(saved["sweep"])
+        res.run(4, measure=False)
+        same = all(np.array_equal(res.get_paths(c), full.get_paths(c)) for c in range(3))
+        assert same == use_counter
+        if use_counter:
+            assert np.array_equal(res.get_next(2), full.get_next(2))
+            assert all(res.com_step(c) == full.com_step(c) for c in range(3))
+            assert res.sweep() == 10
+        res.close()
+    with pytest.raises(A.PimcError):
+        full.set_sweep(-1)
+
+
+def test_set_pair_table_keeps_the_adapted_staging_length():
+    """pimc_set_pair_table replaces only the table of a chain: a staging
+    length adapted on the device (adapt_ell) survives a new table."""
+    from pimc.propagator import PropInt
+    rng = np.random.default_rng(403)
+    tau = 2.0 / 24
+    terms = O.prop_int_terms(48, 1e-20, 6.0, 2.0, tau)
+    p = PropInt(terms, 1e-20, 6.0, 2.0, tau)
+    d = chain_desc(HarmonicTrap(0.5), dim=2, M=24, N=16, L=4.0, beta=2.0, lam=0.5, pair=p, r_a=1.0, ell=4,
+                   com_step=0.5)
+    ctx = make_ctx([d], A.FP64, seed=9, spl=2, adapt_ell=1)
+    ctx.set_paths(0, rng.uniform(-3.0, 3.0, size=(16, 2, 24)))
+    ctx.set_pair_table(0, terms, 1e-20, 6.0, 0.0)
+    ctx.run(20, measure=False)
+    ell = ctx.ell(0)
+    assert ell != 4  # the length moved (staging acceptance of short segments is high)
+    ctx.set_pair_table(0, terms, 1e-20, 6.0, 0.0)
+    assert ctx.ell(0) == ell
+    cnt = ctx.run(2, measure=False)
+    assert cnt.levy_tries > 0
