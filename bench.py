@@ -35,6 +35,10 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_MOVE_FP32 = 16  # read + write of one 2D FP32 bead
 BYTES_PER_MOVE_FP64 = 32
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4 cycles
+# (16 lanes per cycle, packed FP32 off) at the 2.4 GHz peak engine clock
+VALU_PEAK_LANE_OPS = 1024 * 16 * 2.4e9
+MIN_TIMED_LAUNCHES = 10
 
 
 def parse():
@@ -59,6 +63,8 @@ def parse():
     ap.add_argument("--workload", default="trapped_batch",
                     choices=["trapped_batch", "trapped_plain", "gauss_box", "trapped_gauss", "lattice", "contact_box"],
                     help="workload of the timed sweeps (the default is the headline batched form)")
+    ap.add_argument("--check-sweeps", type=int, default=2000,
+                    help="untimed measuring sweeps after the timed ones (at least 8 blocks), for the energy check")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -67,6 +73,47 @@ def parse():
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def rounded_steps(requested: int, spl: int, least: int) -> int:
+    """Sweeps actually run: whole launches of spl sweeps, at least `least` launches."""
+    return max(least, -(-max(0, requested) // spl)) * spl
+
+
+def roofline_from_profile(name: str, moves_per_launch: float, avg_launch_s: float, bytes_per_move: int):
+    """The roofline of the dominant kernel from its committed counter summary
+    (profiles/traffic_<name>.json): VALU issue, the bound of these LDS-resident
+    sweeps, with the instruction count per bead-move from SQ_INSTS_VALU and the
+    launch time measured live; HBM traffic from FETCH_SIZE / WRITE_SIZE, both
+    scaled to this launch's bead-moves.  The algorithmic-bytes model (16 B per
+    bead-move in FP32) is kept as an informational field: the beads stay in LDS
+    for the sweeps of a launch, so those bytes never reach HBM."""
+    hbm_alg = bytes_per_move * moves_per_launch / avg_launch_s / 1e9
+    info = {"achieved": hbm_alg, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_alg / HBM_PEAK_GBS,
+            "note": f"{bytes_per_move} B per bead-move (read + write of its coordinates); LDS-resident, not "
+                    "HBM traffic"}
+    tp = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    try:
+        with open(tp) as f:
+            tj = json.load(f)
+        per_move = tj["valu_wave_instructions_per_launch"] * 64.0 / tj["bead_moves_per_launch"]
+        scale = moves_per_launch / tj["bead_moves_per_launch"]
+        achieved = per_move * moves_per_launch / avg_launch_s
+        return {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_LANE_OPS / 1e12,
+                "unit": "T VALU lane-instructions/s", "frac": achieved / VALU_PEAK_LANE_OPS,
+                "traffic": tj["hbm_bytes_per_launch"] * scale,
+                "kernel": tj.get("kernel"),
+                "valu_lane_instructions_per_bead_move": per_move,
+                "counters_from": os.path.relpath(tp, ROOT),
+                "note": "achieved = SQ_INSTS_VALU x 64 per bead-move (counter run of the same configuration) x "
+                        "bead-moves per launch / live launch time; peak = 1024 SIMDs x 16 lanes x 2.4 GHz; "
+                        "traffic = scaled FETCH_SIZE + WRITE_SIZE bytes per launch",
+                "hbm_algorithmic": info}
+    except (OSError, ValueError, KeyError, ZeroDivisionError):
+        return {"bound": "valu", "achieved": None, "peak": VALU_PEAK_LANE_OPS / 1e12,
+                "unit": "T VALU lane-instructions/s", "frac": None, "traffic": None,
+                "note": f"no counter summary {os.path.relpath(tp, ROOT)} for this workload",
+                "hbm_algorithmic": info}
 
 
 def energy_check(batch, wl):
@@ -169,8 +216,8 @@ def other_workload(args, A):
     b = Batch(wl, sweeps_per_launch=spl, seed=99, tile_beads=args.tile_beads, block_threads=args.threads,
               measure_kernel=args.measure_kernel)
     log(f"{args.workload}: {len(wl.descs)} chains, setup {time.perf_counter() - t:.1f} s")
-    warm = max(spl, (args.warmup // spl) * spl)
-    steps = max(spl, (args.steps // spl) * spl)
+    warm = rounded_steps(args.warmup, spl, 1)
+    steps = rounded_steps(args.steps, spl, MIN_TIMED_LAUNCHES)
     cw = b.run(warm, measure=False)
     t0 = time.perf_counter()
     cnt = b.run(steps, measure=True)
@@ -178,19 +225,19 @@ def other_workload(args, A):
     value = cnt.bead_moves / wall
     launches = max(1, cnt.kernel_launches)
     bpm = BYTES_PER_MOVE_FP64 if wl.precision == A.FP64 else BYTES_PER_MOVE_FP32
-    achieved = bpm * cnt.bead_moves / launches / (cnt.kernel_ms / 1000.0 / launches) / 1e9
+    roof = roofline_from_profile(args.workload, cnt.bead_moves / launches, cnt.kernel_ms / 1000.0 / launches, bpm)
     est = [b.estimates(c) for c in range(len(wl.descs))]
     E = np.array([e.E for e in est])
     Ev = np.array([e.Ev for e in est])
     out = {
         "metric": "bead-moves/sec at 1 GPU, batched vs plain, E within 2σ of FP64 ref; % HBM peak",
         "value": value, "unit": "bead-moves/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "steps_requested": args.steps, "warmup_requested": args.warmup, "launches": cnt.kernel_launches,
         "ms_per_step": 1000.0 * wall / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if wl.precision == A.FP64 else "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {WORKLOAD_TEXT[args.workload]}", "chains": len(wl.descs),
                    "sweeps_per_launch": spl, "bead_moves_per_sweep": int(cnt.bead_moves // steps)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": roof,
         "energy": {"E_mean_over_chains": float(E.mean()), "E_sem": float(E.std(ddof=1) / math.sqrt(len(E)))
                    if len(E) > 1 else float(est[0].E_err),
                    "Ev_mean_over_chains": float(Ev.mean()),
@@ -251,42 +298,28 @@ def main():
         ells, counts = np.unique(batch.tune.ell, return_counts=True)
         log(f"autotune ({batch.tune.seconds:.1f} s): tile_beads {batch.tune.tile_beads}, threads "
             f"{batch.tune.block_threads}, staging lengths {dict(zip(ells.tolist(), counts.tolist()))}")
-    warm = max(spl, (args.warmup // spl) * spl)
-    steps = max(spl, (args.steps // spl) * spl)
+    warm = rounded_steps(args.warmup, spl, 1)
+    steps = rounded_steps(args.steps, spl, MIN_TIMED_LAUNCHES)
     batch.run(warm, measure=False)
-    log(f"warmup {warm} sweeps done")
+    # two untimed measuring launches: the second captures the measuring launch step into a
+    # hipGraph, so every timed launch is a graph replay
+    batch.run(2 * spl, measure=True)
+    log(f"warmup {warm} sweeps and 2 measuring launches done")
     t0 = time.perf_counter()
     cnt = batch.run(steps, measure=True)
     wall = time.perf_counter() - t0
-    log(f"timed {steps} sweeps: wall {wall:.3f} s, device {cnt.wall_ms:.1f} ms, kernels {cnt.kernel_ms:.1f} ms")
+    log(f"timed {steps} sweeps in {cnt.kernel_launches} launches: wall {wall:.3f} s, device {cnt.wall_ms:.1f} ms, "
+        f"kernels {cnt.kernel_ms:.1f} ms")
     value = cnt.bead_moves / wall
     ms_per_step = 1000.0 * wall / steps
     launches = max(1, cnt.kernel_launches)
     avg_launch_s = cnt.kernel_ms / 1000.0 / launches
-    bytes_per_launch = BYTES_PER_MOVE_FP32 * cnt.bead_moves / launches
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    # HBM bytes and VALU counters of one launch of this configuration, from the committed
-    # rocprofv3 runs (counter collection serialises the kernels, so it is not done here)
-    traffic, valu = None, None
-    tp = os.path.join(ROOT, "profiles", "traffic_trapped_batch.json")
-    if os.path.exists(tp):
-        try:
-            with open(tp) as f:
-                tj = json.load(f)
-            if tj.get("sweeps_per_launch") == spl and tj.get("chains") == args.chains \
-                    and tj.get("staging_length") == (batch.tune.ell[0] if batch.tune and len(set(batch.tune.ell)) == 1
-                                                     else args.ell or None):
-                traffic = tj.get("hbm_bytes_per_launch")
-                issue_ms = tj["valu_wave_instructions_per_launch"] * 4 / 1024 / 2.4e9 * 1e3
-                valu = {"bound": "valu",
-                        "wave_instructions_per_launch": tj["valu_wave_instructions_per_launch"],
-                        "issue_ms_per_launch": issue_ms,
-                        "frac_of_launch": issue_ms / (avg_launch_s * 1e3),
-                        "busy_pct_counter": tj.get("valu_busy_pct"),
-                        "note": "4 cycles per wave64 VALU instruction on each of 1024 SIMDs at 2.4 GHz"}
-        except (OSError, ValueError, KeyError):
-            traffic, valu = None, None
+    roof = roofline_from_profile("trapped_batch", cnt.bead_moves / launches, avg_launch_s, BYTES_PER_MOVE_FP32)
+    # untimed measuring sweeps, so that every chain has enough blocks for its error bar
+    check = rounded_steps(args.check_sweeps, spl, 8)
+    batch.run(check, measure=True)
     echeck = energy_check(batch, wl)
+    echeck["blocks_per_chain"] = (2 * spl + steps + check) // spl
     acc = {"levy": cnt.levy_acc / max(1, cnt.levy_tries), "com": cnt.com_acc / max(1, cnt.com_tries)}
     batch.close()
 
@@ -316,6 +349,9 @@ def main():
         "n_gpus": 1,
         "steps": steps,
         "warmup": warm,
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "launches": cnt.kernel_launches,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -332,19 +368,12 @@ def main():
             "tile_beads": batch.tune.tile_beads if batch.tune else (args.tile_beads or 4096),
             "block_threads": batch.tune.block_threads if batch.tune else (args.threads or 256),
             "hipgraph": not args.no_graph,
+            "timed_launches_replayed": (not args.no_graph),
             "sweeps_per_launch": spl,
             "bead_moves_per_sweep": int(cnt.bead_moves // steps),
             "parallelism": "single GPU, one workgroup per tile of particles",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-        },
-        "valu_roofline": valu,
+        "roofline": roof,
         "cpu_baseline": cpu,
         "plain_gpu": {
             "workload": "trapped_plain: one chain N=10, M=64, beta=1, omega=1, FP64",
