@@ -1294,10 +1294,13 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
     double *nrm = (double *)malloc(sizeof(double) * (size_t)(M + 1) * dim);
     uint32_t x[4];
     /* Levy phase: particles in order; the segments of one particle have
-     * disjoint interiors and are independent given their end beads. */
+     * disjoint interiors and are independent given their end beads.  One
+     * staging offset per chain and sweep: the segments of all particles then
+     * share the same slice windows, and moves in different windows touch
+     * disjoint beads, so any order of the windows gives this result. */
+    keyed(seed, 0u, sweep, chain, PUR_OFFSET, x);
+    const int off = (int)(((uint64_t)x[0] * (uint64_t)ell) >> 32);
     for (int p = 0; p < N; ++p) {
-        keyed(seed, (uint32_t)p, sweep, chain, PUR_OFFSET, x);
-        int off = (int)(((uint64_t)x[0] * (uint64_t)ell) >> 32);
         for (int k = 0; k < nseg; ++k) {
             int m = (M - k * ell) < ell ? (M - k * ell) : ell;
             if (m < 2) continue;

@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libpimc_hip.so")
 SOURCES = ["pimc_hip.hip"]
-HEADERS = ["philox.hpp", "physics.hpp", "sweep_kernels.hpp", "hook_kernels.hpp"]
+HEADERS = ["philox.hpp", "physics.hpp", "sweep_kernels.hpp", "hook_kernels.hpp", "window_kernels.hpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

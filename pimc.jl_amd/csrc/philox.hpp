@@ -17,7 +17,7 @@ enum Purpose : uint32_t {
                      // item = particle * M + slice of bead i: words x, y for bead i, z, w for i + 1
     kLevyAccept = 2, // Metropolis uniform of one staging segment, item = particle * M + start slice
     kCom = 3,        // displacement (words 0..dim-1) and Metropolis uniform (word 2), item = particle
-    kOffset = 4,     // staging offset of one particle in one sweep, item = particle
+    kOffset = 4,     // staging offset of one chain in one sweep, item = 0
     kInitStart = 5,  // first bead of a particle's initial path (words 0..dim-1), item = particle
     kInitLevy = 6,   // normals of the initial path, pairs (i, i+1) with i odd, item = particle * M + i
     kSwap = 7,       // swap try a of a sweep: slice, first particle, partner uniform, Metropolis uniform; item = a

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -16,6 +17,7 @@
 #include "pimc_hip.h"
 #include "propagator_kernels.hpp"
 #include "sweep_kernels.hpp"
+#include "window_kernels.hpp"
 
 using namespace pimc;
 
@@ -105,6 +107,25 @@ struct pimc_ctx {
     std::vector<int> whole_cells_ids;  // whole chains with cell lists
     std::vector<int> whole_brute_ids;  // whole chains with brute-force neighbour sums
     std::vector<int> whole_lds_ids;    // ... whose path fits in LDS next to the kernel's buffers
+    // Gaussian-pair chains on the window staging and centre-of-mass kernels, with and without
+    // cell lists for the estimators (index 0: cells, 1: all-pairs); win_all: both
+    std::vector<int> win_ids[2], win_all;
+    int *d_win[2] = {nullptr, nullptr};
+    int *d_win_all = nullptr;
+    int win_seg = 0, win_maxN = 0, win_maxM = 0, win_maxell = 0, win_maxM0 = 0;
+    int cap_stage = 0, cap_cspec = 0, cap_cdec = 0;   // pair-slot pools of the three kernels
+    int com_chunk = 64;                               // centre-of-mass tries per decision chunk
+    size_t lds_stage = 0, lds_cspec = 0, lds_cdec = 0, lds_cells = 0;
+    int cells_wpg = 4, win_ncell = 0;                 // k_cells_win: waves per workgroup, largest cell count
+    double *d_spec = nullptr;                         // centre of mass: Delta V + sum A per (particle, window)
+    void *d_wbnd = nullptr;                           // window bounds (float4) per (particle, window)
+    int spec_nwin = 1;
+    unsigned char *d_cacc = nullptr;                  // centre of mass: accepted flag per particle
+    void *d_cshift = nullptr;                         // and its shift (2 per particle, arithmetic type)
+    void *d_pm = nullptr;                             // particle-major copy of the window chains' paths
+    std::vector<int> cells_all_ids;               // every chain with cell lists
+    int *d_cells_all = nullptr;
+    int n_meas_cells = 0;                         // meas_ids: the chains with cell lists come first
     // each of the three lists holds its Gaussian/pairless chains first, then from
     // this index the chains with the tabulated propagator (a kernel of their own)
     int cells_tab0 = 0, brute_tab0 = 0, lds_tab0 = 0;
@@ -267,13 +288,13 @@ void launch_whole(pimc_ctx *c, const SweepArgs &a)
 // Rebuild the cell lists of one chain (only >= 0) or of all cell-list chains.
 void build_cells(pimc_ctx *c, int only)
 {
-    if (c->whole_cells_ids.empty()) return;
+    if (c->cells_all_ids.empty()) return;
     int maxM = 0;
-    for (int id : c->whole_cells_ids) maxM = std::max(maxM, c->ch[id].M);
-    dim3 grid((maxM + 63) / 64, (unsigned)c->whole_cells_ids.size());
+    for (int id : c->cells_all_ids) maxM = std::max(maxM, c->ch[id].M);
+    dim3 grid((maxM + 63) / 64, (unsigned)c->cells_all_ids.size());
 #define BC(T, D)                                                                                                       \
-    hipLaunchKernelGGL((k_build_cells<T, D>), grid, dim3(64), 0, c->stream, c->d_ch, c->d_whole_cells,                 \
-                       (int)c->whole_cells_ids.size(), c->d_pos, c->d_cell_cnt, c->d_cell_mem, c->d_bins, c->d_status, only)
+    hipLaunchKernelGGL((k_build_cells<T, D>), grid, dim3(64), 0, c->stream, c->d_ch, c->d_cells_all,                   \
+                       (int)c->cells_all_ids.size(), c->d_pos, c->d_cell_cnt, c->d_cell_mem, c->d_bins, c->d_status, only)
     if (c->prec == PIMC_FP64) {
         if (c->dim == 1) BC(double, 1);
         else BC(double, 2);
@@ -441,7 +462,7 @@ template <typename T, int DIM> void launch_meas_d(pimc_ctx *c, int sweep_off)
     m.rs = c->d_rs;
     m.sweep_off = sweep_off;
     m.measure_every = c->opt.measure_every;
-    const int ncell = (int)c->whole_cells_ids.size(), nall = (int)c->meas_ids.size();
+    const int ncell = c->n_meas_cells, nall = (int)c->meas_ids.size();
     if (ncell > 0) {
         MeasArgs mc = m;
         mc.ids = c->d_meas_ids;
@@ -478,6 +499,89 @@ void launch_meas(pimc_ctx *c, int sweep_off)
     }
 }
 
+// One sweep of the Gaussian-pair chains on the window kernels: the staging
+// windows, then the centre-of-mass speculation, decisions and moves.
+template <typename T, int DIM> void launch_win_d(pimc_ctx *c, const SweepArgs &b)
+{
+    const int n = (int)c->win_all.size();
+    if (n == 0) return;
+    SweepArgs w = b;
+    w.whole = c->d_win_all;
+    hipLaunchKernelGGL((k_stage_win<T, DIM>), dim3((unsigned)c->win_seg, (unsigned)n), dim3(256), c->lds_stage,
+                       c->stream, w, c->cap_stage);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((k_com_spec<T, DIM>), dim3((unsigned)c->spec_nwin, (unsigned)n), dim3(256), c->lds_cspec,
+                       c->stream, w, c->d_spec, static_cast<float4 *>(c->d_wbnd), c->spec_nwin, c->cap_cspec,
+                       static_cast<T *>(c->d_pm));
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((k_com_dec<T, DIM>), dim3((unsigned)n), dim3(1024), c->lds_cdec, c->stream, w,
+                       (const double *)c->d_spec, static_cast<const float4 *>(c->d_wbnd), c->spec_nwin, c->cap_cdec,
+                       c->com_chunk, c->d_cacc, static_cast<T *>(c->d_cshift), static_cast<const T *>(c->d_pm));
+    HIPCHK(hipGetLastError());
+    const long long beads = (long long)c->win_maxM * c->win_maxN;
+    hipLaunchKernelGGL((k_com_apply<T, DIM>), dim3((unsigned)((beads + 255) / 256), (unsigned)n), dim3(256), 0,
+                       c->stream, w, (const unsigned char *)c->d_cacc, static_cast<const T *>(c->d_cshift));
+    HIPCHK(hipGetLastError());
+}
+
+void launch_win(pimc_ctx *c, const SweepArgs &b)
+{
+    if (c->prec == PIMC_FP64) {
+        if (c->dim == 1) launch_win_d<double, 1>(c, b);
+        else launch_win_d<double, 2>(c, b);
+    } else {
+        if (c->dim == 1) launch_win_d<float, 1>(c, b);
+        else launch_win_d<float, 2>(c, b);
+    }
+}
+
+// Cell lists of the window chains that have them, rebuilt from the positions.
+template <typename T, int DIM> void launch_cells_win_d(pimc_ctx *c)
+{
+    const int n = (int)c->win_ids[0].size();
+    if (n == 0) return;
+    const int wpg = c->cells_wpg;
+    hipLaunchKernelGGL((k_cells_win<T, DIM>), dim3((unsigned)((c->win_maxM0 + wpg - 1) / wpg), (unsigned)n),
+                       dim3(64 * wpg), c->lds_cells, c->stream, c->d_ch, c->d_prm, c->d_win[0], n, c->d_pos,
+                       c->d_cell_cnt, c->d_cell_mem, c->d_bins, c->d_status, c->win_ncell);
+    HIPCHK(hipGetLastError());
+}
+
+void launch_cells_win(pimc_ctx *c)
+{
+    if (c->prec == PIMC_FP64) {
+        if (c->dim == 1) launch_cells_win_d<double, 1>(c);
+        else launch_cells_win_d<double, 2>(c);
+    } else {
+        if (c->dim == 1) launch_cells_win_d<float, 1>(c);
+        else launch_cells_win_d<float, 2>(c);
+    }
+}
+
+// Dynamic LDS above 64 KiB for the window kernels of this context's type.
+template <typename T, int DIM> void win_attrs_d(pimc_ctx *c)
+{
+    auto set = [](const void *k, size_t lds) {
+        if (lds > 65536) HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    };
+    set((const void *)k_stage_win<T, DIM>, c->lds_stage);
+    set((const void *)k_com_spec<T, DIM>, c->lds_cspec);
+    set((const void *)k_com_dec<T, DIM>, c->lds_cdec);
+    set((const void *)k_cells_win<T, DIM>, c->lds_cells);
+}
+
+void win_attrs(pimc_ctx *c)
+{
+    if (c->win_all.empty()) return;
+    if (c->prec == PIMC_FP64) {
+        if (c->dim == 1) win_attrs_d<double, 1>(c);
+        else win_attrs_d<double, 2>(c);
+    } else {
+        if (c->dim == 1) win_attrs_d<float, 1>(c);
+        else win_attrs_d<float, 2>(c);
+    }
+}
+
 void launch_step(pimc_ctx *c, int k, int measure)
 {
     SweepArgs a = sweep_args(c);
@@ -485,19 +589,44 @@ void launch_step(pimc_ctx *c, int k, int measure)
     a.measure = measure ? 1 : 0;
     a.measure_every = c->opt.measure_every;
     if (!c->tiles.empty()) launch_tiles(c, a);
-    if (!c->whole_ids.empty()) {
-        if (measure && c->split_meas) {
-            // one sweep per launch of the whole-chain kernel, estimators in their own wide kernel
+    const bool win = !c->win_ids[0].empty() || !c->win_ids[1].empty();
+    if (win) {
+        // the window kernels add their counts into the launch's per-chain outputs
+        for (int v = 0; v < 2; ++v)
+            if (!c->win_ids[v].empty()) {
+                hipLaunchKernelGGL(k_clear_chain_meas, dim3((unsigned)c->win_ids[v].size()), dim3(32), 0, c->stream,
+                                   c->d_chain_meas, c->d_win[v], (int)c->win_ids[v].size());
+                HIPCHK(hipGetLastError());
+            }
+    }
+    if (!c->whole_ids.empty() || win) {
+        const bool old = !c->whole_ids.empty();
+        if (measure && (c->split_meas || win)) {
+            // one sweep per launch, estimators in their own wide kernel (the chains on the
+            // whole-chain kernel with in-kernel estimators run all k sweeps in one launch)
+            if (old && !c->split_meas) launch_whole(c, a);
             for (int s = 0; s < k; ++s) {
                 SweepArgs b = a;
                 b.nsweeps = 1;
                 b.sweep_off = s;
                 b.split = 1;
-                launch_whole(c, b);
+                if (old && c->split_meas) launch_whole(c, b);
+                if (win) {
+                    launch_win(c, b);
+                    launch_cells_win(c);
+                }
                 launch_meas(c, s);
             }
         } else {
-            launch_whole(c, a);
+            if (old) launch_whole(c, a);
+            for (int s = 0; win && s < k; ++s) {
+                SweepArgs b = a;
+                b.nsweeps = 1;
+                b.sweep_off = s;
+                b.split = 1;
+                launch_win(c, b);
+            }
+            if (win) launch_cells_win(c);
         }
     }
     PostArgs pa{};
@@ -706,6 +835,11 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 return PIMC_ERR_ARG;
             }
     }
+    if (opt->pair_kernel < 0 || opt->pair_kernel > 2) {
+        c->err = "pair_kernel must be 0, 1 or 2";
+        *out = c;
+        return PIMC_ERR_ARG;
+    }
     if (opt->measure_kernel < 0 || opt->measure_kernel > 2) {
         c->err = "measure_kernel must be 0, 1 or 2";
         *out = c;
@@ -828,7 +962,6 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             pbase += h.N;
             if (c->is_whole[id]) {
                 h.sm = 1; // slice-major: the beads of one slice together
-                c->whole_ids.push_back(id);
                 // cell lists when they beat the all-pairs scan and stay small
                 const long long ncell = (h.dim == 1) ? h.nbins : (long long)h.nbins * h.nbins;
                 const double mean = (double)h.N / (double)ncell;
@@ -836,6 +969,19 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 cap = (cap + 7) & ~7; // whole 16-byte chunks per member list
                 const long long bytes = (long long)h.M * ncell * (cap * 2 + 4);
                 const bool cells = h.pair != PIMC_PAIR_NONE && h.N > 96 && h.nbins >= 3 && bytes < (256ll << 20);
+                // Gaussian-pair chains (no swaps, hard spheres or staging-length adaptation) whose
+                // window state, with every window chain so far, fits in LDS run on the window kernels
+                const int tsz = (int)tsize(c);
+                bool win = h.pair == kPairGaussian && h.swaps == 0 && !(h.hs_a > 0.0) && !opt->adapt_ell &&
+                           h.N <= 4096 && h.M >= 3 && (!cells || ncell <= 16384) &&
+                           (opt->pair_kernel == 2 || (opt->pair_kernel == 0 && cells));
+                if (win) {
+                    const int N = std::max(c->win_maxN, h.N), ell = std::max(c->win_maxell, h.ell);
+                    const int lc = std::max(com_win_len(std::max(c->win_maxM, h.M)), 8);
+                    win = stage_lds(N, ell, h.dim, tsz, N).ck.end <= 160 * 1024 &&
+                          com_spec_lds(N, lc, h.dim, tsz, N).ck.end <= 160 * 1024 &&
+                          com_dec_lds(N, h.dim, tsz, N).ck.end <= 160 * 1024;
+                }
                 if (cells) {
                     h.ncell = (int)ncell;
                     h.cap = cap;
@@ -843,11 +989,28 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                     celloff += (long long)h.M * ncell;
                     h.bin_off = binoff;
                     binoff += (long long)h.N * h.M;
-                    c->whole_cells_ids.push_back(id);
+                    c->cells_all_ids.push_back(id);
                 } else {
                     h.ncell = 0;
                     h.cap = 0;
-                    c->whole_brute_ids.push_back(id);
+                }
+                if (win) {
+                    const int v = cells ? 0 : 1;
+                    c->win_ids[v].push_back(id);
+                    c->win_all.push_back(id);
+                    c->spec_nwin = std::max(c->spec_nwin, com_nwin(h.M));
+                    c->win_seg = std::max(c->win_seg, h.nseg);
+                    c->win_maxN = std::max(c->win_maxN, h.N);
+                    c->win_maxM = std::max(c->win_maxM, h.M);
+                    c->win_maxell = std::max(c->win_maxell, h.ell);
+                    if (cells) {
+                        c->win_maxM0 = std::max(c->win_maxM0, h.M);
+                        c->win_ncell = std::max(c->win_ncell, (int)ncell);
+                    }
+                } else {
+                    c->whole_ids.push_back(id);
+                    if (cells) c->whole_cells_ids.push_back(id);
+                    else c->whole_brute_ids.push_back(id);
                 }
             }
         }
@@ -959,19 +1122,27 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         // estimators of whole chains: in the sweep kernel, or (split) a wide kernel after each sweep;
         // automatic: split when a whole chain with a pair action has >= 8192 beads
         {
-            long long most = 0;
             bool big_pair = false;
             for (int id : c->whole_ids) {
                 const long long nb = (long long)c->ch[id].N * c->ch[id].M;
-                most = std::max(most, nb);
                 big_pair = big_pair || (c->ch[id].pair != kPairNone && nb >= 8192);
             }
             const int mk = c->opt.measure_kernel;
             c->split_meas = !c->whole_ids.empty() && (mk == 2 || (mk == 0 && big_pair));
+            // the wide estimator kernel: chains on the window kernels always, the others when split;
+            // chains with cell lists first
+            c->meas_ids.clear();
+            if (c->split_meas) c->meas_ids = c->whole_cells_ids;
+            c->meas_ids.insert(c->meas_ids.end(), c->win_ids[0].begin(), c->win_ids[0].end());
+            c->n_meas_cells = (int)c->meas_ids.size();
+            if (c->split_meas) {
+                c->meas_ids.insert(c->meas_ids.end(), c->whole_brute_ids.begin(), c->whole_brute_ids.end());
+                c->meas_ids.insert(c->meas_ids.end(), c->whole_lds_ids.begin(), c->whole_lds_ids.end());
+            }
+            c->meas_ids.insert(c->meas_ids.end(), c->win_ids[1].begin(), c->win_ids[1].end());
+            long long most = 0;
+            for (int id : c->meas_ids) most = std::max(most, (long long)c->ch[id].N * c->ch[id].M);
             c->meas_nblk = (int)std::max(1LL, (most + 1023) / 1024);
-            c->meas_ids = c->whole_cells_ids;
-            c->meas_ids.insert(c->meas_ids.end(), c->whole_brute_ids.begin(), c->whole_brute_ids.end());
-            c->meas_ids.insert(c->meas_ids.end(), c->whole_lds_ids.begin(), c->whole_lds_ids.end());
         }
         // ---- device allocations
         c->d_ch = dalloc<ChainDev>(n_chains);
@@ -1027,7 +1198,48 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         c->d_meas_ids = dalloc<int>(c->meas_ids.size());
         if (!c->meas_ids.empty())
             HIPCHK(hipMemcpy(c->d_meas_ids, c->meas_ids.data(), sizeof(int) * c->meas_ids.size(), hipMemcpyHostToDevice));
-        c->d_meas_part = dalloc<double>(c->split_meas ? c->meas_ids.size() * (size_t)c->meas_nblk * 6 : 1);
+        c->d_meas_part = dalloc<double>(!c->meas_ids.empty() ? c->meas_ids.size() * (size_t)c->meas_nblk * 6 : 1);
+        for (int v = 0; v < 2; ++v) {
+            c->d_win[v] = dalloc<int>(c->win_ids[v].size());
+            if (!c->win_ids[v].empty())
+                HIPCHK(hipMemcpy(c->d_win[v], c->win_ids[v].data(), sizeof(int) * c->win_ids[v].size(), hipMemcpyHostToDevice));
+        }
+        c->d_win_all = dalloc<int>(c->win_all.size());
+        if (!c->win_all.empty()) {
+            HIPCHK(hipMemcpy(c->d_win_all, c->win_all.data(), sizeof(int) * c->win_all.size(), hipMemcpyHostToDevice));
+            // pools: at least N slots (one particle's partners), at most a whole chunk's, and
+            // otherwise what keeps the staging and phase-1 kernels within the LDS budget
+            // (PIMC_WIN_LDS_KIB, default 48 KiB: three workgroups per CU) and the decision
+            // kernel (one workgroup per chain) within 96 KiB
+            const int N = c->win_maxN, ts_ = (int)ts, d = c->dim;
+            const int lc = std::max(com_win_len(c->win_maxM), 8);
+            size_t budget = 48 * 1024;
+            if (const char *e = std::getenv("PIMC_WIN_LDS_KIB")) budget = (size_t)std::max(8, std::atoi(e)) * 1024;
+            if (const char *e = std::getenv("PIMC_COM_CHUNK")) c->com_chunk = std::min(64, std::max(1, std::atoi(e)));
+            auto grow = [&](auto bytes, size_t limit) {
+                int cap = std::min(8192, kChunk * std::max(1, N - 1));
+                while (cap > N && bytes(cap) > limit) cap -= 64;
+                return std::max(cap, N);
+            };
+            c->cap_stage = grow([&](int k) { return stage_lds(N, c->win_maxell, d, ts_, k).ck.end; }, budget);
+            c->cap_cspec = grow([&](int k) { return com_spec_lds(N, lc, d, ts_, k).ck.end; }, budget);
+            c->cap_cdec = grow([&](int k) { return com_dec_lds(N, d, ts_, k).ck.end; }, 96 * 1024);
+            c->lds_stage = stage_lds(N, c->win_maxell, d, ts_, c->cap_stage).ck.end;
+            c->lds_cspec = com_spec_lds(N, lc, d, ts_, c->cap_cspec).ck.end;
+            c->lds_cdec = com_dec_lds(N, d, ts_, c->cap_cdec).ck.end;
+            c->cells_wpg = 4;
+            while (c->cells_wpg > 1 && (size_t)c->cells_wpg * c->win_ncell * 4 > 64 * 1024) c->cells_wpg >>= 1;
+            c->lds_cells = (size_t)c->cells_wpg * std::max(c->win_ncell, 1) * 4;
+            c->d_spec = dalloc<double>((size_t)c->n_part * c->spec_nwin);
+            HIPCHK(hipMalloc(&c->d_wbnd, (size_t)c->n_part * c->spec_nwin * sizeof(float4)));
+            c->d_cacc = dalloc<unsigned char>((size_t)c->n_part);
+            HIPCHK(hipMalloc(&c->d_cshift, (size_t)c->n_part * 2 * ts));
+            HIPCHK(hipMalloc(&c->d_pm, std::max<size_t>(1, (size_t)c->n_pos * ts)));
+            win_attrs(c);
+        }
+        c->d_cells_all = dalloc<int>(c->cells_all_ids.size());
+        if (!c->cells_all_ids.empty())
+            HIPCHK(hipMemcpy(c->d_cells_all, c->cells_all_ids.data(), sizeof(int) * c->cells_all_ids.size(), hipMemcpyHostToDevice));
         if (!c->whole_lds_ids.empty())
             HIPCHK(hipMemcpy(c->d_whole_lds, c->whole_lds_ids.data(), sizeof(int) * c->whole_lds_ids.size(), hipMemcpyHostToDevice));
         if (!c->whole_ids.empty())
@@ -1037,7 +1249,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         if (!c->whole_brute_ids.empty())
             HIPCHK(hipMemcpy(c->d_whole_brute, c->whole_brute_ids.data(), sizeof(int) * c->whole_brute_ids.size(), hipMemcpyHostToDevice));
         long long mem_slots = 0;
-        for (int id : c->whole_cells_ids) {
+        for (int id : c->cells_all_ids) {
             ChainDev &h = c->ch[id];
             h.mem_off = mem_slots;
             mem_slots += (long long)h.M * h.ncell * h.cap;
@@ -1083,6 +1295,7 @@ int pimc_destroy(pimc_ctx *c)
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
                     c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next, c->d_meas_ids, c->d_meas_part,
+                    c->d_win[0], c->d_win[1], c->d_cells_all, c->d_spec, c->d_wbnd, c->d_win_all, c->d_cacc, c->d_cshift, c->d_pm,
                     c->d_beams};
     for (void *p : ptrs) dfree(p);
     for (double *p : c->d_tab) dfree(p);
@@ -1209,7 +1422,7 @@ int pimc_init_paths(pimc_ctx *c, int32_t chain, uint64_t seed)
                 HIPCHK(hipMemsetAsync(c->d_status + chain, 0, sizeof(int), c->stream));
                 build_cells(c, chain);
             }
-        } else if (!c->whole_cells_ids.empty()) {
+        } else if (!c->cells_all_ids.empty()) {
             HIPCHK(hipMemsetAsync(c->d_status, 0, sizeof(int) * c->n_chains, c->stream));
             build_cells(c, -1);
         }

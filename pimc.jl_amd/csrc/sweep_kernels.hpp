@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
         for (int q = tid; q < np; q += nt) {
             const int gq = tl.q0 + q;
             const int c = a.part_chain[gq], p = a.part_idx[gq];
-            const U4 x = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kOffset);
+            const U4 x = draw(a.seed, 0u, sw, (uint32_t)c, kOffset); // one offset per chain and sweep
             s_off[q] = (int)(((uint64_t)x.x * (uint64_t)ell) >> 32);
             const U4 y = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kCom);
             const T step2 = (T)a.com_step[c] * T(2);
@@ -1476,10 +1476,10 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
         const bool meas_now = a.measure && !a.split && meas_in == 0;
         meas_in = (meas_in == 0) ? a.measure_every - 1 : meas_in - 1;
         // ---- staging: particles in order, segments (and links) of one particle in parallel
+        const U4 xo = draw(a.seed, 0u, sw, (uint32_t)c, kOffset); // one offset per chain and sweep
+        const int off = (int)(((uint64_t)xo.x * (uint64_t)ell) >> 32);
         for (int p = 0; p < N; ++p) {
             T *buf = (p & 1) ? s_new1 : s_new0;
-            const U4 xo = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kOffset);
-            const int off = (int)(((uint64_t)xo.x * (uint64_t)ell) >> 32);
             // (A1) normals of the interior beads, by slice
             for (int w = tid; w < nseg * hp; w += nt) {
                 const int k = w / hp, i = 2 * (w - k * hp) + 1;

@@ -52,6 +52,7 @@ class Options(C.Structure):
         ("tile_beads", C.c_int32), ("block_threads", C.c_int32),
         ("use_graph", C.c_int32), ("derivatives", C.c_int32), ("seed", C.c_uint64),
         ("measure_every", C.c_int32), ("adapt_ell", C.c_int32), ("measure_kernel", C.c_int32),
+        ("pair_kernel", C.c_int32),
     ]
 
 

@@ -140,11 +140,12 @@ class Batch:
     after every k-th sweep (the reference's measurement cycle, counted in
     sweeps); measure_kernel chooses where whole chains take their estimators
     (pimc_options.measure_kernel: 0 automatic, 1 in the sweep kernel, 2 a
-    wide kernel of their own)."""
+    wide kernel of their own); pair_kernel chooses the kernels of Gaussian-pair
+    chains (pimc_options.pair_kernel: 0 automatic, 1 whole-chain, 2 windows)."""
 
     def __init__(self, wl: Workload, sweeps_per_launch: int = 50, seed: int = 12345, tile_beads: int = 0,
                  block_threads: int = 0, autotune: bool = False, use_graph: bool = False, tune_sweeps: int = 100,
-                 measure_every: int = 1, measure_kernel: int = 0):
+                 measure_every: int = 1, measure_kernel: int = 0, pair_kernel: int = 0):
         import copy
         descs = [copy.copy(d) for d in wl.descs]
         self.tune = None
@@ -167,6 +168,7 @@ class Batch:
         opt.seed = seed
         opt.measure_every = measure_every
         opt.measure_kernel = measure_kernel
+        opt.pair_kernel = pair_kernel
         self.wl = wl
         self.descs = descs
         self.ctx = Context(descs, opt)

@@ -110,7 +110,7 @@ def test_counter_and_adjust_follow_the_reference():
 @pytest.mark.parametrize("case,msg", [
     ("table_without_ra", "r_a"), ("hs_without_table", "hard spheres"), ("hs_above_cell_side", "cell side"),
     ("negative_hs", "hs_a"), ("table_with_derivatives", "derivatives"), ("measure_kernel", "measure_kernel"),
-    ("pair_kind", "pair"),
+    ("pair_kernel", "pair_kernel"), ("pair_kind", "pair"),
 ])
 def test_table_and_hard_sphere_descriptors_are_checked_before_any_device_call(case, msg):
     from pimc.propagator import PropInt
@@ -131,6 +131,8 @@ def test_table_and_hard_sphere_descriptors_are_checked_before_any_device_call(ca
         opt.derivatives = 1
     elif case == "measure_kernel":
         opt.measure_kernel = 3
+    elif case == "pair_kernel":
+        opt.pair_kernel = 3
     elif case == "pair_kind":
         d.pair = 3
     with pytest.raises(A.PimcError) as e:

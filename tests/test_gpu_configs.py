@@ -35,12 +35,28 @@ from test_gpu_parity import assert_blocks, oracle_meas, oracle_sys
 pytestmark = pytest.mark.gpu
 
 
+def hotelling_pvalue(per, want, ncoarse=8):
+    """p-value of Hotelling's T^2 test that the chains' mean histogram (rows of
+    per: one independent chain each) equals want, over ncoarse groups of
+    adjacent bins with about equal expected counts (the bins of one chain are
+    correlated, so a per-bin chi^2 is not calibrated)."""
+    from scipy.stats import f
+    cw = np.cumsum(want) / want.sum()
+    groups = np.split(np.arange(len(want)), np.searchsorted(cw, np.linspace(0, 1, ncoarse + 1)[1:-1]))
+    X = np.stack([per[:, g].sum(1) for g in groups], 1)
+    mu = np.array([want[g].sum() for g in groups])
+    n, p = X.shape
+    d = X.mean(0) - mu
+    T2 = n * d @ np.linalg.solve(np.cov(X, rowvar=False), d)
+    return float(1.0 - f.cdf(T2 * (n - p) / (p * (n - 1)), p, n - p))
+
+
 def test_c1_radial_density_against_closed_form():
     """C1 in FP64, 32 chains x (2k + 10k sweeps): the 64-bin radial histogram
     per measurement against N M times the Rayleigh bin probabilities with
-    the primitive-action <x^2> = (1/M) sum_k 1/a_k per coordinate.  Chains
-    are independent: per bin the mean over chains and its standard error
-    give a chi^2 over the bins that hold at least half a bead."""
+    the primitive-action <x^2> = (1/M) sum_k 1/a_k per coordinate, by
+    Hotelling's T^2 over the independent chains; a profile of 3% wider <x^2>
+    is rejected."""
     d = chain_desc(HarmonicTrap(1.0), dim=2, M=64, N=10, L=100.0, beta=1.0, lam=0.5, ell=8, com_step=1.0)
     nch, nb, rmax = 32, 64, 4.0
     opt = A.Options()
@@ -60,15 +76,11 @@ def test_c1_radial_density_against_closed_form():
     per = np.array(per)
     x2 = AN.x2_primitive(64, 1.0)
     want = AN.radial_expected(10, 64, nb, rmax, x2)
-    mean, sem = per.mean(0), per.std(0, ddof=1) / math.sqrt(nch)
-    use = want > 0.5
-    chi2 = float(np.sum(((mean[use] - want[use]) / sem[use]) ** 2) / use.sum())
-    print("radial chi2/bin", chi2, "bins", int(use.sum()))
-    assert use.sum() > 30
-    assert chi2 < 1.9
-    # a profile of 3% wider <x^2> is rejected
-    wrong = AN.radial_expected(10, 64, nb, rmax, 1.03 * x2)
-    assert float(np.sum(((mean[use] - wrong[use]) / sem[use]) ** 2) / use.sum()) > 5.0
+    pv = hotelling_pvalue(per, want)
+    pw = hotelling_pvalue(per, AN.radial_expected(10, 64, nb, rmax, 1.03 * x2))
+    print("radial Hotelling p-value", pv, "with <x^2> 3% wider", pw)
+    assert pv > 1e-3
+    assert pw < 1e-6
 
 
 def test_c2_batched_autotuned_energies():
