@@ -6,7 +6,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -114,7 +113,6 @@ struct pimc_ctx {
     int *d_win_all = nullptr;
     int win_seg = 0, win_maxN = 0, win_maxM = 0, win_maxell = 0, win_maxM0 = 0;
     int cap_stage = 0, cap_cspec = 0, cap_cdec = 0;   // pair-slot pools of the three kernels
-    int com_chunk = 64;                               // centre-of-mass tries per decision chunk
     size_t lds_stage = 0, lds_cspec = 0, lds_cdec = 0, lds_cells = 0;
     int cells_wpg = 4, win_ncell = 0;                 // k_cells_win: waves per workgroup, largest cell count
     double *d_spec = nullptr;                         // centre of mass: Delta V + sum A per (particle, window)
@@ -507,16 +505,16 @@ template <typename T, int DIM> void launch_win_d(pimc_ctx *c, const SweepArgs &b
     if (n == 0) return;
     SweepArgs w = b;
     w.whole = c->d_win_all;
-    hipLaunchKernelGGL((k_stage_win<T, DIM>), dim3((unsigned)c->win_seg, (unsigned)n), dim3(256), c->lds_stage,
+    hipLaunchKernelGGL((k_stage_win<T, DIM>), dim3((unsigned)c->win_seg, (unsigned)n), dim3(512), c->lds_stage,
                        c->stream, w, c->cap_stage);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_com_spec<T, DIM>), dim3((unsigned)c->spec_nwin, (unsigned)n), dim3(256), c->lds_cspec,
+    hipLaunchKernelGGL((k_com_spec<T, DIM>), dim3((unsigned)c->spec_nwin, (unsigned)n), dim3(512), c->lds_cspec,
                        c->stream, w, c->d_spec, static_cast<float4 *>(c->d_wbnd), c->spec_nwin, c->cap_cspec,
                        static_cast<T *>(c->d_pm));
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((k_com_dec<T, DIM>), dim3((unsigned)n), dim3(1024), c->lds_cdec, c->stream, w,
                        (const double *)c->d_spec, static_cast<const float4 *>(c->d_wbnd), c->spec_nwin, c->cap_cdec,
-                       c->com_chunk, c->d_cacc, static_cast<T *>(c->d_cshift), static_cast<const T *>(c->d_pm));
+                       kChunk, c->d_cacc, static_cast<T *>(c->d_cshift), static_cast<const T *>(c->d_pm));
     HIPCHK(hipGetLastError());
     const long long beads = (long long)c->win_maxM * c->win_maxN;
     hipLaunchKernelGGL((k_com_apply<T, DIM>), dim3((unsigned)((beads + 255) / 256), (unsigned)n), dim3(256), 0,
@@ -1208,21 +1206,18 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         if (!c->win_all.empty()) {
             HIPCHK(hipMemcpy(c->d_win_all, c->win_all.data(), sizeof(int) * c->win_all.size(), hipMemcpyHostToDevice));
             // pools: at least N slots (one particle's partners), at most a whole chunk's, and
-            // otherwise what keeps the staging and phase-1 kernels within the LDS budget
-            // (PIMC_WIN_LDS_KIB, default 48 KiB: three workgroups per CU) and the decision
-            // kernel (one workgroup per chain) within 96 KiB
+            // otherwise what keeps the staging kernel within 64 KiB (two 512-thread workgroups
+            // per CU), the phase-1 kernel within 48 KiB (three) and the decision kernel (one
+            // workgroup per chain) within 96 KiB; measured on gauss_box and lattice
             const int N = c->win_maxN, ts_ = (int)ts, d = c->dim;
             const int lc = std::max(com_win_len(c->win_maxM), 8);
-            size_t budget = 48 * 1024;
-            if (const char *e = std::getenv("PIMC_WIN_LDS_KIB")) budget = (size_t)std::max(8, std::atoi(e)) * 1024;
-            if (const char *e = std::getenv("PIMC_COM_CHUNK")) c->com_chunk = std::min(64, std::max(1, std::atoi(e)));
             auto grow = [&](auto bytes, size_t limit) {
                 int cap = std::min(8192, kChunk * std::max(1, N - 1));
                 while (cap > N && bytes(cap) > limit) cap -= 64;
                 return std::max(cap, N);
             };
-            c->cap_stage = grow([&](int k) { return stage_lds(N, c->win_maxell, d, ts_, k).ck.end; }, budget);
-            c->cap_cspec = grow([&](int k) { return com_spec_lds(N, lc, d, ts_, k).ck.end; }, budget);
+            c->cap_stage = grow([&](int k) { return stage_lds(N, c->win_maxell, d, ts_, k).ck.end; }, 64 * 1024);
+            c->cap_cspec = grow([&](int k) { return com_spec_lds(N, lc, d, ts_, k).ck.end; }, 48 * 1024);
             c->cap_cdec = grow([&](int k) { return com_dec_lds(N, d, ts_, k).ck.end; }, 96 * 1024);
             c->lds_stage = stage_lds(N, c->win_maxell, d, ts_, c->cap_stage).ck.end;
             c->lds_cspec = com_spec_lds(N, lc, d, ts_, c->cap_cspec).ck.end;

@@ -144,8 +144,10 @@ template <typename T, int DIM> struct SliceTerm {
 // A(p, q) and, for q < p (back), C(p, q) over the links 0..m-1 of one window.
 // bead(t, who, ro, rn): old and new bead of particle p (who 0) or q (who 1) at
 // window slice t.  Slices are fetched B at a time before any is used (B > 1
-// lets loads from HBM overlap).  hit: some pairing has a term within the cut-off.
-template <typename T, int DIM, int B, typename BeadFn>
+// lets loads from HBM overlap).  ENDS: the old and new beads coincide at slices
+// 0 and m (staging), so one pairing is evaluated there.  hit: some pairing has
+// a term within the cut-off.
+template <typename T, int DIM, int B, bool ENDS = false, typename BeadFn>
 __device__ __forceinline__ void pair_window(BeadFn bead, int m, bool back, const Prm<T> &P, int nbins, T &A, T &C,
                                             bool &hit)
 {
@@ -168,10 +170,16 @@ __device__ __forceinline__ void pair_window(BeadFn bead, int m, bool back, const
             if (t > m) break;
             SliceTerm<T, DIM> oo1, no1, on1, nn1;
             oo1.eval(po[k], qo[k], P, nbins);
-            no1.eval(pn[k], qo[k], P, nbins);
-            if (back) {
-                on1.eval(po[k], qn[k], P, nbins);
-                nn1.eval(pn[k], qn[k], P, nbins);
+            if (ENDS && (t == 0 || t == m)) {
+                no1 = oo1;
+                on1 = oo1;
+                nn1 = oo1;
+            } else {
+                no1.eval(pn[k], qo[k], P, nbins);
+                if (back) {
+                    on1.eval(po[k], qn[k], P, nbins);
+                    nn1.eval(pn[k], qn[k], P, nbins);
+                }
             }
             if (t > 0) {
                 const T too = oo.link(oo1, ctau), tno = no.link(no1, ctau);
@@ -397,7 +405,7 @@ __host__ __device__ inline StageLds stage_lds(int N, int ell, int dim, int tsize
 
 // One staging window of one chain per workgroup: grid (windows, chains of the
 // list a.whole), one sweep (a.rs->sweep + a.sweep_off).
-template <typename T, int DIM> __global__ void __launch_bounds__(256) k_stage_win(SweepArgs a, int cap)
+template <typename T, int DIM> __global__ void __launch_bounds__(512) k_stage_win(SweepArgs a, int cap)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = a.whole[blockIdx.y];
@@ -529,7 +537,7 @@ template <typename T, int DIM> __global__ void __launch_bounds__(256) k_stage_wi
             const int p = pa + X.po[i], q = X.pq[i];
             T A, C;
             bool hit;
-            pair_window<T, DIM, 1>(bead(p, q), m, q < p, P, ch.nbins, A, C, hit);
+            pair_window<T, DIM, 1, true>(bead(p, q), m, q < p, P, ch.nbins, A, C, hit);
             X.pa[i] = A;
             X.pc[i] = C;
             if (q < p && hit) X.pq[i] = (unsigned short)(q | 0x8000);
@@ -608,7 +616,7 @@ __host__ __device__ inline ComSpecLds com_spec_lds(int N, int lc, int dim, int t
 // at the start of the phase; wbnd[part * nwin + w]: the bound of the
 // particle's old beads in the window.
 template <typename T, int DIM>
-__global__ void __launch_bounds__(256) k_com_spec(SweepArgs a, double *spec, float4 *wbnd, int nwin, int cap, T *pm)
+__global__ void __launch_bounds__(512) k_com_spec(SweepArgs a, double *spec, float4 *wbnd, int nwin, int cap, T *pm)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = a.whole[blockIdx.y];
