@@ -162,10 +162,14 @@ int pimc_device_info(char *name, int32_t len, int32_t *cu_count);
  * particle an M x dim column-major matrix, src/system.jl:1-6). */
 int pimc_set_paths(pimc_ctx *ctx, int32_t chain, const double *r);
 int pimc_get_paths(pimc_ctx *ctx, int32_t chain, double *r);
-/* initial paths on the device, replacing init_world (src/system.jl:36-78) with
- * a = 0: per particle the first bead uniform in [-L, L)^dim, the last bead equal
- * to it and levy! over the M - 1 links in between; Philox-keyed by (seed,
- * chain, particle).  chain = -1 initialises every chain.  Cell lists are rebuilt. */
+/* initial paths on the device, replacing init_world (src/system.jl:36-78): per
+ * particle the first bead uniform in [-L, L)^dim, the last bead equal to it and
+ * levy! over the M - 1 links in between; Philox-keyed by (seed, chain, particle,
+ * try).  For a chain with hard spheres (hs_a = a > 0) the particles are placed
+ * in order and a particle is drawn again while one of its beads lies closer than
+ * a to an earlier particle's bead at the same slice, at most 10 000 times
+ * (src/system.jl:44-64), else PIMC_ERR_CAPACITY.  chain = -1 initialises every
+ * chain.  Cell lists are rebuilt. */
 int pimc_init_paths(pimc_ctx *ctx, int32_t chain, uint64_t seed);
 /* permutation of one chain (Worldline.next, src/system.jl:1-6; next(), src/updates/helper.jl:287-300):
  * next[p] is the particle whose slice 0 follows slice M - 1 of particle p (0-based).  Identity

@@ -23,8 +23,11 @@ PAIR_NONE, PAIR_GAUSSIAN, PAIR_TABLE = 0, 1, 2
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile liboracle.so with the committed Makefile (make rebuilds it when a
+    source is newer; force rebuilds it regardless)."""
+    if force or not os.path.exists(_LIB_PATH) or \
+            any(os.path.getmtime(os.path.join(_HERE, f)) > os.path.getmtime(_LIB_PATH)
+                for f in ("pimc_oracle.c", "pimc_oracle.h", "Makefile") if os.path.exists(os.path.join(_HERE, f))):
         subprocess.run(["make", "-s", "-B" if force else "liboracle.so"], cwd=_HERE, check=True)
     return _LIB_PATH
 
@@ -86,6 +89,9 @@ def lib():
                                    C.c_int, C.c_int, C.c_int, C.POINTER(OSweepStats)]
         L.o_init_world.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                    C.c_double, _D]
+        L.o_init_world_hs.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, _D, C.POINTER(C.c_int)]
+        L.o_init_world_hs.restype = C.c_int
         L.o_box_muller.argtypes = [C.c_uint32, C.c_uint32, _D, _D]
         L.o_distance.argtypes = [C.c_double] * 3; L.o_distance.restype = C.c_double
         L.o_teleport.argtypes = [C.c_double] * 2; L.o_teleport.restype = C.c_double
@@ -452,6 +458,16 @@ def init_world(seed, chain, N, M, dim, L, tau, lam):
     out = np.zeros((N, dim, M))
     lib().o_init_world(seed, chain, N, M, dim, L, tau, lam, out.ctypes.data_as(_D))
     return out
+
+
+def init_world_hs(seed, chain, N, M, dim, L, tau, lam, a):
+    """o_init_world_hs: (paths N x dim x M, tries per particle, rc) of init_world
+    with hard spheres of radius a (rc -1: a particle found no place)."""
+    out = np.zeros((N, dim, M))
+    tries = np.zeros(N, dtype=np.int32)
+    rc = lib().o_init_world_hs(seed, chain, N, M, dim, L, tau, lam, a, out.ctypes.data_as(_D),
+                               tries.ctypes.data_as(C.POINTER(C.c_int)))
+    return out, tries, rc
 
 
 def adapt_step(step, acc, tries, L):

@@ -1581,39 +1581,83 @@ static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int
 }
 
 /* ------------------------------------------------------------------------ */
-/* init_world (src/system.jl:36-78) with a = 0, Philox-keyed                 */
+/* init_world (src/system.jl:36-78), Philox-keyed                            */
 /* ------------------------------------------------------------------------ */
-/* Each particle p: the first bead uniform in [-L, L)^dim (2L (u - 1/2)), the
- * last bead equal to it, and levy! over the M - 1 links in between.  Draws:
- * start = keyed(seed, p, 0, chain, PUR_INIT_START) words 0..dim-1; the
- * normals of interior bead i (1 <= i <= M - 2) come in pairs (i, i+1), i
- * odd, from keyed(seed, p*M + i, 0, chain, PUR_INIT_LEVY): Box-Muller of
- * words (x, y) for bead i and (z, w) for bead i + 1.  out: N x dim x M. */
-void o_init_world(uint64_t seed, uint32_t chain, int N, int M, int dim, double L, double tau, double lambda,
-                  double *out)
+/* One try of particle p: the first bead uniform in [-L, L)^dim (2L (u - 1/2)),
+ * the last bead equal to it, and levy! over the M - 1 links in between.
+ * Draws of try t: start = keyed(seed, p, t, chain, PUR_INIT_START) words
+ * 0..dim-1; the normals of interior bead i (1 <= i <= M - 2) come in pairs
+ * (i, i+1), i odd, from keyed(seed, p*M + i, t, chain, PUR_INIT_LEVY):
+ * Box-Muller of words (x, y) for bead i and (z, w) for bead i + 1.
+ * rp: M x dim, column-major by coordinate (rp[d*M + j]). */
+static void init_try(uint64_t seed, uint32_t chain, int p, uint32_t t, int M, int dim, double L, double tau,
+                     double lambda, double *rp, double *nrm)
+{
+    uint32_t x[4];
+    keyed(seed, (uint32_t)p, t, chain, PUR_INIT_START, x);
+    for (int d = 0; d < dim; ++d) {
+        const double r0 = 2.0 * L * (o_u01(x[d]) - 0.5);
+        rp[d * M] = r0;
+        rp[d * M + (M - 1)] = r0;
+    }
+    for (int i = 1; i <= M - 2; ++i) {
+        double n0, n1;
+        if (i & 1) keyed(seed, (uint32_t)(p * M + i), t, chain, PUR_INIT_LEVY, x);
+        if (i & 1) o_box_muller(x[0], x[1], &n0, &n1);
+        else o_box_muller(x[2], x[3], &n0, &n1);
+        nrm[(i - 1) * dim] = n0;
+        if (dim > 1) nrm[(i - 1) * dim + 1] = n1;
+    }
+    o_levy(rp, M, dim, tau, L, lambda, nrm);
+}
+
+/* init_world with hard spheres of radius a (a = 0: every particle's first
+ * try).  Particles in order; particle p > 0 is drawn again (try t = 0, 1, ...)
+ * while any of its beads lies closer than a to an earlier particle's bead at
+ * the same slice (norm of the per-component |minimum-image| offsets, d < a),
+ * at most 10 000 tries (src/system.jl:44-64).  The reference's extra levy! on
+ * a found overlap only consumes draws: the try is discarded either way.
+ * out: N x dim x M; tries[p] (may be NULL): the accepted try + 1.  Returns 0,
+ * or -1 when a particle found no place. */
+int o_init_world_hs(uint64_t seed, uint32_t chain, int N, int M, int dim, double L, double tau, double lambda,
+                    double a, double *out, int *tries)
 {
     double *rp = (double *)malloc(sizeof(double) * (size_t)M * dim);
     double *nrm = (double *)malloc(sizeof(double) * (size_t)(M > 2 ? M - 2 : 1) * dim);
-    for (int p = 0; p < N; ++p) {
-        uint32_t x[4];
-        keyed(seed, (uint32_t)p, 0u, chain, PUR_INIT_START, x);
-        for (int d = 0; d < dim; ++d) {
-            const double r0 = 2.0 * L * (o_u01(x[d]) - 0.5);
-            rp[d * M] = r0;
-            rp[d * M + (M - 1)] = r0;
+    int rc = 0;
+    for (int p = 0; p < N && rc == 0; ++p) {
+        uint32_t t = 0;
+        for (;; ++t) {
+            if (t >= 10000u) {
+                rc = -1;
+                break;
+            }
+            init_try(seed, chain, p, t, M, dim, L, tau, lambda, rp, nrm);
+            int clash = 0;
+            if (a > 0.0)
+                for (int j = 0; j < M && !clash; ++j)
+                    for (int i = 0; i < p && !clash; ++i) {
+                        double d2 = 0.0;
+                        for (int d = 0; d < dim; ++d) {
+                            const double dd = o_distance(out[((size_t)i * dim + d) * M + j], rp[d * M + j], L);
+                            d2 += dd * dd;
+                        }
+                        clash = sqrt(d2) < a;
+                    }
+            if (!clash) break;
         }
-        for (int i = 1; i <= M - 2; ++i) {
-            double n0, n1;
-            if (i & 1) keyed(seed, (uint32_t)(p * M + i), 0u, chain, PUR_INIT_LEVY, x);
-            if (i & 1) o_box_muller(x[0], x[1], &n0, &n1);
-            else o_box_muller(x[2], x[3], &n0, &n1);
-            nrm[(i - 1) * dim] = n0;
-            if (dim > 1) nrm[(i - 1) * dim + 1] = n1;
-        }
-        o_levy(rp, M, dim, tau, L, lambda, nrm);
+        if (tries) tries[p] = (int)t + 1;
         for (int d = 0; d < dim; ++d)
             for (int j = 0; j < M; ++j) out[((size_t)p * dim + d) * M + j] = rp[d * M + j];
     }
     free(rp);
     free(nrm);
+    return rc;
+}
+
+/* init_world with a = 0 */
+void o_init_world(uint64_t seed, uint32_t chain, int N, int M, int dim, double L, double tau, double lambda,
+                  double *out)
+{
+    (void)o_init_world_hs(seed, chain, N, M, dim, L, tau, lambda, 0.0, out, NULL);
 }

@@ -208,6 +208,12 @@ int64_t o_sweep_bead_moves(int N, int M, int ell);
  * device initialisation pimc_init_paths restates this; out N x dim x M */
 void o_init_world(uint64_t seed, uint32_t chain, int N, int M, int dim, double L, double tau, double lambda,
                   double *out);
+/* init_world with hard spheres of radius a: particle p > 0 redrawn (try t keyed
+ * by sweep word t) while a bead lies closer than a to an earlier particle's at
+ * the same slice, at most 10 000 tries; tries[p] = accepted try + 1 (or NULL).
+ * Returns 0, or -1 when a particle found no place. */
+int o_init_world_hs(uint64_t seed, uint32_t chain, int N, int M, int dim, double L, double tau, double lambda,
+                    double a, double *out, int *tries);
 
 #ifdef __cplusplus
 }

@@ -411,6 +411,96 @@ __global__ void k_init_paths(const ChainDev *chains, const int *part_chain, cons
     }
 }
 
+// init_world (src/system.jl:36-78) with hard spheres of radius a = hs_a > 0:
+// one workgroup per listed chain, particles in order.  Try t of particle p is
+// k_init_paths's path with sweep word t (thread 0 runs the bridge into LDS in
+// FP64); the workgroup then tests every slice against the earlier particles'
+// beads (FP64 copies in hs64, N x M x DIM per chain at its position offset):
+// a bead closer than a (norm of the per-component |minimum-image| offsets)
+// rejects the try.  At most 10 000 tries per particle, else status bit 4 and
+// the chain's remaining particles keep their old paths.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256) k_init_paths_hs(const ChainDev *chains, const int *ids, int n, void *pos,
+                                                       double *hs64, uint64_t seed, int *status)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    if ((int)blockIdx.x >= n) return;
+    const int c = ids[blockIdx.x], tid = threadIdx.x, nt = blockDim.x;
+    const ChainDev ch = chains[c];
+    const Prm<double> P = Prm<double>::make(ch);
+    const int M = ch.M, N = ch.N;
+    const double a = ch.hs_a;
+    double *cand = reinterpret_cast<double *>(smem); // [M][DIM]
+    int *flag = reinterpret_cast<int *>(cand + (size_t)M * DIM);
+    double *done = hs64 + ch.pos_off;                // [p][j][d] of the placed particles
+    T *base = reinterpret_cast<T *>(pos) + ch.pos_off;
+    for (int p = 0; p < N; ++p) {
+        uint32_t t = 0;
+        for (;; ++t) {
+            if (t >= 10000u) {
+                if (tid == 0) atomicOr(status + c, 4);
+                return; // uniform: every thread reads the same flags
+            }
+            if (tid == 0) {
+                U4 x = draw(seed, (uint32_t)p, t, (uint32_t)c, kInitStart);
+                const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+                double r0[DIM], cur[DIM], em[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    r0[d] = 2.0 * ch.L * (u01d(w[d]) - 0.5);
+                    cur[d] = r0[d];
+                    em[d] = shift_end(r0[d], r0[d], P);
+                }
+                const int K = M - 1;
+                for (int i = 1; i < K; ++i) {
+                    if (i & 1) x = draw(seed, (uint32_t)(p * M + i), t, (uint32_t)c, kInitLevy);
+                    double nn[2];
+                    if (i & 1) box_muller(x.x, x.y, nn[0], nn[1]);
+                    else box_muller(x.z, x.w, nn[0], nn[1]);
+                    const double alpha = bridge_alpha<double>(K, i);
+                    const double sd = bridge_sd(alpha, P);
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        cur[d] = bridge_step(cur[d], em[d], alpha, nn[d], sd);
+                        cand[i * DIM + d] = teleport(cur[d], P);
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    cand[d] = teleport(r0[d], P);
+                    cand[(M - 1) * DIM + d] = teleport(em[d], P);
+                }
+                *flag = 0;
+            }
+            __syncthreads();
+            for (int it = tid; it < p * M; it += nt) {
+                const int i = it / M, j = it - i * M;
+                double d2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    const double dd = mindist(done[((size_t)i * M + j) * DIM + d], cand[j * DIM + d], P);
+                    d2 += dd * dd;
+                }
+                if (sqrt(d2) < a) *flag = 1;
+            }
+            __syncthreads();
+            const bool clash = *flag != 0;
+            __syncthreads();
+            if (!clash) break;
+        }
+        for (int it = tid; it < M * DIM; it += nt) {
+            const int j = it / DIM, d = it - j * DIM;
+            const double v = cand[it];
+            done[((size_t)p * M + j) * DIM + d] = v;
+            T x = (T)v;
+            if (x >= (T)ch.L) x = nextafter((T)ch.L, (T)(-ch.L)); // keep [-L, L) after rounding to T
+            if (ch.sm) base[((size_t)j * N + p) * DIM + d] = x;
+            else base[((size_t)p * DIM + d) * M + j] = x;
+        }
+        __syncthreads();
+    }
+}
+
 // The batched kernel's FP32 fast paths on one particle of a tile chain whose
 // staging length ell = MAXL divides M (every segment and centre-of-mass chunk
 // is one skew block of the LDS rows): the particle's ring is staged into LDS

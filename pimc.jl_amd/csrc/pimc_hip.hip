@@ -1412,6 +1412,47 @@ int pimc_init_paths(pimc_ctx *c, int32_t chain, uint64_t seed)
         }
 #undef INIT
         HIPCHK(hipGetLastError());
+        // chains with hard spheres: particles placed in order, redrawn while they overlap
+        std::vector<int> hs;
+        for (int id = 0; id < c->n_chains; ++id)
+            if ((chain < 0 || id == chain) && c->ch[id].hs_a > 0.0) hs.push_back(id);
+        if (!hs.empty()) {
+            int maxM = 0;
+            for (int id : hs) maxM = std::max(maxM, c->ch[id].M);
+            int *d_ids = dalloc<int>(hs.size());
+            double *d_hs64 = dalloc<double>((size_t)std::max<long long>(1, c->n_pos));
+            HIPCHK(hipMemcpyAsync(d_ids, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice, c->stream));
+            for (int id : hs) HIPCHK(hipMemsetAsync(c->d_status + id, 0, sizeof(int), c->stream));
+            const size_t lds = (size_t)maxM * c->dim * sizeof(double) + 16;
+            if (lds > 160 * 1024) throw HipError{hipErrorInvalidValue, "M too large for the hard-sphere init_world"};
+#define INITHS(T, D)                                                                                                   \
+    do {                                                                                                               \
+        if (lds > 65536)                                                                                               \
+            HIPCHK(hipFuncSetAttribute((const void *)k_init_paths_hs<T, D>,                                            \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                         \
+        hipLaunchKernelGGL((k_init_paths_hs<T, D>), dim3((unsigned)hs.size()), dim3(256), lds, c->stream, c->d_ch,     \
+                           d_ids, (int)hs.size(), c->d_pos, d_hs64, seed, c->d_status);                                \
+    } while (0)
+            if (c->prec == PIMC_FP64) {
+                if (c->dim == 1) INITHS(double, 1);
+                else INITHS(double, 2);
+            } else {
+                if (c->dim == 1) INITHS(float, 1);
+                else INITHS(float, 2);
+            }
+#undef INITHS
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(c->stream));
+            std::vector<int> st(c->n_chains);
+            HIPCHK(hipMemcpy(st.data(), c->d_status, sizeof(int) * c->n_chains, hipMemcpyDeviceToHost));
+            dfree(d_ids);
+            dfree(d_hs64);
+            for (int id : hs)
+                if (st[id] & 4)
+                    return set_err(c, PIMC_ERR_CAPACITY, "chain " + std::to_string(id) +
+                                                             ": init_world placed no particle clear of the hard "
+                                                             "spheres within 10000 tries; lower hs_a or the density");
+        }
         if (chain >= 0) {
             if (c->ch[chain].ncell > 0) {
                 HIPCHK(hipMemsetAsync(c->d_status + chain, 0, sizeof(int), c->stream));

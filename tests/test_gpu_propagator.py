@@ -192,6 +192,46 @@ def hard_sphere_case(N, L, ra, hs_a, dim):
     assert tot["levy_acc"] > 0 and tot["com_acc"] > 0
 
 
+@pytest.mark.parametrize("precision", [A.FP64, A.FP32])
+def test_init_paths_with_hard_spheres_matches_oracle(precision):
+    """pimc_init_paths on chains with hard spheres == the oracle's init_world
+    with a > 0 (src/system.jl:44-64: particles placed in order, each redrawn
+    while a bead comes closer than a to an earlier particle's at that slice):
+    the same paths, no two beads of a slice closer than a; a chain without
+    hard spheres keeps the plain init_world; the cell lists are rebuilt, so a
+    sweep with the table runs afterwards."""
+    M, beta, seed = 16, 2.0, 21
+    descs = [table_desc(12, 1.5, M, 1.5, beta=beta, ell=4, hs_a=0.3),
+             table_desc(30, 3.0, M, 1.5, beta=beta, ell=4),
+             table_desc(110, 5.0, M, 2.0, beta=beta, ell=8, hs_a=0.25)]
+    ctx = ctx_for(descs, precision, spl=2)
+    ctx.init_paths(-1, seed)
+    tol = 1e-12 if precision == A.FP64 else 1e-5
+    redrawn = 0
+    for c, d in enumerate(descs):
+        ref, tries, rc = O.init_world_hs(seed, c, d.N, d.M, d.dim, d.L, d.beta / d.M, d.lam, d.hs_a)
+        assert rc == 0
+        redrawn += int((tries > 1).sum())
+        got = ctx.get_paths(c)
+        diff = np.abs(got - ref)
+        diff = np.minimum(diff, 2 * d.L - diff)  # a bead at the wall may round to the other side in FP32
+        assert np.max(diff) <= tol * max(1.0, d.L), (c, np.max(diff))
+        if d.hs_a > 0:
+            best = np.inf
+            for i in range(d.N):
+                dd = np.abs(got[:i] - got[i][None])
+                dd = np.minimum(dd, 2 * d.L - dd)
+                if i:
+                    best = min(best, float(np.sqrt((dd ** 2).sum(axis=1)).min()))
+            assert best >= d.hs_a - tol * d.L, (c, best)
+    assert redrawn > 0  # the redraw loop ran
+    for c, d in enumerate(descs):
+        tab, r0, r1 = table_for(d.L, 2.0, beta / M, 64)
+        ctx.set_pair_table(c, tab, r0, r1, 0.0)
+    cnt = ctx.run(2, measure=False)
+    assert cnt.levy_acc > 0
+
+
 def test_table_chain_fp32_runs():
     rng = np.random.default_rng(29)
     N, L, M, ra = 40, 3.0, 16, 1.5

@@ -322,6 +322,45 @@ def test_init_world_structure_and_keys():
     assert not np.allclose(r, O.init_world(11, 3, 6, 12, 2, 3.0, 0.2, 0.5))
 
 
+def _min_pair_distance_per_slice(r, L):
+    """Smallest distance (norm of the per-component |minimum-image| offsets)
+    between beads of different particles at the same slice, brute force."""
+    N = r.shape[0]
+    best = np.inf
+    for i in range(N):
+        for k in range(i):
+            d = np.abs(r[i] - r[k])
+            d = np.minimum(d, 2 * L - d)
+            best = min(best, float(np.sqrt((d ** 2).sum(axis=0)).min()))
+    return best
+
+
+def test_init_world_hard_spheres():
+    """init_world with hard spheres (src/system.jl:44-64): a = 0 gives the plain
+    init_world; with a > 0 no two beads of a slice end closer than a; a
+    particle placed on its first try keeps the plain path (try 0 draws), and a
+    particle that needed more tries had a first-try path that came closer than
+    a to an earlier particle's beads; an impossible a reports failure."""
+    N, M, dim, L, tau, lam = 12, 16, 2, 1.5, 0.1, 0.5
+    plain = O.init_world(1, 0, N, M, dim, L, tau, lam)
+    r0, t0, rc0 = O.init_world_hs(1, 0, N, M, dim, L, tau, lam, 0.0)
+    assert rc0 == 0 and np.array_equal(r0, plain) and np.all(t0 == 1)
+    a = 0.3
+    r, tries, rc = O.init_world_hs(1, 0, N, M, dim, L, tau, lam, a)
+    assert rc == 0
+    assert _min_pair_distance_per_slice(r, L) >= a
+    assert np.array_equal(r[:, :, 0], r[:, :, -1]) and np.all(r >= -L) and np.all(r < L)
+    assert tries[0] == 1 and tries.max() > 1
+    for p in range(N):
+        if tries[p] == 1:
+            assert np.array_equal(r[p], plain[p])
+        else:
+            # the first try's path overlaps a particle placed before p
+            assert _min_pair_distance_per_slice(np.concatenate([r[:p], plain[p:p + 1]]), L) < a, p
+    _, _, rc_bad = O.init_world_hs(1, 0, 6, 8, dim, 1.0, tau, lam, 2.5)  # a > box diagonal
+    assert rc_bad == -1
+
+
 def test_init_world_statistics():
     """Start beads uniform in the box (Kolmogorov-Smirnov), interior beads a
     Brownian bridge from the start bead back to itself: mean 0 and variance
