@@ -121,6 +121,9 @@ struct pimc_ctx {
     unsigned char *d_cacc = nullptr;                  // centre of mass: accepted flag per particle
     void *d_cshift = nullptr;                         // and its shift (2 per particle, arithmetic type)
     void *d_pm = nullptr;                             // particle-major copy of the window chains' paths
+    double *d_win_part = nullptr;                     // estimator partials of the window chains [chain][window][6]
+    size_t lds_meas = 0;
+    int cap_meas = 0;
     std::vector<int> cells_all_ids;               // every chain with cell lists
     int *d_cells_all = nullptr;
     int n_meas_cells = 0;                         // meas_ids: the chains with cell lists come first
@@ -438,8 +441,9 @@ HookArgs hook_args(pimc_ctx *c, int chain)
         HIPCHK(hipGetLastError());                                                                                     \
     } while (0)
 
-// One launch step: the sweep kernels for k sweeps, the fold into counters and
-// blocks, and the advance of the device-side sweep/block counters.
+// The estimators of one sweep in their own kernels: the whole chains of the
+// wide estimator kernel (k_meas_beads, with or without cell lists) and the
+// window chains (k_meas_win), each folded into its chain's sums by k_meas_fold.
 template <typename T, int DIM> void launch_meas_d(pimc_ctx *c, int sweep_off)
 {
     MeasArgs m{};
@@ -479,11 +483,27 @@ template <typename T, int DIM> void launch_meas_d(pimc_ctx *c, int sweep_off)
                            dim3(256), 0, c->stream, mb);
         HIPCHK(hipGetLastError());
     }
-    m.ids = c->d_meas_ids;
-    m.n_ids = nall;
-    m.part = c->d_meas_part;
-    hipLaunchKernelGGL(k_meas_fold, dim3((unsigned)((nall + 127) / 128)), dim3(128), 0, c->stream, m);
-    HIPCHK(hipGetLastError());
+    if (nall > 0) {
+        m.ids = c->d_meas_ids;
+        m.n_ids = nall;
+        m.part = c->d_meas_part;
+        hipLaunchKernelGGL(k_meas_fold, dim3((unsigned)((nall + 127) / 128)), dim3(128), 0, c->stream, m);
+        HIPCHK(hipGetLastError());
+    }
+    // the window chains: by centre-of-mass windows, then the fold in window order
+    const int nw = (int)c->win_all.size();
+    if (nw > 0) {
+        MeasArgs mw = m;
+        mw.ids = c->d_win_all;
+        mw.n_ids = nw;
+        mw.nblk = c->spec_nwin;
+        mw.part = c->d_win_part;
+        hipLaunchKernelGGL((k_meas_win<T, DIM>), dim3((unsigned)c->spec_nwin, (unsigned)nw), dim3(512), c->lds_meas,
+                           c->stream, mw, c->spec_nwin, c->cap_meas);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_meas_fold, dim3((unsigned)((nw + 127) / 128)), dim3(128), 0, c->stream, mw);
+        HIPCHK(hipGetLastError());
+    }
 }
 
 void launch_meas(pimc_ctx *c, int sweep_off)
@@ -566,6 +586,7 @@ template <typename T, int DIM> void win_attrs_d(pimc_ctx *c)
     set((const void *)k_com_spec<T, DIM>, c->lds_cspec);
     set((const void *)k_com_dec<T, DIM>, c->lds_cdec);
     set((const void *)k_cells_win<T, DIM>, c->lds_cells);
+    set((const void *)k_meas_win<T, DIM>, c->lds_meas);
 }
 
 void win_attrs(pimc_ctx *c)
@@ -609,12 +630,10 @@ void launch_step(pimc_ctx *c, int k, int measure)
                 b.sweep_off = s;
                 b.split = 1;
                 if (old && c->split_meas) launch_whole(c, b);
-                if (win) {
-                    launch_win(c, b);
-                    launch_cells_win(c);
-                }
+                if (win) launch_win(c, b);
                 launch_meas(c, s);
             }
+            if (win) launch_cells_win(c);
         } else {
             if (old) launch_whole(c, a);
             for (int s = 0; win && s < k; ++s) {
@@ -1131,13 +1150,11 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             // chains with cell lists first
             c->meas_ids.clear();
             if (c->split_meas) c->meas_ids = c->whole_cells_ids;
-            c->meas_ids.insert(c->meas_ids.end(), c->win_ids[0].begin(), c->win_ids[0].end());
             c->n_meas_cells = (int)c->meas_ids.size();
             if (c->split_meas) {
                 c->meas_ids.insert(c->meas_ids.end(), c->whole_brute_ids.begin(), c->whole_brute_ids.end());
                 c->meas_ids.insert(c->meas_ids.end(), c->whole_lds_ids.begin(), c->whole_lds_ids.end());
             }
-            c->meas_ids.insert(c->meas_ids.end(), c->win_ids[1].begin(), c->win_ids[1].end());
             long long most = 0;
             for (int id : c->meas_ids) most = std::max(most, (long long)c->ch[id].N * c->ch[id].M);
             c->meas_nblk = (int)std::max(1LL, (most + 1023) / 1024);
@@ -1208,7 +1225,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             // pools: at least N slots (one particle's partners), at most a whole chunk's, and
             // otherwise what keeps the staging kernel within 64 KiB (two 512-thread workgroups
             // per CU), the phase-1 kernel within 48 KiB (three) and the decision kernel (one
-            // workgroup per chain) within 96 KiB; measured on gauss_box and lattice
+            // workgroup per chain) within 128 KiB; measured on gauss_box and lattice
             const int N = c->win_maxN, ts_ = (int)ts, d = c->dim;
             const int lc = std::max(com_win_len(c->win_maxM), 8);
             auto grow = [&](auto bytes, size_t limit) {
@@ -1218,7 +1235,10 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             };
             c->cap_stage = grow([&](int k) { return stage_lds(N, c->win_maxell, d, ts_, k).ck.end; }, 64 * 1024);
             c->cap_cspec = grow([&](int k) { return com_spec_lds(N, lc, d, ts_, k).ck.end; }, 48 * 1024);
-            c->cap_cdec = grow([&](int k) { return com_dec_lds(N, d, ts_, k).ck.end; }, 96 * 1024);
+            c->cap_cdec = grow([&](int k) { return com_dec_lds(N, d, ts_, k).ck.end; }, 128 * 1024);
+            c->cap_meas = grow([&](int k) { return meas_win_lds(N, lc, d, ts_, k).ck.end; }, 48 * 1024);
+            c->lds_meas = meas_win_lds(N, lc, d, ts_, c->cap_meas).ck.end;
+            c->d_win_part = dalloc<double>(c->win_all.size() * (size_t)c->spec_nwin * 6);
             c->lds_stage = stage_lds(N, c->win_maxell, d, ts_, c->cap_stage).ck.end;
             c->lds_cspec = com_spec_lds(N, lc, d, ts_, c->cap_cspec).ck.end;
             c->lds_cdec = com_dec_lds(N, d, ts_, c->cap_cdec).ck.end;
@@ -1290,7 +1310,7 @@ int pimc_destroy(pimc_ctx *c)
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
                     c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next, c->d_meas_ids, c->d_meas_part,
-                    c->d_win[0], c->d_win[1], c->d_cells_all, c->d_spec, c->d_wbnd, c->d_win_all, c->d_cacc, c->d_cshift, c->d_pm,
+                    c->d_win[0], c->d_win[1], c->d_cells_all, c->d_spec, c->d_wbnd, c->d_win_all, c->d_cacc, c->d_cshift, c->d_pm, c->d_win_part,
                     c->d_beams};
     for (void *p : ptrs) dfree(p);
     for (double *p : c->d_tab) dfree(p);

@@ -732,8 +732,12 @@ __global__ void __launch_bounds__(512) k_com_spec(SweepArgs a, double *spec, flo
     }
 }
 
+// Items of the decision kernel: (slot, window) pairs whose bounds come within
+// reach, taken kItemBatch at a time.
+constexpr int kItemBatch = 2048;
+
 struct ComDecLds {
-    size_t spec, su, sd, rb, acc, pwb;
+    size_t spec, su, sd, rb, acc, pwb, wm, ioff, hitf, cw, hw;
     ChunkLds ck;
 };
 __host__ __device__ inline ComDecLds com_dec_lds(int N, int dim, int tsize, int cap)
@@ -752,6 +756,16 @@ __host__ __device__ inline ComDecLds com_dec_lds(int N, int dim, int tsize, int 
     o = align16(o + (size_t)N);
     L.pwb = o; // window bounds of the chunk's particles, kChunk x 32
     o = align16(o + (size_t)kChunk * 32 * 16);
+    L.wm = o; // per slot: mask of the windows to evaluate
+    o = align16(o + (size_t)cap * 4);
+    L.ioff = o; // per slot: first item
+    o = align16(o + (size_t)cap * 4);
+    L.hitf = o; // per slot: a term within the cut-off
+    o = align16(o + (size_t)cap);
+    L.cw = o; // per item of a batch: C of the window
+    o = align16(o + (size_t)kItemBatch * tsize);
+    L.hw = o;
+    o = align16(o + (size_t)kItemBatch);
     L.ck = chunk_lds(o, N, cap, tsize, 1);
     return L;
 }
@@ -783,6 +797,11 @@ __global__ void __launch_bounds__(1024) k_com_dec(SweepArgs a, const double *spe
     float4 *rb = reinterpret_cast<float4 *>(smem + L.rb);
     unsigned char *acc = smem + L.acc;
     float4 *pwb = reinterpret_cast<float4 *>(smem + L.pwb);
+    unsigned *s_wm = reinterpret_cast<unsigned *>(smem + L.wm);
+    int *s_ioff = reinterpret_cast<int *>(smem + L.ioff);
+    unsigned char *s_hitf = smem + L.hitf;
+    T *s_cw = reinterpret_cast<T *>(smem + L.cw);
+    unsigned char *s_hw = smem + L.hw;
     ChunkPtr<T> X(smem, L.ck);
     const T *ring = pm + ch.pos_off; // particle-major copy of the path: [particle][slice][d]
     const float h = (float)ch.h, twoL = (float)(2.0 * ch.L), inv2L = 1.0f / twoL;
@@ -837,35 +856,70 @@ __global__ void __launch_bounds__(1024) k_com_dec(SweepArgs a, const double *spe
             pwb[l * 32 + w] = wb[(size_t)(pa + l) * nwin + w];
         }
         __syncthreads();
+        // the windows of each slot whose bounds (old or shifted) come within reach:
+        // one thread per (slot, 4 windows), q's bounds fetched together
         for (int i = tid; i < total; i += nt) {
+            s_wm[i] = 0u;
+            s_hitf[i] = 0;
+            X.pc[i] = T(0);
+        }
+        __syncthreads();
+        const int ngrp = (nw + 3) / 4;
+        for (int it = tid; it < total * ngrp; it += nt) {
+            const int i = it / ngrp, w0 = (it - i * ngrp) * 4;
             const int l = X.po[i], p = pa + l, q = X.pq[i];
             float dp[2], dq[2];
             fshift(p, dp);
             fshift(q, dq);
-            // windows where the pair's bounds come within the cut-off: q's bounds
-            // fetched 4 at a time before any is tested
-            unsigned wm = 0u;
-            for (int w0 = 0; w0 < nw; w0 += 4) {
-                float4 bq[4];
+            float4 bq[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (w0 + k < nw) bq[k] = wb[(size_t)q * nwin + w0 + k];
+            for (int k = 0; k < 4; ++k)
+                if (w0 + k < nw) bq[k] = wb[(size_t)q * nwin + w0 + k];
+            unsigned bits = 0u;
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (w0 + k < nw && bnd_near4<DIM>(pwb[l * 32 + w0 + k], dp, bq[k], dq, h, twoL, inv2L))
-                        wm |= 1u << (w0 + k);
+            for (int k = 0; k < 4; ++k)
+                if (w0 + k < nw && bnd_near4<DIM>(pwb[l * 32 + w0 + k], dp, bq[k], dq, h, twoL, inv2L))
+                    bits |= 1u << (w0 + k);
+            if (bits) atomicOr(s_wm + i, bits);
+        }
+        __syncthreads();
+        // items = (slot, window) in slot order, windows ascending within a slot
+        if (wave == 0) {
+            const int lane = tid;
+            int base = 0;
+            for (int i0 = 0; i0 < total; i0 += 64) {
+                const int i = i0 + lane;
+                const int v = i < total ? __popc(s_wm[i]) : 0;
+                int tot;
+                const int pre = wave_prefix(v, lane, tot);
+                if (i < total) s_ioff[i] = base + pre;
+                base += tot;
             }
-            T C = T(0);
-            bool hit = false;
-            while (wm) {
+            if (lane == 0) X.misc[2] = base;
+        }
+        __syncthreads();
+        const int nitems = X.misc[2];
+        for (int k0 = 0; k0 < nitems; k0 += kItemBatch) {
+            const int kn = min(kItemBatch, nitems - k0);
+            for (int kk = tid; kk < kn; kk += nt) {
+                const int k = k0 + kk;
+                int lo = 0, hi = total - 1; // the last slot whose first item is <= k
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_ioff[mid] <= k) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int i = lo;
+                unsigned wm = s_wm[i];
+                for (int r = k - s_ioff[i]; r > 0; --r) wm &= wm - 1u;
                 const int w = __ffs((int)wm) - 1;
-                wm &= wm - 1u;
+                const int p = pa + X.po[i], q = X.pq[i];
                 const int s0 = w * lc, m = min(lc, M - s0);
                 auto bead = [&](int t, int who, T *ro, T *rn) {
-                    const int k = who ? q : p;
+                    const int kp = who ? q : p;
                     int j = s0 + t;
                     if (j >= M) j -= M;
-                    const T *r = ring + ((size_t)k * M + j) * DIM;
+                    const T *r = ring + ((size_t)kp * M + j) * DIM;
                     if constexpr (DIM == 2) {
                         using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
                         const V2 v = *reinterpret_cast<const V2 *>(r);
@@ -875,17 +929,31 @@ __global__ void __launch_bounds__(1024) k_com_dec(SweepArgs a, const double *spe
                         ro[0] = r[0];
                     }
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) rn[e] = teleport(ro[e] + sd[k * DIM + e], P);
+                    for (int e = 0; e < DIM; ++e) rn[e] = teleport(ro[e] + sd[kp * DIM + e], P);
                 };
                 T A, Cw;
                 bool hw;
                 pair_window<T, DIM, 4>(bead, m, true, P, ch.nbins, A, Cw, hw);
-                C += Cw;
-                hit = hit || hw;
+                s_cw[kk] = Cw;
+                s_hw[kk] = hw ? 1 : 0;
             }
-            X.pc[i] = C;
-            if (hit) X.pq[i] = (unsigned short)(q | 0x8000);
+            __syncthreads();
+            // each slot adds its items of this batch in window order
+            for (int i = tid; i < total; i += nt) {
+                const int b0 = max(s_ioff[i], k0), b1 = min(s_ioff[i] + __popc(s_wm[i]), k0 + kn);
+                T acc_c = X.pc[i];
+                unsigned char hf = s_hitf[i];
+                for (int k = b0; k < b1; ++k) {
+                    acc_c += s_cw[k - k0];
+                    hf |= s_hw[k - k0];
+                }
+                X.pc[i] = acc_c;
+                s_hitf[i] = hf;
+            }
+            __syncthreads();
         }
+        for (int i = tid; i < total; i += nt)
+            if (s_hitf[i]) X.pq[i] = (unsigned short)(X.pq[i] | 0x8000);
         __syncthreads();
         if (tid < nf) {
             X.spec[tid] = s_spec[pa + tid];
@@ -933,6 +1001,162 @@ __global__ void __launch_bounds__(256) k_com_apply(SweepArgs a, const unsigned c
 #pragma unroll
     for (int e = 0; e < DIM; ++e) r[e] = teleport(r[e] + cshift[q * 2 + e], P);
     path.put(k, j, r);
+}
+
+// --- estimators ---
+
+struct MeasWinLds {
+    size_t ob, bnd;
+    ChunkLds ck;
+};
+__host__ __device__ inline MeasWinLds meas_win_lds(int N, int lc, int dim, int tsize, int cap)
+{
+    MeasWinLds L;
+    size_t o = 0;
+    L.ob = o;
+    o = align16(o + (size_t)(lc + 1) * N * dim * tsize);
+    L.bnd = o;
+    o = align16(o + (size_t)N * 16);
+    L.ck = chunk_lds(o, N, cap, tsize, 1);
+    return L;
+}
+
+// Estimators of Gaussian-pair chains after a sweep, by the centre-of-mass
+// windows: one 512-thread workgroup per (window, chain of a.ids).  The terms of
+// measure_chain for the beads of slices s0 .. s0 + m - 1: kinetic link to the
+// next slice, V, r.grad V, dV/d omega, the radial and Cartesian histograms,
+// and the pair energy and pair virial of each bead as pair_energy_bead takes
+// them (every neighbour within the cut-off at the bead's slice, weight
+// tau/4 times its stencil multiplicity), the neighbours found from window
+// bounds.  part[(y * nwin + w) * 6]: the window's sums (link, V, r.grad V,
+// S_U/g, pair virial, dV/d omega), folded in window order by k_meas_fold.
+template <typename T, int DIM> __global__ void __launch_bounds__(512) k_meas_win(MeasArgs a, int nwin, int cap)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double wbuf[32];
+    if (!split_measured(a)) return;
+    const int y = blockIdx.y, c = a.ids[y], w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const ChainDev ch = a.chains[c];
+    const int N = ch.N, M = ch.M, lc = com_win_len(M);
+    double *out = a.part + ((size_t)y * nwin + w) * 6;
+    if (w * lc >= M || cap < N) { // no such window for this chain (the host sizes cap >= N)
+        if (tid < 6) out[tid] = 0.0;
+        return;
+    }
+    const int m = min(lc, M - w * lc), s0 = w * lc;
+    const Prm<T> P = reinterpret_cast<const Prm<T> *>(a.prm)[c];
+    const MeasWinLds L = meas_win_lds(N, lc, DIM, (int)sizeof(T), cap);
+    T *ob = reinterpret_cast<T *>(smem + L.ob);
+    float4 *bnd = reinterpret_cast<float4 *>(smem + L.bnd);
+    ChunkPtr<T> X(smem, L.ck);
+    const T *const gpath = reinterpret_cast<const T *>(a.pos) + ch.pos_off;
+    for (int t = 0; t <= m; ++t) {
+        const int j = (s0 + t) % M;
+        const T *src = gpath + (size_t)j * N * DIM;
+        T *dst = ob + (size_t)t * N * DIM;
+        for (int i = tid; i < N * DIM; i += nt) dst[i] = src[i];
+    }
+    __syncthreads();
+    for (int p = tid; p < N; p += nt) {
+        float mean[DIM] = {};
+        for (int t = 0; t <= m; ++t) {
+#pragma unroll
+            for (int e = 0; e < DIM; ++e)
+                mean[e] += (float)signed_mindist(ob[((size_t)t * N + p) * DIM + e], ob[(size_t)p * DIM + e], P);
+        }
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) mean[e] /= (float)(m + 1);
+        float r2 = 0.f;
+        for (int t = 0; t <= m; ++t) {
+            float d2 = 0.f;
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) {
+                const float o = (float)signed_mindist(ob[((size_t)t * N + p) * DIM + e], ob[(size_t)p * DIM + e], P) - mean[e];
+                d2 += o * o;
+            }
+            r2 = fmaxf(r2, d2);
+        }
+        bnd[p] = make_float4((float)ob[(size_t)p * DIM] + mean[0],
+                             DIM > 1 ? (float)ob[(size_t)p * DIM + DIM - 1] + mean[DIM - 1] : 0.f, sqrtf(r2), 0.f);
+    }
+    // single-bead terms, one thread per (particle, slice)
+    double l = 0.0, v = 0.0, g = 0.0, r2w = 0.0, su = 0.0, rgp = 0.0;
+    for (int it = tid; it < N * m; it += nt) {
+        const int p = it / m, t = it - p * m;
+        T r0[DIM], r1[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+            r0[e] = ob[((size_t)t * N + p) * DIM + e];
+            r1[e] = ob[((size_t)(t + 1) * N + p) * DIM + e];
+        }
+        T dd = T(0);
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+            const T dx = mindist(r0[e], r1[e], P);
+            dd += dx * dx;
+        }
+        l += (double)dd;
+        v += (double)potV<T, DIM>(r0, P);
+        g += (double)rgradV<T, DIM>(r0, P);
+        if (ch.pot == kPotHarmonic) r2w += (double)dVdomega<T, DIM>(r0, ch.omega);
+        if (a.radial_bins > 0) {
+            T rr = r0[0] * r0[0];
+            if (DIM > 1) rr += r0[1] * r0[1];
+            const double qb = (double)sq(rr) * a.radial_winv;
+            if (qb < (double)a.radial_bins) atomicAdd(&a.radial[(size_t)c * a.radial_bins + (int)qb], 1ull);
+        }
+        if (a.cart_bins > 0) {
+            const double bw = (2.0 * ch.L) / a.cart_bins;
+            long long idx = 0, stride = 1;
+            bool ok = true;
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) {
+                const long long ib = (long long)floor(((double)r0[e] + ch.L) / bw);
+                ok = ok && ib > 0 && ib < a.cart_bins + 1;
+                idx += (ib - 1) * stride;
+                stride *= a.cart_bins;
+            }
+            if (ok) atomicAdd(&a.cart[(size_t)c * stride + idx], 1ull);
+        }
+    }
+    __syncthreads();
+    // pair terms: bead (p, t) with each neighbour q within the cut-off at slice t
+    const float h = (float)ch.h, twoL = (float)(2.0 * ch.L), inv2L = 1.0f / twoL;
+    const double inv_s2 = 1.0 / (ch.sigma * ch.sigma), wq = 0.5 * (0.5 * (double)P.tau);
+    for (int pa = 0; pa < N;) {
+        const int nf = chunk_slots(X, pa, N, cap, kChunk, [&](int p, int q) {
+            if (q == p) return false;
+            const float4 A = bnd[p], B = bnd[q];
+            return bnd_near<DIM>(A.x, A.y, A.z, B.x, B.y, B.z, h, twoL, inv2L);
+        });
+        const int total = X.misc[1];
+        for (int i = tid; i < total; i += nt) {
+            const int p = pa + X.po[i], q = X.pq[i];
+            for (int t = 0; t < m; ++t) {
+                const T *rp = ob + ((size_t)t * N + p) * DIM, *rq = ob + ((size_t)t * N + q) * DIM;
+                const T r2 = dist2<T, DIM>(rq, rp, P);
+                if (!(r2 <= P.h2)) continue;
+                const T mult = stencil_mult<T, DIM>(rp, rq, P, ch.nbins);
+                const T s2 = dist2<T, DIM>(rq + (size_t)N * DIM, rp + (size_t)N * DIM, P);
+                const double ea = (double)ex(-r2 * P.inv2s2), eb = (double)ex(-s2 * P.inv2s2);
+                const double wt = wq * (double)mult;
+                su += wt * (ea + eb);
+                rgp += wt * (-((double)r2 * inv_s2) * ea - ((double)s2 * inv_s2) * eb);
+            }
+        }
+        __syncthreads();
+        pa += nf;
+    }
+    const double sl = block_sum(l, wbuf), sv = block_sum(v, wbuf), sg = block_sum(g, wbuf);
+    const double ssu = block_sum(su, wbuf), srg = block_sum(rgp, wbuf), sw = block_sum(r2w, wbuf);
+    if (tid == 0) {
+        out[0] = sl;
+        out[1] = sv;
+        out[2] = sg;
+        out[3] = ssu;
+        out[4] = srg;
+        out[5] = sw;
+    }
 }
 
 // --- cell lists ---
