@@ -256,26 +256,160 @@ template <typename T> struct ChunkPtr {
     }
 };
 
+// --- grid of window centres ---
+
+// A uniform periodic grid over the particles' bound centres, rebuilt per
+// window: cells of side >= 2 rmax + h (rmax the largest bound radius), so two
+// bounds within reach sit in neighbouring cells and a particle's candidates
+// are the members of its 3^DIM stencil.  ng cells per dimension, at most
+// kGridMax; no grid (ng = 0) when the stencil would cover more than a quarter
+// (2D: ng < 6) or three eighths (1D: ng < 8) of the box, where the full scan
+// is cheaper.
+constexpr int kGridMax = 16;
+
+struct GridLds {
+    size_t start, mem, pcell, misc, end;
+};
+__host__ __device__ inline GridLds grid_lds(size_t o, int N, int dim)
+{
+    GridLds L;
+    const int nc = dim == 1 ? kGridMax : kGridMax * kGridMax;
+    L.start = o;
+    o = align16(o + (size_t)(nc + 1) * 4);
+    L.mem = o;
+    o = align16(o + (size_t)N * 2);
+    L.pcell = o;
+    o = align16(o + (size_t)N * 2);
+    L.misc = o;
+    o = align16(o + 16);
+    L.end = o;
+    return L;
+}
+
+struct WinGrid {
+    int *start;            // after the build: cell c's members are mem[c ? start[c - 1] : 0 .. start[c])
+    unsigned short *mem;   // particles by cell
+    unsigned short *pcell; // each particle's cell
+    int *misc;             // [0] largest radius (float bits)
+    int ng;
+    __device__ __forceinline__ WinGrid(unsigned char *s, const GridLds &L)
+        : start(reinterpret_cast<int *>(s + L.start)), mem(reinterpret_cast<unsigned short *>(s + L.mem)),
+          pcell(reinterpret_cast<unsigned short *>(s + L.pcell)), misc(reinterpret_cast<int *>(s + L.misc)), ng(0)
+    {
+    }
+};
+
+// Build the grid from centre(p, x, y, r) (all threads of the workgroup); sets G.ng.
+template <int DIM, typename Centre>
+__device__ inline void grid_build(WinGrid &G, int N, float h, float twoL, float inv2L, Centre centre)
+{
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) G.misc[0] = 0;
+    __syncthreads();
+    for (int p = tid; p < N; p += nt) {
+        float x, y, r;
+        centre(p, x, y, r);
+        atomicMax(G.misc, __float_as_int(fmaxf(r, 0.f)));
+    }
+    __syncthreads();
+    const float rmax = __int_as_float(G.misc[0]);
+    const float side = (2.f * rmax + h) * 1.0001f + 2e-5f * twoL;
+    int ng = min((int)floorf(twoL / side), kGridMax);
+    G.ng = ng >= (DIM == 1 ? 8 : 6) ? ng : 0;
+    if (G.ng == 0) return; // uniform: every thread read the same rmax
+    const int ncell = DIM == 1 ? ng : ng * ng;
+    for (int c = tid; c <= ncell; c += nt) G.start[c] = 0;
+    __syncthreads();
+    const float scale = (float)ng * inv2L;
+    auto axis = [&](float v) {
+        const float w = v - twoL * floorf(v * inv2L + 0.5f); // into [-L, L]
+        return min(max((int)((w + 0.5f * twoL) * scale), 0), ng - 1);
+    };
+    for (int p = tid; p < N; p += nt) {
+        float x, y, r;
+        centre(p, x, y, r);
+        const int c = DIM == 1 ? axis(x) : axis(x) + ng * axis(y);
+        G.pcell[p] = (unsigned short)c;
+        atomicAdd(G.start + c + 1, 1);
+    }
+    __syncthreads();
+    if (wave == 0) { // inclusive prefix: start[c] = members of the cells before c
+        int base = 0;
+        for (int c0 = 0; c0 <= ncell; c0 += 64) {
+            const int c = c0 + lane;
+            const int v = c <= ncell ? G.start[c] : 0;
+            int tot;
+            const int pre = wave_prefix(v, lane, tot);
+            if (c <= ncell) G.start[c] = base + pre + v;
+            base += tot;
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < N; p += nt) {
+        const int at = atomicAdd(G.start + G.pcell[p], 1); // start[c] ends at cell c's end
+        G.mem[at] = (unsigned short)p;
+    }
+    __syncthreads();
+}
+
 // Slots of the chunk's particles pa, pa + 1, ... (at most kc <= kChunk): for
 // each the partners q with cand(p, q), by index.  Fills as many particles as
 // the pool holds (at least one: cap >= N).  Returns the number of particles filled; X.misc[1] is the
 // slot count.  All threads of the workgroup.
-template <typename T, typename Cand>
-__device__ inline int chunk_slots(ChunkPtr<T> &X, int pa, int N, int cap, int kc, Cand cand)
+template <int DIM = 2, typename T, typename Cand>
+__device__ inline int chunk_slots(ChunkPtr<T> &X, int pa, int N, int cap, int kc, Cand cand,
+                                  const WinGrid *G = nullptr)
 {
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nt >> 6;
     const int nq = (N + 63) / 64, nc = min(kc, N - pa);
-    for (int l = wave; l < nc; l += nwv) {
-        const int p = pa + l;
-        int cnt = 0;
-        for (int q0 = 0; q0 < N; q0 += 64) {
-            const int q = q0 + lane;
-            const bool ok = q < N && cand(p, q);
-            const unsigned long long bal = __ballot(ok);
-            if (lane == 0) X.mask[l * nq + (q0 >> 6)] = bal;
-            cnt += __popcll(bal);
+    if (G && G->ng > 0) {
+        // candidates from the grid stencil (one wave per particle, the stencil's members
+        // flattened over the lanes), set as bits of the particle's mask
+        const int ng = G->ng;
+        constexpr int NS = DIM == 1 ? 3 : 9;
+        for (int i = tid; i < nc * nq; i += nt) X.mask[i] = 0ull;
+        __syncthreads();
+        for (int l = wave; l < nc; l += nwv) {
+            const int p = pa + l, pc = G->pcell[p];
+            const int ix = DIM == 1 ? pc : pc % ng, iy = DIM == 1 ? 0 : pc / ng;
+            int b[NS], pre[NS + 1];
+            pre[0] = 0;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                int cx = ix + k % 3 - 1, cy = iy + (DIM == 1 ? 0 : k / 3 - 1);
+                cx += cx < 0 ? ng : (cx >= ng ? -ng : 0);
+                cy += cy < 0 ? ng : (cy >= ng ? -ng : 0);
+                const int cc = cx + ng * cy;
+                b[k] = cc ? G->start[cc - 1] : 0;
+                pre[k + 1] = pre[k] + G->start[cc] - b[k];
+            }
+            for (int i = lane; i < pre[NS]; i += 64) {
+                int at = b[0] + i;
+#pragma unroll
+                for (int k = 1; k < NS; ++k) at = i >= pre[k] ? b[k] + (i - pre[k]) : at;
+                const int q = G->mem[at];
+                if (cand(p, q)) atomicOr(X.mask + l * nq + (q >> 6), 1ull << (q & 63));
+            }
         }
-        if (lane == 0) X.cnt[l] = cnt;
+        __syncthreads();
+        for (int l = tid; l < nc; l += nt) {
+            int cnt = 0;
+            for (int k = 0; k < nq; ++k) cnt += __popcll(X.mask[l * nq + k]);
+            X.cnt[l] = cnt;
+        }
+    } else {
+        for (int l = wave; l < nc; l += nwv) {
+            const int p = pa + l;
+            int cnt = 0;
+            for (int q0 = 0; q0 < N; q0 += 64) {
+                const int q = q0 + lane;
+                const bool ok = q < N && cand(p, q);
+                const unsigned long long bal = __ballot(ok);
+                if (lane == 0) X.mask[l * nq + (q0 >> 6)] = bal;
+                cnt += __popcll(bal);
+            }
+            if (lane == 0) X.cnt[l] = cnt;
+        }
     }
     __syncthreads();
     if (wave == 0) {
@@ -381,6 +515,7 @@ __global__ void k_clear_chain_meas(double *chain_meas, const int *ids, int n)
 // the chunk machinery with pool capacity cap.
 struct StageLds {
     size_t ob, nb, bnd, dv, su, acc;
+    GridLds gr;
     ChunkLds ck;
 };
 __host__ __device__ inline StageLds stage_lds(int N, int ell, int dim, int tsize, int cap)
@@ -399,7 +534,8 @@ __host__ __device__ inline StageLds stage_lds(int N, int ell, int dim, int tsize
     o = align16(o + (size_t)N * tsize);
     L.acc = o;
     o = align16(o + (size_t)N);
-    L.ck = chunk_lds(o, N, cap, tsize, 2);
+    L.gr = grid_lds(o, N, dim);
+    L.ck = chunk_lds(L.gr.end, N, cap, tsize, 2);
     return L;
 }
 
@@ -525,13 +661,20 @@ template <typename T, int DIM> __global__ void __launch_bounds__(512) k_stage_wi
             }
         };
     };
+    WinGrid G(smem, L.gr);
+    grid_build<DIM>(G, N, h, twoL, inv2L, [&](int p, float &x, float &y, float &r) {
+        const float4 b = bnd[p];
+        x = b.x;
+        y = b.y;
+        r = b.z;
+    });
     int lacc = 0;
     for (int pa = 0; pa < N;) {
-        const int nf = chunk_slots(X, pa, N, cap, kChunk, [&](int p, int q) {
+        const int nf = chunk_slots<DIM>(X, pa, N, cap, kChunk, [&](int p, int q) {
             if (q == p) return false;
             const float4 A = bnd[p], B = bnd[q];
             return bnd_near<DIM>(A.x, A.y, A.z, B.x, B.y, B.z, h, twoL, inv2L);
-        });
+        }, &G);
         const int total = X.misc[1];
         for (int i = tid; i < total; i += nt) {
             const int p = pa + X.po[i], q = X.pq[i];
@@ -592,6 +735,7 @@ __device__ __forceinline__ void com_try(const SweepArgs &a, int c, uint32_t sw, 
 
 struct ComSpecLds {
     size_t ob, sd, bnd, dv;
+    GridLds gr;
     ChunkLds ck;
 };
 __host__ __device__ inline ComSpecLds com_spec_lds(int N, int lc, int dim, int tsize, int cap)
@@ -606,7 +750,8 @@ __host__ __device__ inline ComSpecLds com_spec_lds(int N, int lc, int dim, int t
     o = align16(o + (size_t)N * 16);
     L.dv = o;
     o = align16(o + (size_t)N * tsize);
-    L.ck = chunk_lds(o, N, cap, tsize, 1);
+    L.gr = grid_lds(o, N, dim);
+    L.ck = chunk_lds(L.gr.end, N, cap, tsize, 1);
     return L;
 }
 
@@ -696,14 +841,24 @@ __global__ void __launch_bounds__(512) k_com_spec(SweepArgs a, double *spec, flo
     }
     __syncthreads();
     const float h = (float)ch.h, twoL = (float)(2.0 * ch.L), inv2L = 1.0f / twoL;
+    // grid of the bounds of old and shifted beads together (centre halfway, radius
+    // grown by half the shift): both of a try's pairings lie within them
+    WinGrid G(smem, L.gr);
+    grid_build<DIM>(G, N, h, twoL, inv2L, [&](int p, float &x, float &y, float &r) {
+        const float4 b = bnd[p];
+        const float dx = (float)sd[p * DIM], dy = DIM > 1 ? (float)sd[p * DIM + DIM - 1] : 0.f;
+        x = b.x + 0.5f * dx;
+        y = b.y + 0.5f * dy;
+        r = (b.z + 0.5f * sqrtf(dx * dx + dy * dy)) * 1.00001f;
+    });
     for (int pa = 0; pa < N;) {
-        const int nf = chunk_slots(X, pa, N, cap, kChunk, [&](int p, int q) {
+        const int nf = chunk_slots<DIM>(X, pa, N, cap, kChunk, [&](int p, int q) {
             if (q == p) return false;
             const float4 A = bnd[p], B = bnd[q];
             const float dx = (float)sd[p * DIM], dy = DIM > 1 ? (float)sd[p * DIM + DIM - 1] : 0.f;
             return bnd_near<DIM>(A.x, A.y, A.z, B.x, B.y, B.z, h, twoL, inv2L) ||
                    bnd_near<DIM>(A.x + dx, A.y + dy, A.z, B.x, B.y, B.z, h, twoL, inv2L);
-        });
+        }, &G);
         const int total = X.misc[1];
         for (int i = tid; i < total; i += nt) {
             const int p = pa + X.po[i], q = X.pq[i];
@@ -1007,6 +1162,7 @@ __global__ void __launch_bounds__(256) k_com_apply(SweepArgs a, const unsigned c
 
 struct MeasWinLds {
     size_t ob, bnd;
+    GridLds gr;
     ChunkLds ck;
 };
 __host__ __device__ inline MeasWinLds meas_win_lds(int N, int lc, int dim, int tsize, int cap)
@@ -1017,7 +1173,8 @@ __host__ __device__ inline MeasWinLds meas_win_lds(int N, int lc, int dim, int t
     o = align16(o + (size_t)(lc + 1) * N * dim * tsize);
     L.bnd = o;
     o = align16(o + (size_t)N * 16);
-    L.ck = chunk_lds(o, N, cap, tsize, 1);
+    L.gr = grid_lds(o, N, dim);
+    L.ck = chunk_lds(L.gr.end, N, cap, tsize, 1);
     return L;
 }
 
@@ -1123,12 +1280,19 @@ template <typename T, int DIM> __global__ void __launch_bounds__(512) k_meas_win
     // pair terms: bead (p, t) with each neighbour q within the cut-off at slice t
     const float h = (float)ch.h, twoL = (float)(2.0 * ch.L), inv2L = 1.0f / twoL;
     const double inv_s2 = 1.0 / (ch.sigma * ch.sigma), wq = 0.5 * (0.5 * (double)P.tau);
+    WinGrid G(smem, L.gr);
+    grid_build<DIM>(G, N, h, twoL, inv2L, [&](int p, float &x, float &y, float &r) {
+        const float4 b = bnd[p];
+        x = b.x;
+        y = b.y;
+        r = b.z;
+    });
     for (int pa = 0; pa < N;) {
-        const int nf = chunk_slots(X, pa, N, cap, kChunk, [&](int p, int q) {
+        const int nf = chunk_slots<DIM>(X, pa, N, cap, kChunk, [&](int p, int q) {
             if (q == p) return false;
             const float4 A = bnd[p], B = bnd[q];
             return bnd_near<DIM>(A.x, A.y, A.z, B.x, B.y, B.z, h, twoL, inv2L);
-        });
+        }, &G);
         const int total = X.misc[1];
         for (int i = tid; i < total; i += nt) {
             const int p = pa + X.po[i], q = X.pq[i];
