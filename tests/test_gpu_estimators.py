@@ -25,7 +25,7 @@ from test_gpu_parity import assert_blocks, oracle_meas, oracle_sys, rand_paths
 pytestmark = pytest.mark.gpu
 
 
-def make(descs, precision, seed, spl, radial_bins, radial_rmax, cart_bins, measure_kernel=0):
+def make(descs, precision, seed, spl, radial_bins, radial_rmax, cart_bins, measure_kernel=0, pair_kernel=0):
     opt = A.Options()
     opt.precision = precision
     opt.sweeps_per_launch = spl
@@ -34,6 +34,7 @@ def make(descs, precision, seed, spl, radial_bins, radial_rmax, cart_bins, measu
     opt.radial_rmax = radial_rmax
     opt.cart_bins = cart_bins
     opt.measure_kernel = measure_kernel
+    opt.pair_kernel = pair_kernel
     return A.Context(descs, opt)
 
 
@@ -133,17 +134,19 @@ def pair_descs():
     ]
 
 
-@pytest.mark.parametrize("measure_kernel", [1, 2])
-def test_whole_chain_estimators_and_histograms_match_oracle_fp64(measure_kernel):
-    """Whole-chain kernel, FP64: the E, Ev and Epair blocks and both
-    histograms equal the oracle's, with the estimators inside the sweep
-    kernel and in the wide estimator kernel."""
+@pytest.mark.parametrize("measure_kernel,pair_kernel", [(1, 1), (2, 1), (1, 0)])
+def test_whole_chain_estimators_and_histograms_match_oracle_fp64(measure_kernel, pair_kernel):
+    """Pair and permutation chains, FP64: the E, Ev and Epair blocks and both
+    histograms equal the oracle's, with the whole-chain kernel's estimators
+    inside the sweep kernel and in the wide estimator kernel, and (pair_kernel
+    0) with the cell-list chain on the window kernels and their estimator
+    kernel (k_meas_win)."""
     rng = np.random.default_rng(107)
     descs = pair_descs()
     inits = [rng.uniform(-5.0, 5.0, size=(110, 2, 16)), rng.uniform(-3.0, 3.0, size=(30, 2, 16)),
              rng.uniform(-5.0, 5.0, size=(96, 2, 96)), rand_paths(rng, 4, 12, 5.0, scale=0.6)]
     seed, spl, n_sweeps, rb, rmax, cb = 23, 2, 4, 16, 5.0, 10
-    ctx = make(descs, A.FP64, seed, spl, rb, rmax, cb, measure_kernel=measure_kernel)
+    ctx = make(descs, A.FP64, seed, spl, rb, rmax, cb, measure_kernel=measure_kernel, pair_kernel=pair_kernel)
     for c, r in enumerate(inits):
         ctx.set_paths(c, r)
     ctx.run(n_sweeps, measure=True)
