@@ -523,6 +523,44 @@ def test_sweep_trajectory_pair_fp64(N, L, ra, pair_kernel):
     assert ctx.blocks(0)[2][0] > 0.0
 
 
+@pytest.mark.parametrize("precision", [A.FP64, A.FP32])
+def test_window_kernels_mixed_chains(precision):
+    """Several Gaussian-pair chains on the window kernels in one launch, of
+    different N (not multiples of 64), M and ell: ell dividing M, ell not
+    dividing M (a shorter last window), a last window of one link (no staging
+    bead), M above 256 (centre-of-mass windows of M/32 slices), 1-cycle rings
+    only.  FP64: the oracle's paths to 1e-9, its accept counts and estimator
+    blocks; FP32: the oracle's accept counts and paths to FP32 rounding of the
+    box, on the FP32-rounded start."""
+    rng = np.random.default_rng(71)
+    pair = GaussianPair(1.0, 0.5)
+    specs = [(110, 16, 8, 5.0, 2.0), (130, 20, 8, 6.0, 2.0), (100, 17, 8, 5.0, 1.5), (97, 264, 6, 8.0, 2.0)]
+    descs, inits = [], []
+    for N, M, ell, L, ra in specs:
+        descs.append(chain_desc(HarmonicTrap(0.3), dim=2, M=M, N=N, L=L, beta=2.0, lam=0.5, pair=pair, ell=ell,
+                                r_a=ra, com_step=0.5))
+        r = rng.uniform(-L, L, size=(N, 2, M))
+        if precision == A.FP32:
+            r = r.astype(np.float32).astype(np.float64)
+        inits.append(r)
+    seed, n_sweeps, spl = 13, 2, 2
+    ctx = make_ctx(descs, precision, seed=seed, spl=spl, pair_kernel=2)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    cnt = ctx.run(n_sweeps, measure=True)
+    ref = sweep_oracle(descs, inits, seed, n_sweeps, spl, measure=True)
+    assert cnt.levy_acc == sum(x[1] for x in ref) and cnt.com_acc == sum(x[2] for x in ref)
+    assert cnt.bead_moves == n_sweeps * sum(O.sweep_bead_moves(d.N, d.M, d.ell) for d in descs)
+    for c, d in enumerate(descs):
+        diff = ctx.get_paths(c) - ref[c][0]
+        diff -= 2 * d.L * np.round(diff / (2 * d.L))
+        if precision == A.FP64:
+            assert np.max(np.abs(diff)) < 1e-9, c
+            assert_blocks(ctx, c, ref[c][4], spl)
+        else:
+            assert np.max(np.abs(diff)) <= 1e-5 * d.L, (c, float(np.max(np.abs(diff))))
+
+
 def test_batch_fp32_runs_and_counts():
     from pimc.batch import trapped_batch, Batch
     wl = trapped_batch(n_chains=64, seed=3)
