@@ -90,6 +90,13 @@ typedef struct pimc_chain_desc {
     int32_t n_beams;         /* 1 .. PIMC_MAX_BEAMS */
     int32_t helical;         /* 1: beam b carries the phase a_b (normalized_intensity's helical) */
     double beam_angle[PIMC_MAX_BEAMS]; /* beam directions a_b: the wave vector is scale (sin a_b, cos a_b) */
+    /* the move weights of run!(s, n, updates) (src/simulation.jl:33-36), which draws each update with
+     * weight 1/every: here the staging pass (ReshapeLinear), the centre-of-mass pass (SingleCenterOfMass)
+     * and the swap tries (ReshapeSwapLinear) of a sweep run on the global sweeps s (the Philox sweep word)
+     * with s % every == 0, so each runs on a fraction 1/every of the sweeps.  0 or 1: every sweep (the
+     * default); at most 65536.  Each pass keeps detailed balance on its own, so the weights change the
+     * mixing rate, not the sampled distribution */
+    int32_t staging_every, com_every, swap_every;
 } pimc_chain_desc;
 
 typedef struct pimc_options {

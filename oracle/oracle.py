@@ -87,6 +87,8 @@ def lib():
         L.o_u01.argtypes = [C.c_uint32]; L.o_u01.restype = C.c_double
         L.o_sweep_perm.argtypes = [C.POINTER(OSys), C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_double,
                                    C.c_int, C.c_int, C.c_int, C.POINTER(OSweepStats)]
+        L.o_sweep_every.argtypes = [C.POINTER(OSys), C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_double,
+                                    C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(OSweepStats)]
         L.o_init_world.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                    C.c_double, _D]
         L.o_init_world_hs.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -414,6 +416,14 @@ class System:
         """o_sweep_perm: the sweep with permutation cycles and n_swap swap tries."""
         st = OSweepStats()
         lib().o_sweep_perm(self.ref, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule, C.byref(st))
+        return st
+
+    def sweep_every(self, chain, sweep, seed, ell, com_step, every=(1, 1, 1), n_swap=0, swap_m=0, rule=0):
+        """o_sweep_every: the sweep whose staging pass, COM pass and swap tries
+        run only on the sweeps with sweep % every == 0 (every = (staging, com, swap))."""
+        st = OSweepStats()
+        lib().o_sweep_every(self.ref, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule,
+                            int(every[0]), int(every[1]), int(every[2]), C.byref(st))
         return st
 
     @property

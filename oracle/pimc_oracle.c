@@ -1285,8 +1285,12 @@ static int hs_levy(const o_sys *s, int p, int start, double *rp, int nrows, cons
 }
 
 static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
-                       int n_swap, int swap_m, int rule, o_sweepstats *st)
+                       int n_swap, int swap_m, int rule, const int every[3], o_sweepstats *st)
 {
+    /* move weights: the staging pass, the COM pass and the swap tries run on the sweeps with
+     * sweep % every == 0 (run! draws each update with weight 1/every, src/simulation.jl:33-36) */
+    const int st_on = sweep % (uint32_t)every[0] == 0, com_on = sweep % (uint32_t)every[1] == 0;
+    const int sw_on = sweep % (uint32_t)every[2] == 0;
     const int M = s->M, dim = s->dim, N = s->N;
     const int nseg = (M + ell - 1) / ell;
     const int use_pair = (s->pair.kind != O_PAIR_NONE) && s->pair_in_accept;
@@ -1300,7 +1304,7 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
      * disjoint beads, so any order of the windows gives this result. */
     keyed(seed, 0u, sweep, chain, PUR_OFFSET, x);
     const int off = (int)(((uint64_t)x[0] * (uint64_t)ell) >> 32);
-    for (int p = 0; p < N; ++p) {
+    for (int p = 0; p < (st_on ? N : 0); ++p) {
         for (int k = 0; k < nseg; ++k) {
             int m = (M - k * ell) < ell ? (M - k * ell) : ell;
             if (m < 2) continue;
@@ -1368,7 +1372,7 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
         }
     }
     /* COM phase: 1-cycles only (SingleCenterOfMass picks among next[i] == i) */
-    for (int p = 0; p < N; ++p) {
+    for (int p = 0; p < (com_on ? N : 0); ++p) {
         if (s->next[p] != p) continue;
         keyed(seed, (uint32_t)p, sweep, chain, PUR_COM, x);
         double dsp[3] = {0, 0, 0};
@@ -1426,22 +1430,32 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
                 for (int d = 0; d < dim; ++d) set_r(s, p, d, j, rp[d * M + j]);
         }
     }
-    if (n_swap > 0) o_swaps(s, chain, sweep, seed, n_swap, swap_m, rule, st);
+    if (n_swap > 0 && sw_on) o_swaps(s, chain, sweep, seed, n_swap, swap_m, rule, st);
     o_sys_refresh_V(s);
     free(rp);
     free(nrm);
 }
 
+static const int every1[3] = {1, 1, 1};
+
 void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
              o_sweepstats *st)
 {
-    sweep_core(s, chain, sweep, seed, ell, com_step, 0, 0, 0, st);
+    sweep_core(s, chain, sweep, seed, ell, com_step, 0, 0, 0, every1, st);
 }
 
 void o_sweep_perm(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
                   int swap_m, int rule, o_sweepstats *st)
 {
-    sweep_core(s, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule, st);
+    sweep_core(s, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule, every1, st);
+}
+
+void o_sweep_every(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
+                   int swap_m, int rule, int staging_every, int com_every, int swap_every, o_sweepstats *st)
+{
+    const int every[3] = {staging_every > 1 ? staging_every : 1, com_every > 1 ? com_every : 1,
+                          swap_every > 1 ? swap_every : 1};
+    sweep_core(s, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule, every, st);
 }
 
 /* free-particle propagator over m links, up to its normalisation:

@@ -197,6 +197,11 @@ void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, d
  * exp(w_updated - w_initial) only. */
 void o_sweep_perm(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
                   int swap_m, int rule, o_sweepstats *st);
+/* o_sweep_perm with move weights: the staging pass, the centre-of-mass pass and the swap tries run
+ * only when sweep % every == 0 (every <= 1: always), the sweep-granular form of run!'s draw of one
+ * update with weight 1/every (src/simulation.jl:33-36) */
+void o_sweep_every(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
+                   int swap_m, int rule, int staging_every, int com_every, int swap_every, o_sweepstats *st);
 /* the COM step adaptation applied between launches while thermalising */
 double o_adapt_step(double step, int64_t acc, int64_t tries, double L);
 /* the staging-length adaptation (NumbOfSlices) applied between thermalising launches */

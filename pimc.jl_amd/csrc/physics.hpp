@@ -41,7 +41,21 @@ struct ChainDev {
     int swaps, swap_m, swap_rule; // swap tries per sweep, links per try, acceptance rule
     int tab_n;
     int nbeam;
+    int every_st, every_com, every_sw; // passes run on the sweeps s with s % every == 0
 };
+
+// Whether a pass with move weight 1/every runs in sweep sw (every >= 1).
+__host__ __device__ __forceinline__ bool pass_on(int every, uint32_t sw)
+{
+    return every == 1 || sw % (uint32_t)every == 0u;
+}
+
+// Sweeps s in [s0, s0 + n) with s % every == 0 (every >= 1).
+__host__ __device__ inline long long every_count(unsigned s0, int n, int every)
+{
+    const unsigned long long e = (unsigned long long)every, a = s0, b = (unsigned long long)s0 + (unsigned)n;
+    return (long long)((b + e - 1) / e - (a + e - 1) / e);
+}
 
 // The chain constants converted to the arithmetic type (built once per chain
 // on the host, so the kernels load them instead of dividing per item).

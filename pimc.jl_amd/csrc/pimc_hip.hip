@@ -703,6 +703,8 @@ int validate(const pimc_chain_desc &d, std::string &why)
     if (d.ell != 0 && (d.ell < 2 || d.ell > 32 || d.ell > d.M)) return why = "ell must be in [2, min(32, M)]", -1;
     if (!(d.com_step >= 0)) return why = "com_step must be >= 0", -1;
     if (d.swaps < 0) return why = "swaps must be >= 0", -1;
+    for (int e : {d.staging_every, d.com_every, d.swap_every})
+        if (e < 0 || e > 65536) return why = "staging_every, com_every and swap_every must be in [0, 65536]", -1;
     if (d.swaps > 0) {
         if (d.pair != PIMC_PAIR_NONE) return why = "swap moves are for chains without a pair action", -1;
         if (d.N > 4096) return why = "swap moves need N <= 4096", -1;
@@ -927,6 +929,9 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             h.swap_rule = d.swap_rule;
             h.hs_a = d.hs_a;
             h.nbeam = d.potential == PIMC_POT_OPTICAL ? d.n_beams : 0;
+            h.every_st = std::max(1, d.staging_every);
+            h.every_com = std::max(1, d.com_every);
+            h.every_sw = std::max(1, d.swap_every);
             ellmax = std::max(ellmax, h.ell);
             c->is_whole[i] = (d.pair != PIMC_PAIR_NONE || opt->derivatives || d.swaps > 0) ? 1 : 0;
         }
@@ -967,7 +972,10 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             const int lx = next_pow2_maxl(c->ch[x].ell), ly = next_pow2_maxl(c->ch[y].ell);
             if (lx != ly) return lx < ly;
             if (c->ch[x].M != c->ch[y].M) return c->ch[x].M > c->ch[y].M;
-            return c->ch[x].ell < c->ch[y].ell;
+            if (c->ch[x].ell != c->ch[y].ell) return c->ch[x].ell < c->ch[y].ell;
+            // a tile holds chains of one pair of move weights
+            if (c->ch[x].every_st != c->ch[y].every_st) return c->ch[x].every_st < c->ch[y].every_st;
+            return c->ch[x].every_com < c->ch[y].every_com;
         });
         long long off = 0, celloff = 0, binoff = 0;
         int pbase = 0;
@@ -1060,7 +1068,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 bool fresh = c->tiles.empty();
                 if (!fresh) {
                     TileDev &t = c->tiles.back();
-                    fresh = t.M != h.M || t.ell != h.ell || c->tile_groups.back().pot != h.pot ||
+                    fresh = t.M != h.M || t.ell != h.ell || t.every_st != h.every_st || t.every_com != h.every_com ||
+                            c->tile_groups.back().pot != h.pot ||
                             c->tile_groups.back().maxl != next_pow2_maxl(h.ell) ||
                             (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
                     if (!fresh) {
@@ -1078,6 +1087,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                     t.M = h.M;
                     t.ell = h.ell;
                     t.nseg = h.nseg;
+                    t.every_st = h.every_st;
+                    t.every_com = h.every_com;
                     t.sh = 0;
                     while ((2 << t.sh) <= t.ell) ++t.sh; // floor(log2(ell))
                     t.pst = tile_particle_stride(t.M, t.ell, t.sh, c->dim);

@@ -69,6 +69,8 @@ struct TileDev {
                                // grouped: FP32 tile whose particles' segments are aligned
                                // lane groups (ell = MAXL a power of two dividing M, nseg a
                                // power of two <= 64)
+    int every_st, every_com;   // the staging and COM passes run on the sweeps s with
+                               // s % every == 0 (the same for every chain of the tile)
 };
 
 // Padded LDS row length of M slices skewed by j >> sh.
@@ -534,6 +536,8 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
         const uint32_t sw = sweep0 + (uint32_t)s;
         const bool meas_now = a.measure && meas_in == 0;
         meas_in = (meas_in == 0) ? a.measure_every - 1 : meas_in - 1;
+        // move weights: uniform over the workgroup (one value per tile)
+        const bool st_on = pass_on(tl.every_st, sw), com_on = pass_on(tl.every_com, sw);
         // per-particle draws of this sweep: staging offset, COM displacement and uniform
         for (int q = tid; q < np; q += nt) {
             const int gq = tl.q0 + q;
@@ -549,7 +553,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
         __syncthreads();
 
         // staging segments
-        for (int it = tid, q = q_first, k = k_first; it < items;
+        for (int it = st_on ? tid : items, q = q_first, k = k_first; it < items;
              it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk) {
             const int m = min(ell, M - k * ell);
             if (m < 2) continue;
@@ -575,6 +579,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
         __syncthreads();
 
         // centre of mass: partial Delta V per chunk of `ell` beads
+        if (com_on) {
         unsigned wrap = 0u; // bit r: chunk r of this thread has a moved bead outside the box
         for (int it = tid, q = q_first, k = k_first, r = 0; it < items;
              it += nt, q += dq + (k + dk >= nseg ? 1 : 0), k = (k + dk >= nseg) ? k + dk - nseg : k + dk, ++r) {
@@ -675,6 +680,7 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
             }
         }
         } // !grouped
+        } // com_on
         __syncthreads();
 
         // estimators (src/measurement.jl:45-55, 92-122): T sums over a chunk
@@ -1475,10 +1481,13 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
         const uint32_t sw = sweep0 + (uint32_t)s;
         const bool meas_now = a.measure && !a.split && meas_in == 0;
         meas_in = (meas_in == 0) ? a.measure_every - 1 : meas_in - 1;
+        // move weights: the passes run on the sweeps s with s % every == 0 (uniform per chain)
+        const bool st_on = pass_on(ch.every_st, sw), com_on = pass_on(ch.every_com, sw);
+        const bool sw_on = pass_on(ch.every_sw, sw);
         // ---- staging: particles in order, segments (and links) of one particle in parallel
         const U4 xo = draw(a.seed, 0u, sw, (uint32_t)c, kOffset); // one offset per chain and sweep
         const int off = (int)(((uint64_t)xo.x * (uint64_t)ell) >> 32);
-        for (int p = 0; p < N; ++p) {
+        for (int p = 0; p < (st_on ? N : 0); ++p) {
             T *buf = (p & 1) ? s_new1 : s_new0;
             // (A1) normals of the interior beads, by slice
             for (int w = tid; w < nseg * hp; w += nt) {
@@ -1745,7 +1754,7 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
         __syncthreads();
         // ---- centre of mass: one particle after another, 1-cycles only
         // (SingleCenterOfMass picks among next[i] == i, src/updates/com.jl:141-160)
-        for (int p = 0; p < N; ++p) {
+        for (int p = 0; p < (com_on ? N : 0); ++p) {
             if (perm && s_next[p] != p) continue; // uniform: LDS value
             ctry += (tid == 0) ? 1 : 0;
             const U4 y = draw(a.seed, (uint32_t)p, sw, (uint32_t)c, kCom);
@@ -1826,7 +1835,7 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
         }
         // ---- swap tries (chains without a pair action): the partner choice and the
         // acceptance are serial in particle order on one thread, as in the oracle
-        for (int aa = 0; aa < ch.swaps; ++aa) {
+        for (int aa = 0; aa < (sw_on ? ch.swaps : 0); ++aa) {
             const U4 xs = draw(a.seed, (uint32_t)aa, sw, (uint32_t)c, kSwap);
             const int m = ch.swap_m;
             const int j0 = (int)(((uint64_t)xs.x * (uint64_t)M) >> 32);
@@ -2103,12 +2112,14 @@ __global__ void k_post(PostArgs a)
     ChainDev &ch = a.chains[c];
     long long lt, lb; // staging tries and proposed beads of this launch (the launch's ell)
     staging_counts(ch, lt, lb);
-    lt *= a.nsweeps;
-    lb *= a.nsweeps;
+    // the sweeps of this launch that ran each pass (move weights); rs->sweep is the launch's first
+    const long long nst = every_count(a.rs->sweep, a.nsweeps, ch.every_st);
+    lt *= nst;
+    lb *= nst;
     long long la = 0, ca = 0;
     double q[kNQ];
     for (int k = 0; k < kNQ; ++k) q[k] = 0.0;
-    long long ct = (long long)ch.N * a.nsweeps, sa = 0, st = 0;
+    long long ct = (long long)ch.N * every_count(a.rs->sweep, a.nsweeps, ch.every_com), sa = 0, st = 0;
     if (a.is_whole[c]) {
         const double *src = a.chain_meas + (size_t)c * (kNQ + kCnt);
         for (int k = 0; k < kNQ; ++k) q[k] = src[k];

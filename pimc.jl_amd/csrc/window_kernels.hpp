@@ -551,12 +551,13 @@ template <typename T, int DIM> __global__ void __launch_bounds__(512) k_stage_wi
     if (w >= ch.nseg) return;
     const int m = min(ell, M - w * ell); // links of this window
     if (m < 2) return;                   // no interior bead
+    const uint32_t sw = a.rs->sweep + (uint32_t)a.sweep_off;
+    if (!pass_on(ch.every_st, sw)) return; // no staging pass in this sweep (move weights)
     if (cap < N) { // the host sizes the pool for every particle's partners
         if (tid == 0) atomicOr(a.status + c, 2);
         return;
     }
     const Prm<T> P = reinterpret_cast<const Prm<T> *>(a.prm)[c];
-    const uint32_t sw = a.rs->sweep + (uint32_t)a.sweep_off;
     const U4 xo = draw(a.seed, 0u, sw, (uint32_t)c, kOffset); // the chain's offset of this sweep
     const int off = (int)(((uint64_t)xo.x * (uint64_t)ell) >> 32);
     int s0 = off + w * ell;
@@ -770,12 +771,13 @@ __global__ void __launch_bounds__(512) k_com_spec(SweepArgs a, double *spec, flo
     const int M = ch.M, N = ch.N, lc = com_win_len(M);
     if (w * lc >= M) return;
     const int m = min(lc, M - w * lc), s0 = w * lc;
+    const uint32_t sw = a.rs->sweep + (uint32_t)a.sweep_off;
+    if (!pass_on(ch.every_com, sw)) return; // no centre-of-mass pass in this sweep
     if (cap < N) {
         if (tid == 0) atomicOr(a.status + c, 2);
         return;
     }
     const Prm<T> P = reinterpret_cast<const Prm<T> *>(a.prm)[c];
-    const uint32_t sw = a.rs->sweep + (uint32_t)a.sweep_off;
     const ComSpecLds L = com_spec_lds(N, lc, DIM, (int)sizeof(T), cap);
     T *ob = reinterpret_cast<T *>(smem + L.ob);
     T *sd = reinterpret_cast<T *>(smem + L.sd);
@@ -939,12 +941,13 @@ __global__ void __launch_bounds__(1024) k_com_dec(SweepArgs a, const double *spe
     const ChainDev ch = a.chains[c];
     const int N = ch.N, M = ch.M, tid = threadIdx.x, nt = blockDim.x, wave = tid >> 6;
     const int lc = com_win_len(M), nw = com_nwin(M);
+    const uint32_t sw = a.rs->sweep + (uint32_t)a.sweep_off;
+    if (!pass_on(ch.every_com, sw)) return; // no centre-of-mass pass in this sweep: no tries
     if (cap < N) {
         if (tid == 0) atomicOr(a.status + c, 2);
         return;
     }
     const Prm<T> P = reinterpret_cast<const Prm<T> *>(a.prm)[c];
-    const uint32_t sw = a.rs->sweep + (uint32_t)a.sweep_off;
     const ComDecLds L = com_dec_lds(N, DIM, (int)sizeof(T), cap);
     double *s_spec = reinterpret_cast<double *>(smem + L.spec);
     T *su = reinterpret_cast<T *>(smem + L.su);
@@ -1148,7 +1151,8 @@ __global__ void __launch_bounds__(256) k_com_apply(SweepArgs a, const unsigned c
     if (i >= (long long)ch.M * ch.N) return;
     const int j = (int)(i / ch.N), k = (int)(i - (long long)j * ch.N);
     const size_t q = (size_t)ch.part_base + k;
-    if (!cacc[q]) return;
+    // the flags are those of this sweep's k_com_dec only when the pass ran
+    if (!pass_on(ch.every_com, a.rs->sweep + (uint32_t)a.sweep_off) || !cacc[q]) return;
     const Prm<T> P = reinterpret_cast<const Prm<T> *>(a.prm)[c];
     const Path<T, DIM, true> path{reinterpret_cast<T *>(a.pos) + ch.pos_off, ch.M, ch.N};
     T r[DIM];

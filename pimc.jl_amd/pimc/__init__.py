@@ -22,7 +22,7 @@ from .potentials import GaussianPair, HarmonicTrap, NoPotential, OpticalLattice,
 from .propagator import PropInt, build_prop_int
 from .system import Particle, System, init_world
 from .updates import (Counter, NumbOfSlices, ReshapeLinear, SingleCenterOfMass, HIPSweep, Step, Update, UpdateC,
-                      acceptance, adjust, queue)
+                      acceptance, adjust, queue, sweep_weights)
 from .measurement import Density, Energy, Measurement, ZMeasurement, measurement_Z_sector
 from .simulation import apply, run
 from .batch import Batch
@@ -31,7 +31,7 @@ run_bang = run  # run! in the reference
 
 __all__ = [
     "System", "Particle", "init_world", "run", "run_bang", "apply",
-    "ReshapeLinear", "SingleCenterOfMass", "HIPSweep", "Update", "UpdateC",
+    "ReshapeLinear", "SingleCenterOfMass", "HIPSweep", "Update", "UpdateC", "sweep_weights",
     "Counter", "Step", "NumbOfSlices", "adjust", "queue", "acceptance",
     "Energy", "Density", "Measurement", "ZMeasurement", "measurement_Z_sector",
     "HarmonicTrap", "SquareLattice", "OpticalLattice", "generate_V", "NoPotential", "GaussianPair", "PropInt", "build_prop_int", "prop_int_terms",

@@ -42,6 +42,7 @@ class ChainDesc(C.Structure):
         ("swap_m", C.c_int32), ("swap_rule", C.c_int32), ("hs_a", C.c_double),
         ("beam_scale", C.c_double), ("n_beams", C.c_int32), ("helical", C.c_int32),
         ("beam_angle", C.c_double * 16),
+        ("staging_every", C.c_int32), ("com_every", C.c_int32), ("swap_every", C.c_int32),
     ]
 
 

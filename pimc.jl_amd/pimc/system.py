@@ -51,10 +51,13 @@ def init_world(N: int, M: int, dim: int, L: float, tau: float, lam: float, rng: 
 
 
 def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_in_accept=True, ell=0,
-               com_step=1.0, swaps=0, swap_m=0, swap_rule=0, hs_a=0.0) -> ChainDesc:
+               com_step=1.0, swaps=0, swap_m=0, swap_rule=0, hs_a=0.0, staging_every=1, com_every=1,
+               swap_every=1) -> ChainDesc:
     """pimc_chain_desc of one chain.  swaps > 0 adds that many swap tries per
     sweep (ReshapeSwapLinear) of swap_m links (0: ell); swap_rule 1 keeps the
-    reference's acceptance exp(w_updated - w_initial)."""
+    reference's acceptance exp(w_updated - w_initial).  The staging pass, the
+    centre-of-mass pass and the swap tries run on the sweeps s with
+    s % every == 0 (the move weights 1/every of run!, see sweep_weights)."""
     d = ChainDesc()
     d.N, d.M, d.dim = int(N), int(M), int(dim)
     d.potential = getattr(potential, "kind", 0)
@@ -83,6 +86,7 @@ def chain_desc(potential, *, dim, M, N, L, beta, lam, pair=None, r_a=0.0, pair_i
     d.com_step = float(com_step)
     d.swaps, d.swap_m, d.swap_rule = int(swaps), int(swap_m), int(swap_rule)
     d.hs_a = float(hs_a)
+    d.staging_every, d.com_every, d.swap_every = int(staging_every), int(com_every), int(swap_every)
     return d
 
 
@@ -105,6 +109,9 @@ class System:
     are redrawn on the device while they come closer than a to another
     particle (hardspherelevy!, move_polymer!, src/updates/helper.jl:141-181,
     368-395).  g = 0, the reference's default, gives a = 0.
+    staging_every, com_every and swap_every set the move weights of the
+    batched sweep (HIPSweep): `System(..., **sweep_weights(updates))` takes
+    them from a reference-style update list.
     """
 
     def __init__(self, potential=None, *, dV=None, dim: int = 2, M: int = 100, N: int = 2, mu: float = 0.0,
@@ -112,7 +119,7 @@ class System:
                  r_a: float = 0.0, length_measurement_cycle: int = 10, precision: str = "fp64",
                  seed: int | None = None, r0: np.ndarray | None = None, reference_pair_semantics: bool = False,
                  ell: int = 8, swaps: int = 0, swap_m: int = 0, swap_rule: int = 0, interactions: bool = False,
-                 propint=None, g: float = 0.0):
+                 propint=None, g: float = 0.0, staging_every: int = 1, com_every: int = 1, swap_every: int = 1):
         hs_a = 0.0
         if interactions:
             if propint is None:
@@ -134,7 +141,8 @@ class System:
         self.rng = np.random.default_rng(seed)
         desc = chain_desc(self.potential, dim=dim, M=M, N=N, L=L, beta=self.beta, lam=lam, pair=pair, r_a=r_a,
                           pair_in_accept=not reference_pair_semantics, ell=min(ell, M), swaps=swaps, swap_m=swap_m,
-                          swap_rule=swap_rule, hs_a=hs_a)
+                          swap_rule=swap_rule, hs_a=hs_a, staging_every=staging_every, com_every=com_every,
+                          swap_every=swap_every)
         opt = Options()
         opt.precision = FP64 if precision == "fp64" else FP32
         opt.sweeps_per_launch = 1

@@ -7,6 +7,7 @@ libpimc_hip.so.
 """
 from __future__ import annotations
 
+import math
 from collections import deque
 
 import numpy as np
@@ -130,3 +131,47 @@ class HIPSweep(UpdateC):
         acc = (cnt.levy_acc + cnt.com_acc) > 0
         queue(self.counter_var, acc)
         return acc
+
+
+_SWEEP_PASS = {"ReshapeLinear": "staging_every", "SingleCenterOfMass": "com_every",
+               "ReshapeSwapLinear": "swap_every"}
+
+
+def sweep_weights(updates) -> dict:
+    """The move weights of run!(s, n, updates) (src/simulation.jl:33-36) as the
+    pass periods of the batched sweep.
+
+    run! draws one update per iteration with weight 1/every.  The batched
+    sweep runs each pass (staging, centre of mass, swap tries) on the sweeps s
+    with s % every == 0, so the passes keep the relative frequencies
+    1/every.  The weights are relative, so the periods are divided by their
+    greatest common divisor: [(2, SingleCenterOfMass), (1, ReshapeLinear),
+    (1, ReshapeSwapLinear)], the reference's own test mix
+    (test/testsystem.jl:39-43), gives com_every=2, staging_every=1,
+    swap_every=1.  Entries name their update by class, instance or class
+    name.  A sweep always holds a staging and a centre-of-mass pass, so both
+    kinds must be listed; a list without ReshapeSwapLinear leaves
+    swap_every at 1 (give the System no swap tries).
+    """
+    found: dict[str, int] = {}
+    for every, f in updates:
+        name = f if isinstance(f, str) else (f.__name__ if isinstance(f, type) else type(f).__name__)
+        if name not in _SWEEP_PASS:
+            raise ValueError(f"{name} is not a pass of the batched sweep")
+        if int(every) != every or every < 1:
+            raise ValueError("every must be a positive integer")
+        key = _SWEEP_PASS[name]
+        if key in found:
+            raise ValueError(f"{name} listed twice")
+        found[key] = int(every)
+    for key in ("staging_every", "com_every"):
+        if key not in found:
+            raise ValueError("the batched sweep needs both ReshapeLinear and SingleCenterOfMass")
+    g = 0
+    for v in found.values():
+        g = math.gcd(g, v)
+    out = {k: v // g for k, v in found.items()}
+    out.setdefault("swap_every", 1)
+    if max(out.values()) > 65536:
+        raise ValueError("every / gcd must be <= 65536")
+    return out

@@ -9,7 +9,8 @@ import pytest
 from pimc import _capi as A
 from pimc.potentials import GaussianPair, HarmonicTrap, SquareLattice
 from pimc.system import chain_desc
-from pimc.updates import Counter, NumbOfSlices, Step, acceptance, adjust, queue
+from pimc.updates import (Counter, NumbOfSlices, ReshapeLinear, SingleCenterOfMass, Step, acceptance, adjust,
+                          queue, sweep_weights)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -55,6 +56,7 @@ def test_struct_layouts_match_header(tmp_path, cstruct, pystruct):
 @pytest.mark.parametrize("field,value,msg", [
     ("dim", 3, "dim"), ("N", 0, "N"), ("M", 2, "M"), ("beta", -1.0, "beta"), ("ell", 1, "ell"),
     ("ell", 40, "ell"), ("potential", 7, "potential"), ("sigma", 0.0, "sigma"),
+    ("com_every", -1, "every"), ("staging_every", 70000, "every"), ("swap_every", -3, "every"),
 ])
 def test_invalid_descriptors_are_rejected_before_any_device_call(field, value, msg):
     d = chain_desc(HarmonicTrap(1.0), dim=2, M=32, N=4, L=10.0, beta=1.0, lam=0.5, pair=GaussianPair(1.0, 0.5))
@@ -147,3 +149,23 @@ def test_prop_int_terms_checks_its_arguments_without_a_device():
         with pytest.raises(A.PimcError) as e:
             A.prop_int_terms(*args)
         assert "PIMC_ERR_ARG" in str(e.value)
+
+
+def test_sweep_weights_from_reference_update_lists():
+    """run!'s weights 1/every (src/simulation.jl:33-36) as pass periods: the
+    reference's test mix (test/testsystem.jl:39-43), a scaled copy of it (the
+    weights are relative), names as strings, and the lists refused."""
+    mix = [(2, SingleCenterOfMass), (1, ReshapeLinear), (1, "ReshapeSwapLinear")]
+    assert sweep_weights(mix) == {"com_every": 2, "staging_every": 1, "swap_every": 1}
+    assert sweep_weights([(e * 3, f) for e, f in mix]) == sweep_weights(mix)
+    assert sweep_weights([(1, "SingleCenterOfMass"), (1, "ReshapeLinear")]) == \
+        {"com_every": 1, "staging_every": 1, "swap_every": 1}
+    assert sweep_weights([(6, "SingleCenterOfMass"), (4, "ReshapeLinear")]) == \
+        {"com_every": 3, "staging_every": 2, "swap_every": 1}
+    for bad in ([(1, "ReshapeLinear")], [(1, "ReshapeLinear"), (1, "Energy"), (1, "SingleCenterOfMass")],
+                [(0, "ReshapeLinear"), (1, "SingleCenterOfMass")],
+                [(1, "ReshapeLinear"), (1, "ReshapeLinear"), (1, "SingleCenterOfMass")]):
+        with pytest.raises(ValueError):
+            sweep_weights(bad)
+    d = chain_desc(HarmonicTrap(1.0), dim=2, M=32, N=4, L=10.0, beta=1.0, lam=0.5, **sweep_weights(mix))
+    assert (d.staging_every, d.com_every, d.swap_every) == (1, 2, 1)

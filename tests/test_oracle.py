@@ -604,3 +604,71 @@ def test_thermodynamic_equals_virial_energy_with_pair():
     # the pair terms matter at this size: the pair energy is far from zero
     mp, ep = blockerr(Ep)
     assert mp > 20 * ep and mp > 0.05 * np.mean(Ev)
+
+
+# ---------------------------------------------------------------- move weights
+def _swap_trap(seed=5):
+    rng = np.random.default_rng(seed)
+    return O.System(rng.normal(0, 0.7, size=(4, 2, 16)), 0.5, 100.0, 1.0, O.make_pot(1, 1.0, 0.0, 1.0), None, 0.0,
+                    False)
+
+
+def test_move_weights_every_one_is_the_plain_sweep():
+    """every = (1, 1, 1) is o_sweep_perm bit for bit, swaps included."""
+    a, b = _swap_trap(), _swap_trap()
+    for sw in range(40):
+        sa = a.sweep_perm(0, sw, 7, 4, 0.8, 2, 5)
+        sb = b.sweep_every(0, sw, 7, 4, 0.8, (1, 1, 1), 2, 5)
+        assert (sa.levy_acc, sa.levy_tries, sa.com_acc, sa.com_tries, sa.swap_acc, sa.swap_tries) == \
+            (sb.levy_acc, sb.levy_tries, sb.com_acc, sb.com_tries, sb.swap_acc, sb.swap_tries)
+    assert np.array_equal(a.r, b.r) and np.array_equal(a.next, b.next)
+
+
+def test_move_weights_passes_switched_off():
+    """A sweep without its staging pass moves each ring rigidly (every bead
+    by one shift, modulo the box) or not at all; without its COM pass it is the
+    sweep whose COM shifts are zero (com_step = 0: accepted, nothing moves);
+    without its swap tries it is the sweep with no swap tries."""
+    L = 100.0
+    # staging off (sweeps 1..29 of staging_every = 1000)
+    s = harmonic_sys(5, 24, 1.5, seed=21)
+    for sw in range(1, 30):
+        r0 = s.r.copy()
+        st = s.sweep_every(0, sw, 3, 6, 1.0, (1000, 1, 1))
+        assert st.levy_tries == 0 and st.com_tries == 5
+        d = (s.r - r0 + L) % (2 * L) - L
+        assert np.allclose(d, d[:, :, :1], atol=1e-12)
+        moved = np.abs(d[:, :, 0]).sum(1) > 0
+        assert moved.sum() == st.com_acc
+    # COM off on odd sweeps of com_every = 2 == com_step 0
+    a, b = harmonic_sys(5, 24, 1.5, seed=22), harmonic_sys(5, 24, 1.5, seed=22)
+    for sw in range(1, 40, 2):
+        sa = a.sweep_every(0, sw, 9, 6, 1.0, (1, 2, 1))
+        sb = b.sweep(0, sw, 9, 6, 0.0)
+        assert sa.com_tries == 0 and sb.com_tries == 5 and sb.com_acc == 5
+        assert (sa.levy_acc, sa.levy_tries) == (sb.levy_acc, sb.levy_tries)
+    # the zero shift still passes through teleport, which rounds: equal to rounding
+    assert np.allclose(a.r, b.r, rtol=0, atol=1e-12)
+    # swap tries off on sweeps with sweep % 3 != 0
+    a, b = _swap_trap(8), _swap_trap(8)
+    for sw in range(0, 30):
+        sa = a.sweep_every(0, sw, 11, 4, 0.8, (1, 1, 3), 2, 5)
+        sb = b.sweep_perm(0, sw, 11, 4, 0.8, 2 if sw % 3 == 0 else 0, 5)
+        assert sa.swap_tries == (2 if sw % 3 == 0 else 0) == sb.swap_tries
+    assert np.array_equal(a.r, b.r) and np.array_equal(a.next, b.next)
+
+
+def test_move_weights_keep_the_sampled_distribution():
+    """Passes at weights 1/2 (staging) and 1/3 (COM) still sample the
+    primitive-action distribution: <E> and <x^2> against the closed form."""
+    N, M, beta = 4, 16, 2.0
+    s = harmonic_sys(N, M, beta, seed=31)
+    E, x2 = [], []
+    for sw in range(40000):
+        s.sweep_every(0, sw, 78, 4, 1.0, (2, 3, 1))
+        if sw >= 600:
+            E.append(s.energy()[0])
+            x2.append(np.mean(s.r ** 2))
+    for vals, exact in ((E, AN.energy_boltzmann_primitive(N, M, beta)), (x2, AN.x2_primitive(M, beta))):
+        m, e = blockerr(vals)
+        assert abs(m - exact) < 3.5 * e, (m, e, exact)
