@@ -1,11 +1,13 @@
 """The reference's own tests (test/testsystem.jl, test/testmeasurements.jl),
 restated on the Python mirror of its API over the C ABI, plus the swap moves
 of the batched sweep seen through the same API."""
+import math
+
 import numpy as np
 import pytest
 
-from pimc import (Density, Energy, HIPSweep, HarmonicTrap, ReshapeLinear, SingleCenterOfMass, SquareLattice, System,
-                  run)
+from pimc import (Density, Energy, HIPSweep, HarmonicTrap, NoPotential, OpticalLattice, ReshapeLinear,
+                  SingleCenterOfMass, SquareLattice, System, run, sweep_weights)
 
 pytestmark = pytest.mark.gpu
 
@@ -28,7 +30,6 @@ def test_system_with_the_references_optical_potential():
     scale 0.5, depth 8) of normalized_intensity, System(V; dim=2, M=100, N=5,
     L=4.0, T=1.0); testpotential.jl's intensity 1 at the lattice maxima; and the
     periodic-bounds check of testsystem.jl after run! (no bead outside [-L, L))."""
-    import math
     from pimc import OpticalLattice
     cyclic_4 = [2 * math.pi * k / 4 for k in range(4)]
     V = OpticalLattice(cyclic_4, 0.5, 8.0)
@@ -76,3 +77,67 @@ def test_swaps_through_the_sweep_functor():
     assert abs(d.dens.sum() / d.ndata - 4) < 1e-2
     with pytest.raises(Exception):
         ReshapeLinear(s, 8)(s)
+
+
+# the update mix of the reference's own tests (test/testsystem.jl:39-43, test/testmeasurements.jl:13-17)
+REF_MIX = [(2, "SingleCenterOfMass"), (1, "ReshapeLinear"), (1, "ReshapeSwapLinear")]
+
+
+def v1d_2d(depth=8.0, scale=0.5):
+    """The reference tests' v1d(x) = depth sin^2(2 pi x[1] scale) in the 2D
+    System, a potential of x alone: two counter-propagating beams (angles
+    +-pi/2) give normalized_intensity = cos^2(2 pi scale x), so -depth times it
+    is v1d - depth (a constant shift: same paths, same density)."""
+    return OpticalLattice([math.pi / 2, -math.pi / 2], scale, -depth)
+
+
+def mix_system(potential, seed):
+    """System(potential) with the reference's defaults (dim 2, M 100, N 2, L 4,
+    T 1, lambda 1) sampled by the batched sweep with the reference mix:
+    ReshapeLinear(s, 20) as ell = 20, ReshapeSwapLinear(s, 20) as swap tries
+    of 20 links, as many per sweep as staging tries (equal weights), COM
+    tries every second sweep (weight 1/2)."""
+    w = sweep_weights(REF_MIX)
+    assert w == {"com_every": 2, "staging_every": 1, "swap_every": 1}
+    return System(potential, seed=seed, ell=20, swaps=2 * 5, swap_m=20, **w)
+
+
+def test_reference_v1d_is_an_optical_lattice_of_two_beams():
+    V = v1d_2d()
+    rng = np.random.default_rng(3)
+    for r in rng.uniform(-4, 4, size=(50, 2)):
+        assert V(r) + 8.0 == pytest.approx(8.0 * math.sin(2 * math.pi * r[0] * 0.5) ** 2, abs=1e-12)
+
+
+def test_reference_periodic_bounds_as_written():
+    """testsystem.jl "Periodic Bounds": System(x -> 0), the reference mix
+    with swap tries, run! 10 000 times; no bead lies outside the box (the
+    reference counts beads with every coordinate outside; here no coordinate
+    is)."""
+    s = mix_system(NoPotential(), seed=11)
+    sweep = HIPSweep(s)
+    seen = set()
+    for _ in range(100):
+        sweep(s)
+        seen.add(tuple(s.next))
+    run(s, 10_000, [(1, sweep)])
+    r = s.paths
+    assert np.all(r >= -s.L) and np.all(r < s.L)
+    assert len(seen) > 1  # the swap tries exchange the two particles
+    c = s.ctx.counters(0)
+    assert c[3] > 0 and c[1] > 0
+
+
+def test_reference_density_with_v1d_as_written():
+    """testmeasurements.jl "Number of particles with potential": System(v1d),
+    the reference mix with swap tries, Density measured while running; the
+    density integrates to N within 1e-2."""
+    s = mix_system(v1d_2d(), seed=12)
+    sweep = HIPSweep(s)
+    run(s, 2000, [(1, sweep)])
+    d = Density(s)
+    run(s, 10_000, [(1, sweep)], Zmeasurements=[d])
+    numb = d.dens.sum() / d.ndata
+    assert d.ndata > 0
+    assert abs(numb - s.N) < 1e-2, numb
+    assert np.all(s.paths >= -s.L) and np.all(s.paths < s.L)
