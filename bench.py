@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     ap.add_argument("--measure-kernel", type=int, default=0, choices=[0, 1, 2],
                     help="estimators of whole chains: 0 automatic, 1 in the sweep kernel, 2 a wide kernel")
+    ap.add_argument("--pair-kernel", type=int, default=0, choices=[0, 1, 2],
+                    help="Gaussian-pair chains: 0 automatic, 1 the whole-chain kernel, 2 the window kernels")
     ap.add_argument("--workload", default="trapped_batch",
                     choices=["trapped_batch", "trapped_plain", "gauss_box", "trapped_gauss", "lattice", "contact_box"],
                     help="workload of the timed sweeps (the default is the headline batched form)")
@@ -214,7 +216,7 @@ def other_workload(args, A):
     bt = Batch = B.Batch
     t = time.perf_counter()
     b = Batch(wl, sweeps_per_launch=spl, seed=99, tile_beads=args.tile_beads, block_threads=args.threads,
-              measure_kernel=args.measure_kernel)
+              measure_kernel=args.measure_kernel, pair_kernel=args.pair_kernel)
     log(f"{args.workload}: {len(wl.descs)} chains, setup {time.perf_counter() - t:.1f} s")
     warm = rounded_steps(args.warmup, spl, 1)
     steps = rounded_steps(args.steps, spl, MIN_TIMED_LAUNCHES)
@@ -236,7 +238,8 @@ def other_workload(args, A):
         "ms_per_step": 1000.0 * wall / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if wl.precision == A.FP64 else "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {WORKLOAD_TEXT[args.workload]}", "chains": len(wl.descs),
-                   "sweeps_per_launch": spl, "bead_moves_per_sweep": int(cnt.bead_moves // steps)},
+                   "sweeps_per_launch": spl, "bead_moves_per_sweep": int(cnt.bead_moves // steps),
+                   "pair_kernel": args.pair_kernel},
         "roofline": roof,
         "energy": {"E_mean_over_chains": float(E.mean()), "E_sem": float(E.std(ddof=1) / math.sqrt(len(E)))
                    if len(E) > 1 else float(est[0].E_err),

@@ -87,6 +87,8 @@ def lib():
         L.o_u01.argtypes = [C.c_uint32]; L.o_u01.restype = C.c_double
         L.o_sweep_perm.argtypes = [C.POINTER(OSys), C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_double,
                                    C.c_int, C.c_int, C.c_int, C.POINTER(OSweepStats)]
+        L.o_swap_pair_lw.argtypes = [C.POINTER(OSys), C.c_int, C.c_int, C.c_int, C.c_int, _D, _D, C.c_int]
+        L.o_swap_pair_lw.restype = C.c_double
         L.o_sweep_every.argtypes = [C.POINTER(OSys), C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_double,
                                     C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(OSweepStats)]
         L.o_init_world.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -417,6 +419,12 @@ class System:
         st = OSweepStats()
         lib().o_sweep_perm(self.ref, chain, sweep, seed, ell, com_step, n_swap, swap_m, rule, C.byref(st))
         return st
+
+    def swap_pair_lw(self, n1, n2, j0, m, rp1, rp2, reference):
+        """o_swap_pair_lw: rp1, rp2 are (m + 1) x dim new paths (bead-major)."""
+        a = np.ascontiguousarray(np.asarray(rp1, float).T)
+        b = np.ascontiguousarray(np.asarray(rp2, float).T)
+        return lib().o_swap_pair_lw(self.ref, n1, n2, j0, m, _dp(a), _dp(b), int(reference))
 
     def sweep_every(self, chain, sweep, seed, ell, com_step, every=(1, 1, 1), n_swap=0, swap_m=0, rule=0):
         """o_sweep_every: the sweep whose staging pass, COM pass and swap tries

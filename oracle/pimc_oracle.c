@@ -881,12 +881,13 @@ void o_pair_energy(const o_sys *s, double *Epair, double *Evir_pair)
             bead(s, i, j, ri);
             int cnt = o_find_nns(s, ri, j, i, list, NNCAP);
             if (cnt > NNCAP) cnt = NNCAP;
-            int jn = (j + 1) % M;
             for (int q = 0; q < cnt; ++q) {
-                int k = list[q];
+                int k = list[q], in_, ij, kn, kj;
+                next_bead(s, i, j, &in_, &ij); /* next beads along the cycles */
+                next_bead(s, k, j, &kn, &kj);
                 bead(s, k, j, rk);
-                bead(s, i, jn, rin);
-                bead(s, k, jn, rkn);
+                bead(s, in_, ij, rin);
+                bead(s, kn, kj, rkn);
                 double a2 = 0.0, b2 = 0.0;
                 double da[3], db[3];
                 for (int d = 0; d < dim; ++d) {
@@ -981,14 +982,16 @@ double o_total_lnU(const o_sys *s)
     if (s->pair.kind == O_PAIR_NONE) return 0.0;
     double ri[3], rk[3], rin[3], rkn[3], r1[3], r2[3], w = 0.0;
     for (int j = 0; j < s->M; ++j) {
-        int jn = (j + 1) % s->M;
         for (int i = 0; i < s->N; ++i)
             for (int k = i + 1; k < s->N; ++k) {
                 bead(s, i, j, ri);
                 bead(s, k, j, rk);
                 if (periodic_norm(s, ri, rk) > s->h) continue;
-                bead(s, i, jn, rin);
-                bead(s, k, jn, rkn);
+                int in_, ij, kn, kj; /* next beads along the cycles */
+                next_bead(s, i, j, &in_, &ij);
+                next_bead(s, k, j, &kn, &kj);
+                bead(s, in_, ij, rin);
+                bead(s, kn, kj, rkn);
                 for (int d = 0; d < s->dim; ++d) {
                     r1[d] = o_distance(rk[d], ri[d], s->L);
                     r2[d] = o_distance(rkn[d], rin[d], s->L);
@@ -1209,15 +1212,17 @@ static double link_pair(const o_sys *s, int n, int j, const double *pj, const do
     if (cnt > NNCAP) cnt = NNCAP;
     double w = 0.0;
     for (int q = 0; q < cnt; ++q) {
-        int k = list[q];
+        int k = list[q], kn, kj;
+        next_bead(s, k, j, &kn, &kj); /* the neighbour's next bead along its cycle */
         bead(s, k, j, rk);
-        bead(s, k, jn, rkn);
+        bead(s, kn, kj, rkn);
         for (int d = 0; d < s->dim; ++d) {
             r1[d] = o_distance(rk[d], pj[d], s->L);
             r2[d] = o_distance(rkn[d], pj1[d], s->L);
         }
         w += o_lnU(r1, r2, s->dim, s->tau, &s->pair);
     }
+    (void)jn;
     return w;
 }
 
@@ -1349,12 +1354,15 @@ static void sweep_core(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, 
             if (use_pair) {
                 double du = 0.0, c0[3], c1[3];
                 for (int t = 0; t < m; ++t) {
+                    /* link t of the window: bead u = us + t of the polymer (its owner at slice j)
+                     * to bead u + 1 (along the cycle) */
                     int j = (start + t) % M, jn = (j + 1) % M;
+                    const int q0 = owner(s, p, us + t), q1 = owner(s, p, us + t + 1);
                     for (int d = 0; d < dim; ++d) {
                         a[d] = rp[d * nrows + t]; b[d] = rp[d * nrows + t + 1];
-                        c0[d] = R(s, p, d, j); c1[d] = R(s, p, d, jn);
+                        c0[d] = R(s, q0, d, j); c1[d] = R(s, q1, d, jn);
                     }
-                    du += link_pair(s, p, j, a, b) - link_pair(s, p, j, c0, c1);
+                    du += link_pair(s, q0, j, a, b) - link_pair(s, q0, j, c0, c1);
                 }
                 lw += du;
             }
@@ -1486,6 +1494,110 @@ static double rho0(const o_sys *s, const double *x, const double *y, int m)
  * bridges gives detailed balance.  rule 1 is the reference's exp(Delta lnW).
  * Accepted: new interiors written along the new cycles; for j0 + m < M the
  * beads j0 + m .. M - 1 of n1 and n2 are exchanged; next[n1] <-> next[n2]. */
+/* The number of times find_nns, queried at a, lists a bead at b: the stencil cells of a's cell that
+ * are b's cell (several when nbins < 3), if b lies within h (src/nearest_neighbours.jl:106-154) */
+static int pair_mult(const o_sys *s, const double *a, const double *b)
+{
+    if (periodic_norm(s, a, b) > s->h) return 0;
+    const int nb = s->nbins, dim = s->dim;
+    static const int st2[9][2] = {{0, 0}, {-1, 1}, {0, 1}, {1, 1}, {-1, 0}, {1, 0}, {-1, -1}, {0, -1}, {1, -1}};
+    static const int st1[3] = {0, -1, 1};
+    const int ba = o_bin(s, a), bb = o_bin(s, b);
+    const int bx = ba % nb, by = (dim >= 2) ? (ba / nb) % nb : 0;
+    int mult = 0;
+    for (int cc = 0; cc < ((dim == 1) ? 3 : 9); ++cc) {
+        const int cell = (dim == 1) ? mod1(bx + st1[cc], nb)
+                                    : mod1(bx + st2[cc][0], nb) + nb * mod1(by + st2[cc][1], nb);
+        mult += cell == bb;
+    }
+    return mult;
+}
+
+/* lnU of the link (x at slice j, its next bead xn) with neighbour k's beads at slice j and along k's cycle */
+static double link_term(const o_sys *s, int k, int j, const double *x, const double *xn)
+{
+    double rk[3], rkn[3], r1[3], r2[3];
+    int kn, kj;
+    next_bead(s, k, j, &kn, &kj);
+    bead(s, k, j, rk);
+    bead(s, kn, kj, rkn);
+    for (int d = 0; d < s->dim; ++d) {
+        r1[d] = o_distance(rk[d], x[d], s->L);
+        r2[d] = o_distance(rkn[d], xn[d], s->L);
+    }
+    return o_lnU(r1, r2, s->dim, s->tau, &s->pair);
+}
+
+/* The pair part of a swap try's log acceptance ratio, before the swap (the old cycles in s).
+ * Links t = 0..m-1 of the window from slice j0: the old paths run along the cycles of n1 and n2
+ * (beads of x1 = owner(n1, j0 + t), x2 = owner(n2, j0 + t) at slice j), the new paths are rp1,
+ * rp2 ((m + 1) x dim, column-major).
+ * reference = 1: ReshapeSwapLinear's sums as written (src/updates/reshape.jl:178-245): every lnU
+ * lands in w_initial, the old ones with the neighbour lists of x1 and x2 (each excluding itself
+ * only, so the x1-x2 pair twice), the new ones with both x1 and x2 excluded (no pair between the
+ * two new paths); the ratio is exp(-(those sums)).
+ * reference = 0: lnU(new) - lnU(old) over the same links, each pair once: the neighbours other
+ * than x1, x2 for each path, plus the pair of the two paths themselves. */
+static double swap_pair_lw(const o_sys *s, int n1, int n2, int j0, int m, const double *rp1, const double *rp2,
+                           int reference)
+{
+    const int M = s->M, dim = s->dim, nrows = m + 1;
+    int list[NNCAP];
+    double w_old = 0.0, w_new = 0.0;
+    for (int t = 0; t < m; ++t) {
+        const int u = j0 + t, j = u % M, jn = (j + 1) % M;
+        const int x[2] = {owner(s, n1, u), owner(s, n2, u)};
+        const int y[2] = {owner(s, n1, u + 1), owner(s, n2, u + 1)};
+        double o0[2][3], o1[2][3], a0[2][3], a1[2][3];
+        for (int w = 0; w < 2; ++w) {
+            const double *rp = w == 0 ? rp1 : rp2;
+            for (int d = 0; d < dim; ++d) {
+                o0[w][d] = R(s, x[w], d, j);
+                o1[w][d] = R(s, y[w], d, jn);
+                a0[w][d] = rp[d * nrows + t];
+                a1[w][d] = rp[d * nrows + t + 1];
+            }
+        }
+        for (int w = 0; w < 2; ++w) {
+            /* old: the neighbours of x_w's bead, x_w excluded */
+            int cnt = o_find_nns(s, o0[w], j, x[w], list, NNCAP);
+            if (cnt > NNCAP) cnt = NNCAP;
+            for (int q = 0; q < cnt; ++q) {
+                const int k = list[q];
+                if (!reference && k == x[1 - w] && w == 1) continue; /* the x1-x2 pair once (from x1's list) */
+                w_old += link_term(s, k, j, o0[w], o1[w]);
+            }
+            /* new: the neighbours of the new bead, x1 and x2 excluded */
+            cnt = o_find_nns(s, a0[w], j, x[0], list, NNCAP);
+            if (cnt > NNCAP) cnt = NNCAP;
+            for (int q = 0; q < cnt; ++q) {
+                const int k = list[q];
+                if (k == x[1]) continue;
+                w_new += link_term(s, k, j, a0[w], a1[w]);
+            }
+        }
+        if (!reference) {
+            /* the pair of the two new paths, as find_nns at the first path's bead would list it */
+            const int mult = pair_mult(s, a0[0], a0[1]);
+            if (mult > 0) {
+                double r1[3], r2[3];
+                for (int d = 0; d < dim; ++d) {
+                    r1[d] = o_distance(a0[1][d], a0[0][d], s->L);
+                    r2[d] = o_distance(a1[1][d], a1[0][d], s->L);
+                }
+                w_new += mult * o_lnU(r1, r2, dim, s->tau, &s->pair);
+            }
+        }
+    }
+    return reference ? -(w_old + w_new) : w_new - w_old;
+}
+
+double o_swap_pair_lw(const o_sys *s, int n1, int n2, int j0, int m, const double *rp1, const double *rp2,
+                      int reference)
+{
+    return swap_pair_lw(s, n1, n2, j0, m, rp1, rp2, reference);
+}
+
 static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int n_swap, int m, int rule,
                     o_sweepstats *st)
 {
@@ -1552,7 +1664,11 @@ static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int
                 dv += o_V(&s->pot, na, dim) - o_V(&s->pot, ob, dim);
             }
         }
-        double ratio = exp(-s->tau * dv);
+        double lw = -s->tau * dv;
+        /* pair sums: as the reference writes them when the pair sums of the other moves are left
+         * out of the ratio (pair_in_accept = 0), the physical difference otherwise */
+        if (s->pair.kind != O_PAIR_NONE) lw += swap_pair_lw(s, n1, n2, j0, m, rp1, rp2, !s->pair_in_accept);
+        double ratio = exp(lw);
         if (rule == 0) {
             /* selection normalisations before (E) and after (E' = E with n1, n2's ends exchanged) */
             double W2 = 0.0, W1p = 0.0, W2p = 0.0;

@@ -187,14 +187,17 @@ int  o_run_reference(o_sys *s, o_refstate *st, int64_t n_iter, uint64_t seed, in
 typedef struct { int64_t levy_acc, levy_tries, com_acc, com_tries, bead_moves, swap_acc, swap_tries; } o_sweepstats;
 void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step,
              o_sweepstats *st);
-/* The sweep with permutations (chains without a pair action): the staging
- * window of particle p runs along its permutation cycle (beads past slice
- * M-1 belong to next[p]), the centre-of-mass try moves 1-cycles only
- * (SingleCenterOfMass, src/updates/com.jl:136-160), and n_swap swap tries of
- * m links follow (ReshapeSwapLinear, src/updates/reshape.jl:99-283, with
- * sampleparticles, src/updates/helper.jl:224-267).  rule 0: acceptance with
- * the heat-bath normalisations (detailed balance); rule 1: the reference's
- * exp(w_updated - w_initial) only. */
+/* The sweep with permutations: the staging window of particle p runs along
+ * its permutation cycle (beads past slice M-1 belong to next[p]), the
+ * centre-of-mass try moves 1-cycles only (SingleCenterOfMass,
+ * src/updates/com.jl:136-160), and n_swap swap tries of m links follow
+ * (ReshapeSwapLinear, src/updates/reshape.jl:99-283, with sampleparticles,
+ * src/updates/helper.jl:224-267).  rule 0: acceptance with the heat-bath
+ * normalisations (detailed balance); rule 1: the reference's
+ * exp(w_updated - w_initial) only.  With a pair action every pair link follows
+ * the cycles (next()), and the swap adds its pair sums: as the reference
+ * writes them when pair_in_accept = 0, the pair action's difference when 1
+ * (o_swap_pair_lw). */
 void o_sweep_perm(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
                   int swap_m, int rule, o_sweepstats *st);
 /* o_sweep_perm with move weights: the staging pass, the centre-of-mass pass and the swap tries run
@@ -202,6 +205,12 @@ void o_sweep_perm(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int e
  * update with weight 1/every (src/simulation.jl:33-36) */
 void o_sweep_every(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
                    int swap_m, int rule, int staging_every, int com_every, int swap_every, o_sweepstats *st);
+/* The pair part of the log acceptance ratio of a swap try (window of m links from slice j0, the
+ * paths of n1 and n2 along their cycles replaced by rp1, rp2: (m + 1) x dim, rp[d * (m + 1) + i]):
+ * reference = 1 the sums of ReshapeSwapLinear as written (src/updates/reshape.jl:178-245, all in
+ * w_initial), 0 the difference of the pair action with each pair once */
+double o_swap_pair_lw(const o_sys *s, int n1, int n2, int j0, int m, const double *rp1, const double *rp2,
+                      int reference);
 /* the COM step adaptation applied between launches while thermalising */
 double o_adapt_step(double step, int64_t acc, int64_t tries, double L);
 /* the staging-length adaptation (NumbOfSlices) applied between thermalising launches */

@@ -27,7 +27,7 @@ struct HookArgs {
 template <typename T, int DIM> __device__ inline Cells hook_cells(const HookArgs &a, const ChainDev &ch)
 {
     return Cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, ch.M,
-                 ch.ncell, ch.cap, ch.nbins};
+                 ch.ncell, ch.cap, ch.nbins, ch.swaps > 0 ? a.next : nullptr};
 }
 
 template <typename T, int DIM, bool CELLS, typename PathT>

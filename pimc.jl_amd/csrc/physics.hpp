@@ -477,7 +477,10 @@ struct Cells {
     unsigned short *mem; // [M][ncell][cap]
     int *bin;            // [N][M]
     int M, ncell, cap, nbins;
+    const int *next = nullptr; // permutation (chains with swap moves): bead M-1 of k links to next[k]'s bead 0
     __device__ __forceinline__ int &count(int j, int c) const { return cnt[(long long)j * ncell + c]; }
+    // the particle holding the bead after particle k's bead at slice jn - 1 (jn == 0: across the ring's end)
+    __device__ __forceinline__ int after(int k, int jn) const { return (next != nullptr && jn == 0) ? next[k] : k; }
     __device__ __forceinline__ unsigned short *members(int j, int c) const
     {
         return mem + ((long long)j * ncell + c) * cap;
@@ -528,6 +531,28 @@ __device__ __forceinline__ bool cells_refile(const Cells &C, int n, int j, int o
     ncnt = nc0 + 1;
     b = nc;
     return true;
+}
+
+// Particles n1 and n2 exchange their beads at slice j (a swap move's tails):
+// their ids change places in the lists of slice j.
+__device__ __forceinline__ void cells_exchange(const Cells &C, int n1, int n2, int j)
+{
+    int &b1 = C.bin[(long long)n1 * C.M + j], &b2 = C.bin[(long long)n2 * C.M + j];
+    const int c1 = b1, c2 = b2;
+    if (c1 != c2) {
+        if (c1 >= 0) {
+            unsigned short *m = C.members(j, c1);
+            for (int i = 0, n = C.count(j, c1); i < n; ++i)
+                if (m[i] == (unsigned short)n1) m[i] = (unsigned short)n2;
+        }
+        if (c2 >= 0) {
+            unsigned short *m = C.members(j, c2);
+            for (int i = 0, n = C.count(j, c2); i < n; ++i)
+                if (m[i] == (unsigned short)n2) m[i] = (unsigned short)n1;
+        }
+    }
+    b1 = c2;
+    b2 = c1;
 }
 
 // Move particle n's bead at slice j from its stored cell to the cell of r.

@@ -706,7 +706,8 @@ int validate(const pimc_chain_desc &d, std::string &why)
     for (int e : {d.staging_every, d.com_every, d.swap_every})
         if (e < 0 || e > 65536) return why = "staging_every, com_every and swap_every must be in [0, 65536]", -1;
     if (d.swaps > 0) {
-        if (d.pair != PIMC_PAIR_NONE) return why = "swap moves are for chains without a pair action", -1;
+        if (d.pair == PIMC_PAIR_TABLE)
+            return why = "swap moves are for chains without a pair action or with the Gaussian pair", -1;
         if (d.N > 4096) return why = "swap moves need N <= 4096", -1;
         const int sm = d.swap_m ? d.swap_m : (d.ell ? d.ell : std::min(8, d.M));
         if (sm < 2 || sm > d.M - 1) return why = "swap_m must be in [2, M - 1]", -1;

@@ -964,7 +964,7 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
                     bool need = false;
 #pragma unroll
                     for (int v = 0; v < NQ; ++v) need = need || (lst[v] == u && near[v][r]);
-                    if (need) path.get(k[u][r], jn, rkn[u][r]);
+                    if (need) path.get(C.after(k[u][r], jn), jn, rkn[u][r]);
                 }
             }
 #pragma unroll
@@ -972,7 +972,8 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
 #pragma unroll
                 for (int r = 0; r < R; ++r)
                     if (near[u][r])
-                        term(u, lst[u] == 0 ? rk[0][r] : rk[u][r], lst[u] == 0 ? rkn[0][r] : rkn[u][r], r2[u][r], T(1));
+                        term(u, lst[u] == 0 ? k[0][r] : k[u][r], lst[u] == 0 ? rk[0][r] : rk[u][r],
+                             lst[u] == 0 ? rkn[0][r] : rkn[u][r], r2[u][r], T(1));
             }
         }
     } else {
@@ -1003,13 +1004,13 @@ __device__ __forceinline__ void scan_pairs(const PathT &path, const Cells &C, in
                     mult[u][r] = (cand && mm > 0 && r2[u][r] <= P.h2) ? T(mm) : T(0);
                     need = need || mult[u][r] > T(0);
                 }
-                if (need) path.get(kk, jn, rkn[r]);
+                if (need) path.get(C.after(kk, jn), jn, rkn[r]);
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
 #pragma unroll
                 for (int u = 0; u < NQ; ++u)
-                    if (mult[u][r] > T(0)) term(u, rk[r], rkn[r], r2[u][r], mult[u][r]);
+                    if (mult[u][r] > T(0)) term(u, k0 + r * G, rk[r], rkn[r], r2[u][r], mult[u][r]);
             }
         }
     }
@@ -1028,7 +1029,7 @@ __device__ T link_part(const PathT &path, const Cells &C, int N, int nbins, cons
     for (int e = 0; e < DIM; ++e) q[0][e] = pj[e];
     T acc = T(0);
     scan_pairs<T, DIM, CELLS, 1, 4>(path, C, N, nbins, P, n, j, jn, q, g, G,
-                                    [&](int, const T *rk, const T *rkn, T r2, T mult) {
+                                    [&](int, int, const T *rk, const T *rkn, T r2, T mult) {
                                         const T s2 = dist2<T, DIM>(rkn, pj1, P);
                                         acc += mult * pair_lnU<T, DIM, TAB>(rk, pj, rkn, pj1, r2, s2, P);
                                     });
@@ -1053,7 +1054,7 @@ __device__ __forceinline__ T link_delta(const PathT &path, const Cells &C, int N
         for (int e = 0; e < DIM; ++e) q[0][e] = a0[e];
         T acc = T(0);
         scan_pairs<T, DIM, CELLS, 1, 2 * R>(path, C, N, nbins, P, n, j, jn, q, g - u * h, h,
-                                            [&](int, const T *rk, const T *rkn, T r2, T mult) {
+                                            [&](int, int, const T *rk, const T *rkn, T r2, T mult) {
                                                 const T s2 = dist2<T, DIM>(rkn, a1, P);
                                                 acc += mult * pair_lnU<T, DIM, TAB>(rk, a0, rkn, a1, r2, s2, P);
                                             });
@@ -1067,12 +1068,25 @@ __device__ __forceinline__ T link_delta(const PathT &path, const Cells &C, int N
     }
     T acc[2] = {T(0), T(0)};
     scan_pairs<T, DIM, CELLS, 2, R>(path, C, N, nbins, P, n, j, jn, q, g, G,
-                                    [&](int u, const T *rk, const T *rkn, T r2, T mult) {
+                                    [&](int u, int, const T *rk, const T *rkn, T r2, T mult) {
                                         const T *qn = u == 0 ? n1 : o1;
                                         const T s2 = dist2<T, DIM>(rkn, qn, P);
                                         acc[u] += mult * pair_lnU<T, DIM, TAB>(rk, u == 0 ? n0 : o0, rkn, qn, r2, s2, P);
                                     });
     return acc[0] - acc[1];
+}
+
+// How many times the neighbour scan of a bead at a lists a bead at b: the
+// stencil slots of a's cell holding b's cell (several when nbins < 3), if b
+// lies within the cut-off; scan_pairs' multiplicity for one pair.
+template <typename T, int DIM> __device__ __forceinline__ T pair_mult(const T *a, const T *b, const Prm<T> &P, int nbins)
+{
+    if (!(dist2<T, DIM>(a, b, P) <= P.h2)) return T(0);
+    if (nbins >= 3) return T(1);
+    const int ca = cell_of<T, DIM>(a, P, nbins), cb = cell_of<T, DIM>(b, P, nbins);
+    int mm = 0;
+    for (int s = 0; s < ((DIM == 1) ? 3 : 9); ++s) mm += (stencil_cell(ca, s, nbins, DIM) == cb);
+    return T(mm);
 }
 
 template <typename T, int DIM, bool CELLS, typename PathT>
@@ -1234,14 +1248,14 @@ __device__ inline void pair_energy_bead(const PathT &path, const Cells &C, int N
     const int jn = (j + 1 == M) ? 0 : j + 1;
     T ri[DIM], rin[DIM];
     path.get(i, j, ri);
-    path.get(i, jn, rin);
+    path.get(C.after(i, jn), jn, rin);
     su = 0.0;
     rg = 0.0;
     T q[1][DIM];
 #pragma unroll
     for (int e = 0; e < DIM; ++e) q[0][e] = ri[e];
     scan_pairs<T, DIM, CELLS, 1, 4>(path, C, N, nbins, P, i, j, jn, q, g, G,
-                                    [&](int, const T *, const T *rkn, T r2, T mult) {
+                                    [&](int, int, const T *, const T *rkn, T r2, T mult) {
                                         const T s2 = dist2<T, DIM>(rkn, rin, P);
                                         const double ea = (double)ex(-r2 * P.inv2s2), eb = (double)ex(-s2 * P.inv2s2);
                                         const double w = 0.5 * (0.5 * (double)P.tau) * (double)mult;
@@ -1420,8 +1434,8 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
         __syncthreads();
     }
     const Path<T, DIM, true> path{LDSR ? lpath : gpath, M, N};
-    const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, M, ch.ncell, ch.cap,
-                      ch.nbins};
+    Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, M, ch.ncell, ch.cap,
+                ch.nbins};
     const bool use_pair = (ch.pair != kPairNone) && ch.pair_in_accept;
     // hard spheres (tabulated-propagator chains only): s.ctr of run!, src/simulation.jl:31-32
     const bool hs = TAB && ch.hs_a > 0.0;
@@ -1454,8 +1468,9 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
     o += align16(16 * sizeof(double));
     double *s_q = reinterpret_cast<double *>(smem + o); // estimator sums of the launch
     o += align16(kNQ * sizeof(double));
-    // permutation cycles (chains with swap moves): next[] in LDS for the launch
-    const bool perm = ch.swaps > 0;
+    // permutation cycles (chains with swap moves): next[] in LDS for the launch (never with the
+    // tabulated propagator, which pimc_create refuses with swaps: its kernels leave the code out)
+    const bool perm = !TAB && ch.swaps > 0;
     const int np_ = perm ? N : 0;
     int *s_next = reinterpret_cast<int *>(smem + o);
     o += align16((size_t)np_ * 4);
@@ -1469,6 +1484,7 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
     // the particle holding bead u of the window that starts on particle pp:
     // u < M on pp, u >= M on next[pp] (pp itself without permutations)
     auto owner = [&](int pp, int u) { return (perm && u >= M) ? s_next[pp] : pp; };
+    if (perm) cells.next = s_next; // pair links across the ring's end follow the cycles
 
     if (tid < kNQ) s_q[tid] = 0.0;
     int lacc = 0, cacc = 0, ctry = 0, sacc = 0, stry = 0, overflow = 0;
@@ -1696,9 +1712,9 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                                 for (int e = 0; e < DIM; ++e) s_old[j * DIM + e] = o0[e];
                             }
                         }
-                        if (use_pair)
-                            part = link_delta<T, DIM, CELLS, R, TAB ? 1 : 0>(path, cells, N, ch.nbins, P, p, j, jn, n0, n1, o0, o1, g,
-                                                                G);
+                        if (use_pair) // the neighbours of the bead's holder at slice j
+                            part = link_delta<T, DIM, CELLS, R, TAB ? 1 : 0>(path, cells, N, ch.nbins, P, owner(p, u), j, jn,
+                                                                             n0, n1, o0, o1, g, G);
                     }
                 }
                 if (use_pair) {
@@ -1746,8 +1762,9 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                     T rn[DIM];
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) rn[e] = buf[j * DIM + e];
-                    path.put(owner(p, j >= off ? j : j + M), j, rn);
-                    if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, p, j, rn, s_old + j * DIM);
+                    const int q = owner(p, j >= off ? j : j + M);
+                    path.put(q, j, rn);
+                    if (CELLS) overflow |= !cells_move<T, DIM>(cells, P, q, j, rn, s_old + j * DIM);
                 }
             }
         }
@@ -1833,9 +1850,9 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
             }
             __syncthreads();
         }
-        // ---- swap tries (chains without a pair action): the partner choice and the
-        // acceptance are serial in particle order on one thread, as in the oracle
-        for (int aa = 0; aa < (sw_on ? ch.swaps : 0); ++aa) {
+        // ---- swap tries: the partner choice and the acceptance are serial in particle order
+        // on one thread, as in the oracle; the pair sums are spread over the lane groups
+        for (int aa = 0; aa < ((perm && sw_on) ? ch.swaps : 0); ++aa) {
             const U4 xs = draw(a.seed, (uint32_t)aa, sw, (uint32_t)c, kSwap);
             const int m = ch.swap_m;
             const int j0 = (int)(((uint64_t)xs.x * (uint64_t)M) >> 32);
@@ -1900,6 +1917,66 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                 }
             }
             __syncthreads();
+            // pair sums (chains with a pair action, src/updates/reshape.jl:178-245): per path w and
+            // link t of the window, the neighbours of the old bead (holder x_w at slice j) and of the
+            // new bead, other than the two old holders x1, x2; the x1-x2 pair of the old beads comes
+            // with the old sums, twice as the reference lists it (ref: each holder's list excludes
+            // only itself) or once.  ref also adds the new sums to w_initial (the ratio takes minus
+            // both); otherwise the log ratio takes new minus old, plus the pair of the two new paths
+            // (Gaussian pair only: chains with the tabulated propagator and swaps are refused at create
+            // time, and leaving the code out of the table kernels keeps their registers)
+            const bool pairs = !TAB && ch.pair != kPairNone, ref = !ch.pair_in_accept;
+            auto swap_bead = [&](int w, int i, T *out) { // bead i = 0..m of new path w
+                const T *src = i == 0 ? s_B + (w == 0 ? n1 : n2) * DIM
+                                      : (i == m ? s_E + (w == 0 ? n2 : n1) * DIM : (w == 0 ? s_new0 : s_new1) + i * DIM);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) out[d] = src[d];
+            };
+            if constexpr (!TAB) if (pairs) {
+                for (int it = grp; it < 2 * m; it += ngroups) {
+                    const int w = it >= m ? 1 : 0, t = it - w * m;
+                    const int u = j0 + t, j = u >= M ? u - M : u, jn = (j + 1 == M) ? 0 : j + 1;
+                    const int x1 = owner(n1, u), x2 = owner(n2, u);
+                    const int xw = w == 0 ? x1 : x2, xo = w == 0 ? x2 : x1;
+                    // one scan per query (a rare move: the plain one-query scan keeps the registers low)
+                    T acc[2] = {T(0), T(0)};
+#pragma unroll
+                    for (int uq = 0; uq < 2; ++uq) { // 0: the new bead, 1: the old one
+                        T q[1][DIM], qn[DIM];
+                        if (uq == 0) {
+                            swap_bead(w, t, q[0]);
+                            swap_bead(w, t + 1, qn);
+                        } else {
+                            path.get(xw, j, q[0]);
+                            path.get(owner(w == 0 ? n1 : n2, u + 1), jn, qn);
+                        }
+                        const bool keep_xo = uq == 1 && (ref || w == 0);
+                        scan_pairs<T, DIM, CELLS, 1, 2>(path, cells, N, ch.nbins, P, xw, j, jn, q, g, G,
+                                                        [&](int, int k, const T *rk, const T *rkn, T r2, T mult) {
+                                                            if (k == xo && !keep_xo) return;
+                                                            acc[uq] += mult * pair_lnU<T, DIM, TAB ? 1 : 0>(
+                                                                                  rk, q[0], rkn, qn, r2, dist2<T, DIM>(rkn, qn, P), P);
+                                                        });
+                    }
+                    T part = ref ? -(acc[0] + acc[1]) : acc[0] - acc[1];
+                    part = group_sum_dpp(part, G);
+                    if (g == 0) (w == 0 ? s_du : s_dvb)[t] = part;
+                }
+                if (!ref) {
+                    for (int t = tid; t < m; t += nt) { // the pair of the two new paths at link t
+                        T a0[DIM], a1[DIM], b0[DIM], b1[DIM];
+                        swap_bead(0, t, a0);
+                        swap_bead(0, t + 1, a1);
+                        swap_bead(1, t, b0);
+                        swap_bead(1, t + 1, b1);
+                        const T mult = pair_mult<T, DIM>(a0, b0, P, ch.nbins);
+                        s_old[t] = mult > T(0) ? mult * pair_lnU<T, DIM, TAB ? 1 : 0>(b0, a0, b1, a1, dist2<T, DIM>(b0, a0, P),
+                                                                                      dist2<T, DIM>(b1, a1, P), P)
+                                               : T(0);
+                    }
+                }
+                __syncthreads();
+            }
             if (tid == 0) {
                 // Delta lnW over the interior beads of both paths (old beads along the old cycles)
                 T dv = T(0);
@@ -1913,7 +1990,10 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                         dv += potV<T, DIM>(bw + i * DIM, P) - potV<T, DIM>(ob, P);
                     }
                 }
-                T ratio = ex(-P.tau * dv);
+                T lw = -P.tau * dv;
+                if (pairs)
+                    for (int t = 0; t < m; ++t) lw += s_du[t] + s_dvb[t] + (ref ? T(0) : s_old[t]);
+                T ratio = ex(lw);
                 if (ch.swap_rule == 0) {
                     // reverse over forward partner-choice probability, either particle drawn first
                     const T *A = s_B + n1 * DIM, *C = s_E + n1 * DIM, *Bp = s_B + n2 * DIM, *D = s_E + n2 * DIM;
@@ -1933,11 +2013,21 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
             __syncthreads();
             if (s_sw[1]) { // uniform: LDS value
                 const int q1 = s_next[n1], q2 = s_next[n2];
-                for (int t = tid; t < 2 * (m - 1); t += nt) {
-                    const int w = t / (m - 1), i = t - w * (m - 1) + 1;
-                    const int u = j0 + i;
-                    const int q = u < M ? (w == 0 ? n1 : n2) : (w == 0 ? q2 : q1); // along the new cycles
-                    path.put(q, u >= M ? u - M : u, (w == 0 ? s_new0 : s_new1) + i * DIM);
+                // one thread per slice: the interior slices of the window, then the tails
+                for (int i = 1 + tid; i < m; i += nt) {
+                    const int u = j0 + i, j = u >= M ? u - M : u;
+                    const int qa = u < M ? n1 : q2, qb = u < M ? n2 : q1; // along the new cycles
+                    T oa[DIM], ob[DIM];
+                    if (CELLS) {
+                        path.get(qa, j, oa);
+                        path.get(qb, j, ob);
+                    }
+                    path.put(qa, j, s_new0 + i * DIM);
+                    path.put(qb, j, s_new1 + i * DIM);
+                    if (CELLS) {
+                        overflow |= !cells_move<T, DIM>(cells, P, qa, j, s_new0 + i * DIM, oa);
+                        overflow |= !cells_move<T, DIM>(cells, P, qb, j, s_new1 + i * DIM, ob);
+                    }
                 }
                 if (jm < M) {
                     for (int j = jm + tid; j < M; j += nt) {
@@ -1946,6 +2036,7 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                         path.get(n2, j, r2);
                         path.put(n1, j, r2);
                         path.put(n2, j, r1);
+                        if (CELLS) cells_exchange(cells, n1, n2, j);
                     }
                 }
                 __syncthreads();
@@ -2034,9 +2125,9 @@ template <typename T, int DIM, bool CELLS> __global__ void __launch_bounds__(256
     const int nb = ch.N * ch.M, per = (nb + a.nblk - 1) / a.nblk;
     const int b0 = min(nb, (int)blockIdx.x * per), b1 = min(nb, b0 + per);
     const Path<T, DIM, true> path{reinterpret_cast<T *>(const_cast<void *>(a.pos)) + ch.pos_off, ch.M, ch.N};
-    const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, ch.M, ch.ncell, ch.cap,
-                      ch.nbins};
     const int *nxt = ch.swaps > 0 ? a.next + ch.part_base : nullptr;
+    const Cells cells{a.cell_cnt + ch.cell_off, a.cell_mem + ch.mem_off, a.bins + ch.bin_off, ch.M, ch.ncell, ch.cap,
+                      ch.nbins, nxt};
     const ChainMeas m = measure_chain<T, DIM, CELLS>(path, cells, ch, P, wbuf, a.radial, a.radial_bins, a.radial_winv,
                                                       a.cart, a.cart_bins, c, 2, nxt, b0, b1);
     if (threadIdx.x == 0) {

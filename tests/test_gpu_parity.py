@@ -800,10 +800,11 @@ def test_set_next_validation():
         ctx.set_next(0, [1, 1, 0])  # not a permutation
     with pytest.raises(A.PimcError):
         ctx.set_next(1, [1, 0, 2])  # cycles need swap moves
-    pair = GaussianPair(1.0, 0.5)
-    with pytest.raises(A.PimcError):
-        make_ctx([chain_desc(NoPotential(), dim=2, M=8, N=3, L=4.0, beta=1.0, lam=0.5, pair=pair, ell=4, swaps=1)],
-                 A.FP64)
+    pair = GaussianPair(1.0, 0.5)  # swaps with the Gaussian pair are accepted
+    c2 = make_ctx([chain_desc(NoPotential(), dim=2, M=8, N=3, L=4.0, beta=1.0, lam=0.5, pair=pair, ell=4, swaps=1)],
+                  A.FP64)
+    c2.set_next(0, [2, 0, 1])
+    assert list(c2.get_next(0)) == [2, 0, 1]
 
 
 def test_split_estimators_match_in_kernel_fp64():
@@ -976,3 +977,59 @@ def test_set_pair_table_keeps_the_adapted_staging_length():
     assert ctx.ell(0) == ell
     cnt = ctx.run(2, measure=False)
     assert cnt.levy_tries > 0
+
+
+@pytest.mark.parametrize("N,L,ra", [(9, 2.0, 1.0), (12, 1.5, 1.5), (110, 5.0, 2.0)])
+@pytest.mark.parametrize("pia", [0, 1])
+def test_swap_sweeps_with_pairs_match_oracle_fp64(N, L, ra, pia):
+    """Swap tries on chains with a Gaussian pair action == the oracle, FP64:
+    the pair link sums along the permutation cycles in the staging and COM
+    moves, the swap's pair sums as the reference writes them (pair_in_accept
+    = 0) or as the pair action's difference (1), cell lists kept in step
+    through the swaps (N = 110), the multiplicities of nbins = 2; paths,
+    next[], every counter and the (E, Ev, Epair) blocks."""
+    rng = np.random.default_rng(47 + N)
+    pair = GaussianPair(1.0, 0.5)
+    descs, inits = [], []
+    for rule, m in ((0, 6), (1, 9)):
+        descs.append(chain_desc(HarmonicTrap(0.5), dim=2, M=16, N=N, L=L, beta=1.0, lam=0.5, pair=pair, ell=8, r_a=ra,
+                                com_step=0.4, swaps=6, swap_m=m, swap_rule=rule, pair_in_accept=bool(pia)))
+        inits.append(rng.uniform(-L, L, size=(N, 1, 1)).repeat(2, axis=1).repeat(16, axis=2)
+                     + rng.normal(0, 0.15, size=(N, 2, 16)))
+        inits[-1] = (inits[-1] + L) % (2 * L) - L
+    seed, n_sweeps = 37, 8
+    ctx = make_ctx(descs, A.FP64, seed=seed, spl=n_sweeps)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+    cnt = ctx.run(n_sweeps, measure=True)
+    tot = dict(levy_acc=0, com_acc=0, com_tries=0, swap_acc=0, swap_tries=0)
+    for c, (d, r) in enumerate(zip(descs, inits)):
+        s = oracle_sys(d, r)
+        meas = []
+        for sw in range(n_sweeps):
+            st = s.sweep_perm(c, sw, seed, d.ell, d.com_step, d.swaps, d.swap_m, d.swap_rule)
+            for k in tot:
+                tot[k] += getattr(st, k)
+            meas.append(oracle_meas(s))
+        assert np.array_equal(ctx.get_next(c), s.next), c
+        assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
+        assert_blocks(ctx, c, meas, n_sweeps)
+    for k in tot:
+        assert getattr(cnt, k) == tot[k], (k, getattr(cnt, k), tot[k])
+    assert tot["swap_acc"] > 0
+
+
+def test_swap_sweeps_with_pairs_fp32_run():
+    """FP32 swap tries with a pair action (cell lists): the permutation stays a
+    permutation, the estimators are finite, every try is counted."""
+    rng = np.random.default_rng(48)
+    d = chain_desc(HarmonicTrap(0.5), dim=2, M=32, N=120, L=5.0, beta=1.0, lam=0.5, pair=GaussianPair(1.0, 0.5),
+                   ell=8, r_a=2.0, com_step=0.4, swaps=16, swap_m=8)
+    ctx = make_ctx([d], A.FP32, seed=3, spl=20)
+    ctx.set_paths(0, rng.uniform(-5.0, 5.0, size=(120, 1, 1)).repeat(2, axis=1).repeat(32, axis=2))
+    ctx.run(40, measure=False)
+    cnt = ctx.run(40, measure=True)
+    assert sorted(ctx.get_next(0)) == list(range(120))
+    assert cnt.swap_tries == 40 * 16 and cnt.swap_acc > 0
+    E, Ev, Ep = ctx.blocks(0)
+    assert np.all(np.isfinite(E)) and np.all(np.isfinite(Ep)) and np.all(Ep > 0)

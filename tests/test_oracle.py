@@ -672,3 +672,78 @@ def test_move_weights_keep_the_sampled_distribution():
     for vals, exact in ((E, AN.energy_boltzmann_primitive(N, M, beta)), (x2, AN.x2_primitive(M, beta))):
         m, e = blockerr(vals)
         assert abs(m - exact) < 3.5 * e, (m, e, exact)
+
+
+# ---------------------------------------------------------------- swaps with a pair action
+def _apply_swap(r, nxt, n1, n2, j0, m, rp1, rp2):
+    """The commit of an accepted swap: the new beads along the new cycles, the
+    tails after the window exchanged, next[] of n1 and n2 exchanged."""
+    M = r.shape[2]
+    q1, q2 = int(nxt[n1]), int(nxt[n2])
+    for nw, qn, rp in ((n1, q2, rp1), (n2, q1, rp2)):
+        for i in range(1, m):
+            u = j0 + i
+            r[nw if u < M else qn, :, u % M] = rp[i]
+    if j0 + m < M:
+        r[[n1, n2], :, j0 + m:] = r[[n2, n1], :, j0 + m:]
+    nxt[n1], nxt[n2] = q2, q1
+
+
+def _swap_case(rng, N, M, L, nxt):
+    r = rng.uniform(-L, L, size=(N, 2, M))
+    s = O.System(r, 0.5, L, 1.0, O.make_pot(O.POT_NONE), O.make_pair(O.PAIR_GAUSSIAN, 1.0, 0.5), 1.0, True)
+    s.next = nxt
+    return s, r
+
+
+def _swap_paths(rng, r, nxt, n1, n2, j0, m, L):
+    """New paths of a swap try: the start beads of n1, n2 at j0, the ends of n2,
+    n1 at j0 + m along the cycles, interior beads anywhere in the box."""
+    M = r.shape[2]
+    jm = j0 + m
+    end = lambda n: r[n if jm < M else nxt[n], :, jm % M]
+    rp1 = np.vstack([r[n1, :, j0], rng.uniform(-L, L, size=(m - 1, 2)), end(n2)])
+    rp2 = np.vstack([r[n2, :, j0], rng.uniform(-L, L, size=(m - 1, 2)), end(n1)])
+    return rp1, rp2
+
+
+def test_swap_pair_sums_equal_the_pair_action_change():
+    """The swap's pair log ratio with each pair once (pair_in_accept = 1) is
+    the change of the total pair action (o_total_lnU, along the cycles) from
+    before the swap to after its commit: windows inside the ring and across its
+    end, particles on cycles of several lengths, nbins = 4."""
+    rng = np.random.default_rng(71)
+    N, M, L = 7, 12, 2.0
+    for trial in range(24):
+        nxt = np.array([1, 2, 0, 3, 5, 4, 6]) if trial % 2 else np.arange(N)
+        s, r = _swap_case(rng, N, M, L, nxt)
+        n1, n2 = rng.choice(N, size=2, replace=False)
+        j0, m = int(rng.integers(0, M)), int(rng.integers(2, M))
+        rp1, rp2 = _swap_paths(rng, r, nxt, n1, n2, j0, m, L)
+        lw = s.swap_pair_lw(int(n1), int(n2), j0, m, rp1, rp2, 0)
+        u0 = s.total_lnU()
+        r2, nx2 = r.copy(), nxt.copy()
+        _apply_swap(r2, nx2, n1, n2, j0, m, rp1, rp2)
+        s2, _ = _swap_case(rng, N, M, L, nx2)
+        s2.r = r2
+        assert abs(lw - (s2.total_lnU() - u0)) < 1e-11 * max(1.0, abs(u0)), trial
+
+
+def test_swap_pair_sums_as_the_reference_writes_them():
+    """Two particles (pair_in_accept = 0, ReshapeSwapLinear's sums): the old
+    sums list the pair once from each particle, the new sums exclude both, so
+    the log ratio is minus twice the old pair sums of the window's links, each
+    the reference's interaction_action! of the bead's holder."""
+    rng = np.random.default_rng(72)
+    N, M, L = 2, 10, 1.0
+    nonzero = 0
+    for trial in range(12):
+        nxt = np.array([1, 0]) if trial % 2 else np.arange(N)
+        s, r = _swap_case(rng, N, M, L, nxt)
+        j0, m = int(rng.integers(0, M)), int(rng.integers(2, M))
+        rp1, rp2 = _swap_paths(rng, r, nxt, 0, 1, j0, m, L)
+        lw = s.swap_pair_lw(0, 1, j0, m, rp1, rp2, 1)
+        old = sum(s.interaction_old(0 if j0 + t < M else int(nxt[0]), (j0 + t) % M) for t in range(m))
+        assert abs(lw + 2.0 * old) < 1e-12 * max(1.0, abs(old)), trial
+        nonzero += old != 0.0
+    assert nonzero >= 6
