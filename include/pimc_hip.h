@@ -120,12 +120,13 @@ typedef struct pimc_options {
                                   measurement_Z_sector (src/measurement.jl:1-17), counted in sweeps.
                                   sweeps_per_launch must be a multiple of it, and so must the sweep
                                   count of a measuring pimc_run */
-    int32_t adapt_ell;         /* 1: chains run whole per workgroup (pair action, swaps, derivatives) adapt
-                                  their staging length after every thermalising launch by adjust!(::NumbOfSlices)
-                                  (src/updates/helper.jl:42-52) on that launch's staging acceptance: -1 below
-                                  0.6, +1 above 0.8, within [2, min(32, M - 2)] (ReshapeLinear's minslices and
-                                  maxslices, src/updates/reshape.jl:12-21); tile chains keep theirs (set by the
-                                  autotuner).  pimc_get_ell reads the current value */
+    int32_t adapt_ell;         /* 1: every chain adapts its staging length after every thermalising launch by
+                                  adjust!(::NumbOfSlices) (src/updates/helper.jl:42-52) on that launch's staging
+                                  acceptance: -1 below 0.6, +1 above 0.8, within [2, min(32, M - 2)]
+                                  (ReshapeLinear's minslices and maxslices, src/updates/reshape.jl:12-21).  The
+                                  tiles of chains without a pair action are rebuilt between launches when a length
+                                  changes (a host synchronisation per thermalising launch); Gaussian-pair chains
+                                  then run on the whole-chain kernel.  pimc_get_ell reads the current value */
     int32_t measure_kernel;    /* estimators of chains run whole per workgroup, in measuring runs: 1 inside the
                                   sweep kernel; 2 in a wide kernel of their own after each sweep (the sweep
                                   kernel then runs one sweep per launch); 0: 2 when such a chain has a pair

@@ -136,8 +136,10 @@ struct pimc_ctx {
     std::vector<int> meas_ids;      // whole chains: cell-list chains first, then all-pairs chains
     int *d_meas_ids = nullptr;
     double *d_meas_part = nullptr;  // [id][blk][6] partial sums
+    std::vector<int> order;         // memory order of the chains
     std::vector<TileDev> tiles;
     std::vector<TileGroup> tile_groups;
+    size_t tiles_cap = 0;           // TileDev slots allocated at d_tiles
     std::vector<int> part_chain, part_idx;
     long long n_pos = 0, n_cell_cnt = 0, n_bins = 0;
     int n_part = 0;
@@ -818,6 +820,110 @@ int pimc_device_info(char *name, int32_t len, int32_t *cu_count)
     return PIMC_OK;
 }
 
+// Tiles of the LDS kernel over the tile chains in memory order: consecutive particles of chains
+// with one potential, MAXL, M, ell and pair of move weights share a tile, within the LDS share of 4
+// workgroups per CU; tiles form launch groups by (potential, MAXL).  At create time, and again
+// when a tile chain's staging length changes (adapt_ell).
+void build_tiles(pimc_ctx *c)
+{
+    const int ts = (int)tsize(c);
+    c->tiles.clear();
+    c->tile_groups.clear();
+    c->threads_tile = c->opt.block_threads > 0 ? c->opt.block_threads : 256;
+    if (c->threads_tile % 64 != 0 || c->threads_tile > kTileMaxThreads)
+        throw HipError{hipErrorInvalidValue, "block_threads must be a multiple of 64, <= 256"};
+    int tile_beads = c->opt.tile_beads > 0 ? c->opt.tile_beads : (ts == 4 ? 4096 : 2048);
+    const int max_items = kItemsPerThread * c->threads_tile;
+    size_t lds_max = 0;
+    for (int id : c->order) {
+        if (c->is_whole[id]) continue;
+        const ChainDev &h = c->ch[id];
+        if (h.nseg > max_items) throw HipError{hipErrorInvalidValue, "a particle has more segments than a workgroup holds; raise block_threads or ell"};
+        for (int p = 0; p < h.N; ++p) {
+            const int gq = h.part_base + p;
+            bool fresh = c->tiles.empty();
+            if (!fresh) {
+                TileDev &t = c->tiles.back();
+                fresh = t.M != h.M || t.ell != h.ell || t.every_st != h.every_st || t.every_com != h.every_com ||
+                        c->tile_groups.back().pot != h.pot ||
+                        c->tile_groups.back().maxl != next_pow2_maxl(h.ell) ||
+                        (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
+                if (!fresh) {
+                    // keep every tile within the LDS share of 4 workgroups per CU
+                    const int slots = t.nslot + (p == 0 ? 1 : 0);
+                    fresh = tile_lds_bytes(t.np + 1, c->dim, t.pst, (t.np + 1) * h.nseg, ts, slots,
+                                           c->opt.radial_bins) > kTileLdsBudget;
+                }
+            }
+            if (fresh) {
+                TileDev t{};
+                t.pos_off = h.pos_off + (long long)p * h.dim * h.M;
+                t.q0 = gq;
+                t.np = 0;
+                t.M = h.M;
+                t.ell = h.ell;
+                t.nseg = h.nseg;
+                t.every_st = h.every_st;
+                t.every_com = h.every_com;
+                t.sh = 0;
+                while ((2 << t.sh) <= t.ell) ++t.sh; // floor(log2(ell))
+                t.pst = tile_particle_stride(t.M, t.ell, t.sh, c->dim);
+                t.nslot = 0;
+                {
+                    const int maxl = next_pow2_maxl(h.ell);
+                    t.grouped = ts == 4 && h.ell == maxl && h.M % maxl == 0 && h.nseg <= 64 &&
+                                (h.nseg & (h.nseg - 1)) == 0;
+                }
+                if (c->tile_groups.empty() || c->tile_groups.back().pot != h.pot ||
+                    c->tile_groups.back().maxl != next_pow2_maxl(h.ell))
+                    c->tile_groups.push_back(TileGroup{h.pot, next_pow2_maxl(h.ell), (int)c->tiles.size(), 0});
+                c->tile_groups.back().count += 1;
+                c->tiles.push_back(t);
+            }
+            TileDev &t = c->tiles.back();
+            if (t.np == 0 || p == 0) t.nslot += 1; // chains are contiguous in the particle order
+            t.np += 1;
+            t.items = t.np * t.nseg;
+        }
+    }
+    for (TileDev &t : c->tiles) {
+        int nslot = 1;
+        for (int q = 1; q < t.np; ++q)
+            if (c->part_chain[t.q0 + q] != c->part_chain[t.q0 + q - 1]) ++nslot;
+        if (nslot != t.nslot) throw HipError{hipErrorUnknown, "tile chain-slot count mismatch"};
+        lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.pst, t.items, ts, t.nslot, c->opt.radial_bins));
+    }
+    c->lds_tile = lds_max;
+    if (lds_max > 160 * 1024) throw HipError{hipErrorInvalidValue, "tile exceeds LDS; lower tile_beads"};
+}
+
+// After a thermalising launch under adapt_ell: the staging lengths k_post chose for the tile
+// chains.  Where one changed, the host copy follows and the tiles are rebuilt before the next
+// launch (captured launch sequences hold the old tiles, so they are dropped).
+void retile_after_adapt(pimc_ctx *c)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<ChainDev> dev(c->n_chains);
+    HIPCHK(hipMemcpy(dev.data(), c->d_ch, sizeof(ChainDev) * c->n_chains, hipMemcpyDeviceToHost));
+    bool changed = false;
+    for (int i = 0; i < c->n_chains; ++i)
+        if (!c->is_whole[i] && dev[i].ell != c->ch[i].ell) {
+            c->ch[i].ell = dev[i].ell;
+            c->ch[i].nseg = dev[i].nseg;
+            changed = true;
+        }
+    if (!changed) return;
+    build_tiles(c);
+    if (c->tiles.size() > c->tiles_cap) {
+        HIPCHK(hipFree(c->d_tiles));
+        c->d_tiles = nullptr;
+        c->d_tiles = dalloc<TileDev>(c->tiles.size());
+        c->tiles_cap = c->tiles.size();
+    }
+    HIPCHK(hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice));
+    drop_graphs(c);
+}
+
 int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains, const pimc_options *opt)
 {
     if (!out || !chains || n_chains < 1 || !opt) return PIMC_ERR_ARG;
@@ -965,7 +1071,8 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         }
 
         // ---- memory order: tile chains sorted by (M, ell), then whole chains
-        std::vector<int> order(n_chains);
+        std::vector<int> &order = c->order;
+        order.resize(n_chains);
         for (int i = 0; i < n_chains; ++i) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
             if (c->is_whole[x] != c->is_whole[y]) return c->is_whole[x] < c->is_whole[y];
@@ -1054,72 +1161,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 c->part_idx[c->ch[i].part_base + p] = p;
             }
         const int ts = (int)tsize(c);
-        c->threads_tile = opt->block_threads > 0 ? opt->block_threads : 256;
-        if (c->threads_tile % 64 != 0 || c->threads_tile > kTileMaxThreads)
-            throw HipError{hipErrorInvalidValue, "block_threads must be a multiple of 64, <= 256"};
-        int tile_beads = opt->tile_beads > 0 ? opt->tile_beads : (ts == 4 ? 4096 : 2048);
-        const int max_items = kItemsPerThread * c->threads_tile;
-        size_t lds_max = 0;
-        for (int id : order) {
-            if (c->is_whole[id]) continue;
-            const ChainDev &h = c->ch[id];
-            if (h.nseg > max_items) throw HipError{hipErrorInvalidValue, "a particle has more segments than a workgroup holds; raise block_threads or ell"};
-            for (int p = 0; p < h.N; ++p) {
-                const int gq = h.part_base + p;
-                bool fresh = c->tiles.empty();
-                if (!fresh) {
-                    TileDev &t = c->tiles.back();
-                    fresh = t.M != h.M || t.ell != h.ell || t.every_st != h.every_st || t.every_com != h.every_com ||
-                            c->tile_groups.back().pot != h.pot ||
-                            c->tile_groups.back().maxl != next_pow2_maxl(h.ell) ||
-                            (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
-                    if (!fresh) {
-                        // keep every tile within the LDS share of 4 workgroups per CU
-                        const int slots = t.nslot + (p == 0 ? 1 : 0);
-                        fresh = tile_lds_bytes(t.np + 1, c->dim, t.pst, (t.np + 1) * h.nseg, ts, slots,
-                                               opt->radial_bins) > kTileLdsBudget;
-                    }
-                }
-                if (fresh) {
-                    TileDev t{};
-                    t.pos_off = h.pos_off + (long long)p * h.dim * h.M;
-                    t.q0 = gq;
-                    t.np = 0;
-                    t.M = h.M;
-                    t.ell = h.ell;
-                    t.nseg = h.nseg;
-                    t.every_st = h.every_st;
-                    t.every_com = h.every_com;
-                    t.sh = 0;
-                    while ((2 << t.sh) <= t.ell) ++t.sh; // floor(log2(ell))
-                    t.pst = tile_particle_stride(t.M, t.ell, t.sh, c->dim);
-                    t.nslot = 0;
-                    {
-                        const int maxl = next_pow2_maxl(h.ell);
-                        t.grouped = ts == 4 && h.ell == maxl && h.M % maxl == 0 && h.nseg <= 64 &&
-                                    (h.nseg & (h.nseg - 1)) == 0;
-                    }
-                    if (c->tile_groups.empty() || c->tile_groups.back().pot != h.pot ||
-                        c->tile_groups.back().maxl != next_pow2_maxl(h.ell))
-                        c->tile_groups.push_back(TileGroup{h.pot, next_pow2_maxl(h.ell), (int)c->tiles.size(), 0});
-                    c->tile_groups.back().count += 1;
-                    c->tiles.push_back(t);
-                }
-                TileDev &t = c->tiles.back();
-                if (t.np == 0 || p == 0) t.nslot += 1; // chains are contiguous in the particle order
-                t.np += 1;
-                t.items = t.np * t.nseg;
-            }
-        }
-        for (TileDev &t : c->tiles) {
-            int nslot = 1;
-            for (int q = 1; q < t.np; ++q)
-                if (c->part_chain[t.q0 + q] != c->part_chain[t.q0 + q - 1]) ++nslot;
-            if (nslot != t.nslot) throw HipError{hipErrorUnknown, "tile chain-slot count mismatch"};
-            lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.pst, t.items, ts, t.nslot, opt->radial_bins));
-        }
-        c->lds_tile = lds_max;
-        if (lds_max > 160 * 1024) throw HipError{hipErrorInvalidValue, "tile exceeds LDS; lower tile_beads"};
+        build_tiles(c);
         // whole-chain kernel
         int maxMw = 0;
         for (int id : c->whole_ids) maxMw = std::max(maxMw, c->ch[id].M);
@@ -1186,6 +1228,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             HIPCHK(hipMemcpy(c->d_prm, pr.data(), sizeof(Prm<float>) * n_chains, hipMemcpyHostToDevice));
         }
         c->d_tiles = dalloc<TileDev>(c->tiles.size());
+        c->tiles_cap = c->tiles.size();
         if (!c->tiles.empty())
             HIPCHK(hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice));
         c->d_pchain = dalloc<int>(c->n_part);
@@ -1687,6 +1730,8 @@ int pimc_run(pimc_ctx *c, int32_t n_sweeps, int32_t measure, pimc_counters *out)
                 launch_step(c, k, measure);
             }
             HIPCHK(hipEventRecord(c->events[li].b, c->stream));
+            // tile chains take their adapted staging lengths before the next launch
+            if (!measure && c->opt.adapt_ell && !c->tiles.empty()) retile_after_adapt(c);
             if (measure) c->nblk++;
             c->sweep += (uint32_t)k;
             left -= k;

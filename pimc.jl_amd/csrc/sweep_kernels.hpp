@@ -2256,7 +2256,7 @@ __global__ void k_post(PostArgs a)
     // acceptance: one slice less below 0.6, one more above 0.8, within
     // [minslices, maxslices] = [2, M - 2] (ReshapeLinear's defaults,
     // src/updates/reshape.jl:12-21), capped at 32 and never below 2
-    if (!a.measure && a.adapt_ell && a.is_whole[c] && lt > 0) {
+    if (!a.measure && a.adapt_ell && lt > 0) {
         const double acc = (double)la / (double)lt;
         int ell = ch.ell;
         if (acc < 0.6) ell -= 1;

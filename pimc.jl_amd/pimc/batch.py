@@ -141,11 +141,13 @@ class Batch:
     sweeps); measure_kernel chooses where whole chains take their estimators
     (pimc_options.measure_kernel: 0 automatic, 1 in the sweep kernel, 2 a
     wide kernel of their own); pair_kernel chooses the kernels of Gaussian-pair
-    chains (pimc_options.pair_kernel: 0 automatic, 1 whole-chain, 2 windows)."""
+    chains (pimc_options.pair_kernel: 0 automatic, 1 whole-chain, 2 windows);
+    adapt_ell=True lets every chain adapt its staging length between
+    thermalising launches (adjust!(::NumbOfSlices), pimc_options.adapt_ell)."""
 
     def __init__(self, wl: Workload, sweeps_per_launch: int = 50, seed: int = 12345, tile_beads: int = 0,
                  block_threads: int = 0, autotune: bool = False, use_graph: bool = False, tune_sweeps: int = 100,
-                 measure_every: int = 1, measure_kernel: int = 0, pair_kernel: int = 0):
+                 measure_every: int = 1, measure_kernel: int = 0, pair_kernel: int = 0, adapt_ell: bool = False):
         import copy
         descs = [copy.copy(d) for d in wl.descs]
         self.tune = None
@@ -169,6 +171,7 @@ class Batch:
         opt.measure_every = measure_every
         opt.measure_kernel = measure_kernel
         opt.pair_kernel = pair_kernel
+        opt.adapt_ell = 1 if adapt_ell else 0
         self.wl = wl
         self.descs = descs
         self.ctx = Context(descs, opt)

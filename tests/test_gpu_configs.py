@@ -256,3 +256,27 @@ def test_c5_lattice_matches_oracle_fp64():
     assert np.array_equal(hc, cart)
     assert cart.sum() > 0.99 * n * 400 * 512
     b.close()
+
+
+def test_adapted_staging_lengths_keep_the_energies_fp32():
+    """The batched trapped workload (FP32 tiles) with adapt_ell: the chains
+    leave the shared length 8 for lengths of their own (tiles rebuilt between
+    thermalising launches, graph replay on), within [2, min(32, M - 2)], and the
+    measured energies stay on the primitive-action closed form (per-chain
+    pulls of 16 blocks)."""
+    wl = B.trapped_batch(n_chains=192, seed=9)
+    b = B.Batch(wl, sweeps_per_launch=20, seed=5, use_graph=True, adapt_ell=True)
+    b.run(400)
+    ells = np.array([b.ctx.ell(c) for c in range(len(wl.descs))])
+    assert np.mean(ells != 8) > 0.5
+    assert np.all(ells >= 2) and np.all(ells <= np.minimum(32, wl.meta["M"] - 2))
+    b.run(16 * 20, measure=True)
+    assert np.array_equal(ells, [b.ctx.ell(c) for c in range(len(wl.descs))])
+    pulls = []
+    for c, d in enumerate(wl.descs):
+        e = b.estimates(c)
+        exact = AN.energy_boltzmann_primitive(d.N, d.M, d.beta)
+        pulls.append((e.E - exact) / e.E_err)
+    pulls = np.array(pulls)
+    assert np.mean(np.abs(pulls) < 2.5) > 0.9, np.sort(np.abs(pulls))[-10:]
+    assert abs(np.mean(pulls)) < 0.35

@@ -27,6 +27,7 @@ def make_ctx(descs, precision, seed=7, **kw):
     opt.derivatives = kw.get("derivatives", 0)
     opt.adapt_ell = kw.get("adapt_ell", 0)
     opt.pair_kernel = kw.get("pair_kernel", 0)
+    opt.use_graph = kw.get("use_graph", 0)
     return A.Context(descs, opt)
 
 
@@ -728,13 +729,15 @@ def test_swap_sweeps_match_oracle_fp64():
         assert getattr(cnt, k) == tot[k], (k, getattr(cnt, k), tot[k])
 
 
-def test_adapt_ell_matches_oracle_fp64():
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_adapt_ell_matches_oracle_fp64(use_graph):
     """adapt_ell: whole chains (swap cycles, pair action) change their staging
     length between thermalising launches by adjust!(::NumbOfSlices), on each
     launch's staging acceptance, in step with the oracle's sweep schedule with
     o_adapt_ell between launches: paths, next[], counters, the lengths read
-    back by pimc_get_ell.  The tile chain keeps its length, and so does every
-    chain during a measuring run."""
+    back by pimc_get_ell.  The tile chains adapt too, from one shared tile
+    into tiles of their own lengths (rebuilt between launches); every chain
+    keeps its length during a measuring run."""
     rng = np.random.default_rng(47)
     pair = GaussianPair(3.0, 0.5)
     descs = [
@@ -743,17 +746,20 @@ def test_adapt_ell_matches_oracle_fp64():
         chain_desc(HarmonicTrap(0.3), dim=2, M=20, N=24, L=3.0, beta=8.0, lam=0.5, pair=pair, ell=16, r_a=1.5,
                    com_step=0.5),
         harmonic_desc(N=5, M=16, beta=1.0, ell=4),
+        harmonic_desc(N=6, M=16, beta=6.0, ell=4),
+        chain_desc(HarmonicTrap(1.0), dim=2, M=40, N=3, L=100.0, beta=2.0, lam=0.5, ell=16, com_step=0.8),
     ]
     inits = [rand_paths(rng, 4, 24, 100.0, scale=0.6), rng.uniform(-3.0, 3.0, size=(24, 2, 20)),
-             rand_paths(rng, 5, 16, 100.0, scale=0.6)]
+             rand_paths(rng, 5, 16, 100.0, scale=0.6), rand_paths(rng, 6, 16, 100.0, scale=0.6),
+             rand_paths(rng, 3, 40, 100.0, scale=0.6)]
     seed, spl, n_launch = 31, 2, 6
-    ctx = make_ctx(descs, A.FP64, seed=seed, spl=spl, adapt_ell=1)
+    ctx = make_ctx(descs, A.FP64, seed=seed, spl=spl, adapt_ell=1, use_graph=use_graph)
     for c, r in enumerate(inits):
         ctx.set_paths(c, r)
     cnt_therm = ctx.run(spl * n_launch, measure=False)
-    ells_after = [ctx.ell(c) for c in range(3)]
+    ells_after = [ctx.ell(c) for c in range(len(descs))]
     cnt_meas = ctx.run(spl * 2, measure=True)
-    assert [ctx.ell(c) for c in range(3)] == ells_after
+    assert [ctx.ell(c) for c in range(len(descs))] == ells_after
     tot = dict(levy_acc=0, levy_tries=0, com_acc=0, com_tries=0, swap_acc=0, swap_tries=0)
     ells_oracle, changed = [], False
     for c, (d, r) in enumerate(zip(descs, inits)):
@@ -776,8 +782,7 @@ def test_adapt_ell_matches_oracle_fp64():
                 sweep += 1
             if not measure:
                 step = O.adapt_step(step, ca, ct, d.L)
-                if c < 2:  # whole chains only
-                    ell = O.adapt_ell(ell, la, lt, d.M)
+                ell = O.adapt_ell(ell, la, lt, d.M)
         ells_oracle.append(ell)
         changed |= ell != d.ell
         if d.swaps:
@@ -785,7 +790,8 @@ def test_adapt_ell_matches_oracle_fp64():
         assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
         assert ctx.com_step(c) == pytest.approx(step, rel=1e-12), c
     assert ells_after == ells_oracle, (ells_after, ells_oracle)
-    assert changed and ells_after[2] == descs[2].ell
+    # the tile chains left their shared length, in different directions
+    assert changed and ells_after[2] != descs[2].ell and ells_after[2] != ells_after[3]
     for k in tot:
         assert getattr(cnt_therm, k) + getattr(cnt_meas, k) == tot[k], (k, getattr(cnt_therm, k), tot[k])
 
