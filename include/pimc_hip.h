@@ -70,8 +70,9 @@ typedef struct pimc_chain_desc {
                                 interaction_action! are computed and dropped */
     int32_t ell;             /* staging segment length >= 2 (0: library default) */
     int32_t swaps;           /* swap tries per sweep (ReshapeSwapLinear, src/updates/reshape.jl:99-283);
-                                chains with swaps run whole per workgroup, with no pair action or the
-                                Gaussian pair.  With the pair the swap's pair sums follow pair_in_accept: 0 the reference's
+                                chains with swaps run whole per workgroup (with hard spheres, the bridges of
+                                a swap are redrawn as hardspherelevy! draws them, :157-162).  With a pair
+                                action the swap's pair sums follow pair_in_accept: 0 the reference's
                                 sums (old and new lnU both in w_initial, the pair of the two moved paths
                                 listed twice among the old and not among the new, :178-245); 1 the
                                 difference of the pair action, each pair once */

@@ -60,7 +60,7 @@ void o_box_muller(uint32_t x0, uint32_t x1, double *n0, double *n1)
 
 /* draw purposes: the fourth counter word of every keyed draw */
 enum { PUR_LEVY = 1, PUR_LEVY_ACC = 2, PUR_COM = 3, PUR_OFFSET = 4, PUR_INIT_START = 5, PUR_INIT_LEVY = 6,
-       PUR_SWAP = 7, PUR_SWAP_LEVY = 8, PUR_HS = 9, PUR_HS_COM = 10 };
+       PUR_SWAP = 7, PUR_SWAP_LEVY = 8, PUR_HS = 9, PUR_HS_COM = 10, PUR_HS_SWAP = 11 };
 
 static void keyed(uint64_t seed, uint32_t c0, uint32_t sweep, uint32_t chain, uint32_t purpose,
                   uint32_t out[4])
@@ -1243,14 +1243,27 @@ static int hs_near(const o_sys *s, const double *r, int j, int n)
     return periodic_norm(s, rk, r) < s->hs_a;
 }
 
-/* hardspherelevy! (src/updates/helper.jl:141-181) for particle p, the rows of
- * rp holding the segment from slice `start`: the end shift of levy!, then bead
- * by bead the bridge step, redrawn while the teleported bead lies closer than a
- * to its nearest neighbour.  Draw 1 of interior bead i is normals[i - 1]; draw
- * t >= 2 is Box-Muller of words 0, 1 of keyed(seed, p M + slice, sweep, chain,
- * PUR_HS | t << 8).  Returns 0 when some bead needs more than hs_ctr draws. */
+/* hardspherelevy! (src/updates/helper.jl:141-181), the rows of rp holding the
+ * segment from slice `start`: the end shift of levy!, then bead by bead the
+ * bridge step, redrawn while the teleported bead lies closer than a to its
+ * nearest neighbour other than particle `excl` (the reference's exception,
+ * fpcycle(mod1(j0 + j, M)): the first particle of the cycle the move names).
+ * Draw 1 of interior bead i is normals[i - 1]; draw t >= 2 is Box-Muller of
+ * words 0, 1 of keyed(seed, item, sweep, chain, pur | t << 8), item = base +
+ * slice (by_slice) or base + i.  Returns 0 when some bead needs more than
+ * hs_ctr draws. */
+static int hs_levy_x(const o_sys *s, int excl, int start, double *rp, int nrows, const double *normals,
+                     uint32_t chain, uint32_t sweep, uint64_t seed, uint32_t base, int by_slice, uint32_t pur);
+
+/* staging: particle p, redraws keyed by p M + slice and PUR_HS */
 static int hs_levy(const o_sys *s, int p, int start, double *rp, int nrows, const double *normals, uint32_t chain,
                    uint32_t sweep, uint64_t seed)
+{
+    return hs_levy_x(s, p, start, rp, nrows, normals, chain, sweep, seed, (uint32_t)(p * s->M), 1, PUR_HS);
+}
+
+static int hs_levy_x(const o_sys *s, int excl, int start, double *rp, int nrows, const double *normals,
+                     uint32_t chain, uint32_t sweep, uint64_t seed, uint32_t base, int by_slice, uint32_t pur)
 {
     const int dim = s->dim, last = nrows - 1, M = s->M;
     const double L = s->L;
@@ -1271,7 +1284,7 @@ static int hs_levy(const o_sys *s, int p, int start, double *rp, int nrows, cons
         for (int attempt = 1;; ++attempt) {
             if (attempt > s->hs_ctr) return 0;
             if (attempt > 1) {
-                keyed(seed, (uint32_t)(p * M + slice), sweep, chain, PUR_HS | ((uint32_t)attempt << 8), x);
+                keyed(seed, base + (uint32_t)(by_slice ? slice : j), sweep, chain, pur | ((uint32_t)attempt << 8), x);
                 o_box_muller(x[0], x[1], &nr[0], &nr[1]);
             }
             for (int d = 0; d < dim; ++d) {
@@ -1281,7 +1294,7 @@ static int hs_levy(const o_sys *s, int p, int start, double *rp, int nrows, cons
                 rp[d * nrows + j] = (a + b) + c;
                 t[d] = o_teleport(rp[d * nrows + j], L);
             }
-            if (!hs_near(s, t, slice, p)) break;
+            if (!hs_near(s, t, slice, excl)) break;
         }
     }
     for (int j = 0; j < nrows; ++j)
@@ -1636,7 +1649,11 @@ static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int
         }
         if (n2 == n1) continue;
         const double *Bp = B0 + n2 * dim, *D = E + n2 * dim;
-        /* the two bridges */
+        /* the two bridges; with a > 0 hardspherelevy!(r1, s, j0, fpcycle2) and (r2, s, j0, fpcycle1)
+         * (src/updates/reshape.jl:157-162): path 1's beads keep clear of every particle but n2, path 2's
+         * of every particle but n1; redraws keyed by (2a + w) M + i and PUR_HS_SWAP.  A bead out of draws
+         * rejects the try */
+        int bridged = 1;
         for (int w = 0; w < 2; ++w) {
             double *rp = w == 0 ? rp1 : rp2;
             const double *st0 = w == 0 ? A : Bp, *en = w == 0 ? D : C;
@@ -1650,9 +1667,14 @@ static void o_swaps(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int
                 nrm[(i - 1) * dim] = n0;
                 if (dim > 1) nrm[(i - 1) * dim + 1] = nn1;
             }
-            o_levy(rp, nrows, dim, s->tau, s->L, s->lambda, nrm);
+            if (s->hs_a > 0.0)
+                bridged &= hs_levy_x(s, w == 0 ? n2 : n1, j0, rp, nrows, nrm, chain, sweep, seed,
+                                     (uint32_t)((2 * a + w) * M), 0, PUR_HS_SWAP);
+            else
+                o_levy(rp, nrows, dim, s->tau, s->L, s->lambda, nrm);
             for (int d = 0; d < dim; ++d) { rp[d * nrows] = st0[d]; rp[d * nrows + m] = en[d]; }
         }
+        if (!bridged) continue;
         /* Delta lnW: interior beads of both paths, new minus old along the old cycles */
         double dv = 0.0, na[3], ob[3];
         for (int w = 0; w < 2; ++w) {

@@ -197,7 +197,11 @@ void o_sweep(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, d
  * exp(w_updated - w_initial) only.  With a pair action every pair link follows
  * the cycles (next()), and the swap adds its pair sums: as the reference
  * writes them when pair_in_accept = 0, the pair action's difference when 1
- * (o_swap_pair_lw). */
+ * (o_swap_pair_lw).  With hard spheres the swap's bridges are drawn as
+ * hardspherelevy!(r1, s, j0, fpcycle2) and (r2, s, j0, fpcycle1) draw them
+ * (src/updates/reshape.jl:157-162): redraws keyed by (2a + w) M + i and
+ * PUR_HS_SWAP, path 1 clear of every particle but n2, path 2 of every particle
+ * but n1; a bead out of draws rejects the try. */
 void o_sweep_perm(o_sys *s, uint32_t chain, uint32_t sweep, uint64_t seed, int ell, double com_step, int n_swap,
                   int swap_m, int rule, o_sweepstats *st);
 /* o_sweep_perm with move weights: the staging pass, the centre-of-mass pass and the swap tries run

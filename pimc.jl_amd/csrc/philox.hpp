@@ -25,6 +25,8 @@ enum Purpose : uint32_t {
     kHardSphere = 9, // redraw t >= 2 of a bridge bead (words x, y), item = particle * M + slice; the purpose
                      // word is kHardSphere | t << 8
     kHardSphereCom = 10, // redraw t >= 2 of a COM shift (words 0..dim-1), item = particle; kHardSphereCom | t << 8
+    kHardSphereSwap = 11, // redraw t >= 2 of bead i of swap try a's path w (words x, y), item = (2a + w) M + i;
+                          // kHardSphereSwap | t << 8
 };
 
 struct U4 {

@@ -130,6 +130,7 @@ struct pimc_ctx {
     // each of the three lists holds its Gaussian/pairless chains first, then from
     // this index the chains with the tabulated propagator (a kernel of their own)
     int cells_tab0 = 0, brute_tab0 = 0, lds_tab0 = 0;
+    int cells_tperm0 = 0, brute_tperm0 = 0, lds_tperm0 = 0; // tabulated-propagator chains with swaps
     // measuring steps with the estimators of whole chains in their own wide kernel
     bool split_meas = false;
     int meas_nblk = 1;
@@ -243,7 +244,7 @@ void launch_tiles(pimc_ctx *c, const SweepArgs &a)
     }
 }
 
-template <typename T, int DIM, bool CELLS, bool LDSR, bool TAB>
+template <typename T, int DIM, bool CELLS, bool LDSR, bool TAB, bool TPERM = false>
 void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &all, int lo, int hi, const int *d_ids)
 {
     if (hi <= lo) return;
@@ -260,7 +261,7 @@ void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &all, int l
     int G = 1;
     while (G < 16 && 2 * G * maxM <= nt) G <<= 1;
     a.link_group = std::max(2, G);
-    auto k = k_sweep_whole<T, DIM, CELLS, LDSR, TAB>;
+    auto k = k_sweep_whole<T, DIM, CELLS, LDSR, TAB, TPERM>;
     if (lds > 65536) HIPCHK(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k, dim3((unsigned)(hi - lo)), dim3(nt), lds, c->stream, a);
     HIPCHK(hipGetLastError());
@@ -269,12 +270,16 @@ void launch_whole_t(pimc_ctx *c, SweepArgs a, const std::vector<int> &all, int l
 template <typename T, int DIM> void launch_whole_d(pimc_ctx *c, const SweepArgs &a)
 {
     const auto &wc = c->whole_cells_ids, &wb = c->whole_brute_ids, &wl = c->whole_lds_ids;
+    // per list: analytic pairs, then tabulated-propagator chains, then those with swap moves
     launch_whole_t<T, DIM, true, false, false>(c, a, wc, 0, c->cells_tab0, c->d_whole_cells);
-    launch_whole_t<T, DIM, true, false, true>(c, a, wc, c->cells_tab0, (int)wc.size(), c->d_whole_cells);
+    launch_whole_t<T, DIM, true, false, true>(c, a, wc, c->cells_tab0, c->cells_tperm0, c->d_whole_cells);
+    launch_whole_t<T, DIM, true, false, true, true>(c, a, wc, c->cells_tperm0, (int)wc.size(), c->d_whole_cells);
     launch_whole_t<T, DIM, false, false, false>(c, a, wb, 0, c->brute_tab0, c->d_whole_brute);
-    launch_whole_t<T, DIM, false, false, true>(c, a, wb, c->brute_tab0, (int)wb.size(), c->d_whole_brute);
+    launch_whole_t<T, DIM, false, false, true>(c, a, wb, c->brute_tab0, c->brute_tperm0, c->d_whole_brute);
+    launch_whole_t<T, DIM, false, false, true, true>(c, a, wb, c->brute_tperm0, (int)wb.size(), c->d_whole_brute);
     launch_whole_t<T, DIM, false, true, false>(c, a, wl, 0, c->lds_tab0, c->d_whole_lds);
-    launch_whole_t<T, DIM, false, true, true>(c, a, wl, c->lds_tab0, (int)wl.size(), c->d_whole_lds);
+    launch_whole_t<T, DIM, false, true, true>(c, a, wl, c->lds_tab0, c->lds_tperm0, c->d_whole_lds);
+    launch_whole_t<T, DIM, false, true, true, true>(c, a, wl, c->lds_tperm0, (int)wl.size(), c->d_whole_lds);
 }
 
 void launch_whole(pimc_ctx *c, const SweepArgs &a)
@@ -708,8 +713,6 @@ int validate(const pimc_chain_desc &d, std::string &why)
     for (int e : {d.staging_every, d.com_every, d.swap_every})
         if (e < 0 || e > 65536) return why = "staging_every, com_every and swap_every must be in [0, 65536]", -1;
     if (d.swaps > 0) {
-        if (d.pair == PIMC_PAIR_TABLE)
-            return why = "swap moves are for chains without a pair action or with the Gaussian pair", -1;
         if (d.N > 4096) return why = "swap moves need N <= 4096", -1;
         const int sm = d.swap_m ? d.swap_m : (d.ell ? d.ell : std::min(8, d.M));
         if (sm < 2 || sm > d.M - 1) return why = "swap_m must be in [2, M - 1]", -1;
@@ -1189,6 +1192,14 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             c->cells_tab0 = part(c->whole_cells_ids);
             c->brute_tab0 = part(c->whole_brute_ids);
             c->lds_tab0 = part(c->whole_lds_ids);
+            // ... and among those, the chains with swap moves last (the TPERM instantiation)
+            auto part_perm = [&](std::vector<int> &v, int lo) {
+                return (int)(std::stable_partition(v.begin() + lo, v.end(), [&](int id) { return c->ch[id].swaps == 0; }) -
+                             v.begin());
+            };
+            c->cells_tperm0 = part_perm(c->whole_cells_ids, c->cells_tab0);
+            c->brute_tperm0 = part_perm(c->whole_brute_ids, c->brute_tab0);
+            c->lds_tperm0 = part_perm(c->whole_lds_ids, c->lds_tab0);
         }
         // estimators of whole chains: in the sweep kernel, or (split) a wide kernel after each sweep;
         // automatic: split when a whole chain with a pair action has >= 8192 beads

@@ -1417,7 +1417,7 @@ __host__ __device__ inline size_t whole_path_bytes(const ChainDev &h, int tsize)
 // Workgroup size: 1024 threads, or 512 with the tabulated propagator, whose
 // FP64 table lookups need more than the 128 VGPRs of a 1024-thread launch.
 template <bool TAB> constexpr int whole_threads() { return TAB ? 512 : 1024; }
-template <typename T, int DIM, bool CELLS, bool LDSR, bool TAB>
+template <typename T, int DIM, bool CELLS, bool LDSR, bool TAB, bool TPERM = false>
 __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs a)
 {
     static_assert(!(CELLS && LDSR), "LDS-resident paths are for all-pairs chains");
@@ -1468,9 +1468,11 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
     o += align16(16 * sizeof(double));
     double *s_q = reinterpret_cast<double *>(smem + o); // estimator sums of the launch
     o += align16(kNQ * sizeof(double));
-    // permutation cycles (chains with swap moves): next[] in LDS for the launch (never with the
-    // tabulated propagator, which pimc_create refuses with swaps: its kernels leave the code out)
-    const bool perm = !TAB && ch.swaps > 0;
+    // permutation cycles (chains with swap moves): next[] in LDS for the launch.  Chains with the
+    // tabulated propagator and swaps run in their own instantiation (TPERM), so that the others
+    // leave the permutation code out
+    constexpr bool kPerm = !TAB || TPERM;
+    const bool perm = kPerm && ch.swaps > 0;
     const int np_ = perm ? N : 0;
     int *s_next = reinterpret_cast<int *>(smem + o);
     o += align16((size_t)np_ * 4);
@@ -1589,12 +1591,13 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                         if (lane == 0) s_flag[k] = 0;
                         continue;
                     }
-                    const int start = off + k * ell >= M ? off + k * ell - M : off + k * ell;
+                    const int us = off + k * ell; // window position of the start bead (< 2M)
+                    const int start = us >= M ? us - M : us;
                     int jm = start + m;
                     if (jm >= M) jm -= M;
                     T cur[DIM], em[DIM];
-                    path.get(p, start, cur);
-                    path.get(p, jm, em);
+                    path.get(owner(p, us), start, cur); // the segment's ends along the cycle
+                    path.get(owner(p, us + m), jm, em);
 #pragma unroll
                     for (int e = 0; e < DIM; ++e) em[e] = shift_end(cur[e], em[e], P);
                     // cur: bead i - 1 of the segment; advance(i): bead i from its first draw
@@ -1676,7 +1679,7 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                             int b = start + i;
                             if (b >= M) b -= M;
                             T ro[DIM];
-                            path.get(p, b, ro);
+                            path.get(owner(p, us + i), b, ro);
 #pragma unroll
                             for (int e = 0; e < DIM; ++e) buf[b * DIM + e] = ro[e];
                         }
@@ -1890,7 +1893,11 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
             __syncthreads();
             const int n2 = s_sw[0];
             if (n2 == n1) continue; // uniform: LDS value
-            // the two bridges: n1 from r_n1(j0) to E_n2, n2 from r_n2(j0) to E_n1
+            // the two bridges: n1 from r_n1(j0) to E_n2, n2 from r_n2(j0) to E_n1.  With hard spheres
+            // hardspherelevy!(r1, s, j0, fpcycle2) and (r2, s, j0, fpcycle1) (src/updates/reshape.jl:157-
+            // 162): path 1's beads keep clear of every particle but n2, path 2's of every particle but n1;
+            // bead i is redrawn (draw t >= 2: kHardSphereSwap | t << 8, item (2a + w) M + i) while too
+            // close, and a bead out of draws rejects the try (s_sw[2 + w] = 0)
             if (tid < 2) {
                 const int w = tid;
                 T *bw = w == 0 ? s_new0 : s_new1;
@@ -1902,37 +1909,67 @@ __global__ void __launch_bounds__(whole_threads<TAB>()) k_sweep_whole(SweepArgs 
                     em[d] = shift_end(cur[d], en[d], P);
                 }
                 U4 xr{0u, 0u, 0u, 0u};
-                for (int i = 1; i < m; ++i) {
+                int placed_all = 1;
+                for (int i = 1; i < m && placed_all; ++i) {
                     if (i & 1) xr = draw(a.seed, (uint32_t)((2 * aa + w) * M + i), sw, (uint32_t)c, kSwapLevy);
                     T nn[2];
                     if (i & 1) box_muller(xr.x, xr.y, nn[0], nn[1]);
                     else box_muller(xr.z, xr.w, nn[0], nn[1]);
                     const T alpha = bridge_alpha<T>(m, i);
                     const T sd = bridge_sd(alpha, P);
+                    T nx[DIM], tb[DIM];
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
-                        cur[d] = bridge_step(cur[d], em[d], alpha, nn[d], sd);
-                        bw[i * DIM + d] = teleport(cur[d], P);
+                        nx[d] = bridge_step(cur[d], em[d], alpha, nn[d], sd);
+                        tb[d] = teleport(nx[d], P);
+                    }
+                    if (TAB && hs) {
+                        const int excl = w == 0 ? n2 : n1, slice = (j0 + i) % M;
+                        auto near = [&](const T *q) {
+                            bool r = false;
+                            for (int hg = 0; hg < kHsLanes; ++hg)
+                                r = r || hs_near_part<T, DIM, CELLS>(path, cells, N, P, excl, slice, q, hg);
+                            return r;
+                        };
+                        for (int t = 2; near(tb); ++t) {
+                            if (t > hs_ctr) {
+                                placed_all = 0;
+                                break;
+                            }
+                            const U4 xh = draw(a.seed, (uint32_t)((2 * aa + w) * M + i), sw, (uint32_t)c,
+                                               (uint32_t)kHardSphereSwap | ((uint32_t)t << 8));
+                            box_muller(xh.x, xh.y, nn[0], nn[1]);
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) {
+                                nx[d] = bridge_step(cur[d], em[d], alpha, nn[d], sd);
+                                tb[d] = teleport(nx[d], P);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        cur[d] = nx[d];
+                        bw[i * DIM + d] = tb[d];
                     }
                 }
+                s_sw[2 + w] = placed_all;
             }
             __syncthreads();
+            if (!(s_sw[2] && s_sw[3])) continue; // uniform: LDS values (a rejected try)
             // pair sums (chains with a pair action, src/updates/reshape.jl:178-245): per path w and
             // link t of the window, the neighbours of the old bead (holder x_w at slice j) and of the
             // new bead, other than the two old holders x1, x2; the x1-x2 pair of the old beads comes
             // with the old sums, twice as the reference lists it (ref: each holder's list excludes
             // only itself) or once.  ref also adds the new sums to w_initial (the ratio takes minus
             // both); otherwise the log ratio takes new minus old, plus the pair of the two new paths
-            // (Gaussian pair only: chains with the tabulated propagator and swaps are refused at create
-            // time, and leaving the code out of the table kernels keeps their registers)
-            const bool pairs = !TAB && ch.pair != kPairNone, ref = !ch.pair_in_accept;
+            const bool pairs = kPerm && ch.pair != kPairNone, ref = !ch.pair_in_accept;
             auto swap_bead = [&](int w, int i, T *out) { // bead i = 0..m of new path w
                 const T *src = i == 0 ? s_B + (w == 0 ? n1 : n2) * DIM
                                       : (i == m ? s_E + (w == 0 ? n2 : n1) * DIM : (w == 0 ? s_new0 : s_new1) + i * DIM);
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) out[d] = src[d];
             };
-            if constexpr (!TAB) if (pairs) {
+            if constexpr (kPerm) if (pairs) {
                 for (int it = grp; it < 2 * m; it += ngroups) {
                     const int w = it >= m ? 1 : 0, t = it - w * m;
                     const int u = j0 + t, j = u >= M ? u - M : u, jn = (j + 1 == M) ? 0 : j + 1;

@@ -170,14 +170,3 @@ def test_sweep_weights_from_reference_update_lists():
     d = chain_desc(HarmonicTrap(1.0), dim=2, M=32, N=4, L=10.0, beta=1.0, lam=0.5, **sweep_weights(mix))
     assert (d.staging_every, d.com_every, d.swap_every) == (1, 2, 1)
 
-
-def test_swaps_with_the_tabulated_propagator_are_refused():
-    """Swap tries come without a pair action or with the Gaussian pair; the
-    tabulated propagator with swaps is refused before any device call."""
-    d = chain_desc(SquareLattice(4.0, 1.0), dim=2, M=16, N=4, L=4.0, beta=1.0, lam=0.5, swaps=2)
-    d.pair, d.r_a = A.PAIR_TABLE, 1.0
-    opt = A.Options()
-    opt.precision = A.FP64
-    with pytest.raises(A.PimcError) as e:
-        A.Context([d], opt)
-    assert "PIMC_ERR_ARG" in str(e.value) and "swap" in str(e.value)

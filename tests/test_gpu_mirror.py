@@ -141,3 +141,32 @@ def test_reference_density_with_v1d_as_written():
     assert d.ndata > 0
     assert abs(numb - s.N) < 1e-2, numb
     assert np.all(s.paths >= -s.L) and np.all(s.paths < s.L)
+
+
+def test_reference_srl_example_with_swaps_small():
+    """examples/density_SRL_lattice.jl's system at a small size: the l25
+    superlattice (generate_V(1, 4, :l25; attractive)), interactions=true with
+    build_prop_int(ceil(sqrt(2) L), g, 1/(T M)) and g = 2 (hard spheres of
+    radius exp(-pi)), lambda = 1/pi^2, the update mix (1, SingleCenterOfMass),
+    (1, ReshapeLinear(s, 5)), (1, ReshapeSwapLinear(s, 20)) as move weights and
+    swap tries; the permutation changes and the Density integrates to N up to
+    the beads in its dropped first bins (a strip of 2/500 of the box here,
+    where this lattice leaves beads at the edges)."""
+    from pimc import build_prop_int, generate_V
+    L, M, N, T, g = 4.0, 40, 6, 0.2, 2.0
+    propint = build_prop_int(float(math.ceil(math.sqrt(2 * L * L))), g, 1.0 / (T * M), n=200)
+    w = sweep_weights([(1, "SingleCenterOfMass"), (1, "ReshapeLinear"), (1, "ReshapeSwapLinear")])
+    s = System(generate_V(1.0, 4.0, "l25", attractive=True), lam=1 / math.pi ** 2, M=M, N=N, L=L, T=T,
+               propint=propint, interactions=True, g=g, seed=21, ell=5, swaps=N * (M // 5), swap_m=20, **w)
+    assert s.a == pytest.approx(math.exp(-math.pi))
+    sweep = HIPSweep(s)
+    seen = set()
+    for _ in range(200):
+        sweep(s)
+        seen.add(tuple(s.next))
+    d = Density(s)
+    run(s, 2000, [(1, sweep)], Zmeasurements=[d])
+    assert len(seen) > 1 and sorted(s.next) == list(range(N))
+    assert d.ndata > 0 and abs(d.dens.sum() / d.ndata - N) < 0.01 * N
+    r = s.paths
+    assert np.all(r >= -L) and np.all(r < L)

@@ -300,3 +300,57 @@ def test_system_with_interactions_and_propint():
     sweep = HIPSweep(s)
     run(s, 20, [(1, sweep)])
     assert np.all(np.isfinite(s.paths))
+
+
+@pytest.mark.parametrize("N,L,ra,hs_a,pia", [(30, 3.0, 1.5, 0.0, 1), (30, 3.0, 1.5, 0.3, 0), (110, 5.0, 2.0, 0.25, 1),
+                                             (110, 5.0, 2.0, 0.25, 0)])
+def test_table_swap_sweeps_match_oracle_fp64(N, L, ra, hs_a, pia):
+    """Swap tries on chains with the tabulated propagator, with and without
+    hard spheres (the reference's interacting bosons, System(...;
+    interactions=true) with ReshapeSwapLinear): the bridges of a swap keep
+    clear of every particle but the partner (hardspherelevy!(r1, s, j0,
+    fpcycle2)), the pair sums as the reference writes them (pair_in_accept
+    0) or as the pair action's difference (1), both swap rules, all-pairs and
+    cell lists; thermalising launches (10000 draws), then a measuring one
+    (1000), FP64: paths, next[], every counter and the energy blocks equal
+    the oracle's."""
+    rng = np.random.default_rng(71 + N)
+    M, beta, mu, seed, spl = 16, 2.0, 0.4, 13, 2
+    descs = []
+    for rule, m in ((0, 6), (1, 9)):
+        d = table_desc(N, L, M, ra, beta=beta, ell=8, hs_a=hs_a)
+        d.swaps, d.swap_m, d.swap_rule, d.pair_in_accept = 6, m, rule, pia
+        descs.append(d)
+    tab, r0, r1 = table_for(L, 4.0, beta / M, 64)
+    inits = [rng.uniform(-L, L, size=(N, 1, 1)).repeat(2, axis=1).repeat(M, axis=2) +
+             rng.normal(0, 0.1, size=(N, 2, M)) for _ in descs]
+    inits = [(r + L) % (2 * L) - L for r in inits]
+    ctx = ctx_for(descs, A.FP64, seed=seed, spl=spl)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+        ctx.set_pair_table(c, tab, r0, r1, mu)
+    c1 = ctx.run(2 * spl, measure=False)
+    c2 = ctx.run(spl, measure=True)
+    tot = dict(levy_acc=0, levy_tries=0, com_acc=0, com_tries=0, swap_acc=0, swap_tries=0)
+    for c, (d, r) in enumerate(zip(descs, inits)):
+        s = oracle_table_sys(d, r, tab, r0, r1, mu)
+        step, meas = d.com_step, []
+        for launch in range(3):
+            s.hs_ctr = 10000 if launch < 2 else 1000
+            ca = ct = 0
+            for k in range(spl):
+                st = s.sweep_perm(c, launch * spl + k, seed, d.ell, step, d.swaps, d.swap_m, d.swap_rule)
+                for key in tot:
+                    tot[key] += getattr(st, key)
+                ca += st.com_acc
+                ct += st.com_tries
+                if launch == 2:
+                    meas.append(oracle_meas(s))
+            if launch < 2:
+                step = O.adapt_step(step, ca, ct, d.L)
+        assert np.array_equal(ctx.get_next(c), s.next), c
+        assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
+        assert_blocks(ctx, c, meas, spl)
+    for key in tot:
+        assert getattr(c1, key) + getattr(c2, key) == tot[key], (key, getattr(c1, key) + getattr(c2, key), tot[key])
+    assert tot["swap_acc"] > 0

@@ -747,3 +747,39 @@ def test_swap_pair_sums_as_the_reference_writes_them():
         assert abs(lw + 2.0 * old) < 1e-12 * max(1.0, abs(old)), trial
         nonzero += old != 0.0
     assert nonzero >= 6
+
+
+def _table_swap_sys(seed, hs_a, hs_ctr=10000, N=12, M=12, L=2.0):
+    rng = np.random.default_rng(seed)
+    tau = 2.0 / M
+    r1 = float(math.ceil(math.sqrt(2.0) * L))
+    tab = O.prop_int_terms(48, 1e-20, r1, 4.0, tau)
+    pair = O.make_pair(O.PAIR_TABLE, table=tab, r0=1e-20, r1=r1, mu=0.4)
+    r = rng.uniform(-L, L, size=(N, 2, 1)).repeat(M, axis=2) + rng.normal(0, 0.1, size=(N, 2, M))
+    r = (r + L) % (2 * L) - L
+    s = O.System(r, 0.5, L, 2.0, O.make_pot(O.POT_HARMONIC, 0.3), pair, 1.0, True, hs_a=hs_a, hs_ctr=hs_ctr)
+    s._keep = tab
+    return s
+
+
+def test_hard_sphere_swaps_reduce_to_plain_swaps_and_reject():
+    """Swap tries with the tabulated propagator: a hard-sphere radius no bead
+    comes near gives the sweeps without hard spheres (paths to rounding, the
+    same counters and permutation); a radius that every redraw violates with
+    one draw allowed rejects every swap try while the plain sweeps accept some."""
+    a, b = _table_swap_sys(3, 0.0), _table_swap_sys(3, 1e-9)
+    for sw in range(20):
+        sa = a.sweep_perm(0, sw, 5, 4, 0.5, 6, 5)
+        sb = b.sweep_perm(0, sw, 5, 4, 0.5, 6, 5)
+        assert (sa.levy_acc, sa.com_acc, sa.swap_acc, sa.swap_tries) == (sb.levy_acc, sb.com_acc, sb.swap_acc,
+                                                                         sb.swap_tries)
+    assert np.allclose(a.r, b.r, atol=1e-12) and np.array_equal(a.next, b.next)
+    plain, blocked = _table_swap_sys(4, 0.0), _table_swap_sys(4, 1.0, hs_ctr=1)
+    acc_p = acc_b = tries_b = 0
+    for sw in range(20):
+        acc_p += plain.sweep_perm(0, sw, 6, 4, 0.5, 6, 5).swap_acc
+        st = blocked.sweep_perm(0, sw, 6, 4, 0.5, 6, 5)
+        acc_b += st.swap_acc
+        tries_b += st.swap_tries
+    assert acc_p > 0 and acc_b == 0 and tries_b == 20 * 6
+    assert np.array_equal(blocked.next, np.arange(12))
