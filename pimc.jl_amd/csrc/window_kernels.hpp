@@ -1281,9 +1281,11 @@ template <typename T, int DIM> __global__ void __launch_bounds__(512) k_meas_win
         }
     }
     __syncthreads();
-    // pair terms: bead (p, t) with each neighbour q within the cut-off at slice t
+    // pair terms: bead (p, t) with each neighbour q within the cut-off at slice t.  The term is
+    // symmetric in p and q (distances, stencil multiplicity, the link to the next slice), so each
+    // unordered pair is taken once, from its lower index, with the weight of both ends
     const float h = (float)ch.h, twoL = (float)(2.0 * ch.L), inv2L = 1.0f / twoL;
-    const double inv_s2 = 1.0 / (ch.sigma * ch.sigma), wq = 0.5 * (0.5 * (double)P.tau);
+    const double inv_s2 = 1.0 / (ch.sigma * ch.sigma), wq = 2.0 * (0.5 * (0.5 * (double)P.tau));
     WinGrid G(smem, L.gr);
     grid_build<DIM>(G, N, h, twoL, inv2L, [&](int p, float &x, float &y, float &r) {
         const float4 b = bnd[p];
@@ -1293,7 +1295,7 @@ template <typename T, int DIM> __global__ void __launch_bounds__(512) k_meas_win
     });
     for (int pa = 0; pa < N;) {
         const int nf = chunk_slots<DIM>(X, pa, N, cap, kChunk, [&](int p, int q) {
-            if (q == p) return false;
+            if (q <= p) return false;
             const float4 A = bnd[p], B = bnd[q];
             return bnd_near<DIM>(A.x, A.y, A.z, B.x, B.y, B.z, h, twoL, inv2L);
         }, &G);
