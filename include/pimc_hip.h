@@ -126,7 +126,8 @@ typedef struct pimc_options {
                                   acceptance: -1 below 0.6, +1 above 0.8, within [2, min(32, M - 2)]
                                   (ReshapeLinear's minslices and maxslices, src/updates/reshape.jl:12-21).  The
                                   tiles of chains without a pair action are rebuilt between launches when a length
-                                  changes (a host synchronisation per thermalising launch); Gaussian-pair chains
+                                  changes (a host synchronisation per thermalising launch; a length that would give
+                                  a particle more segments than a workgroup holds is refused); Gaussian-pair chains
                                   then run on the whole-chain kernel.  pimc_get_ell reads the current value */
     int32_t measure_kernel;    /* estimators of chains run whole per workgroup, in measuring runs: 1 inside the
                                   sweep kernel; 2 in a wide kernel of their own after each sweep (the sweep

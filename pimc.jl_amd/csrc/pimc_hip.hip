@@ -830,13 +830,14 @@ int pimc_device_info(char *name, int32_t len, int32_t *cu_count)
 void build_tiles(pimc_ctx *c)
 {
     const int ts = (int)tsize(c);
-    c->tiles.clear();
-    c->tile_groups.clear();
-    c->threads_tile = c->opt.block_threads > 0 ? c->opt.block_threads : 256;
-    if (c->threads_tile % 64 != 0 || c->threads_tile > kTileMaxThreads)
+    // built aside and kept only when every check passes, so a failure leaves the context as it was
+    std::vector<TileDev> tiles;
+    std::vector<TileGroup> tile_groups;
+    const int threads_tile = c->opt.block_threads > 0 ? c->opt.block_threads : 256;
+    if (threads_tile % 64 != 0 || threads_tile > kTileMaxThreads)
         throw HipError{hipErrorInvalidValue, "block_threads must be a multiple of 64, <= 256"};
     int tile_beads = c->opt.tile_beads > 0 ? c->opt.tile_beads : (ts == 4 ? 4096 : 2048);
-    const int max_items = kItemsPerThread * c->threads_tile;
+    const int max_items = kItemsPerThread * threads_tile;
     size_t lds_max = 0;
     for (int id : c->order) {
         if (c->is_whole[id]) continue;
@@ -844,12 +845,12 @@ void build_tiles(pimc_ctx *c)
         if (h.nseg > max_items) throw HipError{hipErrorInvalidValue, "a particle has more segments than a workgroup holds; raise block_threads or ell"};
         for (int p = 0; p < h.N; ++p) {
             const int gq = h.part_base + p;
-            bool fresh = c->tiles.empty();
+            bool fresh = tiles.empty();
             if (!fresh) {
-                TileDev &t = c->tiles.back();
+                TileDev &t = tiles.back();
                 fresh = t.M != h.M || t.ell != h.ell || t.every_st != h.every_st || t.every_com != h.every_com ||
-                        c->tile_groups.back().pot != h.pot ||
-                        c->tile_groups.back().maxl != next_pow2_maxl(h.ell) ||
+                        tile_groups.back().pot != h.pot ||
+                        tile_groups.back().maxl != next_pow2_maxl(h.ell) ||
                         (t.np + 1) * h.M > tile_beads || (t.np + 1) * h.nseg > max_items;
                 if (!fresh) {
                     // keep every tile within the LDS share of 4 workgroups per CU
@@ -877,27 +878,30 @@ void build_tiles(pimc_ctx *c)
                     t.grouped = ts == 4 && h.ell == maxl && h.M % maxl == 0 && h.nseg <= 64 &&
                                 (h.nseg & (h.nseg - 1)) == 0;
                 }
-                if (c->tile_groups.empty() || c->tile_groups.back().pot != h.pot ||
-                    c->tile_groups.back().maxl != next_pow2_maxl(h.ell))
-                    c->tile_groups.push_back(TileGroup{h.pot, next_pow2_maxl(h.ell), (int)c->tiles.size(), 0});
-                c->tile_groups.back().count += 1;
-                c->tiles.push_back(t);
+                if (tile_groups.empty() || tile_groups.back().pot != h.pot ||
+                    tile_groups.back().maxl != next_pow2_maxl(h.ell))
+                    tile_groups.push_back(TileGroup{h.pot, next_pow2_maxl(h.ell), (int)tiles.size(), 0});
+                tile_groups.back().count += 1;
+                tiles.push_back(t);
             }
-            TileDev &t = c->tiles.back();
+            TileDev &t = tiles.back();
             if (t.np == 0 || p == 0) t.nslot += 1; // chains are contiguous in the particle order
             t.np += 1;
             t.items = t.np * t.nseg;
         }
     }
-    for (TileDev &t : c->tiles) {
+    for (TileDev &t : tiles) {
         int nslot = 1;
         for (int q = 1; q < t.np; ++q)
             if (c->part_chain[t.q0 + q] != c->part_chain[t.q0 + q - 1]) ++nslot;
         if (nslot != t.nslot) throw HipError{hipErrorUnknown, "tile chain-slot count mismatch"};
         lds_max = std::max(lds_max, tile_lds_bytes(t.np, c->dim, t.pst, t.items, ts, t.nslot, c->opt.radial_bins));
     }
-    c->lds_tile = lds_max;
     if (lds_max > 160 * 1024) throw HipError{hipErrorInvalidValue, "tile exceeds LDS; lower tile_beads"};
+    c->tiles.swap(tiles);
+    c->tile_groups.swap(tile_groups);
+    c->threads_tile = threads_tile;
+    c->lds_tile = lds_max;
 }
 
 // After a thermalising launch under adapt_ell: the staging lengths k_post chose for the tile
@@ -908,15 +912,25 @@ void retile_after_adapt(pimc_ctx *c)
     HIPCHK(hipStreamSynchronize(c->stream));
     std::vector<ChainDev> dev(c->n_chains);
     HIPCHK(hipMemcpy(dev.data(), c->d_ch, sizeof(ChainDev) * c->n_chains, hipMemcpyDeviceToHost));
-    bool changed = false;
+    std::vector<int> changed;
     for (int i = 0; i < c->n_chains; ++i)
-        if (!c->is_whole[i] && dev[i].ell != c->ch[i].ell) {
-            c->ch[i].ell = dev[i].ell;
-            c->ch[i].nseg = dev[i].nseg;
-            changed = true;
-        }
-    if (!changed) return;
-    build_tiles(c);
+        if (!c->is_whole[i] && dev[i].ell != c->ch[i].ell) changed.push_back(i);
+    if (changed.empty()) return;
+    std::vector<ChainDev> before(c->ch);
+    for (int i : changed) {
+        c->ch[i].ell = dev[i].ell;
+        c->ch[i].nseg = dev[i].nseg;
+    }
+    try {
+        build_tiles(c);
+    } catch (const HipError &) {
+        // a length whose tiles the workgroup cannot hold (more segments than its items): the
+        // chains keep their lengths, on the host and on the device, and the tiles stay
+        c->ch.swap(before);
+        for (int i : changed)
+            HIPCHK(hipMemcpy(c->d_ch + i, &c->ch[i], sizeof(ChainDev), hipMemcpyHostToDevice));
+        return;
+    }
     if (c->tiles.size() > c->tiles_cap) {
         HIPCHK(hipFree(c->d_tiles));
         c->d_tiles = nullptr;

@@ -1039,3 +1039,21 @@ def test_swap_sweeps_with_pairs_fp32_run():
     assert cnt.swap_tries == 40 * 16 and cnt.swap_acc > 0
     E, Ev, Ep = ctx.blocks(0)
     assert np.all(np.isfinite(E)) and np.all(np.isfinite(Ep)) and np.all(Ep > 0)
+
+
+def test_adapt_ell_keeps_tiles_the_workgroup_holds():
+    """A tile chain of M = 1024 slices with a low staging acceptance shortens
+    its segments launch by launch until the next length would give a
+    particle more segments than a workgroup holds (256 items): that length
+    is refused, the chain keeps ell = 4 on the host and the device, and the
+    runs go on."""
+    d = harmonic_desc(N=1, M=1024, beta=1000.0, ell=8)
+    ctx = make_ctx([d], A.FP32, seed=3, spl=1, adapt_ell=1)
+    ctx.set_paths(0, np.zeros((1, 2, 1024)))
+    ells = []
+    for _ in range(8):
+        ctx.run(1, measure=False)
+        ells.append(ctx.ell(0))
+    assert ells[0] < 8 and min(ells) == 4 and ells[-1] == 4, ells
+    cnt = ctx.run(2, measure=True)
+    assert cnt.levy_tries == 2 * (1024 // 4)
