@@ -1020,6 +1020,12 @@ def test_swap_sweeps_with_pairs_match_oracle_fp64(N, L, ra, pia):
         assert np.array_equal(ctx.get_next(c), s.next), c
         assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
         assert_blocks(ctx, c, meas, n_sweeps)
+        # the Energy hook on the permuted configuration (links along the cycles)
+        E, Ev, Ep, Evp = ctx.energy(c)
+        (Eo, Evo), (Po, Pvo) = s.energy(), s.pair_energy()
+        big = d.dim * N / (2 * d.beta / d.M)
+        assert abs(E - Eo) < 1e-10 * big and abs(Ev - Evo) < 1e-9 * max(1.0, abs(Evo)), c
+        assert abs(Ep - Po) < 1e-9 * max(1.0, abs(Po)) and abs(Evp - Pvo) < 1e-9 * max(1.0, abs(Pvo)), c
     for k in tot:
         assert getattr(cnt, k) == tot[k], (k, getattr(cnt, k), tot[k])
     assert tot["swap_acc"] > 0
