@@ -281,3 +281,34 @@ def test_fp32_pair_sampling_matches_fp64(kind):
     if kind != "table_hardsphere":
         assert Ep64.mean() > 0.1
         assert abs(Ep32.mean() - Ep64.mean()) < 3.5 * math.hypot(se(Ep32), se(Ep64))
+
+
+def _pair_boson_run(ell, swaps, swap_m, seed, nch=128, therm=1000, meas=4000):
+    descs = [chain_desc(HarmonicTrap(1.0), dim=2, M=32, N=4, L=100.0, beta=2.0, lam=0.5, pair=GaussianPair(2.0, 0.7),
+                        ell=ell, com_step=1.0, swaps=swaps, swap_m=swap_m) for _ in range(nch)]
+    opt = A.Options()
+    opt.precision, opt.sweeps_per_launch, opt.seed, opt.use_graph = A.FP32, 100, seed, 1
+    ctx = A.Context(descs, opt)
+    ctx.init_paths(-1, seed + 1)
+    ctx.run(therm, measure=False)
+    cnt = ctx.run(meas, measure=True)
+    E = np.array([ctx.estimates(c).E for c in range(nch)])
+    ex = np.array([np.any(ctx.get_next(c) != np.arange(4)) for c in range(nch)], dtype=float)
+    return E.mean(), E.std(ddof=1) / math.sqrt(nch), ex.mean(), cnt
+
+
+def test_pair_swaps_agree_across_move_sets_fp32():
+    """Detailed balance of the swap move with a pair action (pair_in_accept =
+    1: the pair action's difference): four trapped bosons with a Gaussian
+    repulsion, sampled with swap windows of 4 links (staging length 4) and of
+    12 links (staging length 8), give the same energy and the same fraction of
+    chains in an exchanged sector; the repulsion moves the energy well away
+    from the ideal bosons'."""
+    mA, eA, xA, cA = _pair_boson_run(4, 4, 4, 301)
+    mB, eB, xB, cB = _pair_boson_run(8, 2, 12, 302)
+    assert cA.swap_acc > 0.02 * cA.swap_tries and cB.swap_acc > 0.02 * cB.swap_tries
+    assert abs(mA - mB) < 4.0 * math.hypot(eA, eB), (mA, eA, mB, eB)
+    sx = math.sqrt(max(xA * (1 - xA), 0.05) / 128 + max(xB * (1 - xB), 0.05) / 128)
+    assert abs(xA - xB) < 4.0 * sx, (xA, xB)
+    ideal = AN.energy_boson_primitive(4, 32, 2.0)
+    assert mA - ideal > 8 * eA, (mA, eA, ideal)
