@@ -19,7 +19,7 @@ from pimc import _capi as A
 from pimc.propagator import PropInt, build_prop_int
 from pimc.potentials import GaussianPair, HarmonicTrap
 from pimc.system import System, chain_desc
-from test_gpu_parity import assert_blocks, oracle_meas
+from test_gpu_parity import assert_blocks, oracle_meas, oracle_sys
 
 pytestmark = pytest.mark.gpu
 
@@ -354,3 +354,78 @@ def test_table_swap_sweeps_match_oracle_fp64(N, L, ra, hs_a, pia):
     for key in tot:
         assert getattr(c1, key) + getattr(c2, key) == tot[key], (key, getattr(c1, key) + getattr(c2, key), tot[key])
     assert tot["swap_acc"] > 0
+
+
+def test_every_chain_class_in_one_context_fp64():
+    """One context holding every kind of chain: tile chains with and without
+    move weights, Gaussian-pair chains on the window kernels (cell lists) and
+    on the whole-chain kernel, swap chains without and with the Gaussian pair,
+    tabulated-propagator chains without swaps and with swaps and hard spheres
+    (the TPERM instantiation): the launch lists, tiles and instantiations
+    keep every chain equal to the oracle (paths, next[], counters, COM steps,
+    energy blocks) over two thermalising launches and a measuring one."""
+    rng = np.random.default_rng(91)
+    M, beta, mu, seed, spl = 16, 2.0, 0.4, 17, 2
+    gp = GaussianPair(1.0, 0.5)
+    tab, r0, r1 = table_for(3.0, 4.0, beta / M, 64)
+    descs, inits, is_tab = [], [], []
+
+    def add(d, r, t=False):
+        descs.append(d)
+        inits.append(r)
+        is_tab.append(t)
+
+    add(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=5, L=100.0, beta=beta, lam=0.5, ell=8, com_step=1.0),
+        rng.normal(0, 0.7, size=(5, 2, M)))
+    add(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=6, L=100.0, beta=beta, lam=0.5, ell=8, com_step=1.0,
+                   staging_every=2, com_every=3), rng.normal(0, 0.7, size=(6, 2, M)))
+    add(chain_desc(HarmonicTrap(0.3), dim=2, M=M, N=110, L=5.0, beta=beta, lam=0.5, pair=gp, ell=8, r_a=2.0,
+                   com_step=0.5), rng.uniform(-5.0, 5.0, size=(110, 2, M)))
+    add(chain_desc(HarmonicTrap(0.3), dim=2, M=M, N=30, L=3.0, beta=beta, lam=0.5, pair=gp, ell=8, r_a=1.5,
+                   com_step=0.5), rng.uniform(-3.0, 3.0, size=(30, 2, M)))
+    add(chain_desc(HarmonicTrap(1.0), dim=2, M=M, N=4, L=100.0, beta=beta, lam=0.5, ell=4, com_step=0.8, swaps=3,
+                   swap_m=5), rng.normal(0, 0.6, size=(4, 2, M)))
+    add(chain_desc(HarmonicTrap(0.5), dim=2, M=M, N=12, L=1.5, beta=beta, lam=0.5, pair=gp, ell=4, r_a=1.5,
+                   com_step=0.4, swaps=4, swap_m=6, swap_every=2),
+        (rng.uniform(-1.5, 1.5, size=(12, 1, 1)).repeat(2, axis=1).repeat(M, axis=2)
+         + rng.normal(0, 0.1, size=(12, 2, M)) + 1.5) % 3.0 - 1.5)
+    add(table_desc(30, 3.0, M, 1.5, beta=beta, ell=8), rng.uniform(-3.0, 3.0, size=(30, 2, M)), True)
+    dt = table_desc(30, 3.0, M, 1.5, beta=beta, ell=8, hs_a=0.3)
+    dt.swaps, dt.swap_m = 5, 6
+    add(dt, (rng.uniform(-3.0, 3.0, size=(30, 1, 1)).repeat(2, axis=1).repeat(M, axis=2)
+             + rng.normal(0, 0.1, size=(30, 2, M)) + 3.0) % 6.0 - 3.0, True)
+    opt = A.Options()
+    opt.precision, opt.sweeps_per_launch, opt.seed = A.FP64, spl, seed
+    ctx = A.Context(descs, opt)
+    for c, r in enumerate(inits):
+        ctx.set_paths(c, r)
+        if is_tab[c]:
+            ctx.set_pair_table(c, tab, r0, r1, mu)
+    c1 = ctx.run(2 * spl, measure=False)
+    c2 = ctx.run(spl, measure=True)
+    tot = dict(levy_acc=0, levy_tries=0, com_acc=0, com_tries=0, swap_acc=0, swap_tries=0)
+    for c, (d, r) in enumerate(zip(descs, inits)):
+        s = oracle_table_sys(d, r, tab, r0, r1, mu) if is_tab[c] else oracle_sys(d, r)
+        step, meas = d.com_step, []
+        every = (max(1, d.staging_every), max(1, d.com_every), max(1, d.swap_every))
+        for launch in range(3):
+            s.hs_ctr = 10000 if launch < 2 else 1000
+            ca = ct = 0
+            for k in range(spl):
+                st = s.sweep_every(c, launch * spl + k, seed, d.ell, step, every, d.swaps, d.swap_m or d.ell,
+                                   d.swap_rule)
+                for key in tot:
+                    tot[key] += getattr(st, key)
+                ca += st.com_acc
+                ct += st.com_tries
+                if launch == 2:
+                    meas.append(oracle_meas(s))
+            if launch < 2:
+                step = O.adapt_step(step, ca, ct, d.L)
+        if d.swaps:
+            assert np.array_equal(ctx.get_next(c), s.next), c
+        assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
+        assert abs(ctx.com_step(c) - step) < 1e-12, c
+        assert_blocks(ctx, c, meas, spl)
+    for key in tot:
+        assert getattr(c1, key) + getattr(c2, key) == tot[key], (key, getattr(c1, key) + getattr(c2, key), tot[key])
