@@ -7,14 +7,17 @@
 // com.jl:136-224).  Segments that share only end beads touch disjoint
 // interior beads, so the segments of a ring are independent given their ends.
 //
-//  * k_sweep_tiles: chains without pair action.  Particles are then
-//    independent, so a workgroup owns a tile of particles (of one or several
-//    chains with equal M), stages the tile in LDS, runs K sweeps there and
-//    writes it back once.  One thread per (particle, segment).
-//  * k_sweep_whole: chains with a pair action (and chains whose estimators
-//    need whole-chain products).  A workgroup owns a chain in HBM; particles
-//    are updated one after another, the segments (and links) of the current
-//    particle in parallel, pair sums through per-slice cell lists.
+//  * k_sweep_tiles: chains without pair action or swap moves.  Particles are
+//    then independent, so a workgroup owns a tile of particles (of one or
+//    several chains with equal M), stages the tile in LDS, runs K sweeps there
+//    and writes it back once.  One thread per (particle, segment).
+//  * k_sweep_whole: chains with a pair action, swap moves (permutation
+//    cycles) or estimators that need whole-chain products.  A workgroup owns a
+//    chain in HBM; particles are updated one after another, the segments (and
+//    links) of the current particle in parallel, pair sums through per-slice
+//    cell lists; swap tries follow the centre-of-mass pass.
+//  * the window kernels (window_kernels.hpp) take the Gaussian-pair chains with
+//    cell lists instead, many particles at once.
 #pragma once
 #include "physics.hpp"
 
@@ -2213,7 +2216,7 @@ struct PostArgs {
     double *blocks;      // [blk][chain][kNQ]
     const RunState *rs; // block index of this launch
     int measure, nsweeps;
-    int adapt_ell;      // NumbOfSlices rule on whole chains while thermalising
+    int adapt_ell;      // NumbOfSlices rule on every chain while thermalising
 };
 
 // Staging tries and proposed beads of one sweep of a chain (segments of ell
