@@ -429,3 +429,24 @@ def test_every_chain_class_in_one_context_fp64():
         assert_blocks(ctx, c, meas, spl)
     for key in tot:
         assert getattr(c1, key) + getattr(c2, key) == tot[key], (key, getattr(c1, key) + getattr(c2, key), tot[key])
+
+
+def test_table_swaps_with_hard_spheres_fp32_run():
+    """FP32 swap tries with the tabulated propagator and hard spheres (cell
+    lists): the permutation stays a permutation, every try is counted, some
+    are accepted, and the paths end finite and inside the box."""
+    rng = np.random.default_rng(93)
+    M, beta, L = 16, 2.0, 5.0
+    d = table_desc(120, L, M, 2.0, beta=beta, ell=8, hs_a=0.2)
+    d.swaps, d.swap_m = 12, 6
+    tab, r0, r1 = table_for(L, 4.0, beta / M, 64)
+    ctx = ctx_for([d], A.FP32, seed=5, spl=10)
+    x = rng.uniform(-L, L, size=(120, 2, 1)).repeat(M, axis=2)
+    ctx.set_paths(0, (x + rng.normal(0, 0.05, size=x.shape) + L) % (2 * L) - L)
+    ctx.set_pair_table(0, tab, r0, r1, 0.4)
+    ctx.run(40, measure=False)
+    cnt = ctx.run(40, measure=True)
+    assert sorted(ctx.get_next(0)) == list(range(120))
+    assert cnt.swap_tries == 40 * 12 and cnt.swap_acc > 0
+    r = ctx.get_paths(0)
+    assert np.all(np.isfinite(r)) and np.all(r >= -L) and np.all(r < L)
