@@ -721,16 +721,17 @@ __global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_p
                             }
                             l += dd;
                         }
+                        // |r|^2 of the bead: the radial histogram's radius and, in the FP32
+                        // harmonic trap, the potential sum's term
+                        T rr = r0[0] * r0[0];
+                        if (DIM > 1) rr += r0[1] * r0[1];
                         if constexpr (kSumR2) {
-#pragma unroll
-                            for (int e = 0; e < DIM; ++e) v = fma(r0[e], r0[e], v);
+                            v += rr;
                         } else {
                             v += potVk<T, DIM, POT>(r0, P);
                             g += rgradVk<T, DIM, POT>(r0, P);
                         }
                         if (rbins > 0) {
-                            T rr = r0[0] * r0[0];
-                            if (DIM > 1) rr += r0[1] * r0[1];
                             const T qb = sq(rr) * (T)a.radial_winv;
                             if (qb < (T)rbins) atomicAdd(&hist[(int)qb], 1u);
                         }
