@@ -133,8 +133,8 @@ typedef struct pimc_options {
                                   sweep kernel; 2 in a wide kernel of their own after each sweep (the sweep
                                   kernel then runs one sweep per launch); 0: 2 when such a chain has a pair
                                   action and at least 8192 beads, else 1.  Same samples either way */
-    int32_t pair_kernel;       /* chains with a Gaussian pair action and no swaps, hard spheres or staging-length
-                                  adaptation: 1 the whole-chain kernel (one workgroup per chain, tries in
+    int32_t pair_kernel;       /* chains with a Gaussian pair action in the acceptance (pair_in_accept 1) and no
+                                  swaps, hard spheres or staging-length adaptation: 1 the whole-chain kernel (one workgroup per chain, tries in
                                   order); 2 the window kernels (staging by slice windows, centre-of-mass tries
                                   decided in dependency rounds) wherever their LDS state fits; 0: 2 for chains
                                   with cell lists (more than 96 particles), else 1.  Same sampling either way */

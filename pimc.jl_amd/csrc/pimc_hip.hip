@@ -1119,10 +1119,12 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 cap = (cap + 7) & ~7; // whole 16-byte chunks per member list
                 const long long bytes = (long long)h.M * ncell * (cap * 2 + 4);
                 const bool cells = h.pair != PIMC_PAIR_NONE && h.N > 96 && h.nbins >= 3 && bytes < (256ll << 20);
-                // Gaussian-pair chains (no swaps, hard spheres or staging-length adaptation) whose
-                // window state, with every window chain so far, fits in LDS run on the window kernels
+                // Gaussian-pair chains (the pair action in the acceptance; no swaps, hard spheres or
+                // staging-length adaptation) whose window state, with every window chain so far, fits
+                // in LDS run on the window kernels
                 const int tsz = (int)tsize(c);
-                bool win = h.pair == kPairGaussian && h.swaps == 0 && !(h.hs_a > 0.0) && !opt->adapt_ell &&
+                bool win = h.pair == kPairGaussian && h.pair_in_accept && h.swaps == 0 && !(h.hs_a > 0.0) &&
+                           !opt->adapt_ell &&
                            h.N <= 4096 && h.M >= 3 && (!cells || ncell <= 16384) &&
                            (opt->pair_kernel == 2 || (opt->pair_kernel == 0 && cells));
                 if (win) {

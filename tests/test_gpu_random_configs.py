@@ -45,44 +45,98 @@ def case(trial):
     return d, (r + L) % (2 * L) - L, kind, every
 
 
+def oracle_of(d, r, kind, tab, r1):
+    if kind == "table":
+        return O.System(r, d.lam, d.L, d.beta, O.pot_from_desc(d),
+                        O.make_pair(O.PAIR_TABLE, table=tab, r0=1e-20, r1=r1, mu=0.4), d.r_a, bool(d.pair_in_accept),
+                        hs_a=d.hs_a)
+    return oracle_sys(d, r)
+
+
+def table_of(d):
+    r1 = float(math.ceil(math.sqrt(2.0) * d.L))
+    return O.prop_int_terms(48, 1e-20, r1, 4.0, d.beta / d.M), r1
+
+
+def run_and_check(chains, seed, spl, pair_kernel=0):
+    """chains: (desc, paths, kind, every).  Two thermalising launches and a
+    measuring one; every chain against the oracle under its own chain index."""
+    opt = A.Options()
+    opt.precision, opt.sweeps_per_launch, opt.seed, opt.pair_kernel = A.FP64, spl, seed, pair_kernel
+    ctx = A.Context([d for d, _, _, _ in chains], opt)
+    refs = []
+    for c, (d, r, kind, every) in enumerate(chains):
+        ctx.set_paths(c, r)
+        tab, r1 = table_of(d) if kind == "table" else (None, None)
+        if kind == "table":
+            ctx.set_pair_table(c, tab, 1e-20, r1, 0.4)
+        refs.append(oracle_of(d, r, kind, tab, r1))
+    c1 = ctx.run(2 * spl, measure=False)
+    c2 = ctx.run(spl, measure=True)
+    keys = ("levy_acc", "levy_tries", "com_acc", "com_tries", "swap_acc", "swap_tries")
+    grand = dict.fromkeys(keys, 0)
+    for c, ((d, _, _, every), s) in enumerate(zip(chains, refs)):
+        step, meas = d.com_step, []
+        tot = dict.fromkeys(keys, 0)
+        for launch in range(3):
+            s.hs_ctr = 10000 if launch < 2 else 1000
+            ca = ct = 0
+            for k in range(spl):
+                st = s.sweep_every(c, launch * spl + k, seed, d.ell, step, every, d.swaps, d.swap_m or d.ell,
+                                   d.swap_rule)
+                for key in tot:
+                    tot[key] += getattr(st, key)
+                ca += st.com_acc
+                ct += st.com_tries
+                if launch == 2:
+                    meas.append(oracle_meas(s))
+            if launch < 2:
+                step = O.adapt_step(step, ca, ct, d.L)
+        assert np.max(np.abs(ctx.get_paths(c) - s.r)) < 1e-9, c
+        if d.swaps:
+            assert np.array_equal(ctx.get_next(c), s.next), c
+        assert list(ctx.counters(c)) == [tot["levy_acc"], tot["levy_tries"], tot["com_acc"], tot["com_tries"]], c
+        Eo = np.mean([m[0] for m in meas])
+        assert abs(ctx.blocks(c)[0][0] - Eo) < 1e-8 * max(1.0, abs(Eo)), c
+        for key in keys:
+            grand[key] += tot[key]
+    for key in keys:
+        assert getattr(c1, key) + getattr(c2, key) == grand[key], key
+
+
 @pytest.mark.parametrize("trial", range(24))
 def test_random_configuration_matches_oracle_fp64(trial):
     d, r, kind, every = case(trial)
-    seed, spl = 77 + trial, 3
-    opt = A.Options()
-    opt.precision, opt.sweeps_per_launch, opt.seed = A.FP64, spl, seed
-    ctx = A.Context([d], opt)
-    ctx.set_paths(0, r)
-    if kind == "table":
-        r1 = float(math.ceil(math.sqrt(2.0) * d.L))
-        tab = O.prop_int_terms(48, 1e-20, r1, 4.0, d.beta / d.M)
-        ctx.set_pair_table(0, tab, 1e-20, r1, 0.4)
-        s = O.System(r, d.lam, d.L, d.beta, O.pot_from_desc(d),
-                     O.make_pair(O.PAIR_TABLE, table=tab, r0=1e-20, r1=r1, mu=0.4), d.r_a, bool(d.pair_in_accept),
-                     hs_a=d.hs_a)
-    else:
-        s = oracle_sys(d, r)
-    c1 = ctx.run(2 * spl, measure=False)
-    c2 = ctx.run(spl, measure=True)
-    step, meas = d.com_step, []
-    tot = dict(levy_acc=0, levy_tries=0, com_acc=0, com_tries=0, swap_acc=0, swap_tries=0)
-    for launch in range(3):
-        s.hs_ctr = 10000 if launch < 2 else 1000
-        ca = ct = 0
-        for k in range(spl):
-            st = s.sweep_every(0, launch * spl + k, seed, d.ell, step, every, d.swaps, d.swap_m or d.ell, d.swap_rule)
-            for key in tot:
-                tot[key] += getattr(st, key)
-            ca += st.com_acc
-            ct += st.com_tries
-            if launch == 2:
-                meas.append(oracle_meas(s))
-        if launch < 2:
-            step = O.adapt_step(step, ca, ct, d.L)
-    assert np.max(np.abs(ctx.get_paths(0) - s.r)) < 1e-9
-    if d.swaps:
-        assert np.array_equal(ctx.get_next(0), s.next)
-    for key in tot:
-        assert getattr(c1, key) + getattr(c2, key) == tot[key], key
-    Eo = np.mean([m[0] for m in meas])
-    assert abs(ctx.blocks(0)[0][0] - Eo) < 1e-8 * max(1.0, abs(Eo))
+    run_and_check([(d, r, kind, every)], 77 + trial, 3)
+
+
+def big_gauss(trial, pair_in_accept=True):
+    """A Gaussian-pair chain with cell lists (N = 110, 2L / r_a = 5 cells a
+    side): the window kernels take it by default when its pair action enters
+    the acceptance."""
+    rng = np.random.default_rng(5000 + trial)
+    every = tuple(int(x) for x in rng.integers(1, 4, size=3))
+    d = chain_desc(HarmonicTrap(0.3), dim=2, M=16, N=110, L=5.0, beta=2.0, lam=0.5,
+                   pair=GaussianPair(1.0, 0.5), ell=int(rng.integers(2, 9)), r_a=2.0, com_step=0.5,
+                   staging_every=every[0], com_every=every[1], swap_every=every[2], pair_in_accept=pair_in_accept)
+    return d, rng.uniform(-5.0, 5.0, size=(110, 2, 16)), "gauss", every
+
+
+@pytest.mark.parametrize("trial", range(8))
+def test_random_mixed_contexts_match_oracle_fp64(trial):
+    """Four random chains and a window-kernel chain in one context (tile,
+    whole-chain, table, swap and window launches side by side), pair_kernel
+    0 or 2 (small Gaussian chains without swaps then run on the window
+    kernels too): every chain equals the oracle under its own chain index."""
+    chains = [case(100 + 4 * trial + k) for k in range(4)] + [big_gauss(trial)]
+    run_and_check(chains, 300 + trial, 3, pair_kernel=0 if trial % 2 else 2)
+
+
+@pytest.mark.parametrize("pair_kernel", [0, 2])
+def test_reference_pair_semantics_match_oracle_fp64(pair_kernel):
+    """A Gaussian-pair chain with cell lists and pair_in_accept = 0 (the
+    reference's interaction_action! sums computed and dropped from the
+    acceptance) beside one with the pair action in the acceptance: the first
+    samples without its pair action whichever pair kernel is asked for, and
+    both equal the oracle."""
+    run_and_check([big_gauss(11, pair_in_accept=False), big_gauss(12)], 401, 3, pair_kernel=pair_kernel)
