@@ -1063,3 +1063,46 @@ def test_adapt_ell_keeps_tiles_the_workgroup_holds():
     assert ells[0] < 8 and min(ells) == 4 and ells[-1] == 4, ells
     cnt = ctx.run(2, measure=True)
     assert cnt.levy_tries == 2 * (1024 // 4)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_estimator_kernels_agree_with_derivatives_fp64(dim):
+    """Reweighting derivatives on window-size Gaussian-pair chains (cell
+    lists, N = 110 and 130): the same samples measured three ways (the
+    whole-chain kernel's in-kernel estimators, its wide k_meas_beads kernel,
+    and the window kernels' k_meas_win) give the same E, E_virial, E_pair,
+    dS/d omega, dS/dg, d<E>/d omega and d<E>/dg to 1e-9, block by block for
+    the energies; the paths equal the oracle's."""
+    rng = np.random.default_rng(71)
+    pair = GaussianPair(1.0, 0.5)
+    descs, inits = [], []
+    for N, L, omega in [(110, 5.0 if dim == 2 else 20.0, 0.3), (130, 6.0 if dim == 2 else 24.0, 0.5)]:
+        descs.append(chain_desc(HarmonicTrap(omega), dim=dim, M=16, N=N, L=L, beta=2.0, lam=0.5, pair=pair, ell=6,
+                                r_a=2.0, com_step=0.5))
+        inits.append(rng.uniform(-L, L, size=(N, dim, 16)))
+    seed, spl = 29, 3
+    fields = ("E", "Ev", "Epair", "dSdomega", "dSdg", "dE_domega", "dE_dg")
+    runs = []
+    for pk, mk in [(1, 1), (1, 2), (2, 0)]:
+        opt = A.Options()
+        opt.precision, opt.sweeps_per_launch, opt.seed = A.FP64, spl, seed
+        opt.derivatives, opt.pair_kernel, opt.measure_kernel = 1, pk, mk
+        ctx = A.Context(descs, opt)
+        for c, r in enumerate(inits):
+            ctx.set_paths(c, r)
+        ctx.run(2 * spl, measure=False)
+        therm = [ctx.get_paths(c) for c in range(len(descs))]
+        ctx.run(4 * spl, measure=True)
+        runs.append([(ctx.get_paths(c), ctx.blocks(c), ctx.estimates(c), therm[c]) for c in range(len(descs))])
+        ctx.close()
+    ref = sweep_oracle(descs, inits, seed, 2 * spl, spl)
+    for c in range(len(descs)):
+        assert np.max(np.abs(runs[0][c][3] - ref[c][0])) < 1e-9
+        for k in (1, 2):
+            assert np.max(np.abs(runs[k][c][0] - runs[0][c][0])) < 1e-9, (k, c)
+            for a, b in zip(runs[k][c][1], runs[0][c][1]):
+                assert np.allclose(a, b, rtol=1e-9, atol=1e-9), (k, c)
+            for f in fields:
+                x, y = getattr(runs[k][c][2], f), getattr(runs[0][c][2], f)
+                assert abs(x - y) <= 1e-9 * max(1.0, abs(y)), (k, c, f, x, y)
+        assert runs[0][c][2].dSdg != 0.0 and runs[0][c][2].dSdomega != 0.0
