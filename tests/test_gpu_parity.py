@@ -1072,7 +1072,8 @@ def test_estimator_kernels_agree_with_derivatives_fp64(dim):
     whole-chain kernel's in-kernel estimators, its wide k_meas_beads kernel,
     and the window kernels' k_meas_win) give the same E, E_virial, E_pair,
     dS/d omega, dS/dg, d<E>/d omega and d<E>/dg to 1e-9, block by block for
-    the energies; the paths equal the oracle's."""
+    the energies, and the same radial and Cartesian histograms bin for bin;
+    the paths equal the oracle's."""
     rng = np.random.default_rng(71)
     pair = GaussianPair(1.0, 0.5)
     descs, inits = [], []
@@ -1087,13 +1088,15 @@ def test_estimator_kernels_agree_with_derivatives_fp64(dim):
         opt = A.Options()
         opt.precision, opt.sweeps_per_launch, opt.seed = A.FP64, spl, seed
         opt.derivatives, opt.pair_kernel, opt.measure_kernel = 1, pk, mk
+        opt.radial_bins, opt.radial_rmax, opt.cart_bins = 40, 4.0, 32
         ctx = A.Context(descs, opt)
         for c, r in enumerate(inits):
             ctx.set_paths(c, r)
         ctx.run(2 * spl, measure=False)
         therm = [ctx.get_paths(c) for c in range(len(descs))]
         ctx.run(4 * spl, measure=True)
-        runs.append([(ctx.get_paths(c), ctx.blocks(c), ctx.estimates(c), therm[c]) for c in range(len(descs))])
+        runs.append([(ctx.get_paths(c), ctx.blocks(c), ctx.estimates(c), therm[c], ctx.radial(c), ctx.cartesian(c))
+                     for c in range(len(descs))])
         ctx.close()
     ref = sweep_oracle(descs, inits, seed, 2 * spl, spl)
     for c in range(len(descs)):
@@ -1105,4 +1108,7 @@ def test_estimator_kernels_agree_with_derivatives_fp64(dim):
             for f in fields:
                 x, y = getattr(runs[k][c][2], f), getattr(runs[0][c][2], f)
                 assert abs(x - y) <= 1e-9 * max(1.0, abs(y)), (k, c, f, x, y)
+            for h in (4, 5):
+                assert runs[k][c][h][1] == runs[0][c][h][1] and np.array_equal(runs[k][c][h][0], runs[0][c][h][0]), (k, c, h)
         assert runs[0][c][2].dSdg != 0.0 and runs[0][c][2].dSdomega != 0.0
+        assert runs[0][c][4][0].sum() > 0 and runs[0][c][5][0].sum() > 0
