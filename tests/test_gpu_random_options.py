@@ -5,9 +5,10 @@ pair mode; move weights; traps, square and optical lattices, no potential;
 cut-off radii down to one cell a side), a Gaussian chain the window kernels
 take beside them in half the contexts, and random run options:
 sweeps_per_launch, measure_every, adapt_ell, graph replay, pair_kernel,
-measure_kernel and the starting sweep word.  Thermalising launches then
-measuring ones: paths, next[], every counter, the adapted COM steps and
-staging lengths and the estimator blocks equal the oracle's schedule."""
+measure_kernel, reweighting derivatives and the starting sweep word.
+Thermalising launches then measuring ones: paths, next[], every counter, the
+adapted COM steps and staging lengths and the estimator blocks equal the
+oracle's schedule."""
 import math
 
 import numpy as np
@@ -85,6 +86,8 @@ def context(trial):
                adapt_ell=int(rng.random() < 0.4), use_graph=int(rng.random() < 0.5),
                pair_kernel=int(rng.integers(0, 3)), measure_kernel=int(rng.integers(0, 3)),
                sweep0=int(rng.integers(0, 7)), n_therm=int(rng.integers(1, 4)), n_meas=int(rng.integers(1, 3)))
+    # reweighting derivatives (every chain then runs whole; refused with a tabulated propagator)
+    opt["derivatives"] = int(rng.random() < 0.3 and all(k != "table" for _, _, k, _ in chains))
     return chains, opt
 
 
@@ -95,7 +98,7 @@ def test_random_context_with_options_matches_oracle_fp64(trial):
     opt = A.Options()
     opt.precision, opt.sweeps_per_launch, opt.seed = A.FP64, spl, seed
     opt.measure_every, opt.adapt_ell, opt.use_graph = o["measure_every"], o["adapt_ell"], o["use_graph"]
-    opt.pair_kernel, opt.measure_kernel = o["pair_kernel"], o["measure_kernel"]
+    opt.pair_kernel, opt.measure_kernel, opt.derivatives = o["pair_kernel"], o["measure_kernel"], o["derivatives"]
     print(f"trial {trial}: options {o}; chains {[(k, d.N, d.M, d.dim, d.swaps, d.hs_a > 0) for d, _, k, _ in chains]}",
           flush=True)
     ctx = A.Context([d for d, _, _, _ in chains], opt)
