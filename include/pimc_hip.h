@@ -1,8 +1,10 @@
 /*
  * pimc_hip.h -- C ABI of the MI355X (gfx950) sampling path of PIMC.jl.
  *
- * The library owns device memory, one HIP stream and the kernels; the caller
- * owns every host buffer (copies are synchronous).  No exception or abort
+ * The library owns device memory, its HIP streams (one per context, and a
+ * second that a context's window-kernel chains may run on, joined back into
+ * the first within every launch) and the kernels; the caller owns every host
+ * buffer (copies are synchronous).  No exception or abort
  * crosses this boundary: every entry point returns PIMC_OK (0) or a negative
  * PIMC_ERR_* code, and pimc_last_error() describes the last failure.  A
  * context is used by one host thread at a time; separate contexts may run

@@ -109,7 +109,6 @@ struct pimc_ctx {
     // Gaussian-pair chains on the window staging and centre-of-mass kernels, with and without
     // cell lists for the estimators (index 0: cells, 1: all-pairs); win_all: both
     std::vector<int> win_ids[2], win_all;
-    int *d_win[2] = {nullptr, nullptr};
     int *d_win_all = nullptr;
     int win_seg = 0, win_maxN = 0, win_maxM = 0, win_maxell = 0, win_maxM0 = 0;
     int cap_stage = 0, cap_cspec = 0, cap_cdec = 0;   // pair-slot pools of the three kernels
@@ -123,6 +122,21 @@ struct pimc_ctx {
     void *d_pm = nullptr;                             // particle-major copy of the window chains' paths
     double *d_win_part = nullptr;                     // estimator partials of the window chains [chain][window][6]
     size_t lds_meas = 0;
+    // The window chains in one group, or in two of about equal beads when there are two or more
+    // and they fill at most half of the CUs: group g is win_all[wg_off[g] .. wg_off[g] + wg_n[g]);
+    // group 1 runs on stream2, which each launch step forks from stream (ev_fork) and joins back
+    // into it (ev_join).  Chains share no buffer, so the groups overlap freely: one group's
+    // decision kernel (a workgroup per chain) runs beside the other group's staging and
+    // speculative sums, half a sweep apart (ev_stagger).
+    int n_wgroup = 0;
+    int wg_off[2] = {0, 0}, wg_n[2] = {0, 0};
+    std::vector<int> wg_cells[2];                     // the group's chains with cell lists
+    int *d_wg_cells[2] = {nullptr, nullptr};
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // per sweep of a launch: group 0's speculative sums done; group 1's sweep starts after it, so
+    // the groups run half a sweep apart instead of in step
+    std::vector<hipEvent_t> ev_stagger;
     int cap_meas = 0;
     std::vector<int> cells_all_ids;               // every chain with cell lists
     int *d_cells_all = nullptr;
@@ -497,20 +511,6 @@ template <typename T, int DIM> void launch_meas_d(pimc_ctx *c, int sweep_off)
         hipLaunchKernelGGL(k_meas_fold, dim3((unsigned)((nall + 127) / 128)), dim3(128), 0, c->stream, m);
         HIPCHK(hipGetLastError());
     }
-    // the window chains: by centre-of-mass windows, then the fold in window order
-    const int nw = (int)c->win_all.size();
-    if (nw > 0) {
-        MeasArgs mw = m;
-        mw.ids = c->d_win_all;
-        mw.n_ids = nw;
-        mw.nblk = c->spec_nwin;
-        mw.part = c->d_win_part;
-        hipLaunchKernelGGL((k_meas_win<T, DIM>), dim3((unsigned)c->spec_nwin, (unsigned)nw), dim3(512), c->lds_meas,
-                           c->stream, mw, c->spec_nwin, c->cap_meas);
-        HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_meas_fold, dim3((unsigned)((nw + 127) / 128)), dim3(128), 0, c->stream, mw);
-        HIPCHK(hipGetLastError());
-    }
 }
 
 void launch_meas(pimc_ctx *c, int sweep_off)
@@ -524,62 +524,116 @@ void launch_meas(pimc_ctx *c, int sweep_off)
     }
 }
 
-// One sweep of the Gaussian-pair chains on the window kernels: the staging
-// windows, then the centre-of-mass speculation, decisions and moves.
-template <typename T, int DIM> void launch_win_d(pimc_ctx *c, const SweepArgs &b)
+// The stream of window group g.
+hipStream_t win_stream(const pimc_ctx *c, int g) { return g == 0 ? c->stream : c->stream2; }
+
+// The estimators of one sweep of window group g: by centre-of-mass windows, then the fold in
+// window order (the partials of the group's chains start at its offset in win_all).
+template <typename T, int DIM> void launch_meas_win_d(pimc_ctx *c, int sweep_off, int g)
 {
-    const int n = (int)c->win_all.size();
+    const int nw = c->wg_n[g];
+    if (nw == 0) return;
+    MeasArgs mw{};
+    mw.chains = c->d_ch;
+    mw.prm = c->d_prm;
+    mw.pos = c->d_pos;
+    mw.cell_cnt = c->d_cell_cnt;
+    mw.cell_mem = c->d_cell_mem;
+    mw.bins = c->d_bins;
+    mw.next = c->d_next;
+    mw.radial = c->d_radial;
+    mw.cart = c->d_cart;
+    mw.radial_bins = c->opt.radial_bins;
+    mw.radial_winv = c->opt.radial_bins > 0 ? c->opt.radial_bins / c->opt.radial_rmax : 0.0;
+    mw.cart_bins = c->opt.cart_bins;
+    mw.chain_meas = c->d_chain_meas;
+    mw.rs = c->d_rs;
+    mw.sweep_off = sweep_off;
+    mw.measure_every = c->opt.measure_every;
+    mw.ids = c->d_win_all + c->wg_off[g];
+    mw.n_ids = nw;
+    mw.nblk = c->spec_nwin;
+    mw.part = c->d_win_part + (size_t)c->wg_off[g] * c->spec_nwin * 6;
+    const hipStream_t st = win_stream(c, g);
+    hipLaunchKernelGGL((k_meas_win<T, DIM>), dim3((unsigned)c->spec_nwin, (unsigned)nw), dim3(512), c->lds_meas, st,
+                       mw, c->spec_nwin, c->cap_meas);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_meas_fold, dim3((unsigned)((nw + 127) / 128)), dim3(128), 0, st, mw);
+    HIPCHK(hipGetLastError());
+}
+
+void launch_meas_win(pimc_ctx *c, int sweep_off, int g)
+{
+    if (c->prec == PIMC_FP64) {
+        if (c->dim == 1) launch_meas_win_d<double, 1>(c, sweep_off, g);
+        else launch_meas_win_d<double, 2>(c, sweep_off, g);
+    } else {
+        if (c->dim == 1) launch_meas_win_d<float, 1>(c, sweep_off, g);
+        else launch_meas_win_d<float, 2>(c, sweep_off, g);
+    }
+}
+
+// One sweep of window group g on the window kernels: the staging windows, then the
+// centre-of-mass speculation, decisions and moves.
+template <typename T, int DIM> void launch_win_d(pimc_ctx *c, const SweepArgs &b, int g, hipEvent_t stagger)
+{
+    const int n = c->wg_n[g];
     if (n == 0) return;
     SweepArgs w = b;
-    w.whole = c->d_win_all;
-    hipLaunchKernelGGL((k_stage_win<T, DIM>), dim3((unsigned)c->win_seg, (unsigned)n), dim3(512), c->lds_stage,
-                       c->stream, w, c->cap_stage);
+    w.whole = c->d_win_all + c->wg_off[g];
+    const hipStream_t st = win_stream(c, g);
+    if (stagger && g == 1) HIPCHK(hipStreamWaitEvent(st, stagger, 0));
+    hipLaunchKernelGGL((k_stage_win<T, DIM>), dim3((unsigned)c->win_seg, (unsigned)n), dim3(512), c->lds_stage, st,
+                       w, c->cap_stage);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_com_spec<T, DIM>), dim3((unsigned)c->spec_nwin, (unsigned)n), dim3(512), c->lds_cspec,
-                       c->stream, w, c->d_spec, static_cast<float4 *>(c->d_wbnd), c->spec_nwin, c->cap_cspec,
+    hipLaunchKernelGGL((k_com_spec<T, DIM>), dim3((unsigned)c->spec_nwin, (unsigned)n), dim3(512), c->lds_cspec, st,
+                       w, c->d_spec, static_cast<float4 *>(c->d_wbnd), c->spec_nwin, c->cap_cspec,
                        static_cast<T *>(c->d_pm));
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_com_dec<T, DIM>), dim3((unsigned)n), dim3(1024), c->lds_cdec, c->stream, w,
+    if (stagger && g == 0) HIPCHK(hipEventRecord(stagger, st));
+    hipLaunchKernelGGL((k_com_dec<T, DIM>), dim3((unsigned)n), dim3(1024), c->lds_cdec, st, w,
                        (const double *)c->d_spec, static_cast<const float4 *>(c->d_wbnd), c->spec_nwin, c->cap_cdec,
                        kChunk, c->d_cacc, static_cast<T *>(c->d_cshift), static_cast<const T *>(c->d_pm));
     HIPCHK(hipGetLastError());
     const long long beads = (long long)c->win_maxM * c->win_maxN;
-    hipLaunchKernelGGL((k_com_apply<T, DIM>), dim3((unsigned)((beads + 255) / 256), (unsigned)n), dim3(256), 0,
-                       c->stream, w, (const unsigned char *)c->d_cacc, static_cast<const T *>(c->d_cshift));
+    hipLaunchKernelGGL((k_com_apply<T, DIM>), dim3((unsigned)((beads + 255) / 256), (unsigned)n), dim3(256), 0, st, w,
+                       (const unsigned char *)c->d_cacc, static_cast<const T *>(c->d_cshift));
     HIPCHK(hipGetLastError());
 }
 
-void launch_win(pimc_ctx *c, const SweepArgs &b)
+void launch_win(pimc_ctx *c, const SweepArgs &b, int g)
 {
+    // sweep b.sweep_off of a launch with two groups: group 0 records its event, group 1 waits on it
+    hipEvent_t stagger = c->n_wgroup == 2 ? c->ev_stagger[b.sweep_off] : nullptr;
     if (c->prec == PIMC_FP64) {
-        if (c->dim == 1) launch_win_d<double, 1>(c, b);
-        else launch_win_d<double, 2>(c, b);
+        if (c->dim == 1) launch_win_d<double, 1>(c, b, g, stagger);
+        else launch_win_d<double, 2>(c, b, g, stagger);
     } else {
-        if (c->dim == 1) launch_win_d<float, 1>(c, b);
-        else launch_win_d<float, 2>(c, b);
+        if (c->dim == 1) launch_win_d<float, 1>(c, b, g, stagger);
+        else launch_win_d<float, 2>(c, b, g, stagger);
     }
 }
 
-// Cell lists of the window chains that have them, rebuilt from the positions.
-template <typename T, int DIM> void launch_cells_win_d(pimc_ctx *c)
+// Cell lists of window group g's chains that have them, rebuilt from the positions.
+template <typename T, int DIM> void launch_cells_win_d(pimc_ctx *c, int g)
 {
-    const int n = (int)c->win_ids[0].size();
+    const int n = (int)c->wg_cells[g].size();
     if (n == 0) return;
     const int wpg = c->cells_wpg;
     hipLaunchKernelGGL((k_cells_win<T, DIM>), dim3((unsigned)((c->win_maxM0 + wpg - 1) / wpg), (unsigned)n),
-                       dim3(64 * wpg), c->lds_cells, c->stream, c->d_ch, c->d_prm, c->d_win[0], n, c->d_pos,
-                       c->d_cell_cnt, c->d_cell_mem, c->d_bins, c->d_status, c->win_ncell);
+                       dim3(64 * wpg), c->lds_cells, win_stream(c, g), c->d_ch, c->d_prm, c->d_wg_cells[g], n,
+                       c->d_pos, c->d_cell_cnt, c->d_cell_mem, c->d_bins, c->d_status, c->win_ncell);
     HIPCHK(hipGetLastError());
 }
 
-void launch_cells_win(pimc_ctx *c)
+void launch_cells_win(pimc_ctx *c, int g)
 {
     if (c->prec == PIMC_FP64) {
-        if (c->dim == 1) launch_cells_win_d<double, 1>(c);
-        else launch_cells_win_d<double, 2>(c);
+        if (c->dim == 1) launch_cells_win_d<double, 1>(c, g);
+        else launch_cells_win_d<double, 2>(c, g);
     } else {
-        if (c->dim == 1) launch_cells_win_d<float, 1>(c);
-        else launch_cells_win_d<float, 2>(c);
+        if (c->dim == 1) launch_cells_win_d<float, 1>(c, g);
+        else launch_cells_win_d<float, 2>(c, g);
     }
 }
 
@@ -614,17 +668,21 @@ void launch_step(pimc_ctx *c, int k, int measure)
     a.nsweeps = k;
     a.measure = measure ? 1 : 0;
     a.measure_every = c->opt.measure_every;
-    if (!c->tiles.empty()) launch_tiles(c, a);
-    const bool win = !c->win_ids[0].empty() || !c->win_ids[1].empty();
-    if (win) {
-        // the window kernels add their counts into the launch's per-chain outputs
-        for (int v = 0; v < 2; ++v)
-            if (!c->win_ids[v].empty()) {
-                hipLaunchKernelGGL(k_clear_chain_meas, dim3((unsigned)c->win_ids[v].size()), dim3(32), 0, c->stream,
-                                   c->d_chain_meas, c->d_win[v], (int)c->win_ids[v].size());
-                HIPCHK(hipGetLastError());
-            }
+    const bool win = !c->win_all.empty();
+    const int ng = win ? c->n_wgroup : 0;
+    if (ng == 2) {
+        // window group 1 starts after the previous launch step (sweep word advanced)
+        HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
     }
+    if (!c->tiles.empty()) launch_tiles(c, a);
+    // the window kernels add their counts into the launch's per-chain outputs
+    for (int g = 0; g < ng; ++g)
+        if (c->wg_n[g] > 0) {
+            hipLaunchKernelGGL(k_clear_chain_meas, dim3((unsigned)c->wg_n[g]), dim3(32), 0, win_stream(c, g),
+                               c->d_chain_meas, c->d_win_all + c->wg_off[g], c->wg_n[g]);
+            HIPCHK(hipGetLastError());
+        }
     if (!c->whole_ids.empty() || win) {
         const bool old = !c->whole_ids.empty();
         if (measure && (c->split_meas || win)) {
@@ -637,10 +695,10 @@ void launch_step(pimc_ctx *c, int k, int measure)
                 b.sweep_off = s;
                 b.split = 1;
                 if (old && c->split_meas) launch_whole(c, b);
-                if (win) launch_win(c, b);
+                for (int g = 0; g < ng; ++g) launch_win(c, b, g);
                 launch_meas(c, s);
+                for (int g = 0; g < ng; ++g) launch_meas_win(c, s, g);
             }
-            if (win) launch_cells_win(c);
         } else {
             if (old) launch_whole(c, a);
             for (int s = 0; win && s < k; ++s) {
@@ -648,10 +706,15 @@ void launch_step(pimc_ctx *c, int k, int measure)
                 b.nsweeps = 1;
                 b.sweep_off = s;
                 b.split = 1;
-                launch_win(c, b);
+                for (int g = 0; g < ng; ++g) launch_win(c, b, g);
             }
-            if (win) launch_cells_win(c);
         }
+        for (int g = 0; g < ng; ++g) launch_cells_win(c, g);
+    }
+    if (ng == 2) {
+        // every launch step ends on stream: k_post reads the window chains' outputs
+        HIPCHK(hipEventRecord(c->ev_join, c->stream2));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
     PostArgs pa{};
     pa.chains = c->d_ch;
@@ -1017,7 +1080,6 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             HIPCHK(hipGetDeviceProperties(&p, c->dev));
             cu = p.multiProcessorCount;
         }
-        (void)cu;
 
         // ---- chain table; whole-chain kernel for pair chains (and all chains with derivatives)
         c->ch.resize(n_chains);
@@ -1296,11 +1358,6 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
         if (!c->meas_ids.empty())
             HIPCHK(hipMemcpy(c->d_meas_ids, c->meas_ids.data(), sizeof(int) * c->meas_ids.size(), hipMemcpyHostToDevice));
         c->d_meas_part = dalloc<double>(!c->meas_ids.empty() ? c->meas_ids.size() * (size_t)c->meas_nblk * 6 : 1);
-        for (int v = 0; v < 2; ++v) {
-            c->d_win[v] = dalloc<int>(c->win_ids[v].size());
-            if (!c->win_ids[v].empty())
-                HIPCHK(hipMemcpy(c->d_win[v], c->win_ids[v].data(), sizeof(int) * c->win_ids[v].size(), hipMemcpyHostToDevice));
-        }
         c->d_win_all = dalloc<int>(c->win_all.size());
         if (!c->win_all.empty()) {
             HIPCHK(hipMemcpy(c->d_win_all, c->win_all.data(), sizeof(int) * c->win_all.size(), hipMemcpyHostToDevice));
@@ -1333,6 +1390,47 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
             HIPCHK(hipMalloc(&c->d_cshift, (size_t)c->n_part * 2 * ts));
             HIPCHK(hipMalloc(&c->d_pm, std::max<size_t>(1, (size_t)c->n_pos * ts)));
             win_attrs(c);
+            // groups: the shortest prefix of win_all holding at least half of the beads, when the
+            // window chains fill at most half of the CUs (one decision workgroup a chain: with
+            // more, the decision kernels already occupy the chip and a second group only contends
+            // for it; measured on lattice, 64 chains, and gauss_box, 256)
+            const int nwc = (int)c->win_all.size();
+            const bool two = nwc >= 2 && 2 * nwc <= cu;
+            int split = nwc;
+            if (two) {
+                long long tot = 0, acc = 0;
+                for (int id : c->win_all) tot += (long long)c->ch[id].N * c->ch[id].M;
+                split = nwc - 1;
+                for (int i = 0; i < nwc - 1; ++i) {
+                    acc += (long long)c->ch[c->win_all[i]].N * c->ch[c->win_all[i]].M;
+                    if (2 * acc >= tot) {
+                        split = i + 1;
+                        break;
+                    }
+                }
+            }
+            c->n_wgroup = two ? 2 : 1;
+            c->wg_off[0] = 0;
+            c->wg_n[0] = split;
+            c->wg_off[1] = split;
+            c->wg_n[1] = nwc - split;
+            std::vector<char> has_cells(n_chains, 0);
+            for (int id : c->win_ids[0]) has_cells[id] = 1;
+            for (int g = 0; g < c->n_wgroup; ++g) {
+                for (int i = c->wg_off[g]; i < c->wg_off[g] + c->wg_n[g]; ++i)
+                    if (has_cells[c->win_all[i]]) c->wg_cells[g].push_back(c->win_all[i]);
+                c->d_wg_cells[g] = dalloc<int>(c->wg_cells[g].size());
+                if (!c->wg_cells[g].empty())
+                    HIPCHK(hipMemcpy(c->d_wg_cells[g], c->wg_cells[g].data(), sizeof(int) * c->wg_cells[g].size(),
+                                     hipMemcpyHostToDevice));
+            }
+            if (c->n_wgroup == 2) {
+                HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+                HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+                HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+                c->ev_stagger.assign(c->opt.sweeps_per_launch, nullptr);
+                for (auto &e : c->ev_stagger) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            }
         }
         c->d_cells_all = dalloc<int>(c->cells_all_ids.size());
         if (!c->cells_all_ids.empty())
@@ -1392,8 +1490,8 @@ int pimc_destroy(pimc_ctx *c)
                     c->d_step, c->d_cnt, c->d_chain_meas, c->d_blocks, c->d_radial, c->d_cart, c->d_whole,
                     c->d_whole_cells, c->d_whole_brute, c->d_is_whole, c->d_cell_cnt, c->d_bins, c->d_cell_mem,
                     c->d_status, c->d_work, c->d_io, c->d_whole_lds, c->d_next, c->d_meas_ids, c->d_meas_part,
-                    c->d_win[0], c->d_win[1], c->d_cells_all, c->d_spec, c->d_wbnd, c->d_win_all, c->d_cacc, c->d_cshift, c->d_pm, c->d_win_part,
-                    c->d_beams};
+                    c->d_cells_all, c->d_spec, c->d_wbnd, c->d_win_all, c->d_cacc, c->d_cshift, c->d_pm, c->d_win_part,
+                    c->d_beams, c->d_wg_cells[0], c->d_wg_cells[1]};
     for (void *p : ptrs) dfree(p);
     for (double *p : c->d_tab) dfree(p);
     for (auto &e : c->events) {
@@ -1402,6 +1500,11 @@ int pimc_destroy(pimc_ctx *c)
     }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->ev_stagger)
+        if (e) (void)hipEventDestroy(e);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PIMC_OK;

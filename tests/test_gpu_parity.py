@@ -652,6 +652,41 @@ def test_graph_replay_with_split_estimators_and_tables():
         assert all(np.array_equal(x, y) for x, y in zip(ctxs[0].blocks(c), ctxs[1].blocks(c)))
 
 
+def test_graph_replay_with_two_window_groups():
+    """Three window-kernel chains (two groups on two streams, the second half a
+    sweep behind the first): graph replays of thermalising and measuring steps
+    are bit-identical to eager launches, paths, counters, estimator blocks and
+    both histograms, replay after replay."""
+    rng = np.random.default_rng(62)
+    pair = GaussianPair(1.0, 0.5)
+    descs, inits = [], []
+    for N, M, ell in [(110, 32, 8), (130, 24, 6), (120, 16, 5)]:
+        descs.append(chain_desc(HarmonicTrap(0.3), dim=2, M=M, N=N, L=5.0, beta=2.0, lam=0.5, pair=pair, ell=ell,
+                                r_a=2.0, com_step=0.5))
+        inits.append(rng.uniform(-5.0, 5.0, size=(N, 2, M)))
+    ctxs = []
+    for g in (0, 1):
+        opt = A.Options()
+        opt.precision, opt.sweeps_per_launch, opt.seed, opt.use_graph = A.FP32, 3, 9, g
+        opt.radial_bins, opt.radial_rmax, opt.cart_bins = 24, 4.0, 16
+        ctx = A.Context(descs, opt)
+        for c, r in enumerate(inits):
+            ctx.set_paths(c, r)
+        ctxs.append(ctx)
+    for rep in range(4):
+        measure = rep % 2 == 1
+        cnts = [ctx.run(9, measure=measure) for ctx in ctxs]
+        assert (cnts[0].levy_acc, cnts[0].com_acc) == (cnts[1].levy_acc, cnts[1].com_acc), rep
+        for c in range(3):
+            assert np.array_equal(ctxs[0].get_paths(c), ctxs[1].get_paths(c)), (rep, c)
+            assert np.array_equal(ctxs[0].counters(c), ctxs[1].counters(c)), (rep, c)
+    for c in range(3):
+        assert all(np.array_equal(x, y) for x, y in zip(ctxs[0].blocks(c), ctxs[1].blocks(c)))
+        assert np.array_equal(ctxs[0].radial(c)[0], ctxs[1].radial(c)[0])
+        assert np.array_equal(ctxs[0].cartesian(c)[0], ctxs[1].cartesian(c)[0])
+        assert ctxs[0].counters(c)[3] == 36 * descs[c].N
+
+
 def test_cell_list_overflow_is_reported():
     """All particles of a pair chain in one cell at every slice: set_paths
     reports PIMC_ERR_CAPACITY for that chain instead of sampling with a
