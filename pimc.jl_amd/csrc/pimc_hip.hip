@@ -1543,7 +1543,8 @@ int pimc_set_paths(pimc_ctx *c, int32_t chain, const double *r)
             HIPCHK(hipMemcpy((float *)c->d_pos + h.pos_off, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
         }
         if (h.ncell > 0) {
-            HIPCHK(hipMemset(c->d_status + chain, 0, sizeof(int)));
+            // on the library stream, ahead of the cell build (not in a captured graph)
+            HIPCHK(hipMemsetAsync(c->d_status + chain, 0, sizeof(int), c->stream));
             build_cells(c, chain);
             HIPCHK(hipStreamSynchronize(c->stream));
             return check_status(c);
@@ -2102,8 +2103,15 @@ int pimc_reset_estimators(pimc_ctx *c)
             rs.blk = 0;
             HIPCHK(hipMemcpy(c->d_rs, &rs, sizeof(RunState), hipMemcpyHostToDevice));
         }
-        if (c->d_radial) HIPCHK(hipMemset(c->d_radial, 0, sizeof(unsigned long long) * c->n_chains * c->opt.radial_bins));
-        if (c->d_cart) HIPCHK(hipMemset(c->d_cart, 0, sizeof(unsigned long long) * c->n_chains * c->cart_per_chain));
+        // the histograms are cleared on the library stream, which the next run's kernels follow
+        auto zero = [&](unsigned long long *p, long long n) {
+            hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<long long>(1024, (n + 255) / 256)), dim3(256), 0,
+                               c->stream, p, n);
+            HIPCHK(hipGetLastError());
+        };
+        if (c->d_radial) zero(c->d_radial, (long long)c->n_chains * c->opt.radial_bins);
+        if (c->d_cart) zero(c->d_cart, (long long)c->n_chains * c->cart_per_chain);
+        HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError &e) {
         return hip_err(c, e);
     }
@@ -2248,8 +2256,11 @@ int pimc_density(pimc_ctx *c, int32_t chain, int32_t nbins, double *dens, int64_
     unsigned long long *d_h = nullptr;
     try {
         d_h = dalloc<unsigned long long>(nh);
-        HIPCHK(hipMemset(d_h, 0, sizeof(unsigned long long) * nh));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        // cleared on the library stream, ahead of the counting kernel: a plain hipMemset runs on the
+        // null stream, which the library's non-blocking stream does not wait for
+        const unsigned fb = (unsigned)std::min<size_t>(1024, (nh + 255) / 256);
+        hipLaunchKernelGGL(k_zero_u64, dim3(fb), dim3(256), 0, c->stream, d_h, (long long)nh);
+        HIPCHK(hipGetLastError());
         HookArgs h = hook_args(c, chain);
         const int nb = std::max(1, std::min(1024, (hc.N * hc.M + 255) / 256));
 #define DENS(T, D, S) hipLaunchKernelGGL((k_hook_density<T, D, S>), dim3(nb), dim3(256), 0, c->stream, h, nbins, d_h)

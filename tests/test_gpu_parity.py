@@ -207,6 +207,23 @@ def test_fp32_fast_path_moves_with_injected_draws(pot, ell, M):
         assert np.array_equal(after[n, :, (j0 + i) % M], r_prop[i].astype(np.float32).astype(np.float64))
 
 
+def test_density_after_every_sweep_matches_oracle():
+    """pimc_density (and pimc_density_noalloc) called right after each of 400
+    one-sweep runs: every call's histogram equals the oracle's of the same
+    paths bin for bin (the histogram is cleared on the library stream ahead
+    of the counting kernel; a clear that landed after the counts would show
+    as a deficit)."""
+    rng = np.random.default_rng(16)
+    d = harmonic_desc(N=8, M=64, beta=1.5, L=6.0)
+    ctx = make_ctx([d], A.FP64, spl=1)
+    ctx.set_paths(0, rand_paths(rng, 8, 64, 6.0, scale=1.5))
+    for it in range(400):
+        ctx.run(1, measure=False)
+        dg, ndg = (ctx.density if it % 2 == 0 else ctx.density_noalloc)(0, 40)
+        do, ndo = oracle_sys(d, ctx.get_paths(0)).density(40)
+        assert ndg == ndo == 64 and np.array_equal(dg, do), it
+
+
 @pytest.mark.parametrize("precision", [A.FP64, A.FP32])
 def test_energy_density_hooks(precision):
     rng = np.random.default_rng(15)
