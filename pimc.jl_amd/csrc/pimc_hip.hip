@@ -374,6 +374,14 @@ bool grow_blocks(pimc_ctx *c, int need)
 }
 
 // Block b of one chain from the host copy, refreshed after each run.
+// Host to device, ordered on the library stream: that stream is non-blocking, so a plain
+// hipMemcpy (null stream) is not ordered before its kernels.  Returns when the copy is complete.
+void h2d(pimc_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 const double *host_block(pimc_ctx *c, int b, int chain)
 {
     if (c->hblk_n != c->nblk) {
@@ -991,7 +999,7 @@ void retile_after_adapt(pimc_ctx *c)
         // chains keep their lengths, on the host and on the device, and the tiles stay
         c->ch.swap(before);
         for (int i : changed)
-            HIPCHK(hipMemcpy(c->d_ch + i, &c->ch[i], sizeof(ChainDev), hipMemcpyHostToDevice));
+            h2d(c, c->d_ch + i, &c->ch[i], sizeof(ChainDev));
         return;
     }
     if (c->tiles.size() > c->tiles_cap) {
@@ -1000,7 +1008,7 @@ void retile_after_adapt(pimc_ctx *c)
         c->d_tiles = dalloc<TileDev>(c->tiles.size());
         c->tiles_cap = c->tiles.size();
     }
-    HIPCHK(hipMemcpy(c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size(), hipMemcpyHostToDevice));
+    h2d(c, c->d_tiles, c->tiles.data(), sizeof(TileDev) * c->tiles.size());
     drop_graphs(c);
 }
 
@@ -1530,7 +1538,7 @@ int pimc_set_paths(pimc_ctx *c, int32_t chain, const double *r)
             for (int p = 0; p < h.N; ++p)
                 for (int d = 0; d < h.dim; ++d)
                     for (int j = 0; j < h.M; ++j) f[dst_index(p, d, j)] = r[((size_t)p * h.dim + d) * h.M + j];
-            HIPCHK(hipMemcpy((double *)c->d_pos + h.pos_off, f.data(), n * sizeof(double), hipMemcpyHostToDevice));
+            h2d(c, (double *)c->d_pos + h.pos_off, f.data(), n * sizeof(double));
         } else {
             std::vector<float> f(n);
             for (int p = 0; p < h.N; ++p)
@@ -1540,7 +1548,7 @@ int pimc_set_paths(pimc_ctx *c, int32_t chain, const double *r)
                         if (v >= (float)h.L) v = std::nextafter((float)h.L, -INFINITY);
                         f[dst_index(p, d, j)] = v;
                     }
-            HIPCHK(hipMemcpy((float *)c->d_pos + h.pos_off, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+            h2d(c, (float *)c->d_pos + h.pos_off, f.data(), n * sizeof(float));
         }
         if (h.ncell > 0) {
             // on the library stream, ahead of the cell build (not in a captured graph)
@@ -1571,7 +1579,7 @@ int pimc_set_next(pimc_ctx *c, int32_t chain, const int32_t *next)
     if (!ident && h.swaps == 0) return set_err(c, PIMC_ERR_ARG, "permutation cycles need a chain with swap moves");
     try {
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipMemcpy(c->d_next + h.part_base, next, sizeof(int) * h.N, hipMemcpyHostToDevice));
+        h2d(c, c->d_next + h.part_base, next, sizeof(int) * h.N);
     } catch (const HipError &e) {
         return hip_err(c, e);
     }
@@ -1726,7 +1734,7 @@ int pimc_set_com_step(pimc_ctx *c, int32_t chain, double step)
     if (!(step >= 0)) return set_err(c, PIMC_ERR_ARG, "step must be >= 0");
     try {
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipMemcpy(c->d_step + chain, &step, sizeof(double), hipMemcpyHostToDevice));
+        h2d(c, c->d_step + chain, &step, sizeof(double));
     } catch (const HipError &e) {
         return hip_err(c, e);
     }
@@ -1763,7 +1771,8 @@ int pimc_set_pair_table(pimc_ctx *c, int32_t chain, int32_t n, double r0, double
     try {
         HIPCHK(hipStreamSynchronize(c->stream));
         double *t = dalloc<double>((size_t)n * n);
-        if (hipMemcpy(t, terms, sizeof(double) * n * n, hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMemcpyAsync(t, terms, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
             dfree(t);
             throw HipError{hipErrorInvalidValue, "hipMemcpy(pair table)"};
         }
@@ -1776,17 +1785,17 @@ int pimc_set_pair_table(pimc_ctx *c, int32_t chain, int32_t n, double r0, double
         h.mu = mu;
         // only the table fields: the device copy's staging length may have been adapted (adapt_ell)
         char *dc = reinterpret_cast<char *>(c->d_ch + chain);
-        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, tab), &h.tab, sizeof(h.tab), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, tab_n), &h.tab_n, sizeof(h.tab_n), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, tab_r0), &h.tab_r0, sizeof(h.tab_r0), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, tab_h), &h.tab_h, sizeof(h.tab_h), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dc + offsetof(ChainDev, mu), &h.mu, sizeof(h.mu), hipMemcpyHostToDevice));
+        h2d(c, dc + offsetof(ChainDev, tab), &h.tab, sizeof(h.tab));
+        h2d(c, dc + offsetof(ChainDev, tab_n), &h.tab_n, sizeof(h.tab_n));
+        h2d(c, dc + offsetof(ChainDev, tab_r0), &h.tab_r0, sizeof(h.tab_r0));
+        h2d(c, dc + offsetof(ChainDev, tab_h), &h.tab_h, sizeof(h.tab_h));
+        h2d(c, dc + offsetof(ChainDev, mu), &h.mu, sizeof(h.mu));
         if (c->prec == PIMC_FP64) {
             const Prm<double> pr = Prm<double>::make(h);
-            HIPCHK(hipMemcpy(reinterpret_cast<Prm<double> *>(c->d_prm) + chain, &pr, sizeof(pr), hipMemcpyHostToDevice));
+            h2d(c, reinterpret_cast<Prm<double> *>(c->d_prm) + chain, &pr, sizeof(pr));
         } else {
             const Prm<float> pr = Prm<float>::make(h);
-            HIPCHK(hipMemcpy(reinterpret_cast<Prm<float> *>(c->d_prm) + chain, &pr, sizeof(pr), hipMemcpyHostToDevice));
+            h2d(c, reinterpret_cast<Prm<float> *>(c->d_prm) + chain, &pr, sizeof(pr));
         }
     } catch (const HipError &e) {
         return hip_err(c, e);
@@ -1810,8 +1819,7 @@ int pimc_set_sweep(pimc_ctx *c, int64_t sweep)
         HIPCHK(hipStreamSynchronize(c->stream));
         const unsigned v = (unsigned)sweep;
         // the kernels read the counter from the device run state, so captured graphs stay valid
-        HIPCHK(hipMemcpy(reinterpret_cast<char *>(c->d_rs) + offsetof(RunState, sweep), &v, sizeof(v),
-                         hipMemcpyHostToDevice));
+        h2d(c, reinterpret_cast<char *>(c->d_rs) + offsetof(RunState, sweep), &v, sizeof(v));
         c->sweep = v;
     } catch (const HipError &e) {
         return hip_err(c, e);
@@ -2101,7 +2109,7 @@ int pimc_reset_estimators(pimc_ctx *c)
             RunState rs{};
             HIPCHK(hipMemcpy(&rs, c->d_rs, sizeof(RunState), hipMemcpyDeviceToHost));
             rs.blk = 0;
-            HIPCHK(hipMemcpy(c->d_rs, &rs, sizeof(RunState), hipMemcpyHostToDevice));
+            h2d(c, c->d_rs, &rs, sizeof(RunState));
         }
         // the histograms are cleared on the library stream, which the next run's kernels follow
         auto zero = [&](unsigned long long *p, long long n) {
@@ -2150,7 +2158,7 @@ static int hook_move(pimc_ctx *c, int chain, int n, int j0, int m, const double 
         ensure_io(c, nout + (size_t)nin + 8);
         double *d_in = c->d_io + nout;
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipMemcpy(d_in, in, sizeof(double) * nin, hipMemcpyHostToDevice));
+        h2d(c, d_in, in, sizeof(double) * nin);
         HookArgs h = hook_args(c, chain);
         h.n = n;
         h.j0 = j0;
