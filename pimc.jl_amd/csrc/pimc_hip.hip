@@ -134,9 +134,10 @@ struct pimc_ctx {
     int *d_wg_cells[2] = {nullptr, nullptr};
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    // per sweep of a launch: group 0's speculative sums done; group 1's sweep starts after it, so
-    // the groups run half a sweep apart instead of in step
-    std::vector<hipEvent_t> ev_stagger;
+    // group 0's speculative sums of the current sweep done; group 1's sweep starts after it, so the
+    // groups run half a sweep apart instead of in step (recorded and waited on once per sweep, in
+    // that order, so one event serves every sweep)
+    hipEvent_t ev_stagger = nullptr;
     int cap_meas = 0;
     std::vector<int> cells_all_ids;               // every chain with cell lists
     int *d_cells_all = nullptr;
@@ -611,8 +612,8 @@ template <typename T, int DIM> void launch_win_d(pimc_ctx *c, const SweepArgs &b
 
 void launch_win(pimc_ctx *c, const SweepArgs &b, int g)
 {
-    // sweep b.sweep_off of a launch with two groups: group 0 records its event, group 1 waits on it
-    hipEvent_t stagger = c->n_wgroup == 2 ? c->ev_stagger[b.sweep_off] : nullptr;
+    // with two groups, group 0 records the sweep's event and group 1 (launched next) waits on it
+    hipEvent_t stagger = c->n_wgroup == 2 ? c->ev_stagger : nullptr;
     if (c->prec == PIMC_FP64) {
         if (c->dim == 1) launch_win_d<double, 1>(c, b, g, stagger);
         else launch_win_d<double, 2>(c, b, g, stagger);
@@ -1436,8 +1437,7 @@ int pimc_create(pimc_ctx **out, const pimc_chain_desc *chains, int32_t n_chains,
                 HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
                 HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
                 HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-                c->ev_stagger.assign(c->opt.sweeps_per_launch, nullptr);
-                for (auto &e : c->ev_stagger) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                HIPCHK(hipEventCreateWithFlags(&c->ev_stagger, hipEventDisableTiming));
             }
         }
         c->d_cells_all = dalloc<int>(c->cells_all_ids.size());
@@ -1508,8 +1508,7 @@ int pimc_destroy(pimc_ctx *c)
     }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    for (hipEvent_t e : c->ev_stagger)
-        if (e) (void)hipEventDestroy(e);
+    if (c->ev_stagger) (void)hipEventDestroy(c->ev_stagger);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
